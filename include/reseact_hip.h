@@ -1,0 +1,231 @@
+/*
+ * reseact_hip.h -- C ABI of the operator-split step library (transport SSPRK43 +
+ * per-cell Rosenbrock23 chemistry + discrete adjoint), FP64 throughout.
+ *
+ * Two shared libraries export exactly this ABI:
+ *   libreseact_hip.so   hand-written HIP kernels for gfx950 (the product);
+ *   libreseact_cpu.so   the CPU restatement used as the parity oracle and the
+ *                       CPU baseline (built from oracle/, never shipped).
+ *
+ * Conventions
+ *   * State vectors are SPECIES-MAJOR, longitude fastest: element
+ *     u[s*NC + c] with c = i + NLON*(j + NLAT*k), i = lon, j = lat, k = level
+ *     (k = 0 is the surface).  Same contract as the reference's
+ *     tools/reactant_handoff/rx_traced_integrator.jl:39-43.
+ *   * Every call returns an int status: RS_OK (0) or one of RS_E_*; the
+ *     message of the last failure is rs_last_error(ctx).
+ *   * A context owns all device memory; the caller owns host buffers.  Calls on
+ *     one context are not thread-safe; use one context per host thread.
+ *   * The transport operator acts on species 0..NT-1; species NT..NS-1 are
+ *     left bit-for-bit untouched by it.  Error norms are always taken over the
+ *     full NS*NC state, as the reference's operators both see the full state
+ *     (tools/reactant_handoff/op_split.jl:107-109).
+ */
+#ifndef RESEACT_HIP_H
+#define RESEACT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define RS_API __attribute__((visibility("default")))
+#else
+#define RS_API
+#endif
+
+#define RS_OK           0
+#define RS_E_ARG        1   /* bad argument / call order                       */
+#define RS_E_HIP        2   /* a HIP runtime call failed                       */
+#define RS_E_NONFINITE  3   /* a non-finite value where none is allowed        */
+#define RS_E_MAXITERS   4   /* adaptive loop hit maxiters (adjoint_gradient.jl:580) */
+#define RS_E_STATE      5   /* missing setup (grid / mechanism / state)        */
+#define RS_E_JIT        6   /* runtime compilation of the mechanism kernels    */
+
+typedef struct rs_ctx rs_ctx;
+
+/* PI step-size controller.  Reference: PICtrl, rx_traced_integrator.jl:86-95;
+ * defaults pictrl_ros23 / pictrl_ssprk43 at :98-99. */
+typedef struct rs_pictrl {
+    double beta1, beta2;
+    double qmin, qmax, gamma;
+    double qsteady_min, qsteady_max;
+    double qoldinit;
+} rs_pictrl;
+
+/* Outcome of one adaptive solve (adaptive_solve returns (u, t, dt, nacc, nrej),
+ * rx_traced_integrator.jl:455-507). */
+typedef struct rs_counts {
+    long long naccept, nreject;
+    double t_end;     /* time reached                                          */
+    double dt_end;    /* proposed dt carried into the next solve                */
+} rs_counts;
+
+/* Options of one Lie-Trotter macro step (adjoint_gradient.jl:609-619 and the
+ * tolerances/controllers of run_reseact_reactant.jl:144,310-355). */
+typedef struct rs_split_opts {
+    rs_pictrl ctrl_T, ctrl_C;
+    double atol_T, rtol_T, atol_C, rtol_C;
+    int clamp_nonneg;        /* max(u,0) after every accepted inner step       */
+    int chem_percell;        /* 0: one global error norm (reference), 1: per cell */
+    long long maxiters;
+} rs_split_opts;
+
+/* Per-cell (work-stealing) chemistry statistics. */
+typedef struct rs_percell_stats {
+    long long attempts_total, accepts_total, rejects_total;
+    long long attempts_max, attempts_min;   /* over cells                       */
+    long long n_failed;                     /* cells that hit maxiters          */
+} rs_percell_stats;
+
+/* Statistics of a forward + backward (adjoint) run. */
+typedef struct rs_adjoint_stats {
+    long long naccept_T, naccept_C, nreject_T, nreject_C;
+    long long nvjp_T, nvjp_C;
+    double seconds_forward, seconds_backward, seconds_replay;
+    double replay_maxrel;    /* replayed end state vs checkpoint (adjoint_gradient.jl:802-808) */
+} rs_adjoint_stats;
+
+/* ---- lifecycle ------------------------------------------------------------ */
+RS_API int         rs_create(rs_ctx** out, int device);
+RS_API int         rs_destroy(rs_ctx* ctx);
+RS_API const char* rs_last_error(const rs_ctx* ctx);
+RS_API const char* rs_backend(void);                 /* "hip-gfx950" or "cpu-oracle" */
+
+/* ---- setup ----------------------------------------------------------------
+ * rs_set_grid: NLON x NLAT x NLEV cells, NS species of which the first NT are
+ * transported.  Replaces the model build `(f, u0, p, tspan, var_map)` of
+ * prototypes/reseact_3d_chem/split_common.jl:530-533 with explicit sizes.     */
+RS_API int rs_set_grid(rs_ctx* ctx, int nlon, int nlat, int nlev, int ns, int nt);
+
+/* Flux-form advection coefficients (units 1/s), periodic in longitude:
+ *   cx[j]      zonal rate u_j / (a cos(phi_j) dlambda) of latitude row j;
+ *   wz[k]      vertical face profile w(z_k)/dz at faces k = 0..NLEV
+ *              (wz[0] = wz[NLEV] = 0: closed bottom and top);
+ *   wl[i]      longitude factor; vertical rate at face (i,j,k) = wz[k]*wl[i].
+ * Replaces the transport RHS f_T of reseact.esm:1564-3100 (the synthetic
+ * 3rd-order upwind operator of the workloads). */
+RS_API int rs_set_wind(rs_ctx* ctx, const double* cx, const double* wz, const double* wl);
+
+/* Mass-action mechanism, NR reactions: reactants[2r], reactants[2r+1]
+ * (second = -1 for unimolecular; equal entries = 2A), products[2r],
+ * products[2r+1] (-1 = none), rate constants k[r] (1/s or 1/(conc s)),
+ * photo[r] != 0 marks a photolysis reaction whose rate is k[r]*jfac[cell].
+ * Replaces the chemistry RHS f_C and its block Jacobian
+ * (rx_traced_integrator.jl:151-253, prototypes/reseact_3d_chem/block_jac.jl:57-136). */
+RS_API int rs_set_mechanism(rs_ctx* ctx, int nr, const int* reactants, const int* products,
+                     const double* k, const int* photo);
+/* Per-cell photolysis factor (NC values, e.g. max(0, cos SZA)); NULL = 1. */
+RS_API int rs_set_photolysis(rs_ctx* ctx, const double* jfac);
+/* Surface emission source E[s*NLAT*NLON + j*NLON + i] (conc/s) added to the
+ * chemistry tendency of level-0 cells (reseact.esm:12132); NULL = none. */
+RS_API int rs_set_emissions(rs_ctx* ctx, const double* emis);
+
+/* ---- state ---------------------------------------------------------------- */
+RS_API int rs_set_state(rs_ctx* ctx, const double* u);   /* NS*NC, host -> device   */
+RS_API int rs_get_state(rs_ctx* ctx, double* u);         /* NS*NC, device -> host   */
+
+/* ---- single attempts ------------------------------------------------------
+ * One SSPRK43 / ROS23 attempt from the current state with step dt; the raw
+ * result becomes the "candidate" and *eest its Hairer RMS error.
+ * Reference: ssprk43_step_unrolled rx_traced_integrator.jl:404-437,
+ *            ros23_step :279-349 (jac = :sym, an exact Jacobian).           */
+RS_API int rs_ssprk43_step(rs_ctx* ctx, double t, double dt, double atol, double rtol, double* eest);
+RS_API int rs_ros23_step(rs_ctx* ctx, double t, double dt, double atol, double rtol, double* eest);
+RS_API int rs_get_candidate(rs_ctx* ctx, double* u);
+RS_API int rs_accept_candidate(rs_ctx* ctx, int clamp_nonneg);
+
+/* ---- adaptive solves ------------------------------------------------------
+ * The PI-controlled accept/reject loop over [t0, t1] with a single global
+ * error norm.  *dt_io: initial dt in, proposed next dt out.
+ * Reference: adaptive_solve rx_traced_integrator.jl:455-507 and its host twin
+ * host_adaptive! tools/adjoint_gradient.jl:538-582.                         */
+RS_API int rs_adaptive_transport(rs_ctx* ctx, double t0, double t1, double* dt_io,
+                          const rs_pictrl* ctrl, double atol, double rtol,
+                          int clamp_nonneg, long long maxiters, rs_counts* out);
+RS_API int rs_adaptive_chem(rs_ctx* ctx, double t0, double t1, double* dt_io,
+                     const rs_pictrl* ctrl, double atol, double rtol,
+                     int clamp_nonneg, long long maxiters, rs_counts* out);
+
+/* Per-cell adaptive chemistry: every cell integrates [t0, t1] as its own ODE
+ * (the reference loop applied with NC = 1), all cells in one persistent
+ * work-stealing launch.  dt_cells (NC values, may be NULL = dt0 everywhere)
+ * carries each cell's proposed dt in and out. */
+RS_API int rs_chem_percell(rs_ctx* ctx, double t0, double t1, double dt0, double* dt_cells,
+                    const rs_pictrl* ctrl, double atol, double rtol,
+                    int clamp_nonneg, long long maxiters, rs_percell_stats* out);
+
+/* One Lie-Trotter macro step: transport over [t0,t1] then chemistry over the
+ * same interval from the transported state.
+ * Reference: macro_step tools/adjoint_gradient.jl:609-619;
+ *            lie_trotter_solve tools/reactant_handoff/op_split.jl:100-164. */
+RS_API int rs_macro_step(rs_ctx* ctx, double t0, double t1, double* dtT_io, double* dtC_io,
+                  const rs_split_opts* opts, rs_counts* cT, rs_counts* cC);
+
+/* ---- discrete adjoint of one step (fixed dt) ------------------------------
+ * lambda <- (d unew / d u)^T lambda at the current state; for ROS23 also
+ * grad_emis += (d unew / d E)^T lambda.  The Rosenbrock stages are recomputed
+ * from the state, not stored.
+ * Reference: ssprk43_step_vjp / ros23_step_vjp rx_traced_integrator.jl:610-623. */
+RS_API int rs_set_lambda(rs_ctx* ctx, const double* lam);      /* NS*NC               */
+RS_API int rs_get_lambda(rs_ctx* ctx, double* lam);
+RS_API int rs_zero_grad(rs_ctx* ctx);
+RS_API int rs_get_grad_emis(rs_ctx* ctx, double* g);           /* NS*NLAT*NLON        */
+RS_API int rs_ssprk43_vjp(rs_ctx* ctx, double t, double dt);
+RS_API int rs_ros23_vjp(rs_ctx* ctx, double t, double dt);
+
+/* ---- time-loop adjoint ----------------------------------------------------
+ * Forward pass over macro stops tstops[0..nmacro] with a checkpoint of the
+ * state and the accepted (t, dt) sequences at every macro step, then the
+ * backward sweep: per macro step (last first) a fixed-sequence replay from
+ * its checkpoint, chemistry VJPs (last step first), then transport VJPs, with
+ * the clamp mask lambda *= 1[u_raw > 0] before every step.
+ * Loss (loss_kind 0): J = sum over level-0 cells of u[loss_species]^2.
+ * Outputs: *J_out, grad_u0 (NS*NC), grad_emis (NS*NLAT*NLON); either
+ * gradient pointer may be NULL.
+ * Reference: forward_pass / replay_fixed / backward_stage! / backward_sweep,
+ * tools/adjoint_gradient.jl:647-834.                                        */
+RS_API int rs_adjoint_run(rs_ctx* ctx, int nmacro, const double* tstops,
+                   double dtT0, double dtC0, const rs_split_opts* opts,
+                   int loss_kind, int loss_species,
+                   double* J_out, double* grad_u0, double* grad_emis,
+                   rs_adjoint_stats* stats);
+
+/* Frozen-sequence forward map used by the finite-difference checks of the
+ * time-loop adjoint (adjoint_gradient.jl:1183-1244 "fdtape"): re-runs the
+ * accepted step sequences recorded by the last rs_adjoint_run from the
+ * current state with the current emissions, and returns the loss. */
+RS_API int rs_replay_loss(rs_ctx* ctx, int loss_kind, int loss_species, double* J_out);
+
+
+/* ---- extensions with no counterpart in the reference ----------------------
+ * Elimination order of the per-cell sparse LU, set before rs_set_mechanism:
+ * 0 = natural species order (the reference blocksolve's order, operation for
+ * operation), 1 = Markowitz fill-reducing order (default; fewer flops). */
+RS_API int rs_set_lu_ordering(rs_ctx* ctx, int ordering);
+
+/* Static work of the compiled mechanism kernels (roofline accounting). */
+typedef struct rs_mech_info {
+    int ns, nr, nnz_J, nnz_LU, ordering;
+    long long flops_attempt;     /* FP64 flops of one ROS23 attempt on one cell  */
+    long long flops_vjp;         /* FP64 flops of one ROS23 step VJP on one cell */
+    int regs_attempt, regs_percell, regs_vjp;   /* VGPRs per lane (0 on CPU)   */
+    long long scratch_attempt, scratch_percell, scratch_vjp;  /* spill bytes/lane */
+} rs_mech_info;
+RS_API int rs_mechanism_info(rs_ctx* ctx, rs_mech_info* out);
+
+/* Per-kernel timing with HIP events recorded on the library's stream around
+ * every launch while enabled (rs_profile(ctx, 1)); rs_profile_read returns
+ * launches and total milliseconds per kernel kind. */
+typedef struct rs_kernel_time {
+    char name[32];
+    long long launches;
+    double total_ms;
+} rs_kernel_time;
+RS_API int rs_profile(rs_ctx* ctx, int enable);
+RS_API int rs_profile_read(rs_ctx* ctx, rs_kernel_time* out, int max, int* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RESEACT_HIP_H */

@@ -1,0 +1,21 @@
+/* oracle_extra.h -- oracle-only entry points of libreseact_cpu.so (test
+ * infrastructure: tangents and operator-level probes used by tests/ as the
+ * checker).  The HIP library does not export these. */
+#ifndef RESEACT_ORACLE_EXTRA_H
+#define RESEACT_ORACLE_EXTRA_H
+#include <stddef.h>
+#include "../include/reseact_hip.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+int rs_oracle_ros23_jvp(rs_ctx* c, double dt, const double* v, const double* Ev, double* out);
+int rs_oracle_ssprk43_jvp(rs_ctx* c, double dt, const double* v, double* out);
+int rs_oracle_transport_rhs(rs_ctx* c, const double* q, double* f);
+int rs_oracle_transport_rhs_adj(rs_ctx* c, const double* lam, double* g);
+int rs_oracle_chem_rhs(rs_ctx* c, const double* u, double* f);
+int rs_oracle_chem_jac(rs_ctx* c, const double* u, size_t cell, double* J);
+int rs_oracle_threads(void);
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1173 @@
+// reseact_cpu.cpp -- the CPU ORACLE of the operator-split step.
+//
+// TEST INFRASTRUCTURE, NOT PRODUCT CODE.  Only tests/, __graft_entry__.smoke()
+// and bench.py's cpu_baseline leg may load libreseact_cpu.so, and only as the
+// checker (or as the timed CPU baseline, kind "port").  The product path is
+// libreseact_hip.so; nothing in reseact_esm_amd/ loads this library.
+//
+// It is a plain FP64 restatement of the reference's Julia algebra, behind the
+// same C ABI as the HIP library (include/reseact_hip.h), so every parity test is
+// "same call, two libraries, compare".  Built with -ffp-contract=off so that
+// each statement rounds exactly as the reference's one-operation broadcasts do.
+// OpenMP parallelises over cells / rows only; every reduction is done in a fixed
+// order, so results do not depend on the thread count.
+//
+// Function -> reference map (file:line under EarthSciML/reseact.esm):
+//   pi_update            PICtrl + adaptive loop body, tools/reactant_handoff/
+//                        rx_traced_integrator.jl:86-99, :480-505;
+//                        host_adaptive! tools/adjoint_gradient.jl:538-582
+//   blocksolve           rx_traced_integrator.jl:110-135 (pivot-free elimination)
+//   ssprk43_attempt      ssprk43_step_unrolled rx_traced_integrator.jl:404-437
+//   ros23_cell           ros23_step rx_traced_integrator.jl:279-349, jac = :sym
+//   macro step           macro_step tools/adjoint_gradient.jl:609-619
+//   replay / adjoint     replay_fixed :647-661, forward_pass :671-688,
+//                        backward_stage! :731-784, backward_sweep :813-834
+//   one-step VJPs        hand-derived adjoints of the two step maps (the
+//                        reference obtains them with Enzyme, :610-623)
+// The transport RHS (3rd-order upwind, periodic longitude) and the synthetic
+// mass-action chemistry are the workloads' own definitions; the reference's
+// PPM / SuperFast right-hand sides are not on this path.
+//
+// Parity pins (tests/): SSPRK43 order 3 and ROS23 order 2 / L-stability on
+// linear problems against expm, transport mass conservation and freestream,
+// upwind order-3 convergence, analytic exp(Kt) for first-order mechanisms,
+// FD-vs-analytic Jacobian, JVP-vs-FD, dot-product identity VJP<->JVP, and
+// the reference's own toy problem thresholds (tools/rx_adjoint_toy.jl:17-24).
+
+#include "../include/reseact_hip.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+const double ROS23_D = 1.0 / (2.0 + std::sqrt(2.0));   // rx_traced_integrator.jl:256
+const double ROS23_C32 = 6.0 + std::sqrt(2.0);          // :257
+const double C16 = 1.0 / 6.0;
+
+struct Seq { std::vector<std::pair<double, double>> ts; };
+
+}  // namespace
+
+struct rs_ctx {
+    int nlon = 0, nlat = 0, nlev = 0, ns = 0, nt = 0;
+    size_t nc = 0, ncol = 0;
+    std::vector<double> cx, wz, wl;
+    bool have_wind = false, have_mech = false, have_state = false;
+    // mechanism
+    int nr = 0;
+    std::vector<int> ra, rb, pa, pb, photo;
+    std::vector<double> kr;
+    std::vector<std::vector<std::pair<int, int>>> net;   // per reaction: (species, nu), nu != 0
+    std::vector<double> jfac;   // nc
+    std::vector<double> emis;   // ns*ncol, empty = none
+    // state
+    std::vector<double> u, cand;
+    bool have_cand = false;
+    std::vector<double> lam, gemis;
+    // time-loop adjoint records (last rs_adjoint_run)
+    std::vector<Seq> seqT, seqC;
+    rs_split_opts last_opts{};
+    bool have_record = false;
+    std::string err;
+};
+
+namespace {
+
+int fail(rs_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+// ----------------------------------------------------------------------------
+// PI controller (rx_traced_integrator.jl:480-505; adjoint_gradient.jl:553-577)
+// ----------------------------------------------------------------------------
+struct PIState {
+    double t, dt, qold;
+    long long nacc = 0, nrej = 0, iters = 0;
+};
+
+// Returns true on accept.  dtc is the attempted step.
+bool pi_update(const rs_pictrl& c, PIState& s, double dtc, double eest_raw) {
+    const double invqmax = 1.0 / c.qmax, invqmin = 1.0 / c.qmin;
+    double EEst = std::isnan(eest_raw) ? 1.0e10 : eest_raw;       // :484
+    double q11 = std::pow(std::max(EEst, 1.0e-35), c.beta1);      // :486
+    double qden = std::pow(s.qold, c.beta2);                      // :487
+    double q = q11 / qden;                                         // :488
+    q = std::max(invqmax, std::min(invqmin, q / c.gamma));        // :489
+    bool accept = EEst <= 1.0;                                     // :490
+    bool insteady = (c.qsteady_min <= q) && (q <= c.qsteady_max); // :492
+    double qa = insteady ? 1.0 : q;                                // :493
+    if (accept) {
+        s.t = s.t + dtc;
+        s.dt = dtc / qa;                                           // :494
+        s.qold = std::max(EEst, c.qoldinit);                       // :501
+        s.nacc += 1;
+    } else {
+        double qr = std::min(invqmin, q11 / c.gamma);              // :496
+        s.dt = dtc / qr;                                           // :497
+        s.nrej += 1;
+    }
+    s.iters += 1;
+    return accept;
+}
+
+// ----------------------------------------------------------------------------
+// Transport operator: 3rd-order upwind flux form, periodic in longitude,
+// closed bottom/top, vertical stencil indices clamped to the column.
+// ----------------------------------------------------------------------------
+inline double face3(double qmm, double qm, double qp) {
+    // upwind-biased 3rd-order face value (-q_{m-1} + 5 q_m + 2 q_{m+1}) / 6
+    return (5.0 * qm - qmm + 2.0 * qp) * C16;
+}
+
+struct TGeom {
+    int nlon, nlat, nlev;
+    size_t nc;
+};
+
+// f = L q for one species block (q, f of length nc).
+void transport_rhs_block(const rs_ctx& c, const double* q, double* f) {
+    const int nlon = c.nlon, nlat = c.nlat, nlev = c.nlev;
+    const size_t rowlen = (size_t)nlon, plane = (size_t)nlon * nlat;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < nlev; ++k) {
+        for (int j = 0; j < nlat; ++j) {
+            const double cxj = c.cx[j];
+            const double* qr = q + k * plane + j * rowlen;
+            for (int i = 0; i < nlon; ++i) {
+                auto Q = [&](int ii) { ii %= nlon; if (ii < 0) ii += nlon; return qr[ii]; };
+                double fe, fw;
+                if (cxj >= 0.0) {
+                    fe = cxj * face3(Q(i - 1), Q(i), Q(i + 1));
+                    fw = cxj * face3(Q(i - 2), Q(i - 1), Q(i));
+                } else {
+                    fe = cxj * face3(Q(i + 2), Q(i + 1), Q(i));
+                    fw = cxj * face3(Q(i + 1), Q(i), Q(i - 1));
+                }
+                auto V = [&](int kk) { return q[(size_t)kk * plane + j * rowlen + i]; };
+                auto G = [&](int kf) -> double {   // flux through face kf (between kf-1 and kf)
+                    if (kf <= 0 || kf >= nlev) return 0.0;
+                    double w = c.wz[kf] * c.wl[i];
+                    if (w >= 0.0) {
+                        int mm = std::max(kf - 2, 0);
+                        return w * face3(V(mm), V(kf - 1), V(kf));
+                    } else {
+                        int mm = std::min(kf + 1, nlev - 1);
+                        return w * face3(V(mm), V(kf), V(kf - 1));
+                    }
+                };
+                double gt = G(k + 1), gb = G(k);
+                f[k * plane + j * rowlen + i] = -(fe - fw) - (gt - gb);
+            }
+        }
+    }
+}
+
+// g += L^T lam for one species block, by reversing the face fluxes: every
+// face flux F leaves its lower/western cell and enters its upper/eastern cell.
+void transport_rhs_adj_block(const rs_ctx& c, const double* lam, double* g) {
+    const int nlon = c.nlon, nlat = c.nlat, nlev = c.nlev;
+    const size_t rowlen = (size_t)nlon, plane = (size_t)nlon * nlat;
+    // zonal faces: face (i+1/2) between i (west) and i+1 (east); rows are
+    // independent, so parallelise over (k, j).
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < nlev; ++k) {
+        for (int j = 0; j < nlat; ++j) {
+            const double cxj = c.cx[j];
+            const size_t b = k * plane + j * rowlen;
+            auto W = [&](int ii) { ii %= nlon; if (ii < 0) ii += nlon; return (size_t)ii; };
+            for (int i = 0; i < nlon; ++i) {
+                // F_{i+1/2}: f_i -= F, f_{i+1} += F  => Fbar = lam_{i+1} - lam_i
+                double Fb = lam[b + W(i + 1)] - lam[b + W(i)];
+                int mm, m, p;
+                if (cxj >= 0.0) { mm = i - 1; m = i; p = i + 1; }
+                else { mm = i + 2; m = i + 1; p = i; }
+                double a = cxj * Fb * C16;
+                g[b + W(mm)] += -a;
+                g[b + W(m)] += 5.0 * a;
+                g[b + W(p)] += 2.0 * a;
+            }
+        }
+    }
+    // vertical faces kf = 1..nlev-1 between kf-1 (below) and kf (above);
+    // columns are independent, parallelise over (j, i).
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = 0; j < nlat; ++j) {
+        for (int i = 0; i < nlon; ++i) {
+            const size_t b = j * rowlen + i;
+            for (int kf = 1; kf < nlev; ++kf) {
+                double w = c.wz[kf] * c.wl[i];
+                double Gb = lam[b + kf * plane] - lam[b + (kf - 1) * plane];
+                int mm, m, p;
+                if (w >= 0.0) { mm = std::max(kf - 2, 0); m = kf - 1; p = kf; }
+                else { mm = std::min(kf + 1, nlev - 1); m = kf; p = kf - 1; }
+                double a = w * Gb * C16;
+                g[b + mm * plane] += -a;
+                g[b + m * plane] += 5.0 * a;
+                g[b + p * plane] += 2.0 * a;
+            }
+        }
+    }
+}
+
+// Blocked sum in a fixed order (independent of the OpenMP thread count).
+double fixed_sum(const std::vector<double>& v) {
+    const size_t B = 4096;
+    size_t nb = (v.size() + B - 1) / B;
+    std::vector<double> part(nb, 0.0);
+#pragma omp parallel for schedule(static)
+    for (long long b = 0; b < (long long)nb; ++b) {
+        double s = 0.0;
+        size_t e = std::min(v.size(), (size_t)(b + 1) * B);
+        for (size_t x = (size_t)b * B; x < e; ++x) s += v[x];
+        part[b] = s;
+    }
+    double s = 0.0;
+    for (double x : part) s += x;
+    return s;
+}
+
+// One SSPRK43 attempt (ssprk43_step_unrolled, rx_traced_integrator.jl:404-437).
+// Writes the raw result into unew (all species; non-transported copied), returns EEst.
+double ssprk43_attempt(const rs_ctx& c, const std::vector<double>& u, std::vector<double>& unew,
+                       double dt, double atol, double rtol) {
+    const size_t nc = c.nc, nT = (size_t)c.nt * nc;
+    std::vector<double> k(nc), u1(nT), u2(nT), u3(nT), u4(nT);
+    unew.assign(u.begin(), u.end());
+    const double dt_2 = dt / 2;
+    std::vector<double> sq(nc * (size_t)c.nt);
+    for (int s = 0; s < c.nt; ++s) {
+        const double* us = u.data() + s * nc;
+        double* a1 = u1.data() + s * nc;
+        double* a2 = u2.data() + s * nc;
+        double* a3 = u3.data() + s * nc;
+        double* a4 = u4.data() + s * nc;
+        transport_rhs_block(c, us, k.data());
+        for (size_t x = 0; x < nc; ++x) { double du = dt_2 * k[x]; a1[x] = us[x] + du; }
+        transport_rhs_block(c, a1, k.data());
+        for (size_t x = 0; x < nc; ++x) { double du = dt_2 * k[x]; a2[x] = a1[x] + du; }
+        transport_rhs_block(c, a2, k.data());
+        for (size_t x = 0; x < nc; ++x) { double du = dt_2 * k[x]; a3[x] = a2[x] + du; }
+        for (size_t x = 0; x < nc; ++x) {
+            double a = (2.0 / 3.0) * us[x];
+            double b = (1.0 / 3.0) * a3[x];
+            a4[x] = a + b;
+        }
+        transport_rhs_block(c, a4, k.data());
+        double* un = unew.data() + s * nc;
+        for (size_t x = 0; x < nc; ++x) {
+            double du = dt_2 * k[x];
+            un[x] = a4[x] + du;
+            double a = (1.0 / 3.0) * us[x];
+            double b = (2.0 / 3.0) * a3[x];
+            double ut = a + b;
+            double ud = ut - un[x];
+            ut = 0.5 * ud;
+            double am = std::max(std::fabs(us[x]), std::fabs(un[x]));
+            double den = am * rtol;
+            den = den + atol;
+            double at = ut / den;
+            sq[s * nc + x] = at * at;
+        }
+    }
+    double sse = fixed_sum(sq);
+    double N = (double)c.ns * (double)nc;
+    return std::sqrt(sse / N);
+}
+
+// ----------------------------------------------------------------------------
+// Chemistry (mass action), one cell at a time, dense NS x NS blocks.
+// ----------------------------------------------------------------------------
+struct CellChem {
+    const rs_ctx& c;
+    size_t cell;
+    double jf;        // photolysis factor
+    const double* E;  // emission of this cell per species (stride ncol) or null
+    size_t estride;
+    CellChem(const rs_ctx& cc, size_t cell_) : c(cc), cell(cell_) {
+        jf = cc.jfac.empty() ? 1.0 : cc.jfac[cell_];
+        E = nullptr; estride = cc.ncol;
+        if (!cc.emis.empty() && cell_ < cc.ncol) E = cc.emis.data() + cell_;
+    }
+    double kc(int r) const { return c.photo[r] ? c.kr[r] * jf : c.kr[r]; }
+    double rate(int r, const double* u) const {
+        double k = kc(r);
+        if (c.rb[r] < 0) return k * u[c.ra[r]];
+        return (k * u[c.ra[r]]) * u[c.rb[r]];
+    }
+    // f = sum_r nu_r rate_r (+ E on level-0 cells), accumulation in reaction order.
+    void rhs(const double* u, double* f) const {
+        const int ns = c.ns;
+        for (int i = 0; i < ns; ++i) f[i] = E ? E[(size_t)i * estride] : 0.0;
+        for (int r = 0; r < c.nr; ++r) {
+            double rt = rate(r, u);
+            for (auto& pr : c.net[r]) f[pr.first] += (double)pr.second * rt;
+        }
+    }
+    // dense analytic Jacobian J[i*ns + a]
+    void jac(const double* u, double* J) const {
+        const int ns = c.ns;
+        std::fill(J, J + (size_t)ns * ns, 0.0);
+        for (int r = 0; r < c.nr; ++r) {
+            double k = kc(r);
+            int a = c.ra[r], b = c.rb[r];
+            double da, db = 0.0;
+            if (b < 0) da = k;
+            else if (a == b) da = (2.0 * k) * u[a];
+            else { da = k * u[b]; db = k * u[a]; }
+            for (auto& pr : c.net[r]) {
+                double nu = (double)pr.second;
+                J[(size_t)pr.first * ns + a] += nu * da;
+                if (b >= 0 && b != a) J[(size_t)pr.first * ns + b] += nu * db;
+            }
+        }
+    }
+};
+
+// blocksolve (rx_traced_integrator.jl:110-135): pivot-free Gaussian
+// elimination of W x = b, W row-major NS x NS.  W and b are copied.
+void blocksolve(int ns, const double* Win, const double* bin, double* x) {
+    std::vector<double> W(Win, Win + (size_t)ns * ns), b(bin, bin + ns);
+    for (int k = 0; k < ns; ++k) {
+        double piv = 1.0 / W[(size_t)k * ns + k];
+        for (int i = k + 1; i < ns; ++i) {
+            double m = W[(size_t)i * ns + k] * piv;
+            for (int j = k + 1; j < ns; ++j) {
+                double mw = m * W[(size_t)k * ns + j];
+                W[(size_t)i * ns + j] = W[(size_t)i * ns + j] - mw;
+            }
+            double mb = m * b[k];
+            b[i] = b[i] - mb;
+        }
+    }
+    for (int i = ns - 1; i >= 0; --i) {
+        double acc = b[i];
+        for (int j = i + 1; j < ns; ++j) {
+            double wx = W[(size_t)i * ns + j] * x[j];
+            acc = acc - wx;
+        }
+        x[i] = acc / W[(size_t)i * ns + i];
+    }
+}
+
+// W^T x = b by the same elimination applied to the transpose.
+void blocksolve_T(int ns, const double* W, const double* b, double* x) {
+    std::vector<double> WT((size_t)ns * ns);
+    for (int i = 0; i < ns; ++i)
+        for (int j = 0; j < ns; ++j) WT[(size_t)j * ns + i] = W[(size_t)i * ns + j];
+    blocksolve(ns, WT.data(), b, x);
+}
+
+struct Ros23Work {
+    std::vector<double> f0, f1, f2, k1, k2, k3, u1, J, W, b;
+    explicit Ros23Work(int ns)
+        : f0(ns), f1(ns), f2(ns), k1(ns), k2(ns), k3(ns), u1(ns),
+          J((size_t)ns * ns), W((size_t)ns * ns), b(ns) {}
+};
+
+// Build W = I - dt*d*J (rx_traced_integrator.jl:291-296).
+void build_W(int ns, double dt, const double* J, double* W) {
+    double dtgamma = dt * ROS23_D;
+    for (int r = 0; r < ns; ++r)
+        for (int s = 0; s < ns; ++s) {
+            double tmp = dtgamma * J[(size_t)r * ns + s];
+            W[(size_t)r * ns + s] = (r == s) ? 1.0 - tmp : 0.0 - tmp;
+        }
+}
+
+// One ROS23 attempt on one cell (ros23_step, rx_traced_integrator.jl:279-349).
+// u, unew: NS values; returns the cell's sum of squared scaled residuals.
+double ros23_cell(const CellChem& cc, const double* u, double* unew, double dt,
+                  double atol, double rtol, Ros23Work& w) {
+    const int ns = cc.c.ns;
+    cc.rhs(u, w.f0.data());                                           // :282
+    cc.jac(u, w.J.data());                                            // :287
+    build_W(ns, dt, w.J.data(), w.W.data());                          // :291-296
+    blocksolve(ns, w.W.data(), w.f0.data(), w.k1.data());             // :298
+    double dto2 = dt / 2;
+    for (int i = 0; i < ns; ++i) { double du1 = dto2 * w.k1[i]; w.u1[i] = u[i] + du1; }
+    cc.rhs(w.u1.data(), w.f1.data());                                 // :303
+    for (int i = 0; i < ns; ++i) w.b[i] = w.f1[i] - w.k1[i];          // :306-309
+    blocksolve(ns, w.W.data(), w.b.data(), w.k2.data());              // :310
+    for (int i = 0; i < ns; ++i) w.k2[i] = w.k2[i] + w.k1[i];         // :311-313
+    for (int i = 0; i < ns; ++i) { double du2 = dt * w.k2[i]; unew[i] = u[i] + du2; }
+    cc.rhs(unew, w.f2.data());                                        // :318
+    for (int i = 0; i < ns; ++i) {                                    // :319-327
+        double a = w.k2[i] - w.f1[i];
+        a = ROS23_C32 * a;
+        double c = w.k1[i] - w.f0[i];
+        c = 2.0 * c;
+        double d3 = w.f2[i] - a;
+        w.b[i] = d3 - c;
+    }
+    blocksolve(ns, w.W.data(), w.b.data(), w.k3.data());              // :328
+    double dto6 = dt / 6, sse = 0.0;
+    for (int i = 0; i < ns; ++i) {                                    // :330-346
+        double w2 = 2.0 * w.k2[i];
+        double ut = w.k1[i] - w2;
+        ut = ut + w.k3[i];
+        ut = dto6 * ut;
+        double am = std::max(std::fabs(u[i]), std::fabs(unew[i]));
+        double den = am * rtol;
+        den = den + atol;
+        double at = ut / den;
+        sse += at * at;
+    }
+    return sse;
+}
+
+// Gather / scatter one cell of a species-major vector.
+inline void gather(const rs_ctx& c, const double* v, size_t cell, double* out) {
+    for (int s = 0; s < c.ns; ++s) out[s] = v[(size_t)s * c.nc + cell];
+}
+inline void scatter(const rs_ctx& c, const double* in, size_t cell, double* v) {
+    for (int s = 0; s < c.ns; ++s) v[(size_t)s * c.nc + cell] = in[s];
+}
+
+double ros23_attempt(const rs_ctx& c, const std::vector<double>& u, std::vector<double>& unew,
+                     double dt, double atol, double rtol) {
+    unew.resize(u.size());
+    std::vector<double> sq(c.nc);
+#pragma omp parallel
+    {
+        Ros23Work w(c.ns);
+        std::vector<double> uc(c.ns), un(c.ns);
+#pragma omp for schedule(static)
+        for (long long cell = 0; cell < (long long)c.nc; ++cell) {
+            CellChem cc(c, (size_t)cell);
+            gather(c, u.data(), cell, uc.data());
+            sq[cell] = ros23_cell(cc, uc.data(), un.data(), dt, atol, rtol, w);
+            scatter(c, un.data(), cell, unew.data());
+        }
+    }
+    double sse = fixed_sum(sq);
+    return std::sqrt(sse / ((double)c.ns * (double)c.nc));
+}
+
+// ----------------------------------------------------------------------------
+// One-step adjoints and tangents.
+// ----------------------------------------------------------------------------
+// SSPRK43 at fixed dt, transport linear: unew = P u with
+// P = (2/3 I + 1/3 S^3)(I + h/2 L)... reversed stage by stage.
+void ssprk43_vjp(const rs_ctx& c, std::vector<double>& lam, double dt) {
+    const size_t nc = c.nc;
+    const double h2 = dt / 2;
+    std::vector<double> g(nc), l4(nc), l3(nc), l2(nc), l1(nc), l0(nc), ub(nc);
+    auto apply = [&](const double* in, double* out) {    // out = in + h/2 L^T in
+        std::fill(g.begin(), g.end(), 0.0);
+        transport_rhs_adj_block(c, in, g.data());
+        for (size_t x = 0; x < nc; ++x) out[x] = in[x] + h2 * g[x];
+    };
+    for (int s = 0; s < c.nt; ++s) {
+        double* L = lam.data() + s * nc;
+        apply(L, l4.data());                                        // unew = u4 + h/2 L u4
+        for (size_t x = 0; x < nc; ++x) { ub[x] = (2.0 / 3.0) * l4[x]; l3[x] = (1.0 / 3.0) * l4[x]; }
+        apply(l3.data(), l2.data());                                // u3 = u2 + h/2 L u2
+        apply(l2.data(), l1.data());                                // u2 = u1 + h/2 L u1
+        apply(l1.data(), l0.data());                                // u1 = u + h/2 L u
+        for (size_t x = 0; x < nc; ++x) L[x] = ub[x] + l0[x];
+    }
+}
+
+void ssprk43_jvp(const rs_ctx& c, const std::vector<double>& v, std::vector<double>& out, double dt) {
+    // The transport step is linear in u: its JVP is the step applied to v.
+    std::vector<double> tmp;
+    ssprk43_attempt(c, v, tmp, dt, 1.0, 0.0);
+    out = tmp;
+}
+
+// Products of the analytic Jacobian with vectors and its u-derivative terms.
+void jac_T_vec(const rs_ctx& c, const CellChem& cc, const double* u, const double* z, double* out) {
+    // out += J(u)^T z
+    std::vector<double> J((size_t)c.ns * c.ns);
+    cc.jac(u, J.data());
+    for (int a = 0; a < c.ns; ++a) {
+        double s = 0.0;
+        for (int i = 0; i < c.ns; ++i) s += J[(size_t)i * c.ns + a] * z[i];
+        out[a] += s;
+    }
+}
+
+// ubar += coef * d/du [ z^T J(u) v ]  (only bimolecular reactions contribute)
+void hess_term(const rs_ctx& c, const CellChem& cc, const double* z, const double* v,
+               double coef, double* ubar) {
+    for (int r = 0; r < c.nr; ++r) {
+        int a = c.ra[r], b = c.rb[r];
+        if (b < 0) continue;
+        double sr = 0.0;
+        for (auto& pr : c.net[r]) sr += (double)pr.second * z[pr.first];
+        double k = cc.kc(r);
+        if (a == b) ubar[a] += coef * (2.0 * k) * v[a] * sr;
+        else {
+            ubar[a] += coef * k * v[b] * sr;
+            ubar[b] += coef * k * v[a] * sr;
+        }
+    }
+}
+
+// Reverse of the part of ros23_step that reaches unew (f2, k3, EEst are dead):
+//   f0=f(u); k1=W\f0; u1=u+h/2 k1; f1=f(u1); k2=W\(f1-k1)+k1; unew=u+h k2.
+void ros23_vjp_cell(const CellChem& cc, const double* u, double dt, const double* lam_in,
+                    double* lam_out, double* ebar /*ns, surface only, may be null*/) {
+    const rs_ctx& c = cc.c;
+    const int ns = c.ns;
+    Ros23Work w(ns);
+    cc.rhs(u, w.f0.data());
+    cc.jac(u, w.J.data());
+    build_W(ns, dt, w.J.data(), w.W.data());
+    blocksolve(ns, w.W.data(), w.f0.data(), w.k1.data());
+    double dto2 = dt / 2;
+    for (int i = 0; i < ns; ++i) w.u1[i] = u[i] + dto2 * w.k1[i];
+    cc.rhs(w.u1.data(), w.f1.data());
+    for (int i = 0; i < ns; ++i) w.b[i] = w.f1[i] - w.k1[i];
+    blocksolve(ns, w.W.data(), w.b.data(), w.k2.data());
+    for (int i = 0; i < ns; ++i) w.k2[i] = w.k2[i] + w.k1[i];
+    const double hg = dt * ROS23_D;
+    std::vector<double> ub(lam_in, lam_in + ns), k2b(ns), z2(ns), k1b(ns), u1b(ns, 0.0), z1(ns), v(ns);
+    for (int i = 0; i < ns; ++i) k2b[i] = dt * lam_in[i];
+    blocksolve_T(ns, w.W.data(), k2b.data(), z2.data());
+    for (int i = 0; i < ns; ++i) k1b[i] = k2b[i] - z2[i];
+    jac_T_vec(c, cc, w.u1.data(), z2.data(), u1b.data());
+    for (int i = 0; i < ns; ++i) { ub[i] += u1b[i]; k1b[i] += dto2 * u1b[i]; }
+    blocksolve_T(ns, w.W.data(), k1b.data(), z1.data());
+    jac_T_vec(c, cc, u, z1.data(), ub.data());
+    for (int i = 0; i < ns; ++i) v[i] = w.k2[i] - w.k1[i];
+    hess_term(c, cc, z2.data(), v.data(), hg, ub.data());
+    hess_term(c, cc, z1.data(), w.k1.data(), hg, ub.data());
+    for (int i = 0; i < ns; ++i) lam_out[i] = ub[i];
+    if (ebar)
+        for (int i = 0; i < ns; ++i) ebar[i] += z1[i] + z2[i];
+}
+
+// Forward-mode tangent of the same map, derived independently of the VJP.
+void ros23_jvp_cell(const CellChem& cc, const double* u, double dt, const double* ud,
+                    const double* Ed /*ns or null*/, double* outd) {
+    const rs_ctx& c = cc.c;
+    const int ns = c.ns;
+    Ros23Work w(ns);
+    cc.rhs(u, w.f0.data());
+    cc.jac(u, w.J.data());
+    build_W(ns, dt, w.J.data(), w.W.data());
+    blocksolve(ns, w.W.data(), w.f0.data(), w.k1.data());
+    double dto2 = dt / 2;
+    for (int i = 0; i < ns; ++i) w.u1[i] = u[i] + dto2 * w.k1[i];
+    cc.rhs(w.u1.data(), w.f1.data());
+    std::vector<double> y(ns);
+    for (int i = 0; i < ns; ++i) w.b[i] = w.f1[i] - w.k1[i];
+    blocksolve(ns, w.W.data(), w.b.data(), y.data());
+    for (int i = 0; i < ns; ++i) w.k2[i] = y[i] + w.k1[i];
+    // Jdot = dJ/du[ud]
+    std::vector<double> Jd((size_t)ns * ns, 0.0);
+    for (int r = 0; r < c.nr; ++r) {
+        int a = c.ra[r], b = c.rb[r];
+        if (b < 0) continue;
+        double k = cc.kc(r);
+        for (auto& pr : c.net[r]) {
+            double nu = (double)pr.second;
+            if (a == b) Jd[(size_t)pr.first * ns + a] += nu * (2.0 * k) * ud[a];
+            else {
+                Jd[(size_t)pr.first * ns + a] += nu * k * ud[b];
+                Jd[(size_t)pr.first * ns + b] += nu * k * ud[a];
+            }
+        }
+    }
+    const double hg = dt * ROS23_D;
+    auto Wd_mul = [&](const double* x, double* out) {   // out = Wdot x = -hg Jd x
+        for (int i = 0; i < ns; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < ns; ++j) s += Jd[(size_t)i * ns + j] * x[j];
+            out[i] = -hg * s;
+        }
+    };
+    auto J_mul = [&](const double* uu, const double* x, double* out) {
+        std::vector<double> JJ((size_t)ns * ns);
+        cc.jac(uu, JJ.data());
+        for (int i = 0; i < ns; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < ns; ++j) s += JJ[(size_t)i * ns + j] * x[j];
+            out[i] = s;
+        }
+    };
+    std::vector<double> f0d(ns), tmp(ns), k1d(ns), u1d(ns), f1d(ns), yd(ns);
+    J_mul(u, ud, f0d.data());
+    if (Ed) for (int i = 0; i < ns; ++i) f0d[i] += Ed[i];
+    Wd_mul(w.k1.data(), tmp.data());
+    for (int i = 0; i < ns; ++i) tmp[i] = f0d[i] - tmp[i];
+    blocksolve(ns, w.W.data(), tmp.data(), k1d.data());
+    for (int i = 0; i < ns; ++i) u1d[i] = ud[i] + dto2 * k1d[i];
+    J_mul(w.u1.data(), u1d.data(), f1d.data());
+    if (Ed) for (int i = 0; i < ns; ++i) f1d[i] += Ed[i];
+    Wd_mul(y.data(), tmp.data());
+    for (int i = 0; i < ns; ++i) tmp[i] = f1d[i] - k1d[i] - tmp[i];
+    blocksolve(ns, w.W.data(), tmp.data(), yd.data());
+    for (int i = 0; i < ns; ++i) outd[i] = ud[i] + dt * (yd[i] + k1d[i]);
+}
+
+// ----------------------------------------------------------------------------
+// Adaptive loops.
+// ----------------------------------------------------------------------------
+enum class Op { T, C };
+
+int adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs_pictrl& ctrl,
+             double atol, double rtol, int clamp, long long maxiters, rs_counts* out,
+             Seq* seq) {
+    PIState s{t0, *dt_io, ctrl.qoldinit};
+    const double tlim = t1 - 1.0e-9;                              // :479
+    std::vector<double> raw;
+    while (s.t < tlim && s.iters < maxiters) {                    // :480
+        double dtc = std::min(s.dt, t1 - s.t);                     // :482
+        double ee = (op == Op::T) ? ssprk43_attempt(*c, c->u, raw, dtc, atol, rtol)
+                                  : ros23_attempt(*c, c->u, raw, dtc, atol, rtol);
+        double tbefore = s.t;
+        if (pi_update(ctrl, s, dtc, ee)) {
+            if (clamp) for (double& x : raw) x = std::max(x, 0.0);
+            c->u.swap(raw);
+            if (seq) seq->ts.emplace_back(tbefore, dtc);
+        }
+    }
+    *dt_io = s.dt;
+    if (out) { out->naccept = s.nacc; out->nreject = s.nrej; out->t_end = s.t; out->dt_end = s.dt; }
+    if (s.iters >= maxiters && s.t < tlim)
+        return fail(c, RS_E_MAXITERS, "adaptive loop hit maxiters at t=" + std::to_string(s.t));
+    return RS_OK;
+}
+
+// Per-cell adaptive chemistry (the reference loop with NC = 1).
+int percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells, const rs_pictrl& ctrl,
+            double atol, double rtol, int clamp, long long maxiters, rs_percell_stats* st) {
+    const int ns = c->ns;
+    long long att = 0, acc = 0, rej = 0, amax = 0, amin = -1, nfail = 0;
+#pragma omp parallel reduction(+ : att, acc, rej, nfail) reduction(max : amax)
+    {
+        Ros23Work w(ns);
+        std::vector<double> uc(ns), un(ns);
+        long long lmin = -1;
+#pragma omp for schedule(dynamic, 64)
+        for (long long cell = 0; cell < (long long)c->nc; ++cell) {
+            CellChem cc(*c, (size_t)cell);
+            gather(*c, c->u.data(), cell, uc.data());
+            PIState s{t0, dt_cells ? dt_cells[cell] : dt0, ctrl.qoldinit};
+            const double tlim = t1 - 1.0e-9;
+            while (s.t < tlim && s.iters < maxiters) {
+                double dtc = std::min(s.dt, t1 - s.t);
+                double sse = ros23_cell(cc, uc.data(), un.data(), dtc, atol, rtol, w);
+                double ee = std::sqrt(sse / (double)ns);
+                if (pi_update(ctrl, s, dtc, ee)) {
+                    for (int i = 0; i < ns; ++i) uc[i] = clamp ? std::max(un[i], 0.0) : un[i];
+                }
+            }
+            if (s.iters >= maxiters && s.t < tlim) nfail += 1;
+            scatter(*c, uc.data(), cell, c->u.data());
+            if (dt_cells) dt_cells[cell] = s.dt;
+            att += s.iters; acc += s.nacc; rej += s.nrej;
+            amax = std::max(amax, s.iters);
+            lmin = (lmin < 0) ? s.iters : std::min(lmin, s.iters);
+        }
+#pragma omp critical
+        { if (lmin >= 0) amin = (amin < 0) ? lmin : std::min(amin, lmin); }
+    }
+    if (st) {
+        st->attempts_total = att; st->accepts_total = acc; st->rejects_total = rej;
+        st->attempts_max = amax; st->attempts_min = amin < 0 ? 0 : amin; st->n_failed = nfail;
+    }
+    if (nfail) return fail(c, RS_E_MAXITERS, std::to_string(nfail) + " cells hit maxiters");
+    return RS_OK;
+}
+
+int check_ready(rs_ctx* c, bool need_mech, bool need_wind) {
+    if (!c) return RS_E_ARG;
+    if (c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!c->have_state) return fail(c, RS_E_STATE, "state not set");
+    if (need_mech && !c->have_mech) return fail(c, RS_E_STATE, "mechanism not set");
+    if (need_wind && !c->have_wind && c->nt > 0) return fail(c, RS_E_STATE, "wind not set");
+    return RS_OK;
+}
+
+double loss_value(const rs_ctx& c, int species) {
+    double J = 0.0;
+    const double* q = c.u.data() + (size_t)species * c.nc;
+    for (size_t x = 0; x < c.ncol; ++x) J += q[x] * q[x];
+    return J;
+}
+
+// One macro step: transport then chemistry, recording the accepted sequences.
+int macro(rs_ctx* c, double t0, double t1, double* dtT, double* dtC, const rs_split_opts& o,
+          rs_counts* cT, rs_counts* cC, Seq* sT, Seq* sC) {
+    int rc = RS_OK;
+    if (c->nt > 0) {
+        rc = adaptive(c, Op::T, t0, t1, dtT, o.ctrl_T, o.atol_T, o.rtol_T, o.clamp_nonneg,
+                      o.maxiters, cT, sT);
+        if (rc) return rc;
+    }
+    if (o.chem_percell) {
+        rs_percell_stats st{};
+        return percell(c, t0, t1, *dtC, nullptr, o.ctrl_C, o.atol_C, o.rtol_C, o.clamp_nonneg,
+                       o.maxiters, &st);
+    }
+    return adaptive(c, Op::C, t0, t1, dtC, o.ctrl_C, o.atol_C, o.rtol_C, o.clamp_nonneg,
+                    o.maxiters, cC, sC);
+}
+
+void ros23_vjp_all(rs_ctx* c, const std::vector<double>& u, double dt) {
+    std::vector<double> out(c->lam.size());
+#pragma omp parallel
+    {
+        std::vector<double> uc(c->ns), lc(c->ns), lo(c->ns), eb(c->ns);
+#pragma omp for schedule(static)
+        for (long long cell = 0; cell < (long long)c->nc; ++cell) {
+            CellChem cc(*c, (size_t)cell);
+            gather(*c, u.data(), cell, uc.data());
+            gather(*c, c->lam.data(), cell, lc.data());
+            bool surf = (size_t)cell < c->ncol;
+            std::fill(eb.begin(), eb.end(), 0.0);
+            ros23_vjp_cell(cc, uc.data(), dt, lc.data(), lo.data(), surf ? eb.data() : nullptr);
+            scatter(*c, lo.data(), cell, out.data());
+            if (surf)
+                for (int s = 0; s < c->ns; ++s) c->gemis[(size_t)s * c->ncol + cell] += eb[s];
+        }
+    }
+    c->lam.swap(out);
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+int rs_create(rs_ctx** out, int device) {
+    (void)device;
+    if (!out) return RS_E_ARG;
+    *out = new rs_ctx();
+    return RS_OK;
+}
+int rs_destroy(rs_ctx* c) { delete c; return RS_OK; }
+const char* rs_last_error(const rs_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* rs_backend(void) { return "cpu-oracle"; }
+
+int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
+    if (!c) return RS_E_ARG;
+    if (nlon < 1 || nlat < 1 || nlev < 1 || ns < 1 || nt < 0 || nt > ns)
+        return fail(c, RS_E_ARG, "bad grid sizes");
+    c->nlon = nlon; c->nlat = nlat; c->nlev = nlev; c->ns = ns; c->nt = nt;
+    c->ncol = (size_t)nlon * nlat;
+    c->nc = c->ncol * nlev;
+    c->u.assign((size_t)ns * c->nc, 0.0);
+    c->lam.assign((size_t)ns * c->nc, 0.0);
+    c->gemis.assign((size_t)ns * c->ncol, 0.0);
+    c->jfac.clear(); c->emis.clear();
+    c->have_state = c->have_mech = c->have_wind = false;
+    return RS_OK;
+}
+
+int rs_set_wind(rs_ctx* c, const double* cx, const double* wz, const double* wl) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!cx || !wz || !wl) return fail(c, RS_E_ARG, "null wind array");
+    c->cx.assign(cx, cx + c->nlat);
+    c->wz.assign(wz, wz + c->nlev + 1);
+    c->wl.assign(wl, wl + c->nlon);
+    for (double x : c->cx) if (!std::isfinite(x)) return fail(c, RS_E_NONFINITE, "cx");
+    for (double x : c->wz) if (!std::isfinite(x)) return fail(c, RS_E_NONFINITE, "wz");
+    for (double x : c->wl) if (!std::isfinite(x)) return fail(c, RS_E_NONFINITE, "wl");
+    c->have_wind = true;
+    return RS_OK;
+}
+
+int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const double* k,
+                     const int* photo) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (nr < 0 || (nr > 0 && (!re || !pr || !k))) return fail(c, RS_E_ARG, "bad mechanism arrays");
+    c->nr = nr;
+    c->ra.resize(nr); c->rb.resize(nr); c->pa.resize(nr); c->pb.resize(nr);
+    c->photo.assign(nr, 0); c->kr.resize(nr); c->net.assign(nr, {});
+    for (int r = 0; r < nr; ++r) {
+        int a = re[2 * r], b = re[2 * r + 1], p = pr[2 * r], q = pr[2 * r + 1];
+        if (a < 0 || a >= c->ns || b < -1 || b >= c->ns || p < -1 || p >= c->ns || q < -1 ||
+            q >= c->ns)
+            return fail(c, RS_E_ARG, "species index out of range in reaction " + std::to_string(r));
+        if (!std::isfinite(k[r])) return fail(c, RS_E_NONFINITE, "rate constant");
+        c->ra[r] = a; c->rb[r] = b; c->pa[r] = p; c->pb[r] = q;
+        c->kr[r] = k[r];
+        c->photo[r] = photo ? (photo[r] != 0) : 0;
+        std::vector<int> nu(c->ns, 0);
+        nu[a] -= 1;
+        if (b >= 0) nu[b] -= 1;
+        if (p >= 0) nu[p] += 1;
+        if (q >= 0) nu[q] += 1;
+        for (int s = 0; s < c->ns; ++s)
+            if (nu[s] != 0) c->net[r].emplace_back(s, nu[s]);
+    }
+    c->have_mech = true;
+    return RS_OK;
+}
+
+int rs_set_photolysis(rs_ctx* c, const double* jfac) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!jfac) { c->jfac.clear(); return RS_OK; }
+    c->jfac.assign(jfac, jfac + c->nc);
+    return RS_OK;
+}
+
+int rs_set_emissions(rs_ctx* c, const double* e) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!e) { c->emis.clear(); return RS_OK; }
+    c->emis.assign(e, e + (size_t)c->ns * c->ncol);
+    return RS_OK;
+}
+
+int rs_set_state(rs_ctx* c, const double* u) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!u) return fail(c, RS_E_ARG, "null state");
+    c->u.assign(u, u + (size_t)c->ns * c->nc);
+    c->have_state = true;
+    c->have_cand = false;
+    return RS_OK;
+}
+
+int rs_get_state(rs_ctx* c, double* u) {
+    if (!c || !c->have_state) return fail(c, RS_E_STATE, "state not set");
+    if (!u) return fail(c, RS_E_ARG, "null output");
+    std::copy(c->u.begin(), c->u.end(), u);
+    return RS_OK;
+}
+
+int rs_ssprk43_step(rs_ctx* c, double t, double dt, double atol, double rtol, double* eest) {
+    (void)t;
+    int rc = check_ready(c, false, true);
+    if (rc) return rc;
+    double e = ssprk43_attempt(*c, c->u, c->cand, dt, atol, rtol);
+    c->have_cand = true;
+    if (eest) *eest = e;
+    return RS_OK;
+}
+
+int rs_ros23_step(rs_ctx* c, double t, double dt, double atol, double rtol, double* eest) {
+    (void)t;
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    double e = ros23_attempt(*c, c->u, c->cand, dt, atol, rtol);
+    c->have_cand = true;
+    if (eest) *eest = e;
+    return RS_OK;
+}
+
+int rs_get_candidate(rs_ctx* c, double* u) {
+    if (!c || !c->have_cand) return fail(c, RS_E_STATE, "no candidate");
+    std::copy(c->cand.begin(), c->cand.end(), u);
+    return RS_OK;
+}
+
+int rs_accept_candidate(rs_ctx* c, int clamp) {
+    if (!c || !c->have_cand) return fail(c, RS_E_STATE, "no candidate");
+    if (clamp) for (double& x : c->cand) x = std::max(x, 0.0);
+    c->u.swap(c->cand);
+    c->have_cand = false;
+    return RS_OK;
+}
+
+int rs_adaptive_transport(rs_ctx* c, double t0, double t1, double* dt_io, const rs_pictrl* ctrl,
+                          double atol, double rtol, int clamp, long long maxiters, rs_counts* out) {
+    int rc = check_ready(c, false, true);
+    if (rc) return rc;
+    if (!dt_io || !ctrl || !(*dt_io > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
+    return adaptive(c, Op::T, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out, nullptr);
+}
+
+int rs_adaptive_chem(rs_ctx* c, double t0, double t1, double* dt_io, const rs_pictrl* ctrl,
+                     double atol, double rtol, int clamp, long long maxiters, rs_counts* out) {
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    if (!dt_io || !ctrl || !(*dt_io > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
+    return adaptive(c, Op::C, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out, nullptr);
+}
+
+int rs_chem_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells,
+                    const rs_pictrl* ctrl, double atol, double rtol, int clamp,
+                    long long maxiters, rs_percell_stats* out) {
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    if (!ctrl || !(dt0 > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
+    return percell(c, t0, t1, dt0, dt_cells, *ctrl, atol, rtol, clamp, maxiters, out);
+}
+
+int rs_macro_step(rs_ctx* c, double t0, double t1, double* dtT, double* dtC,
+                  const rs_split_opts* o, rs_counts* cT, rs_counts* cC) {
+    int rc = check_ready(c, true, true);
+    if (rc) return rc;
+    if (!o || !dtT || !dtC) return fail(c, RS_E_ARG, "null argument");
+    if (cT) *cT = rs_counts{};
+    if (cC) *cC = rs_counts{};
+    return macro(c, t0, t1, dtT, dtC, *o, cT, cC, nullptr, nullptr);
+}
+
+int rs_set_lambda(rs_ctx* c, const double* lam) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    c->lam.assign(lam, lam + (size_t)c->ns * c->nc);
+    return RS_OK;
+}
+int rs_get_lambda(rs_ctx* c, double* lam) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    std::copy(c->lam.begin(), c->lam.end(), lam);
+    return RS_OK;
+}
+int rs_zero_grad(rs_ctx* c) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    std::fill(c->gemis.begin(), c->gemis.end(), 0.0);
+    return RS_OK;
+}
+int rs_get_grad_emis(rs_ctx* c, double* g) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    std::copy(c->gemis.begin(), c->gemis.end(), g);
+    return RS_OK;
+}
+
+int rs_ssprk43_vjp(rs_ctx* c, double t, double dt) {
+    (void)t;
+    int rc = check_ready(c, false, true);
+    if (rc) return rc;
+    ssprk43_vjp(*c, c->lam, dt);
+    return RS_OK;
+}
+
+int rs_ros23_vjp(rs_ctx* c, double t, double dt) {
+    (void)t;
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    ros23_vjp_all(c, c->u, dt);
+    return RS_OK;
+}
+
+int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, double dtC0,
+                   const rs_split_opts* o, int loss_kind, int loss_species, double* J_out,
+                   double* grad_u0, double* grad_emis, rs_adjoint_stats* st) {
+    int rc = check_ready(c, true, true);
+    if (rc) return rc;
+    if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
+    if (o->chem_percell) return fail(c, RS_E_ARG, "adjoint needs the global-norm chemistry");
+    if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
+        return fail(c, RS_E_ARG, "bad loss");
+    using clk = std::chrono::steady_clock;
+    rs_adjoint_stats S{};
+    auto tf0 = clk::now();
+    std::vector<std::vector<double>> ck(nmacro);
+    c->seqT.assign(nmacro, {}); c->seqC.assign(nmacro, {});
+    double dtT = dtT0, dtC = dtC0;
+    for (int m = 0; m < nmacro; ++m) {
+        ck[m] = c->u;
+        rs_counts cT{}, cC{};
+        rc = macro(c, tstops[m], tstops[m + 1], &dtT, &dtC, *o, &cT, &cC, &c->seqT[m], &c->seqC[m]);
+        if (rc) return rc;
+        S.naccept_T += cT.naccept; S.nreject_T += cT.nreject;
+        S.naccept_C += cC.naccept; S.nreject_C += cC.nreject;
+    }
+    std::vector<double> uend = c->u;
+    if (J_out) *J_out = loss_value(*c, loss_species);
+    S.seconds_forward = std::chrono::duration<double>(clk::now() - tf0).count();
+    c->last_opts = *o;
+    c->have_record = true;
+
+    auto tb0 = clk::now();
+    double trep = 0.0;
+    std::fill(c->lam.begin(), c->lam.end(), 0.0);
+    std::fill(c->gemis.begin(), c->gemis.end(), 0.0);
+    {
+        const double* q = uend.data() + (size_t)loss_species * c->nc;
+        double* L = c->lam.data() + (size_t)loss_species * c->nc;
+        for (size_t x = 0; x < c->ncol; ++x) L[x] = 2.0 * q[x];
+    }
+    const int clampf = o->clamp_nonneg;
+    for (int m = nmacro - 1; m >= 0; --m) {
+        auto tr0 = clk::now();
+        // replay_fixed: tape of step inputs, each followed by its clamped output
+        std::vector<std::vector<double>> tapeT, tapeC;
+        std::vector<double> u = ck[m], raw;
+        for (auto& p : c->seqT[m].ts) {
+            tapeT.push_back(u);
+            ssprk43_attempt(*c, u, raw, p.second, o->atol_T, o->rtol_T);
+            if (clampf) for (double& x : raw) x = std::max(x, 0.0);
+            u.swap(raw);
+        }
+        std::vector<double> uT = u;
+        for (auto& p : c->seqC[m].ts) {
+            tapeC.push_back(u);
+            ros23_attempt(*c, u, raw, p.second, o->atol_C, o->rtol_C);
+            if (clampf) for (double& x : raw) x = std::max(x, 0.0);
+            u.swap(raw);
+        }
+        const std::vector<double>& ref = (m + 1 < nmacro) ? ck[m + 1] : uend;
+        double d = 0.0;
+        for (size_t x = 0; x < u.size(); ++x)
+            d = std::max(d, std::fabs(u[x] - ref[x]) / std::max(std::fabs(ref[x]), 1e-30));
+        S.replay_maxrel = std::max(S.replay_maxrel, d);
+        trep += std::chrono::duration<double>(clk::now() - tr0).count();
+        // chemistry backwards (adjoint_gradient.jl:821)
+        auto mask = [&](const std::vector<double>& out) {
+            if (!clampf) return;
+            for (size_t x = 0; x < c->lam.size(); ++x) if (!(out[x] > 0.0)) c->lam[x] = 0.0;
+        };
+        for (int jx = (int)tapeC.size() - 1; jx >= 0; --jx) {
+            const std::vector<double>& outj = (jx + 1 < (int)tapeC.size()) ? tapeC[jx + 1] : u;
+            mask(outj);
+            ros23_vjp_all(c, tapeC[jx], c->seqC[m].ts[jx].second);
+            S.nvjp_C += 1;
+        }
+        for (int jx = (int)tapeT.size() - 1; jx >= 0; --jx) {
+            const std::vector<double>& outj = (jx + 1 < (int)tapeT.size()) ? tapeT[jx + 1] : uT;
+            mask(outj);
+            ssprk43_vjp(*c, c->lam, c->seqT[m].ts[jx].second);
+            S.nvjp_T += 1;
+        }
+    }
+    S.seconds_backward = std::chrono::duration<double>(clk::now() - tb0).count();
+    S.seconds_replay = trep;
+    if (grad_u0) std::copy(c->lam.begin(), c->lam.end(), grad_u0);
+    if (grad_emis) std::copy(c->gemis.begin(), c->gemis.end(), grad_emis);
+    if (st) *st = S;
+    return RS_OK;
+}
+
+int rs_replay_loss(rs_ctx* c, int loss_kind, int loss_species, double* J_out) {
+    int rc = check_ready(c, true, true);
+    if (rc) return rc;
+    if (!c->have_record) return fail(c, RS_E_STATE, "no recorded sequence");
+    if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
+        return fail(c, RS_E_ARG, "bad loss");
+    const rs_split_opts& o = c->last_opts;
+    std::vector<double> raw;
+    for (size_t m = 0; m < c->seqT.size(); ++m) {
+        for (auto& p : c->seqT[m].ts) {
+            ssprk43_attempt(*c, c->u, raw, p.second, o.atol_T, o.rtol_T);
+            if (o.clamp_nonneg) for (double& x : raw) x = std::max(x, 0.0);
+            c->u.swap(raw);
+        }
+        for (auto& p : c->seqC[m].ts) {
+            ros23_attempt(*c, c->u, raw, p.second, o.atol_C, o.rtol_C);
+            if (o.clamp_nonneg) for (double& x : raw) x = std::max(x, 0.0);
+            c->u.swap(raw);
+        }
+    }
+    if (J_out) *J_out = loss_value(*c, loss_species);
+    return RS_OK;
+}
+
+// ---- ABI extensions (same symbols as the HIP library) -----------------------
+int rs_set_lu_ordering(rs_ctx* c, int ordering) {
+    if (!c) return RS_E_ARG;
+    if (ordering != 0 && ordering != 1) return fail(c, RS_E_ARG, "ordering must be 0 or 1");
+    return RS_OK;   // the oracle always eliminates densely in natural order
+}
+
+int rs_mechanism_info(rs_ctx* c, rs_mech_info* out) {
+    if (!c || !c->have_mech || !out) return fail(c, RS_E_STATE, "mechanism not set");
+    rs_mech_info m{};
+    m.ns = c->ns;
+    m.nr = c->nr;
+    m.nnz_J = 0;
+    m.nnz_LU = c->ns * c->ns;
+    m.ordering = 0;
+    const long long n = c->ns;
+    m.flops_attempt = (2 * n * n * n) / 3 + 3 * 2 * n * n;   // dense LU + 3 solves
+    m.flops_vjp = m.flops_attempt;
+    *out = m;
+    return RS_OK;
+}
+
+int rs_profile(rs_ctx* c, int enable) { (void)enable; return c ? RS_OK : RS_E_ARG; }
+
+int rs_profile_read(rs_ctx* c, rs_kernel_time* out, int max, int* n_out) {
+    (void)out; (void)max;
+    if (!c) return RS_E_ARG;
+    if (n_out) *n_out = 0;
+    return RS_OK;
+}
+
+// ---- oracle-only extras (declared in oracle/oracle_extra.h) -----------------
+// Tangent of one step at the current state: out = (d unew/d u) v + (d unew/d E) Ev.
+int rs_oracle_ros23_jvp(rs_ctx* c, double dt, const double* v, const double* Ev, double* out) {
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    std::vector<double> res((size_t)c->ns * c->nc);
+#pragma omp parallel
+    {
+        std::vector<double> uc(c->ns), vc(c->ns), oc(c->ns), ec(c->ns);
+#pragma omp for schedule(static)
+        for (long long cell = 0; cell < (long long)c->nc; ++cell) {
+            CellChem cc(*c, (size_t)cell);
+            gather(*c, c->u.data(), cell, uc.data());
+            gather(*c, v, cell, vc.data());
+            bool surf = Ev && (size_t)cell < c->ncol;
+            if (surf) for (int s = 0; s < c->ns; ++s) ec[s] = Ev[(size_t)s * c->ncol + cell];
+            ros23_jvp_cell(cc, uc.data(), dt, vc.data(), surf ? ec.data() : nullptr, oc.data());
+            scatter(*c, oc.data(), cell, res.data());
+        }
+    }
+    std::copy(res.begin(), res.end(), out);
+    return RS_OK;
+}
+
+int rs_oracle_ssprk43_jvp(rs_ctx* c, double dt, const double* v, double* out) {
+    int rc = check_ready(c, false, true);
+    if (rc) return rc;
+    std::vector<double> vv(v, v + (size_t)c->ns * c->nc), o;
+    ssprk43_jvp(*c, vv, o, dt);
+    std::copy(o.begin(), o.end(), out);
+    return RS_OK;
+}
+
+// Transport tendency f = L q (species < NT) and the dense per-cell Jacobian,
+// exposed for the operator-level checks.
+int rs_oracle_transport_rhs(rs_ctx* c, const double* q, double* f) {
+    if (!c || !c->have_wind) return fail(c, RS_E_STATE, "wind not set");
+    for (int s = 0; s < c->ns; ++s) {
+        if (s < c->nt) transport_rhs_block(*c, q + (size_t)s * c->nc, f + (size_t)s * c->nc);
+        else std::fill(f + (size_t)s * c->nc, f + (size_t)(s + 1) * c->nc, 0.0);
+    }
+    return RS_OK;
+}
+int rs_oracle_transport_rhs_adj(rs_ctx* c, const double* lam, double* g) {
+    if (!c || !c->have_wind) return fail(c, RS_E_STATE, "wind not set");
+    std::fill(g, g + (size_t)c->ns * c->nc, 0.0);
+    for (int s = 0; s < c->nt; ++s)
+        transport_rhs_adj_block(*c, lam + (size_t)s * c->nc, g + (size_t)s * c->nc);
+    return RS_OK;
+}
+int rs_oracle_chem_rhs(rs_ctx* c, const double* u, double* f) {
+    if (!c || !c->have_mech) return fail(c, RS_E_STATE, "mechanism not set");
+    std::vector<double> uc(c->ns), fc(c->ns);
+    for (size_t cell = 0; cell < c->nc; ++cell) {
+        CellChem cc(*c, cell);
+        gather(*c, u, cell, uc.data());
+        cc.rhs(uc.data(), fc.data());
+        scatter(*c, fc.data(), cell, f);
+    }
+    return RS_OK;
+}
+int rs_oracle_chem_jac(rs_ctx* c, const double* u, size_t cell, double* J) {
+    if (!c || !c->have_mech) return fail(c, RS_E_STATE, "mechanism not set");
+    std::vector<double> uc(c->ns);
+    gather(*c, u, cell, uc.data());
+    CellChem cc(*c, cell);
+    cc.jac(uc.data(), J);
+    return RS_OK;
+}
+int rs_oracle_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

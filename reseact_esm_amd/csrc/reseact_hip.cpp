@@ -1,0 +1,1172 @@
+// reseact_hip.cpp -- host side of libreseact_hip.so: the C ABI of
+// include/reseact_hip.h on one MI355X (gfx950).
+//
+// Ownership: a context owns one HIP stream (default priority) and all device
+// memory; every launch goes to that stream.  Adaptive solves never stop for
+// the host per attempt: the PI controller runs on the device (rs_ctrl_update)
+// and the host enqueues attempts in growing chunks, reading the controller
+// block back once per chunk; kernels of a finished solve exit at their first
+// instruction.  Chemistry kernels are compiled at rs_set_mechanism time with
+// hipRTC from the mechanism compiler's straight-line code (mechgen.cpp).
+//
+// Reference interfaces replaced (EarthSciML/reseact.esm):
+//   ssprk43_step / ros23_step / adaptive_solve / ssprk43_step_vjp /
+//   ros23_step_vjp     tools/reactant_handoff/rx_traced_integrator.jl:279-623
+//   macro_step, forward_pass, replay_fixed, backward_stage!, backward_sweep
+//                      tools/adjoint_gradient.jl:538-834
+//   lie_trotter_solve  tools/reactant_handoff/op_split.jl:100-164
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include "../../include/reseact_hip.h"
+#include "mechgen.h"
+#include "rs_launch.h"
+
+// launch wrappers from transport.hip
+extern "C" {
+hipError_t rs_launch_transport(int kind, const RsTransArgs* a, hipStream_t st);
+long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt);
+hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog, hipStream_t st);
+hipError_t rs_launch_fill(double* x, long long n, double v, hipStream_t st);
+hipError_t rs_launch_fill_u64(unsigned long long* x, int n, unsigned long long v, hipStream_t st);
+hipError_t rs_launch_mask(double* lam, const double* uout, long long n, hipStream_t st);
+hipError_t rs_launch_clamp(double* x, long long n, hipStream_t st);
+int rs_reduce_blocks(long long n);
+hipError_t rs_launch_relmax(const double* a, const double* b, long long n, double* part, hipStream_t st);
+hipError_t rs_launch_sumsq(const double* q, long long n, double* part, hipStream_t st);
+hipError_t rs_launch_loss_seed(double* lam, const double* u, long long ntot, long long sbase,
+                               long long ncol, hipStream_t st);
+}
+
+// embedded sources of the runtime-compiled chemistry kernels
+#include "jit_sources.inc"
+
+#define KIND_STAGE 0
+#define KIND_LAST 1
+#define KIND_ADJ_A 2
+#define KIND_ADJ_B 3
+#define KIND_ADJ_C 4
+#define KIND_ADJ_D 5
+
+namespace {
+
+enum ProfKind { P_TSTAGE, P_TLAST, P_CTRL, P_CHEM, P_PERCELL, P_CHEMVJP, P_TADJ, P_UTIL, P_COUNT };
+const char* kProfNames[P_COUNT] = {"transport_stage", "transport_last", "ctrl_update",
+                                   "chem_attempt",    "chem_percell",   "chem_vjp",
+                                   "transport_adj",   "util"};
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+#define HIPCHK(x)                                       \
+    do {                                                \
+        hipError_t e_ = (x);                            \
+        if (e_ != hipSuccess) throw HipError{e_, #x};   \
+    } while (0)
+
+struct ArgError { std::string msg; int code; };
+
+template <class T>
+struct DevArray {
+    T* p = nullptr;
+    size_t n = 0;
+    DevArray() = default;
+    DevArray(const DevArray&) = delete;
+    DevArray& operator=(const DevArray&) = delete;
+    ~DevArray() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void resize(size_t m) {
+        if (m == n) return;
+        release();
+        if (m) HIPCHK(hipMalloc((void**)&p, m * sizeof(T)));
+        n = m;
+    }
+    void ensure(size_t m) { if (m > n) resize(m); }
+};
+
+struct ChemModule {
+    hipModule_t mod = nullptr;
+    hipFunction_t attempt = nullptr, percell = nullptr, vjp = nullptr;
+    rs::MechPlan plan;
+    ~ChemModule() { if (mod) (void)hipModuleUnload(mod); }
+};
+
+// Compiled code objects, shared by every context of the process.
+std::mutex g_jit_mu;
+std::map<std::string, std::vector<char>> g_jit_cache;
+
+unsigned long long fnv1a(const std::string& s) {
+    unsigned long long h = 1469598103934665603ULL;
+    for (unsigned char c : s) { h ^= c; h *= 1099511628211ULL; }
+    return h;
+}
+
+}  // namespace
+
+struct rs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ncu = 0;
+    int nlon = 0, nlat = 0, nlev = 0, ns = 0, nt = 0;
+    long long nc = 0, ncol = 0;
+    bool have_wind = false, have_mech = false, have_state = false, have_cand = false;
+    int cur = 0;
+    int lu_order = 1;
+    DevArray<double> S[2], X, Y, cx, wz, wl, K, jfac, E, lam, LA, LB, LC, gE, partial, seqlog;
+    DevArray<double> red;
+    DevArray<unsigned long long> head;
+    DevArray<long long> stats;
+    DevArray<double> dtcell;
+    RsCtrl* dctrl = nullptr;
+    RsCtrl* hctrl = nullptr;       // pinned
+    long long* hstats = nullptr;   // pinned, 8
+    double* hred = nullptr;        // pinned, 1024
+    bool has_jfac = false, has_E = false;
+    int nr = 0;
+    std::unique_ptr<ChemModule> chem;
+    // time-loop adjoint records
+    std::vector<std::vector<std::pair<double, double>>> seqT, seqC;
+    rs_split_opts last_opts{};
+    bool have_record = false;
+    DevArray<double> ckpt, tape;
+    // profiling
+    bool prof = false;
+    struct EvPair { hipEvent_t a, b; int kind; };
+    std::vector<EvPair> ev_pool;
+    std::vector<hipEvent_t> ev_free;
+    long long prof_n[P_COUNT] = {0};
+    double prof_ms[P_COUNT] = {0};
+    std::string err;
+
+    size_t N() const { return (size_t)ns * (size_t)nc; }
+};
+
+namespace {
+
+int fail(rs_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    return code;
+}
+
+// ---- profiling --------------------------------------------------------------
+void prof_flush(rs_ctx* c) {
+    if (c->ev_pool.empty()) return;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& e : c->ev_pool) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+        c->prof_n[e.kind] += 1;
+        c->prof_ms[e.kind] += ms;
+        c->ev_free.push_back(e.a);
+        c->ev_free.push_back(e.b);
+    }
+    c->ev_pool.clear();
+}
+
+hipEvent_t ev_get(rs_ctx* c) {
+    if (c->ev_free.empty()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t e = c->ev_free.back();
+    c->ev_free.pop_back();
+    return e;
+}
+
+struct ProfScope {
+    rs_ctx* c;
+    hipEvent_t b{};
+    int kind;
+    bool on;
+    ProfScope(rs_ctx* cc, int k) : c(cc), kind(k), on(cc->prof) {
+        if (!on) return;
+        if (c->ev_pool.size() >= 4096) prof_flush(c);
+        hipEvent_t a = ev_get(c);
+        b = ev_get(c);
+        HIPCHK(hipEventRecord(a, c->stream));
+        c->ev_pool.push_back({a, b, kind});
+    }
+    ~ProfScope() noexcept(false) {
+        if (on) HIPCHK(hipEventRecord(b, c->stream));
+    }
+};
+
+// ---- JIT --------------------------------------------------------------------
+std::string chem_source(const rs::MechSpec& m, const rs::MechPlan& p) {
+    std::string src;
+    src += kJitPrelude;
+    src += kJitLaunch;
+    src += rs::generate_mech_source(m, p);
+    src += kJitChemKernels;
+    return src;
+}
+
+// On-disk cache of compiled code objects: $RS_JIT_CACHE_DIR, else
+// $TMPDIR/reseact_jit_<uid>; RS_JIT_CACHE_DIR="" disables it.
+std::string jit_cache_path(const std::string& key) {
+    const char* env = std::getenv("RS_JIT_CACHE_DIR");
+    std::string dir;
+    if (env) {
+        if (!*env) return "";
+        dir = env;
+    } else {
+        const char* tmp = std::getenv("TMPDIR");
+        dir = std::string(tmp && *tmp ? tmp : "/tmp") + "/reseact_jit_" + std::to_string(getuid());
+    }
+    mkdir(dir.c_str(), 0700);
+    return dir + "/" + key + ".hsaco";
+}
+
+std::vector<char> jit_compile(const std::string& src, std::string& log) {
+    const std::string key = std::to_string(fnv1a(src)) + "_" + std::to_string(src.size());
+    {
+        std::lock_guard<std::mutex> g(g_jit_mu);
+        auto it = g_jit_cache.find(key);
+        if (it != g_jit_cache.end()) return it->second;
+    }
+    const std::string path = jit_cache_path(key);
+    if (!path.empty()) {
+        if (FILE* f = std::fopen(path.c_str(), "rb")) {
+            std::vector<char> code;
+            char buf[65536];
+            size_t n;
+            while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) code.insert(code.end(), buf, buf + n);
+            std::fclose(f);
+            if (!code.empty()) {
+                std::lock_guard<std::mutex> g(g_jit_mu);
+                g_jit_cache[key] = code;
+                return code;
+            }
+        }
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "rs_chem.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        log = "hiprtcCreateProgram failed";
+        return {};
+    }
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
+                          "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    size_t ln = 0;
+    hiprtcGetProgramLogSize(prog, &ln);
+    if (ln > 1) {
+        std::vector<char> lb(ln + 1, 0);
+        hiprtcGetProgramLog(prog, lb.data());
+        log = lb.data();
+    }
+    std::vector<char> code;
+    if (r == HIPRTC_SUCCESS) {
+        size_t cs = 0;
+        hiprtcGetCodeSize(prog, &cs);
+        code.resize(cs);
+        hiprtcGetCode(prog, code.data());
+    }
+    hiprtcDestroyProgram(&prog);
+    if (!code.empty()) {
+        std::lock_guard<std::mutex> g(g_jit_mu);
+        g_jit_cache[key] = code;
+        if (!path.empty()) {
+            const std::string tmpp = path + ".tmp" + std::to_string(getpid());
+            if (FILE* f = std::fopen(tmpp.c_str(), "wb")) {
+                const bool ok = std::fwrite(code.data(), 1, code.size(), f) == code.size();
+                std::fclose(f);
+                if (ok) std::rename(tmpp.c_str(), path.c_str());
+                else std::remove(tmpp.c_str());
+            }
+        }
+    }
+    return code;
+}
+
+// ---- helpers -----------------------------------------------------------------
+void h2d(double* d, const double* h, size_t n, hipStream_t st) {
+    HIPCHK(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, st));
+}
+void d2h(double* h, const double* d, size_t n, hipStream_t st) {
+    HIPCHK(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+}
+void d2d(double* d, const double* s, size_t n, hipStream_t st) {
+    if (n) HIPCHK(hipMemcpyAsync(d, s, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+}
+
+RsTransArgs trans_args(rs_ctx* c) {
+    RsTransArgs a{};
+    a.S0 = c->S[0].p;
+    a.S1 = c->S[1].p;
+    a.cx = c->cx.p;
+    a.wz = c->wz.p;
+    a.wl = c->wl.p;
+    a.nc = c->nc;
+    a.nlon = c->nlon;
+    a.nlat = c->nlat;
+    a.nlev = c->nlev;
+    a.nt = c->nt;
+    a.partial = c->partial.p;
+    return a;
+}
+
+long long trans_nblocks(rs_ctx* c) { return rs_transport_nblocks(c->nlon, c->nlat, c->nlev, c->nt); }
+
+// One SSPRK43 attempt: either from the state buffers under a controller (ctrl
+// != null: dt/cur/done from the device), or between explicit buffers.
+void enqueue_ssprk43(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* uout, double dt,
+                     double atol, double rtol, int clamp) {
+    RsTransArgs a = trans_args(c);
+    a.ctrl = ctrl;
+    a.dt_static = dt;
+    a.cur_static = c->cur;
+    a.atol = atol;
+    a.rtol = rtol;
+    const bool st = (uin == nullptr);
+    // stage 1: X = u + h/2 L u
+    a.a_is_state = st ? 1 : 0;
+    a.a = uin;
+    a.out_is_state = 0;
+    a.out = c->X.p;
+    a.clamp = 0;
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_transport(KIND_STAGE, &a, c->stream)); }
+    a.a_is_state = 0;
+    a.a = c->X.p;
+    a.out = c->Y.p;
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_transport(KIND_STAGE, &a, c->stream)); }
+    a.a = c->Y.p;
+    a.out = c->X.p;
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_transport(KIND_STAGE, &a, c->stream)); }
+    a.a_is_state = st ? 1 : 0;
+    a.a = uin;
+    a.b = c->X.p;
+    a.out_is_state = st ? 1 : 0;
+    a.out = uout;
+    a.clamp = clamp;
+    { ProfScope ps(c, P_TLAST); HIPCHK(rs_launch_transport(KIND_LAST, &a, c->stream)); }
+}
+
+void chem_launch(rs_ctx* c, hipFunction_t f, void* argp, size_t argsz, unsigned grid,
+                 unsigned block, int kind) {
+    void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, argp, HIP_LAUNCH_PARAM_BUFFER_SIZE, &argsz,
+                   HIP_LAUNCH_PARAM_END};
+    ProfScope ps(c, kind);
+    HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, c->stream, nullptr, cfg));
+}
+
+const unsigned CHEM_BLOCK = 64;
+
+unsigned chem_grid(rs_ctx* c) { return (unsigned)((c->nc + CHEM_BLOCK - 1) / CHEM_BLOCK); }
+
+void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cur_static,
+                   double dt, double atol, double rtol, int clamp) {
+    RsChemArgs a{};
+    a.S0 = S0;
+    a.S1 = S1;
+    a.ctrl = ctrl;
+    a.K = c->K.p;
+    a.jfac = c->has_jfac ? c->jfac.p : nullptr;
+    a.E = c->has_E ? c->E.p : nullptr;
+    a.partial = c->partial.p;
+    a.nc = c->nc;
+    a.ncol = c->ncol;
+    a.dt_static = dt;
+    a.atol = atol;
+    a.rtol = rtol;
+    a.cur_static = cur_static;
+    a.clamp = clamp;
+    chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), CHEM_BLOCK, P_CHEM);
+}
+
+double host_sum(rs_ctx* c, const double* dpart, int n) {
+    HIPCHK(hipMemcpyAsync(c->hred, dpart, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += c->hred[i];
+    return s;
+}
+
+double host_max(rs_ctx* c, const double* dpart, int n) {
+    HIPCHK(hipMemcpyAsync(c->hred, dpart, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s = std::max(s, c->hred[i]);
+    return s;
+}
+
+enum class Op { T, C };
+
+// Global-norm adaptive solve on the state buffers (rx_traced_integrator.jl:455-507).
+int run_adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs_pictrl& pc,
+                 double atol, double rtol, int clamp, long long maxiters, rs_counts* out,
+                 std::vector<std::pair<double, double>>* seq) {
+    RsCtrl h{};
+    h.t = t0;
+    h.t1 = t1;
+    h.tlim = t1 - 1.0e-9;
+    h.dt = *dt_io;
+    h.qold = pc.qoldinit;
+    h.nnorm = (double)c->ns * (double)c->nc;
+    h.beta1 = pc.beta1;
+    h.beta2 = pc.beta2;
+    h.invqmin = 1.0 / pc.qmin;
+    h.invqmax = 1.0 / pc.qmax;
+    h.gamma = pc.gamma;
+    h.qsmin = pc.qsteady_min;
+    h.qsmax = pc.qsteady_max;
+    h.qoldinit = pc.qoldinit;
+    h.maxiters = maxiters;
+    h.cur = c->cur;
+    h.done = (!(t0 < h.tlim) || maxiters <= 0) ? 1 : 0;
+    h.dtc = std::min(h.dt, t1 - t0);
+    long long cap = seq ? std::min<long long>(maxiters, 1 << 22) : 0;
+    if (seq) c->seqlog.ensure((size_t)(2 * std::max<long long>(cap, 1)));
+    h.logcap = (int)cap;
+    const int start_cur = c->cur;
+    *c->hctrl = h;
+    HIPCHK(hipMemcpyAsync(c->dctrl, c->hctrl, sizeof(RsCtrl), hipMemcpyHostToDevice, c->stream));
+    const int npart = (op == Op::T) ? (int)trans_nblocks(c) : (int)chem_grid(c);
+    double* logp = seq ? c->seqlog.p : nullptr;
+    int chunk = 2;
+    while (!c->hctrl->done) {
+        for (int q = 0; q < chunk; ++q) {
+            if (op == Op::T) enqueue_ssprk43(c, c->dctrl, nullptr, nullptr, 0.0, atol, rtol, clamp);
+            else enqueue_ros23(c, c->dctrl, c->S[0].p, c->S[1].p, 0, 0.0, atol, rtol, clamp);
+            ProfScope ps(c, P_CTRL);
+            HIPCHK(rs_launch_ctrl(c->dctrl, c->partial.p, npart, logp, c->stream));
+        }
+        HIPCHK(hipMemcpyAsync(c->hctrl, c->dctrl, sizeof(RsCtrl), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        chunk = std::min(chunk * 2, 64);
+    }
+    const RsCtrl r = *c->hctrl;
+    c->cur = r.cur;
+    if (op == Op::T && c->nt < c->ns && c->cur != start_cur) {
+        // species nt..ns-1 are untouched by transport: carry them over
+        const size_t off = (size_t)c->nt * c->nc, n = (size_t)(c->ns - c->nt) * c->nc;
+        d2d(c->S[c->cur].p + off, c->S[start_cur].p + off, n, c->stream);
+    }
+    if (seq) {
+        seq->clear();
+        if (r.nlog > 0) {
+            std::vector<double> lg(2 * (size_t)r.nlog);
+            d2h(lg.data(), c->seqlog.p, lg.size(), c->stream);
+            for (int i = 0; i < r.nlog; ++i) seq->emplace_back(lg[2 * i], lg[2 * i + 1]);
+        }
+        if (r.nacc > r.nlog) return fail(c, RS_E_ARG, "accepted-step log overflow");
+    }
+    *dt_io = r.dt;
+    if (out) {
+        out->naccept = r.nacc;
+        out->nreject = r.nrej;
+        out->t_end = r.t;
+        out->dt_end = r.dt;
+    }
+    if (r.iters >= maxiters && r.t < r.tlim)
+        return fail(c, RS_E_MAXITERS, "adaptive loop hit maxiters at t=" + std::to_string(r.t));
+    return RS_OK;
+}
+
+int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_host,
+                const rs_pictrl& pc, double atol, double rtol, int clamp, long long maxiters,
+                rs_percell_stats* out) {
+    RsPercellArgs a{};
+    a.u = c->S[c->cur].p;
+    a.K = c->K.p;
+    a.jfac = c->has_jfac ? c->jfac.p : nullptr;
+    a.E = c->has_E ? c->E.p : nullptr;
+    a.dtcell = nullptr;
+    if (dt_cells_host) {
+        c->dtcell.ensure((size_t)c->nc);
+        h2d(c->dtcell.p, dt_cells_host, (size_t)c->nc, c->stream);
+        a.dtcell = c->dtcell.p;
+    }
+    c->head.ensure(1);
+    c->stats.ensure(8);
+    a.head = c->head.p;
+    a.stats = c->stats.p;
+    a.nc = c->nc;
+    a.ncol = c->ncol;
+    a.maxiters = maxiters;
+    a.t0 = t0;
+    a.t1 = t1;
+    a.dt0 = dt0;
+    a.atol = atol;
+    a.rtol = rtol;
+    a.beta1 = pc.beta1;
+    a.beta2 = pc.beta2;
+    a.invqmin = 1.0 / pc.qmin;
+    a.invqmax = 1.0 / pc.qmax;
+    a.gamma = pc.gamma;
+    a.qsmin = pc.qsteady_min;
+    a.qsmax = pc.qsteady_max;
+    a.qoldinit = pc.qoldinit;
+    a.clamp = clamp;
+    // queue head and statistics (min starts at the largest value)
+    long long init[8] = {0, 0, 0, 0, LLONG_MAX, 0, 0, 0};
+    std::memcpy(c->hstats, init, sizeof(init));
+    HIPCHK(hipMemcpyAsync(c->stats.p, c->hstats, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(rs_launch_fill_u64(c->head.p, 1, 0ULL, c->stream));
+    int per_cu = 0;
+    HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c->chem->percell,
+                                                              (int)CHEM_BLOCK, 0));
+    per_cu = std::max(per_cu, 1);
+    long long grid = (long long)per_cu * c->ncu;
+    grid = std::min<long long>(grid, (c->nc + CHEM_BLOCK - 1) / CHEM_BLOCK);
+    grid = std::max<long long>(grid, 1);
+    chem_launch(c, c->chem->percell, &a, sizeof(a), (unsigned)grid, CHEM_BLOCK, P_PERCELL);
+    HIPCHK(hipMemcpyAsync(c->hstats, c->stats.p, 8 * sizeof(long long), hipMemcpyDeviceToHost,
+                          c->stream));
+    if (dt_cells_host) d2h(dt_cells_host, c->dtcell.p, (size_t)c->nc, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (out) {
+        out->attempts_total = c->hstats[0];
+        out->accepts_total = c->hstats[1];
+        out->rejects_total = c->hstats[2];
+        out->attempts_max = c->hstats[3];
+        out->attempts_min = c->hstats[4] == LLONG_MAX ? 0 : c->hstats[4];
+        out->n_failed = c->hstats[5];
+    }
+    if (c->hstats[5] > 0)
+        return fail(c, RS_E_MAXITERS, std::to_string(c->hstats[5]) + " cells hit maxiters");
+    return RS_OK;
+}
+
+int check_ready(rs_ctx* c, bool need_mech, bool need_wind) {
+    if (!c) return RS_E_ARG;
+    if (c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!c->have_state) return fail(c, RS_E_STATE, "state not set");
+    if (need_mech && !c->have_mech) return fail(c, RS_E_STATE, "mechanism not set");
+    if (need_wind && c->nt > 0 && !c->have_wind) return fail(c, RS_E_STATE, "wind not set");
+    return RS_OK;
+}
+
+int macro(rs_ctx* c, double t0, double t1, double* dtT, double* dtC, const rs_split_opts& o,
+          rs_counts* cT, rs_counts* cC, std::vector<std::pair<double, double>>* sT,
+          std::vector<std::pair<double, double>>* sC) {
+    int rc = RS_OK;
+    if (c->nt > 0) {
+        rc = run_adaptive(c, Op::T, t0, t1, dtT, o.ctrl_T, o.atol_T, o.rtol_T, o.clamp_nonneg,
+                          o.maxiters, cT, sT);
+        if (rc) return rc;
+    }
+    if (o.chem_percell) {
+        rs_percell_stats st{};
+        rc = run_percell(c, t0, t1, *dtC, nullptr, o.ctrl_C, o.atol_C, o.rtol_C, o.clamp_nonneg,
+                         o.maxiters, &st);
+        if (cC) { cC->naccept = st.accepts_total; cC->nreject = st.rejects_total; cC->t_end = t1; cC->dt_end = *dtC; }
+        return rc;
+    }
+    return run_adaptive(c, Op::C, t0, t1, dtC, o.ctrl_C, o.atol_C, o.rtol_C, o.clamp_nonneg,
+                        o.maxiters, cC, sC);
+}
+
+// lam <- SSPRK43 VJP (lam in c->lam; species >= nt untouched: identity)
+void enqueue_ssprk43_vjp(rs_ctx* c, double dt) {
+    RsTransArgs a = trans_args(c);
+    a.dt_static = dt;
+    a.a_is_state = 0;
+    a.out_is_state = 0;
+    a.a = c->lam.p;
+    a.out = c->LA.p;
+    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_A, &a, c->stream)); }
+    a.a = c->LA.p;
+    a.out = c->LB.p;
+    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_B, &a, c->stream)); }
+    a.a = c->LB.p;
+    a.out = c->LC.p;
+    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_C, &a, c->stream)); }
+    a.a = c->LC.p;
+    a.b = c->LA.p;
+    a.out = c->lam.p;
+    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_D, &a, c->stream)); }
+}
+
+void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt) {
+    RsChemVjpArgs a{};
+    a.u = u;
+    a.uout = uout;
+    a.lam = c->lam.p;
+    a.gE = c->gE.p;
+    a.K = c->K.p;
+    a.jfac = c->has_jfac ? c->jfac.p : nullptr;
+    a.E = c->has_E ? c->E.p : nullptr;
+    a.nc = c->nc;
+    a.ncol = c->ncol;
+    a.dt = dt;
+    chem_launch(c, c->chem->vjp, &a, sizeof(a), chem_grid(c), CHEM_BLOCK, P_CHEMVJP);
+}
+
+double loss_on_device(rs_ctx* c, const double* u, int species) {
+    const double* q = u + (size_t)species * c->nc;
+    const int nb = rs_reduce_blocks(c->ncol);
+    c->red.ensure(1024);
+    HIPCHK(rs_launch_sumsq(q, c->ncol, c->red.p, c->stream));
+    return host_sum(c, c->red.p, nb);
+}
+
+// Fixed-sequence replay of one macro step from `u0` into the tape; tape slot j
+// holds the input of step j (transport steps first, then chemistry), the last
+// slot the end state.  Returns the number of slots written.
+size_t replay_into_tape(rs_ctx* c, const double* u0, const std::vector<std::pair<double, double>>& sT,
+                        const std::vector<std::pair<double, double>>& sC, const rs_split_opts& o) {
+    const size_t N = c->N();
+    const size_t nslots = sT.size() + sC.size() + 1;
+    c->tape.ensure(nslots * N);
+    double* tp = c->tape.p;
+    d2d(tp, u0, N, c->stream);
+    size_t j = 0;
+    for (auto& p : sT) {
+        double* in = tp + j * N;
+        double* outp = tp + (j + 1) * N;
+        enqueue_ssprk43(c, nullptr, in, outp, p.second, o.atol_T, o.rtol_T, o.clamp_nonneg);
+        if (c->nt < c->ns) {
+            const size_t off = (size_t)c->nt * c->nc;
+            d2d(outp + off, in + off, (size_t)(c->ns - c->nt) * c->nc, c->stream);
+        }
+        ++j;
+    }
+    for (auto& p : sC) {
+        enqueue_ros23(c, nullptr, tp + j * N, tp + (j + 1) * N, 0, p.second, o.atol_C, o.rtol_C,
+                      o.clamp_nonneg);
+        ++j;
+    }
+    return nslots;
+}
+
+}  // namespace
+
+namespace {
+int guarded(rs_ctx* c, const std::function<int()>& f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return fail(c, RS_E_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
+    } catch (const ArgError& e) {
+        return fail(c, e.code, e.msg);
+    } catch (const std::bad_alloc&) {
+        return fail(c, RS_E_ARG, "host out of memory");
+    }
+}
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char* rs_backend(void) { return "hip-gfx950"; }
+
+int rs_create(rs_ctx** out, int device) {
+    if (!out) return RS_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RS_E_HIP;
+    if (device < 0 || device >= ndev) return RS_E_ARG;
+    auto* c = new rs_ctx();
+    c->device = device;
+    int rc = guarded(c, [&]() {
+        HIPCHK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        c->ncu = prop.multiProcessorCount;
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHK(hipMalloc((void**)&c->dctrl, sizeof(RsCtrl)));
+        HIPCHK(hipHostMalloc((void**)&c->hctrl, sizeof(RsCtrl), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&c->hstats, 8 * sizeof(long long), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&c->hred, 1024 * sizeof(double), hipHostMallocDefault));
+        std::memset(c->hctrl, 0, sizeof(RsCtrl));
+        return RS_OK;
+    });
+    if (rc) {
+        rs_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return RS_OK;
+}
+
+int rs_destroy(rs_ctx* c) {
+    if (!c) return RS_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->ev_pool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    for (auto& e : c->ev_free) (void)hipEventDestroy(e);
+    c->chem.reset();
+    if (c->dctrl) (void)hipFree(c->dctrl);
+    if (c->hctrl) (void)hipHostFree(c->hctrl);
+    if (c->hstats) (void)hipHostFree(c->hstats);
+    if (c->hred) (void)hipHostFree(c->hred);
+    hipStream_t st = c->stream;
+    delete c;   // DevArray destructors free device memory
+    if (st) (void)hipStreamDestroy(st);
+    return RS_OK;
+}
+
+const char* rs_last_error(const rs_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
+    if (!c) return RS_E_ARG;
+    if (nlon < 1 || nlat < 1 || nlev < 1 || ns < 1 || nt < 0 || nt > ns)
+        return fail(c, RS_E_ARG, "bad grid sizes");
+    if (nt > 0 && nlon < 4)
+        return fail(c, RS_E_ARG, "transport needs nlon >= 4 (periodic 5-point stencil)");
+    return guarded(c, [&]() {
+        HIPCHK(hipSetDevice(c->device));
+        c->nlon = nlon; c->nlat = nlat; c->nlev = nlev; c->ns = ns; c->nt = nt;
+        c->ncol = (long long)nlon * nlat;
+        c->nc = c->ncol * nlev;
+        const size_t N = c->N(), NT = (size_t)nt * (size_t)c->nc;
+        c->S[0].resize(N); c->S[1].resize(N);
+        c->X.resize(NT); c->Y.resize(NT);
+        c->lam.resize(N);
+        c->LA.resize(NT); c->LB.resize(NT); c->LC.resize(NT);
+        c->gE.resize((size_t)ns * c->ncol);
+        const long long np = std::max<long long>(rs_transport_nblocks(nlon, nlat, nlev, std::max(nt, 1)),
+                                                 (c->nc + CHEM_BLOCK - 1) / CHEM_BLOCK);
+        c->partial.resize((size_t)np);
+        c->red.resize(1024);
+        HIPCHK(rs_launch_fill(c->lam.p, (long long)N, 0.0, c->stream));
+        HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
+        c->cur = 0;
+        c->have_state = c->have_wind = c->have_mech = c->have_cand = false;
+        c->has_jfac = c->has_E = false;
+        c->chem.reset();
+        c->have_record = false;
+        return RS_OK;
+    });
+}
+
+int rs_set_wind(rs_ctx* c, const double* cx, const double* wz, const double* wl) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!cx || !wz || !wl) return fail(c, RS_E_ARG, "null wind array");
+    for (int j = 0; j < c->nlat; ++j) if (!std::isfinite(cx[j])) return fail(c, RS_E_NONFINITE, "cx");
+    for (int k = 0; k <= c->nlev; ++k) if (!std::isfinite(wz[k])) return fail(c, RS_E_NONFINITE, "wz");
+    for (int i = 0; i < c->nlon; ++i) if (!std::isfinite(wl[i])) return fail(c, RS_E_NONFINITE, "wl");
+    return guarded(c, [&]() {
+        c->cx.resize(c->nlat); c->wz.resize(c->nlev + 1); c->wl.resize(c->nlon);
+        h2d(c->cx.p, cx, c->nlat, c->stream);
+        h2d(c->wz.p, wz, c->nlev + 1, c->stream);
+        h2d(c->wl.p, wl, c->nlon, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->have_wind = true;
+        return RS_OK;
+    });
+}
+
+int rs_set_lu_ordering(rs_ctx* c, int ordering) {
+    if (!c) return RS_E_ARG;
+    if (ordering != 0 && ordering != 1) return fail(c, RS_E_ARG, "ordering must be 0 or 1");
+    c->lu_order = ordering;
+    return RS_OK;
+}
+
+int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const double* k,
+                     const int* photo) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (nr < 0 || (nr > 0 && (!re || !pr || !k))) return fail(c, RS_E_ARG, "bad mechanism arrays");
+    rs::MechSpec m;
+    m.ns = c->ns;
+    m.nr = nr;
+    for (int r = 0; r < nr; ++r) {
+        m.ra.push_back(re[2 * r]); m.rb.push_back(re[2 * r + 1]);
+        m.pa.push_back(pr[2 * r]); m.pb.push_back(pr[2 * r + 1]);
+        m.photo.push_back(photo ? (photo[r] != 0) : 0);
+        if (!std::isfinite(k[r])) return fail(c, RS_E_NONFINITE, "rate constant");
+    }
+    std::string v = rs::validate_mech(m);
+    if (!v.empty()) return fail(c, RS_E_ARG, v);
+    return guarded(c, [&]() {
+        auto cm = std::make_unique<ChemModule>();
+        cm->plan = rs::plan_mechanism(m, c->lu_order);
+        std::string log;
+        std::vector<char> code = jit_compile(chem_source(m, cm->plan), log);
+        if (code.empty()) return fail(c, RS_E_JIT, "hipRTC compile failed: " + log.substr(0, 2000));
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipModuleLoadData(&cm->mod, code.data()));
+        HIPCHK(hipModuleGetFunction(&cm->attempt, cm->mod, "rs_chem_attempt"));
+        HIPCHK(hipModuleGetFunction(&cm->percell, cm->mod, "rs_chem_percell"));
+        HIPCHK(hipModuleGetFunction(&cm->vjp, cm->mod, "rs_chem_vjp"));
+        c->K.resize(std::max(nr, 1));
+        if (nr) h2d(c->K.p, k, nr, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->nr = nr;
+        c->chem = std::move(cm);
+        c->have_mech = true;
+        return RS_OK;
+    });
+}
+
+int rs_set_photolysis(rs_ctx* c, const double* jfac) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    return guarded(c, [&]() {
+        if (!jfac) { c->has_jfac = false; return RS_OK; }
+        c->jfac.resize((size_t)c->nc);
+        h2d(c->jfac.p, jfac, (size_t)c->nc, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->has_jfac = true;
+        return RS_OK;
+    });
+}
+
+int rs_set_emissions(rs_ctx* c, const double* e) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    return guarded(c, [&]() {
+        if (!e) { c->has_E = false; return RS_OK; }
+        c->E.resize((size_t)c->ns * c->ncol);
+        h2d(c->E.p, e, c->E.n, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->has_E = true;
+        return RS_OK;
+    });
+}
+
+int rs_set_state(rs_ctx* c, const double* u) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!u) return fail(c, RS_E_ARG, "null state");
+    return guarded(c, [&]() {
+        h2d(c->S[c->cur].p, u, c->N(), c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->have_state = true;
+        c->have_cand = false;
+        return RS_OK;
+    });
+}
+
+int rs_get_state(rs_ctx* c, double* u) {
+    if (!c || !c->have_state) return fail(c, RS_E_STATE, "state not set");
+    if (!u) return fail(c, RS_E_ARG, "null output");
+    return guarded(c, [&]() { d2h(u, c->S[c->cur].p, c->N(), c->stream); return RS_OK; });
+}
+
+int rs_ssprk43_step(rs_ctx* c, double t, double dt, double atol, double rtol, double* eest) {
+    (void)t;
+    int rc = check_ready(c, false, true);
+    if (rc) return rc;
+    return guarded(c, [&]() {
+        double* in = c->S[c->cur].p;
+        double* outp = c->S[c->cur ^ 1].p;
+        if (c->nt > 0) {
+            enqueue_ssprk43(c, nullptr, in, outp, dt, atol, rtol, 0);
+        }
+        if (c->nt < c->ns) {
+            const size_t off = (size_t)c->nt * c->nc;
+            d2d(outp + off, in + off, (size_t)(c->ns - c->nt) * c->nc, c->stream);
+        }
+        double sse = c->nt > 0 ? host_sum(c, c->partial.p, (int)trans_nblocks(c)) : 0.0;
+        if (eest) *eest = std::sqrt(sse / ((double)c->ns * (double)c->nc));
+        c->have_cand = true;
+        return RS_OK;
+    });
+}
+
+int rs_ros23_step(rs_ctx* c, double t, double dt, double atol, double rtol, double* eest) {
+    (void)t;
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    return guarded(c, [&]() {
+        enqueue_ros23(c, nullptr, c->S[0].p, c->S[1].p, c->cur, dt, atol, rtol, 0);
+        double sse = host_sum(c, c->partial.p, (int)chem_grid(c));
+        if (eest) *eest = std::sqrt(sse / ((double)c->ns * (double)c->nc));
+        c->have_cand = true;
+        return RS_OK;
+    });
+}
+
+int rs_get_candidate(rs_ctx* c, double* u) {
+    if (!c || !c->have_cand) return fail(c, RS_E_STATE, "no candidate");
+    return guarded(c, [&]() { d2h(u, c->S[c->cur ^ 1].p, c->N(), c->stream); return RS_OK; });
+}
+
+int rs_accept_candidate(rs_ctx* c, int clamp) {
+    if (!c || !c->have_cand) return fail(c, RS_E_STATE, "no candidate");
+    return guarded(c, [&]() {
+        c->cur ^= 1;
+        if (clamp) HIPCHK(rs_launch_clamp(c->S[c->cur].p, (long long)c->N(), c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->have_cand = false;
+        return RS_OK;
+    });
+}
+
+int rs_adaptive_transport(rs_ctx* c, double t0, double t1, double* dt_io, const rs_pictrl* ctrl,
+                          double atol, double rtol, int clamp, long long maxiters, rs_counts* out) {
+    int rc = check_ready(c, false, true);
+    if (rc) return rc;
+    if (!dt_io || !ctrl || !(*dt_io > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
+    if (c->nt == 0) return fail(c, RS_E_STATE, "no transported species");
+    c->have_cand = false;
+    return guarded(c, [&]() {
+        return run_adaptive(c, Op::T, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out, nullptr);
+    });
+}
+
+int rs_adaptive_chem(rs_ctx* c, double t0, double t1, double* dt_io, const rs_pictrl* ctrl,
+                     double atol, double rtol, int clamp, long long maxiters, rs_counts* out) {
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    if (!dt_io || !ctrl || !(*dt_io > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
+    c->have_cand = false;
+    return guarded(c, [&]() {
+        return run_adaptive(c, Op::C, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out, nullptr);
+    });
+}
+
+int rs_chem_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells,
+                    const rs_pictrl* ctrl, double atol, double rtol, int clamp,
+                    long long maxiters, rs_percell_stats* out) {
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    if (!ctrl || !(dt0 > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
+    c->have_cand = false;
+    return guarded(c, [&]() {
+        return run_percell(c, t0, t1, dt0, dt_cells, *ctrl, atol, rtol, clamp, maxiters, out);
+    });
+}
+
+int rs_macro_step(rs_ctx* c, double t0, double t1, double* dtT, double* dtC,
+                  const rs_split_opts* o, rs_counts* cT, rs_counts* cC) {
+    int rc = check_ready(c, true, true);
+    if (rc) return rc;
+    if (!o || !dtT || !dtC) return fail(c, RS_E_ARG, "null argument");
+    if (cT) *cT = rs_counts{};
+    if (cC) *cC = rs_counts{};
+    c->have_cand = false;
+    return guarded(c, [&]() { return macro(c, t0, t1, dtT, dtC, *o, cT, cC, nullptr, nullptr); });
+}
+
+int rs_set_lambda(rs_ctx* c, const double* lam) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    return guarded(c, [&]() {
+        h2d(c->lam.p, lam, c->N(), c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return RS_OK;
+    });
+}
+
+int rs_get_lambda(rs_ctx* c, double* lam) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    return guarded(c, [&]() { d2h(lam, c->lam.p, c->N(), c->stream); return RS_OK; });
+}
+
+int rs_zero_grad(rs_ctx* c) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    return guarded(c, [&]() {
+        HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return RS_OK;
+    });
+}
+
+int rs_get_grad_emis(rs_ctx* c, double* g) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    return guarded(c, [&]() { d2h(g, c->gE.p, c->gE.n, c->stream); return RS_OK; });
+}
+
+int rs_ssprk43_vjp(rs_ctx* c, double t, double dt) {
+    (void)t;
+    int rc = check_ready(c, false, true);
+    if (rc) return rc;
+    return guarded(c, [&]() {
+        if (c->nt > 0) enqueue_ssprk43_vjp(c, dt);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return RS_OK;
+    });
+}
+
+int rs_ros23_vjp(rs_ctx* c, double t, double dt) {
+    (void)t;
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    return guarded(c, [&]() {
+        enqueue_ros23_vjp(c, c->S[c->cur].p, nullptr, dt);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return RS_OK;
+    });
+}
+
+int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, double dtC0,
+                   const rs_split_opts* o, int loss_kind, int loss_species, double* J_out,
+                   double* grad_u0, double* grad_emis, rs_adjoint_stats* st) {
+    int rc = check_ready(c, true, true);
+    if (rc) return rc;
+    if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
+    if (o->chem_percell) return fail(c, RS_E_ARG, "adjoint needs the global-norm chemistry");
+    if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
+        return fail(c, RS_E_ARG, "bad loss");
+    c->have_cand = false;
+    return guarded(c, [&]() -> int {
+        using clk = std::chrono::steady_clock;
+        rs_adjoint_stats S{};
+        const size_t N = c->N();
+        c->ckpt.ensure((size_t)(nmacro + 1) * N);
+        c->seqT.assign(nmacro, {});
+        c->seqC.assign(nmacro, {});
+        HIPCHK(hipStreamSynchronize(c->stream));
+        auto tf0 = clk::now();
+        double dtT = dtT0, dtC = dtC0;
+        for (int m = 0; m < nmacro; ++m) {
+            d2d(c->ckpt.p + (size_t)m * N, c->S[c->cur].p, N, c->stream);
+            rs_counts cT{}, cC{};
+            int r = macro(c, tstops[m], tstops[m + 1], &dtT, &dtC, *o, &cT, &cC, &c->seqT[m], &c->seqC[m]);
+            if (r) return r;
+            S.naccept_T += cT.naccept; S.nreject_T += cT.nreject;
+            S.naccept_C += cC.naccept; S.nreject_C += cC.nreject;
+        }
+        d2d(c->ckpt.p + (size_t)nmacro * N, c->S[c->cur].p, N, c->stream);
+        const double J = loss_on_device(c, c->S[c->cur].p, loss_species);
+        if (J_out) *J_out = J;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        S.seconds_forward = std::chrono::duration<double>(clk::now() - tf0).count();
+        c->last_opts = *o;
+        c->have_record = true;
+
+        auto tb0 = clk::now();
+        double trep = 0.0;
+        HIPCHK(rs_launch_loss_seed(c->lam.p, c->S[c->cur].p, (long long)N,
+                                   (long long)loss_species * c->nc, c->ncol, c->stream));
+        HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
+        const int nbr = rs_reduce_blocks((long long)N);
+        for (int m = nmacro - 1; m >= 0; --m) {
+            auto tr0 = clk::now();
+            const auto& sT = c->seqT[m];
+            const auto& sC = c->seqC[m];
+            const size_t ns = replay_into_tape(c, c->ckpt.p + (size_t)m * N, sT, sC, *o);
+            double* tp = c->tape.p;
+            HIPCHK(rs_launch_relmax(tp + (ns - 1) * N, c->ckpt.p + (size_t)(m + 1) * N, (long long)N,
+                                    c->red.p, c->stream));
+            S.replay_maxrel = std::max(S.replay_maxrel, host_max(c, c->red.p, nbr));
+            trep += std::chrono::duration<double>(clk::now() - tr0).count();
+            // chemistry VJPs, last step first (adjoint_gradient.jl:821)
+            const size_t nT = sT.size();
+            for (size_t q = sC.size(); q-- > 0;) {
+                const size_t slot = nT + q;
+                enqueue_ros23_vjp(c, tp + slot * N, o->clamp_nonneg ? tp + (slot + 1) * N : nullptr,
+                                  sC[q].second);
+                S.nvjp_C += 1;
+            }
+            // transport VJPs (:822)
+            for (size_t q = nT; q-- > 0;) {
+                if (o->clamp_nonneg) {
+                    ProfScope ps(c, P_UTIL);
+                    HIPCHK(rs_launch_mask(c->lam.p, tp + (q + 1) * N, (long long)N, c->stream));
+                }
+                enqueue_ssprk43_vjp(c, sT[q].second);
+                S.nvjp_T += 1;
+            }
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        S.seconds_backward = std::chrono::duration<double>(clk::now() - tb0).count();
+        S.seconds_replay = trep;
+        if (grad_u0) d2h(grad_u0, c->lam.p, N, c->stream);
+        if (grad_emis) d2h(grad_emis, c->gE.p, c->gE.n, c->stream);
+        if (st) *st = S;
+        return RS_OK;
+    });
+}
+
+int rs_replay_loss(rs_ctx* c, int loss_kind, int loss_species, double* J_out) {
+    int rc = check_ready(c, true, true);
+    if (rc) return rc;
+    if (!c->have_record) return fail(c, RS_E_STATE, "no recorded sequence");
+    if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
+        return fail(c, RS_E_ARG, "bad loss");
+    return guarded(c, [&]() {
+        const size_t N = c->N();
+        const double* u = c->S[c->cur].p;
+        // replay each macro step through the tape; the end state seeds the next
+        DevArray<double> tmp;
+        tmp.resize(N);
+        d2d(tmp.p, u, N, c->stream);
+        for (size_t m = 0; m < c->seqT.size(); ++m) {
+            const size_t ns = replay_into_tape(c, tmp.p, c->seqT[m], c->seqC[m], c->last_opts);
+            d2d(tmp.p, c->tape.p + (ns - 1) * N, N, c->stream);
+        }
+        const double J = loss_on_device(c, tmp.p, loss_species);
+        if (J_out) *J_out = J;
+        d2d(c->S[c->cur].p, tmp.p, N, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return RS_OK;
+    });
+}
+
+int rs_mechanism_info(rs_ctx* c, rs_mech_info* out) {
+    if (!c || !c->have_mech || !out) return fail(c, RS_E_STATE, "mechanism not set");
+    return guarded(c, [&]() {
+        const rs::MechPlan& p = c->chem->plan;
+        rs_mech_info m{};
+        m.ns = p.ns;
+        m.nr = p.nr;
+        m.nnz_J = (int)p.work.nnz_J;
+        m.nnz_LU = (int)p.work.nnz_LU;
+        m.ordering = p.ordering;
+        m.flops_attempt = p.work.attempt();
+        m.flops_vjp = p.work.vjp();
+        hipFunction_t fs[3] = {c->chem->attempt, c->chem->percell, c->chem->vjp};
+        int regs[3] = {0, 0, 0};
+        long long scr[3] = {0, 0, 0};
+        for (int i = 0; i < 3; ++i) {
+            int v = 0;
+            HIPCHK(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, fs[i]));
+            regs[i] = v;
+            HIPCHK(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fs[i]));
+            scr[i] = v;
+        }
+        m.regs_attempt = regs[0]; m.regs_percell = regs[1]; m.regs_vjp = regs[2];
+        m.scratch_attempt = scr[0]; m.scratch_percell = scr[1]; m.scratch_vjp = scr[2];
+        *out = m;
+        return RS_OK;
+    });
+}
+
+int rs_profile(rs_ctx* c, int enable) {
+    if (!c) return RS_E_ARG;
+    return guarded(c, [&]() {
+        if (enable && !c->prof) {
+            prof_flush(c);
+            for (int k = 0; k < P_COUNT; ++k) { c->prof_n[k] = 0; c->prof_ms[k] = 0; }
+        }
+        if (!enable && c->prof) prof_flush(c);
+        c->prof = enable != 0;
+        return RS_OK;
+    });
+}
+
+int rs_profile_read(rs_ctx* c, rs_kernel_time* out, int max, int* n_out) {
+    if (!c) return RS_E_ARG;
+    return guarded(c, [&]() {
+        prof_flush(c);
+        int n = 0;
+        for (int k = 0; k < P_COUNT && n < max; ++k) {
+            if (!c->prof_n[k]) continue;
+            std::snprintf(out[n].name, sizeof(out[n].name), "%s", kProfNames[k]);
+            out[n].launches = c->prof_n[k];
+            out[n].total_ms = c->prof_ms[k];
+            ++n;
+        }
+        if (n_out) *n_out = n;
+        return RS_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+/* rs_launch.h -- argument blocks shared by the host code, the statically built
+ * kernels and the runtime-compiled (per-mechanism) chemistry kernels.  Plain C
+ * layout: the same text is compiled by hipcc and embedded into the hipRTC
+ * source, so both sides agree on every offset. */
+#ifndef RS_LAUNCH_H
+#define RS_LAUNCH_H
+
+/* Device-resident state of one PI-controlled adaptive solve with a single
+ * global error norm (rx_traced_integrator.jl:455-507). */
+typedef struct RsCtrl {
+    double t, t1, tlim, dt, dtc, qold, eest, nnorm;
+    double beta1, beta2, invqmin, invqmax, gamma, qsmin, qsmax, qoldinit;
+    long long nacc, nrej, iters, maxiters;
+    int done, accept, cur, nlog, logcap, pad;
+} RsCtrl;
+
+/* One ROS23 attempt over all cells.  With ctrl != 0 the step size, the
+ * current state buffer and the done flag are read from it (adaptive mode). */
+typedef struct RsChemArgs {
+    double* S0;
+    double* S1;
+    const RsCtrl* ctrl;
+    const double* K;        /* rate constants, nr          */
+    const double* jfac;     /* photolysis factor, nc or 0  */
+    const double* E;        /* surface emissions, ns*ncol or 0 */
+    double* partial;        /* one sse per block           */
+    long long nc, ncol;
+    double dt_static, atol, rtol;
+    int cur_static, clamp;
+} RsChemArgs;
+
+/* Per-cell adaptive chemistry, persistent work-stealing launch. */
+typedef struct RsPercellArgs {
+    double* u;              /* state, in place             */
+    const double* K;
+    const double* jfac;
+    const double* E;
+    double* dtcell;         /* per-cell dt in/out, or 0    */
+    unsigned long long* head;   /* work-queue head (zeroed before launch) */
+    long long* stats;       /* attempts, accepts, rejects, max, min, failed */
+    long long nc, ncol, maxiters;
+    double t0, t1, dt0, atol, rtol;
+    double beta1, beta2, invqmin, invqmax, gamma, qsmin, qsmax, qoldinit;
+    int clamp, pad;
+} RsPercellArgs;
+
+/* One ROS23 step VJP over all cells (stages recomputed from u). */
+typedef struct RsChemVjpArgs {
+    const double* u;        /* step input state            */
+    const double* uout;     /* step output (clamp mask), or 0 */
+    double* lam;            /* in/out                      */
+    double* gE;             /* emission gradient, ns*ncol, or 0 */
+    const double* K;
+    const double* jfac;
+    const double* E;
+    long long nc, ncol;
+    double dt;
+} RsChemVjpArgs;
+
+/* One transport stage over all transported species (see transport.hip). */
+typedef struct RsTransArgs {
+    double* S0;
+    double* S1;
+    const double* a;        /* stage input                 */
+    const double* b;        /* second input (last stage / adjoint) */
+    double* out;
+    const double* cx;       /* nlat   */
+    const double* wz;       /* nlev+1 */
+    const double* wl;       /* nlon   */
+    const RsCtrl* ctrl;
+    double* partial;
+    long long nc;
+    double dt_static, atol, rtol;
+    int nlon, nlat, nlev, nt;
+    int cur_static, a_is_state, out_is_state, clamp;
+} RsTransArgs;
+
+#endif

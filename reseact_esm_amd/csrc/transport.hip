@@ -1,0 +1,419 @@
+// transport.hip -- SSPRK43 flux-form transport stages, their adjoints, the
+// device-side PI controller and small utility kernels (FP64, gfx950).
+//
+// Transport operator (the workloads' 3rd-order upwind scheme): for tracer q,
+//   f = -(F_e - F_w) - (G_t - G_b),
+//   F_{i+1/2} = cx_j * q_f,   q_f = (-q_mm + 5 q_m + 2 q_p) / 6 (upwind side m),
+//   G_{k}     = wz_k * wl_i * q_f  at interior vertical faces, 0 at bottom/top,
+// periodic in longitude; vertical stencil indices clamped to the column.
+//
+// Layout and tiling: species-major, longitude fastest.  A 256-lane workgroup
+// (4 waves) owns a 64 (lon) x 16 (level) tile of one (tracer, latitude) slab;
+// it stages the tile plus a 2-cell halo in every direction in LDS with
+// coalesced loads (periodic wrap in longitude, clamp in the vertical), and each
+// lane computes 4 cells from LDS.  v = 0, so latitude rows never interact.
+//
+// Stages (ssprk43_step_unrolled, rx_traced_integrator.jl:404-437):
+//   KIND_STAGE : out = a + dt/2 * L a                    (stages 1-3)
+//   KIND_LAST  : u4 = 2/3 u + 1/3 u3 on load;  out = u4 + dt/2 * L u4, plus the
+//                scaled error of utilde = (1/3 u + 2/3 u3 - unew)/2 summed per
+//                block (deterministic tree) for the global Hairer norm.
+// Adjoint stages (transpose of the above, L^T as a gather over the faces each
+// cell takes part in):
+//   KIND_ADJ_A : out = a + dt/2 L^T a                     (lambda4)
+//   KIND_ADJ_B : x = a/3 on load;  out = x + dt/2 L^T x    (lambda2)
+//   KIND_ADJ_C : out = a + dt/2 L^T a                     (lambda1)
+//   KIND_ADJ_D : out = 2/3 b + a + dt/2 L^T a             (lambda_in; b = lambda4)
+#include <hip/hip_runtime.h>
+
+#include "rs_launch.h"
+
+#define TX 64
+#define TK 16
+#define LXW (TX + 4)
+#define LKW (TK + 4)
+#define TB 256
+
+#define KIND_STAGE 0
+#define KIND_LAST 1
+#define KIND_ADJ_A 2
+#define KIND_ADJ_B 3
+#define KIND_ADJ_C 4
+#define KIND_ADJ_D 5
+
+namespace {
+
+constexpr double C16 = 1.0 / 6.0;
+constexpr double C13 = 1.0 / 3.0;
+constexpr double C23 = 2.0 / 3.0;
+
+__device__ __forceinline__ double face3(double qmm, double qm, double qp) {
+    return (5.0 * qm - qmm + 2.0 * qp) * C16;
+}
+
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+// Forward tendency of cell (kk+2, ii+2) of the LDS tile (level k, lon i).
+__device__ __forceinline__ double tend_fwd(const double (*T)[LXW], int kk, int ii, int k, int i,
+                                           int nlev, double cxj, const double* __restrict__ wz,
+                                           double wli) {
+    const int r = kk + 2, c = ii + 2;
+    double fe, fw;
+    if (cxj >= 0.0) {
+        fe = cxj * face3(T[r][c - 1], T[r][c], T[r][c + 1]);
+        fw = cxj * face3(T[r][c - 2], T[r][c - 1], T[r][c]);
+    } else {
+        fe = cxj * face3(T[r][c + 2], T[r][c + 1], T[r][c]);
+        fw = cxj * face3(T[r][c + 1], T[r][c], T[r][c - 1]);
+    }
+    // vertical face kf between levels kf-1 and kf; LDS row of level L is L - k + r
+    auto G = [&](int kf) -> double {
+        if (kf <= 0 || kf >= nlev) return 0.0;
+        const double w = wz[kf] * wli;
+        if (w >= 0.0) {
+            const int mm = kf - 2 < 0 ? 0 : kf - 2;
+            return w * face3(T[mm - k + r][c], T[kf - 1 - k + r][c], T[kf - k + r][c]);
+        } else {
+            const int mm = kf + 1 > nlev - 1 ? nlev - 1 : kf + 1;
+            return w * face3(T[mm - k + r][c], T[kf - k + r][c], T[kf - 1 - k + r][c]);
+        }
+    };
+    const double gt = G(k + 1), gb = G(k);
+    return -(fe - fw) - (gt - gb);
+}
+
+// (L^T lam) of cell (level k, lon i): every face flux F leaves its lower /
+// western cell and enters its upper / eastern one, so Fbar = lam_east - lam_west;
+// the cell receives Fbar * rate * (its stencil weight) from each face whose
+// 3-point stencil contains it.  Faces: zonal i-2..i+1 (face f between f and
+// f+1), vertical k-1..k+2 (face kf between kf-1 and kf); needs nlon >= 4.
+__device__ __forceinline__ double tend_adj(const double (*T)[LXW], int kk, int ii, int k, int i,
+                                           int nlon, int nlev, double cxj,
+                                           const double* __restrict__ wz,
+                                           const double* __restrict__ wl) {
+    const int r = kk + 2, c = ii + 2;
+    double acc = 0.0;
+    // zonal: offsets relative to i, valid for nlon >= 4 (all distinct mod nlon)
+#pragma unroll
+    for (int d = -2; d <= 1; ++d) {
+        // face between (i+d) and (i+d+1)
+        const double Fb = T[r][c + d + 1] - T[r][c + d];
+        int omm, om, op;   // stencil offsets relative to i
+        if (cxj >= 0.0) { omm = d - 1; om = d; op = d + 1; }
+        else { omm = d + 2; om = d + 1; op = d; }
+        double wgt = 0.0;
+        if (omm == 0) wgt += -1.0;
+        if (om == 0) wgt += 5.0;
+        if (op == 0) wgt += 2.0;
+        acc += (cxj * Fb * C16) * wgt;
+    }
+    const double wli = wl[i];
+#pragma unroll
+    for (int kf = k - 1; kf <= k + 2; ++kf) {
+        if (kf <= 0 || kf >= nlev) continue;
+        const double w = wz[kf] * wli;
+        const double Gb = T[kf - k + r][c] - T[kf - 1 - k + r][c];
+        int mm, m, p;
+        if (w >= 0.0) { mm = kf - 2 < 0 ? 0 : kf - 2; m = kf - 1; p = kf; }
+        else { mm = kf + 1 > nlev - 1 ? nlev - 1 : kf + 1; m = kf; p = kf - 1; }
+        double wgt = 0.0;
+        if (mm == k) wgt += -1.0;
+        if (m == k) wgt += 5.0;
+        if (p == k) wgt += 2.0;
+        acc += (w * Gb * C16) * wgt;
+    }
+    (void)nlon;
+    return acc;
+}
+
+}  // namespace
+
+template <int KIND>
+__global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
+    __shared__ double T[LKW][LXW];
+    __shared__ double red[TB / 64];
+    const RsCtrl* ctl = A.ctrl;
+    if (ctl && ctl->done) return;
+    const int cur = ctl ? ctl->cur : A.cur_static;
+    const double dt = ctl ? ctl->dtc : A.dt_static;
+    const double h2 = dt / 2;
+    const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev;
+    const int i0 = blockIdx.x * TX, k0 = blockIdx.y * TK;
+    const int j = blockIdx.z % nlat, s = blockIdx.z / nlat;
+    const long long sb = (long long)s * A.nc;
+    const long long plane = (long long)nlon * nlat;
+    const double* __restrict__ pa = A.a_is_state ? (cur ? A.S1 : A.S0) : A.a;
+    const double* __restrict__ pb = A.b;
+    double* __restrict__ po = A.out_is_state ? (cur ? A.S0 : A.S1) : A.out;
+    const int tid = threadIdx.x;
+    // ---- stage the tile + halo
+    for (int x = tid; x < LKW * LXW; x += TB) {
+        const int kk = x / LXW, ii = x - kk * LXW;
+        const int gk = k0 - 2 + kk;
+        double v = 0.0;
+        if (gk >= 0 && gk < nlev) {
+            const int gi = wrapi(i0 - 2 + ii, nlon);
+            const long long g = sb + gk * plane + (long long)j * nlon + gi;
+            if (KIND == KIND_LAST) {
+                const double a = C23 * pa[g];
+                const double b = C13 * pb[g];
+                v = a + b;
+            } else if (KIND == KIND_ADJ_B) {
+                v = C13 * pa[g];
+            } else {
+                v = pa[g];
+            }
+        }
+        T[kk][ii] = v;
+    }
+    __syncthreads();
+    const int ii = tid & 63, ky = tid >> 6;
+    const int i = i0 + ii;
+    const double cxj = A.cx[j];
+    double sq = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < TK / 4; ++rr) {
+        const int kk = ky + 4 * rr;
+        const int k = k0 + kk;
+        if (i >= nlon || k >= nlev) continue;
+        const long long g = sb + k * plane + (long long)j * nlon + i;
+        const double xc = T[kk + 2][ii + 2];
+        if (KIND == KIND_STAGE || KIND == KIND_LAST) {
+            const double f = tend_fwd(T, kk, ii, k, i, nlev, cxj, A.wz, A.wl[i]);
+            const double du = h2 * f;
+            const double un = xc + du;
+            if (KIND == KIND_LAST) {
+                const double u = pa[g], u3 = pb[g];
+                const double a = C13 * u;
+                const double b = C23 * u3;
+                double ut = a + b;
+                const double ud = ut - un;
+                ut = 0.5 * ud;
+                const double am = dmax(fabs(u), fabs(un));
+                double den = am * A.rtol;
+                den = den + A.atol;
+                const double at = ut / den;
+                sq += at * at;
+                po[g] = A.clamp ? dmax(un, 0.0) : un;
+            } else {
+                po[g] = un;
+            }
+        } else {
+            const double g2 = tend_adj(T, kk, ii, k, i, nlon, nlev, cxj, A.wz, A.wl);
+            const double v = xc + h2 * g2;
+            if (KIND == KIND_ADJ_D) po[g] = C23 * pb[g] + v;
+            else po[g] = v;
+        }
+    }
+    if (KIND == KIND_LAST) {
+        for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = sq;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < TB / 64; ++w) t += red[w];
+            const long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+            A.partial[bid] = t;
+        }
+    }
+}
+
+// ---- device-side PI controller (rx_traced_integrator.jl:480-505) ----------
+__global__ void __launch_bounds__(256) rs_ctrl_update(RsCtrl* c, const double* partial,
+                                                      int npart, double* seqlog) {
+    __shared__ double sh[256];
+    if (c->done) return;
+    double s = 0.0;
+    for (int x = threadIdx.x; x < npart; x += 256) s += partial[x];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double sse = sh[0];
+    const double raw = sqrt(sse / c->nnorm);
+    const double EEst = (raw != raw) ? 1.0e10 : raw;
+    const double q11 = pow(dmax(EEst, 1.0e-35), c->beta1);
+    const double qden = pow(c->qold, c->beta2);
+    double q = q11 / qden;
+    const double qg = q / c->gamma;
+    q = dmax(c->invqmax, (c->invqmin < qg) ? c->invqmin : qg);
+    const int accept = EEst <= 1.0;
+    const int insteady = (c->qsmin <= q) && (q <= c->qsmax);
+    const double qa = insteady ? 1.0 : q;
+    const double dtc = c->dtc;
+    c->eest = EEst;
+    c->accept = accept;
+    if (accept) {
+        if (seqlog && c->nlog < c->logcap) {
+            seqlog[2 * c->nlog] = c->t;
+            seqlog[2 * c->nlog + 1] = dtc;
+            c->nlog += 1;
+        }
+        c->t = c->t + dtc;
+        c->dt = dtc / qa;
+        c->qold = dmax(EEst, c->qoldinit);
+        c->nacc += 1;
+        c->cur ^= 1;
+    } else {
+        const double q11g = q11 / c->gamma;
+        const double qr = (c->invqmin < q11g) ? c->invqmin : q11g;
+        c->dt = dtc / qr;
+        c->nrej += 1;
+    }
+    c->iters += 1;
+    if (!(c->t < c->tlim) || c->iters >= c->maxiters) {
+        c->done = 1;
+    } else {
+        const double rem = c->t1 - c->t;
+        c->dtc = (c->dt < rem) ? c->dt : rem;
+    }
+}
+
+// ---- utilities ---------------------------------------------------------------
+__global__ void rs_fill(double* x, long long n, double v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+__global__ void rs_fill_u64(unsigned long long* x, int n, unsigned long long v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+// lam *= 1[uout > 0]
+__global__ void rs_mask(double* lam, const double* uout, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        if (!(uout[i] > 0.0)) lam[i] = 0.0;
+}
+
+// x = max(x, 0)
+__global__ void rs_clamp(double* x, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = dmax(x[i], 0.0);
+}
+
+// per-block max of |a-b| / max(|b|, 1e-30)
+__global__ void __launch_bounds__(256) rs_relmax(const double* a, const double* b, long long n,
+                                                 double* part) {
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const double d = fabs(a[i] - b[i]) / dmax(fabs(b[i]), 1e-30);
+        m = dmax(m, d);
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = dmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// J = sum over level-0 cells of q^2 (per block), q = species block of the state
+__global__ void __launch_bounds__(256) rs_sumsq(const double* q, long long n, double* part) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        s += q[i] * q[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// lam = 2 q on level-0 cells of species s, 0 elsewhere (dJ/du of the loss)
+__global__ void rs_loss_seed(double* lam, const double* u, long long ntot, long long sbase,
+                             long long ncol) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ntot;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long o = i - sbase;
+        lam[i] = (o >= 0 && o < ncol) ? 2.0 * u[i] : 0.0;
+    }
+}
+
+// ---- host-visible launch wrappers ------------------------------------------
+extern "C" {
+
+hipError_t rs_launch_transport(int kind, const RsTransArgs* a, hipStream_t st) {
+    dim3 grid((a->nlon + TX - 1) / TX, (a->nlev + TK - 1) / TK, (unsigned)(a->nlat * a->nt));
+    dim3 block(TB);
+    switch (kind) {
+        case KIND_STAGE: hipLaunchKernelGGL(rs_transport_stage<KIND_STAGE>, grid, block, 0, st, *a); break;
+        case KIND_LAST: hipLaunchKernelGGL(rs_transport_stage<KIND_LAST>, grid, block, 0, st, *a); break;
+        case KIND_ADJ_A: hipLaunchKernelGGL(rs_transport_stage<KIND_ADJ_A>, grid, block, 0, st, *a); break;
+        case KIND_ADJ_B: hipLaunchKernelGGL(rs_transport_stage<KIND_ADJ_B>, grid, block, 0, st, *a); break;
+        case KIND_ADJ_C: hipLaunchKernelGGL(rs_transport_stage<KIND_ADJ_C>, grid, block, 0, st, *a); break;
+        case KIND_ADJ_D: hipLaunchKernelGGL(rs_transport_stage<KIND_ADJ_D>, grid, block, 0, st, *a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt) {
+    return (long long)((nlon + TX - 1) / TX) * ((nlev + TK - 1) / TK) * (long long)nlat * nt;
+}
+
+hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(rs_ctrl_update, dim3(1), dim3(256), 0, st, c, partial, npart, seqlog);
+    return hipGetLastError();
+}
+
+static unsigned util_grid(long long n) {
+    long long b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+hipError_t rs_launch_fill(double* x, long long n, double v, hipStream_t st) {
+    hipLaunchKernelGGL(rs_fill, dim3(util_grid(n)), dim3(256), 0, st, x, n, v);
+    return hipGetLastError();
+}
+hipError_t rs_launch_fill_u64(unsigned long long* x, int n, unsigned long long v, hipStream_t st) {
+    hipLaunchKernelGGL(rs_fill_u64, dim3((n + 255) / 256), dim3(256), 0, st, x, n, v);
+    return hipGetLastError();
+}
+hipError_t rs_launch_mask(double* lam, const double* uout, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(rs_mask, dim3(util_grid(n)), dim3(256), 0, st, lam, uout, n);
+    return hipGetLastError();
+}
+hipError_t rs_launch_clamp(double* x, long long n, hipStream_t st) {
+    hipLaunchKernelGGL(rs_clamp, dim3(util_grid(n)), dim3(256), 0, st, x, n);
+    return hipGetLastError();
+}
+int rs_reduce_blocks(long long n) { return (int)util_grid(n) > 1024 ? 1024 : (int)util_grid(n); }
+hipError_t rs_launch_relmax(const double* a, const double* b, long long n, double* part,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(rs_relmax, dim3(rs_reduce_blocks(n)), dim3(256), 0, st, a, b, n, part);
+    return hipGetLastError();
+}
+hipError_t rs_launch_sumsq(const double* q, long long n, double* part, hipStream_t st) {
+    hipLaunchKernelGGL(rs_sumsq, dim3(rs_reduce_blocks(n)), dim3(256), 0, st, q, n, part);
+    return hipGetLastError();
+}
+hipError_t rs_launch_loss_seed(double* lam, const double* u, long long ntot, long long sbase,
+                               long long ncol, hipStream_t st) {
+    hipLaunchKernelGGL(rs_loss_seed, dim3(util_grid(ntot)), dim3(256), 0, st, lam, u, ntot, sbase,
+                       ncol);
+    return hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,341 @@
+"""Synthetic workloads of the operator-split step: grid, wind, mechanism,
+initial state, emissions and photolysis factors.
+
+Every array is generated here, on the host, from a seed, and handed to the C
+ABI; the GPU library and the CPU oracle therefore see bit-identical inputs.
+
+Definitions (shared by every workload):
+  * sphere radius a = 6.371e6 m, model top H = 10 km, NLEV equal layers;
+    longitude centres (i+1/2) dlambda (periodic), latitude centres
+    -90 deg + (j+1/2) dphi, level centres (k+1/2) H/NLEV, k = 0 at the surface;
+  * wind: u = 20 cos(phi) m/s, v = 0, w = w0 sin(pi z/H) cos(lambda) at the
+    vertical faces (zero at bottom and top), w0 chosen so that the largest
+    vertical Courant number at the reference step equals the target;
+  * mechanisms: 1-2 reactants and 1-2 products per reaction, with
+    sum(nu_products) <= sum(nu_reactants); rate constants log-uniform;
+    photolysis reactions are first order and scale with max(0, cos SZA);
+  * cos SZA is the reference's solar formula (run_reseact.jl:164-177) at
+    t = 5400 s + model time, evaluated once per split interval (forcing is
+    held constant inside a window, rx_traced_integrator.jl:9-10).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+A_EARTH = 6.371e6
+H_TOP = 10000.0
+U_SOLID = 20.0
+K_GAMMA = 0.01721420632103996     # run_reseact.jl:163
+D2R = 0.017453292519943295
+
+
+@dataclasses.dataclass
+class Grid:
+    nlon: int
+    nlat: int
+    nlev: int
+
+    @property
+    def nc(self) -> int:
+        return self.nlon * self.nlat * self.nlev
+
+    @property
+    def ncol(self) -> int:
+        return self.nlon * self.nlat
+
+    @property
+    def dlam(self) -> float:
+        return 2.0 * math.pi / self.nlon
+
+    @property
+    def dphi(self) -> float:
+        return math.pi / self.nlat
+
+    @property
+    def dz(self) -> float:
+        return H_TOP / self.nlev
+
+    @property
+    def lon(self) -> np.ndarray:          # radians, cell centres
+        return (np.arange(self.nlon) + 0.5) * self.dlam
+
+    @property
+    def lat(self) -> np.ndarray:          # radians, cell centres
+        return -0.5 * math.pi + (np.arange(self.nlat) + 0.5) * self.dphi
+
+    def cell_volume_weights(self) -> np.ndarray:
+        """Relative cell volume (proportional to cos(phi)), shape (nlev, nlat, nlon)."""
+        w = np.cos(self.lat)[None, :, None] * np.ones((self.nlev, 1, self.nlon))
+        return w
+
+
+@dataclasses.dataclass
+class Wind:
+    cx: np.ndarray       # (nlat,)   1/s
+    wz: np.ndarray       # (nlev+1,) 1/s (divided by dz)
+    wl: np.ndarray       # (nlon,)
+    w0: float            # m/s
+
+    def max_courant(self, grid: Grid, dt: float) -> tuple[float, float]:
+        zon = float(np.max(np.abs(self.cx)) * dt)
+        ver = float(np.max(np.abs(self.wz)) * np.max(np.abs(self.wl)) * dt)
+        return zon, ver
+
+
+def solid_body_wind(grid: Grid, dt_ref: float, courant: float = 0.8) -> Wind:
+    """u = 20 cos(phi), v = 0, w = w0 sin(pi z/H) cos(lambda)."""
+    lat = grid.lat
+    cx = U_SOLID * np.cos(lat) / (A_EARTH * np.cos(lat) * grid.dlam)
+    zf = np.arange(grid.nlev + 1) * grid.dz
+    prof = np.sin(np.pi * zf / H_TOP)
+    prof[0] = 0.0
+    prof[-1] = 0.0
+    wl = np.cos(grid.lon)
+    pmax = float(np.max(np.abs(prof))) if grid.nlev > 1 else 0.0
+    wmax = pmax * float(np.max(np.abs(wl)))
+    w0 = courant * grid.dz / (dt_ref * wmax) if wmax > 0 else 0.0
+    wz = w0 * prof / grid.dz
+    return Wind(cx=np.ascontiguousarray(cx), wz=np.ascontiguousarray(wz),
+                wl=np.ascontiguousarray(wl), w0=w0)
+
+
+def courant_dt(grid: Grid, courant: float = 0.8) -> float:
+    """Largest dt with zonal Courant number `courant` (independent of phi)."""
+    return courant * A_EARTH * grid.dlam / U_SOLID
+
+
+@dataclasses.dataclass
+class Mechanism:
+    ns: int
+    reactants: np.ndarray   # (nr, 2) int32, -1 = none
+    products: np.ndarray    # (nr, 2) int32, -1 = none
+    k: np.ndarray           # (nr,) float64
+    photo: np.ndarray       # (nr,) int32
+
+    @property
+    def nr(self) -> int:
+        return int(self.k.shape[0])
+
+    def net(self) -> list[list[tuple[int, int]]]:
+        out = []
+        for r in range(self.nr):
+            nu: dict[int, int] = {}
+            for a in self.reactants[r]:
+                if a >= 0:
+                    nu[int(a)] = nu.get(int(a), 0) - 1
+            for p in self.products[r]:
+                if p >= 0:
+                    nu[int(p)] = nu.get(int(p), 0) + 1
+            out.append(sorted((s, v) for s, v in nu.items() if v != 0))
+        return out
+
+    def flops(self) -> dict:
+        """Per-attempt flop terms of one ROS23 attempt on one cell."""
+        net = self.net()
+        f_rhs = 0
+        f_jac = 0
+        for r in range(self.nr):
+            nre = int(np.sum(self.reactants[r] >= 0))
+            order_cost = 1 + (1 if nre == 2 else 0) + (1 if self.photo[r] else 0)
+            f_rhs += order_cost + 2 * len(net[r])
+            f_jac += nre * 2 * len(net[r]) + (2 if nre == 2 else 0)
+        return {"rhs": f_rhs, "jac": f_jac}
+
+
+def random_mechanism(ns: int, nr: int, seed: int, klo: float, khi: float,
+                     nphoto: int = 0) -> Mechanism:
+    """Synthetic mass-action mechanism; reactions 0..nphoto-1 are photolysis."""
+    rng = np.random.default_rng([seed, 1])
+    re = -np.ones((nr, 2), dtype=np.int32)
+    pr = -np.ones((nr, 2), dtype=np.int32)
+    photo = np.zeros(nr, dtype=np.int32)
+    for r in range(nr):
+        nre = 1 if r < nphoto else int(rng.integers(1, 3))
+        a = int(rng.integers(0, ns))
+        re[r, 0] = a
+        if nre == 2:
+            re[r, 1] = int(rng.integers(0, ns))
+        npr = 1 if nre == 1 else int(rng.integers(1, 3))
+        for q in range(npr):
+            p = int(rng.integers(0, ns))
+            while nre == 1 and p == a and ns > 1:
+                p = int(rng.integers(0, ns))
+            pr[r, q] = p
+        photo[r] = 1 if r < nphoto else 0
+    k = 10.0 ** rng.uniform(math.log10(klo), math.log10(khi), nr)
+    return Mechanism(ns=ns, reactants=re, products=pr, k=k, photo=photo)
+
+
+def cos_sza(t: float, lat_deg: np.ndarray, lon_deg: np.ndarray) -> np.ndarray:
+    """Solar zenith cosine, the formula of run_reseact.jl:168-176."""
+    g = K_GAMMA * (-12 / 24 + t / 86400)
+    dec = (0.006918 - 0.399912 * math.cos(g) + 0.070257 * math.sin(g)
+           - 0.006758 * math.cos(2 * g) + 0.000907 * math.sin(2 * g)
+           - 0.002697 * math.cos(3 * g) + 0.00148 * math.sin(3 * g))
+    eqt = 229.18 * (0.000075 + 0.001868 * math.cos(g) - 0.032077 * math.sin(g)
+                    - 0.014615 * math.cos(2 * g) - 0.040849 * math.sin(2 * g))
+    w = D2R * ((t / 60 + eqt + 4 * lon_deg) / 4 - 180)
+    v = np.sin(D2R * lat_deg) * math.sin(dec) + np.cos(D2R * lat_deg) * math.cos(dec) * np.cos(w)
+    return np.clip(v, -1.0, 1.0)
+
+
+def photolysis_factor(grid: Grid, t_model: float) -> np.ndarray:
+    """max(0, cos SZA) per cell (all levels of a column share it), length NC."""
+    lat_deg = np.degrees(grid.lat)[:, None]
+    lon_deg = np.degrees(grid.lon)[None, :]
+    c = np.maximum(cos_sza(5400.0 + t_model, lat_deg, lon_deg), 0.0)   # (nlat, nlon)
+    return np.ascontiguousarray(np.broadcast_to(c, (grid.nlev, grid.nlat, grid.nlon)).reshape(-1))
+
+
+def lognormal_state(grid: Grid, ns: int, seed: int, sigma: float = 1.0) -> np.ndarray:
+    rng = np.random.default_rng([seed, 2])
+    return rng.lognormal(0.0, sigma, size=ns * grid.nc)
+
+
+def blob_state(grid: Grid, ns: int, seed: int, nblob: int = 3, amp: float = 5.0) -> np.ndarray:
+    """Lognormal background plus Gaussian blobs, species-major."""
+    rng = np.random.default_rng([seed, 2])
+    u = rng.lognormal(0.0, 1.0, size=(ns, grid.nlev, grid.nlat, grid.nlon))
+    lon = grid.lon[None, None, :]
+    lat = grid.lat[None, :, None]
+    z = ((np.arange(grid.nlev) + 0.5) * grid.dz)[:, None, None]
+    for s in range(ns):
+        for _ in range(nblob):
+            l0 = rng.uniform(0, 2 * math.pi)
+            p0 = rng.uniform(-1.2, 1.2)
+            z0 = rng.uniform(0.2, 0.8) * H_TOP
+            dl = np.angle(np.exp(1j * (lon - l0)))
+            r2 = (dl * np.cos(lat)) ** 2 + (lat - p0) ** 2
+            blob = np.exp(-r2 / (2 * 0.15 ** 2)) * np.exp(-((z - z0) ** 2) / (2 * (0.15 * H_TOP) ** 2))
+            u[s] += amp * blob
+    return np.ascontiguousarray(u.reshape(-1))
+
+
+def lognormal_emissions(grid: Grid, ns: int, seed: int, scale: float = 1e-3,
+                        perturb_sigma: float = 0.0, perturb_seed: int | None = None) -> np.ndarray:
+    rng = np.random.default_rng([seed, 3])
+    e = rng.lognormal(0.0, 1.0, size=ns * grid.ncol) * scale
+    if perturb_sigma > 0:
+        prng = np.random.default_rng([seed if perturb_seed is None else perturb_seed, 4])
+        e = e * prng.lognormal(0.0, perturb_sigma, size=e.shape)
+    return e
+
+
+@dataclasses.dataclass
+class Problem:
+    """One workload: everything the C ABI needs plus its run parameters."""
+    name: str
+    grid: Grid
+    ns: int
+    nt: int
+    wind: Wind
+    mech: Mechanism | None
+    u0: np.ndarray
+    jfac: np.ndarray | None = None
+    emis: np.ndarray | None = None
+    dt_macro: float = 600.0
+    nmacro: int = 1
+    t0: float = 0.0
+    rtol_T: float = 1e-3
+    atol_T: float = 1e-6
+    rtol_C: float = 1e-3
+    atol_C: float = 1e-6
+    dt0_T: float = 15.0      # run_reseact_reactant.jl:134
+    dt0_C: float = 0.5       # run_reseact_reactant.jl:135
+    extra: dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def nc(self) -> int:
+        return self.grid.nc
+
+    def tstops(self) -> np.ndarray:
+        return self.t0 + self.dt_macro * np.arange(self.nmacro + 1, dtype=np.float64)
+
+
+def config1(nlon: int = 72, nlat: int = 46, nlev: int = 10, seed: int = 1001,
+            nmacro: int = 24) -> Problem:
+    """Plain coupled forward: 72x46x10, 4 tracers, 10 species / 20 reactions,
+    24 split steps of 600 s, rtol 1e-3, atol 1e-6."""
+    g = Grid(nlon, nlat, nlev)
+    dt = 600.0
+    mech = random_mechanism(10, 20, seed, 1e-6, 1e2)
+    return Problem(name="coupled-72x46x10-10sp", grid=g, ns=10, nt=4,
+                   wind=solid_body_wind(g, dt, 0.8), mech=mech,
+                   u0=lognormal_state(g, 10, seed), dt_macro=dt, nmacro=nmacro)
+
+
+def config2(nlon: int = 360, nlat: int = 180, nlev: int = 72, nt: int = 20,
+            seed: int = 1002, nsteps: int = 60) -> Problem:
+    """Transport only: 360x180x72, 20 tracers, fixed dt at Courant 0.8, 60 steps."""
+    g = Grid(nlon, nlat, nlev)
+    dt = courant_dt(g, 0.8)
+    return Problem(name="transport-360x180x72-20tr", grid=g, ns=nt, nt=nt,
+                   wind=solid_body_wind(g, dt, 0.8), mech=None,
+                   u0=blob_state(g, nt, seed), dt_macro=dt, nmacro=nsteps,
+                   extra={"dt_fixed": dt})
+
+
+def config3(nlon: int = 360, nlat: int = 180, nlev: int = 72, seed: int = 1003) -> Problem:
+    """Batched chemistry: 4.67M cells, 50 species / 120 reactions (8 photolysis),
+    k log-uniform 1e-8..1e4, adaptive per cell over one 1800 s interval."""
+    g = Grid(nlon, nlat, nlev)
+    mech = random_mechanism(50, 120, seed, 1e-8, 1e4, nphoto=8)
+    return Problem(name="chem-360x180x72-50sp", grid=g, ns=50, nt=0,
+                   wind=solid_body_wind(g, 1800.0, 0.8), mech=mech,
+                   u0=lognormal_state(g, 50, seed), jfac=photolysis_factor(g, 0.0),
+                   dt_macro=1800.0, nmacro=1)
+
+
+def config4(nlon: int = 144, nlat: int = 91, nlev: int = 47, seed: int = 1004,
+            nmacro: int = 12) -> Problem:
+    """Coupled forward + adjoint: 144x91x47, 20 species / 40 reactions,
+    12 split steps of 900 s, surface emissions, loss sum_surface u0^2."""
+    g = Grid(nlon, nlat, nlev)
+    dt = 900.0
+    mech = random_mechanism(20, 40, seed, 1e-6, 1e2)
+    return Problem(name="coupled-adjoint-144x91x47-20sp", grid=g, ns=20, nt=20,
+                   wind=solid_body_wind(g, dt, 0.8), mech=mech,
+                   u0=lognormal_state(g, 20, seed),
+                   emis=lognormal_emissions(g, 20, seed), dt_macro=dt, nmacro=nmacro)
+
+
+def config5_domains(ndom: int = 32, seed: int = 1005) -> list[tuple[int, int, int]]:
+    rng = np.random.default_rng([seed, 5])
+    return [(int(rng.integers(64, 257)), int(rng.integers(64, 193)), int(rng.integers(20, 48)))
+            for _ in range(ndom)]
+
+
+def config5(ndom: int = 32, seed: int = 1005, nmacro: int = 6,
+            sizes: list[tuple[int, int, int]] | None = None) -> list[Problem]:
+    """Ensemble: regional domains 64x64x20 .. 256x192x47, 20-species mechanism,
+    emissions perturbed lognormally (sigma 0.3) per domain, 6 split steps."""
+    sizes = config5_domains(ndom, seed) if sizes is None else sizes
+    mech = random_mechanism(20, 40, 1004, 1e-6, 1e2)
+    out = []
+    for d, (nx, ny, nz) in enumerate(sizes):
+        g = Grid(nx, ny, nz)
+        out.append(Problem(name=f"ensemble-{d}-{nx}x{ny}x{nz}", grid=g, ns=20, nt=20,
+                           wind=solid_body_wind(g, 900.0, 0.8), mech=mech,
+                           u0=lognormal_state(g, 20, seed + d),
+                           emis=lognormal_emissions(g, 20, 1004, perturb_sigma=0.3,
+                                                    perturb_seed=seed + d),
+                           dt_macro=900.0, nmacro=nmacro))
+    return out
+
+
+def toy_problem() -> Problem:
+    """3 species x 5 cells: A+B -> C, C -> A+B, 2C -> B (the shape of the
+    reference's model-free adjoint toy, tools/rx_adjoint_toy.jl:63-74, with
+    the time factor s(t) fixed)."""
+    g = Grid(5, 1, 1)
+    re = np.array([[0, 1], [2, -1], [2, 2]], dtype=np.int32)
+    pr = np.array([[2, -1], [0, 1], [1, -1]], dtype=np.int32)
+    mech = Mechanism(ns=3, reactants=re, products=pr, k=np.array([0.7, 0.3, 0.15]),
+                     photo=np.zeros(3, dtype=np.int32))
+    rng = np.random.default_rng(11)
+    u0 = 0.5 + rng.random(15)
+    return Problem(name="toy-3sp-5cells", grid=g, ns=3, nt=0,
+                   wind=solid_body_wind(g, 1.0, 0.0), mech=mech, u0=u0)

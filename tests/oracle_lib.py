@@ -1,0 +1,39 @@
+"""Loader of the CPU oracle (oracle/libreseact_cpu.so) for tests and the CPU
+baseline.  The oracle is the checker, never the thing under test."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+from reseact_esm_amd.host import Library, _P, _D
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(REPO, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "libreseact_cpu.so")
+
+EXTRA = {
+    "rs_oracle_ros23_jvp": [_P, C.c_double, _D, _D, _D],
+    "rs_oracle_ssprk43_jvp": [_P, C.c_double, _D, _D],
+    "rs_oracle_transport_rhs": [_P, _D, _D],
+    "rs_oracle_transport_rhs_adj": [_P, _D, _D],
+    "rs_oracle_chem_rhs": [_P, _D, _D],
+    "rs_oracle_chem_jac": [_P, _D, C.c_size_t, _D],
+    "rs_oracle_threads": [],
+}
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+_LIB = None
+
+
+def oracle_library() -> Library:
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(ORACLE_LIB):
+            build_oracle()
+        _LIB = Library(ORACLE_LIB, EXTRA)
+    return _LIB

@@ -1,0 +1,213 @@
+"""GPU parity: the HIP library against the CPU oracle, through the C ABI, on
+the same seeded inputs.
+
+Tolerances (relative, max over all entries): transport 1e-12, chemistry 1e-8
+with the same accepted/rejected step sequence, adjoint gradients 1e-6; the
+natural-order LU reproduces the reference's elimination and is held to 1e-12.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from reseact_esm_amd import problems as P
+from reseact_esm_amd.host import pictrl_ros23, pictrl_ssprk43, split_opts
+
+pytestmark = pytest.mark.gpu
+
+TOL_T = 1e-12
+TOL_C = 1e-8
+TOL_G = 1e-6
+
+
+def relmax(a, b, floor=1e-300):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+def relnorm(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def both(hip, oracle, p, setup=None):
+    cg = hip.context().load(p)
+    co = oracle.context().load(p)
+    if setup:
+        setup(cg)
+        setup(co)
+    return cg, co
+
+
+# ---------------------------------------------------------------- transport
+@pytest.mark.parametrize("dims,nt,ns", [((8, 5, 4), 3, 3), ((70, 9, 19), 2, 5),
+                                        ((144, 7, 33), 4, 4), ((4, 3, 1), 2, 2)])
+def test_ssprk43_step(hip, oracle, dims, nt, ns):
+    print(f"ssprk43 step dims={dims} nt={nt} ns={ns}", flush=True)
+    g = P.Grid(*dims)
+    p = P.Problem(name="t", grid=g, ns=ns, nt=nt, wind=P.solid_body_wind(g, 600.0, 0.8),
+                  mech=None, u0=P.blob_state(g, ns, 5))
+    cg, co = both(hip, oracle, p)
+    for dt in (60.0, 600.0):
+        eg = cg.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+        eo = co.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+        ug, uo = cg.get_candidate(), co.get_candidate()
+        assert relmax(ug, uo) <= TOL_T
+        assert abs(eg - eo) <= TOL_T * abs(eo)
+        # untransported species bit-for-bit untouched
+        assert np.array_equal(ug[nt * g.nc:], p.u0[nt * g.nc:])
+
+
+def test_ssprk43_negative_wind(hip, oracle):
+    g = P.Grid(16, 6, 9)
+    w = P.solid_body_wind(g, 300.0, 0.7)
+    w.cx = -w.cx
+    w.wl = np.roll(w.wl, 3)
+    p = P.Problem(name="t", grid=g, ns=2, nt=2, wind=w, mech=None, u0=P.blob_state(g, 2, 9))
+    cg, co = both(hip, oracle, p)
+    eg = cg.ssprk43_step(0.0, 300.0, 1e-6, 1e-3)
+    eo = co.ssprk43_step(0.0, 300.0, 1e-6, 1e-3)
+    assert relmax(cg.get_candidate(), co.get_candidate()) <= TOL_T
+    assert abs(eg - eo) <= TOL_T * eo
+
+
+def test_adaptive_transport_same_sequence(hip, oracle):
+    p = P.config1(nlon=36, nlat=10, nlev=10)
+    cg, co = both(hip, oracle, p)
+    dg, kg = cg.adaptive_transport(0.0, 600.0, 15.0, pictrl_ssprk43(), 1e-6, 1e-3, True)
+    do, ko = co.adaptive_transport(0.0, 600.0, 15.0, pictrl_ssprk43(), 1e-6, 1e-3, True)
+    assert (kg.naccept, kg.nreject) == (ko.naccept, ko.nreject)
+    assert abs(dg - do) <= 1e-10 * do
+    assert relmax(cg.get_state(), co.get_state()) <= TOL_T
+
+
+# ---------------------------------------------------------------- chemistry
+@pytest.mark.parametrize("cfg", ["c1", "c4", "c3"])
+def test_ros23_step(hip, oracle, cfg):
+    print(f"ros23 step {cfg}", flush=True)
+    if cfg == "c1":
+        p = P.config1(nlon=9, nlat=6, nlev=4)
+    elif cfg == "c4":
+        p = P.config4(nlon=8, nlat=6, nlev=3)
+    else:
+        p = P.config3(nlon=12, nlat=8, nlev=2)
+    cg, co = both(hip, oracle, p)
+    for dt in (0.5, 30.0):
+        eg = cg.ros23_step(0.0, dt, 1e-6, 1e-3)
+        eo = co.ros23_step(0.0, dt, 1e-6, 1e-3)
+        assert relmax(cg.get_candidate(), co.get_candidate()) <= TOL_C
+        assert abs(eg - eo) <= TOL_C * eo
+
+
+def test_ros23_natural_order_matches_reference_elimination(hip, oracle):
+    p = P.config4(nlon=8, nlat=6, nlev=3)
+    cg = hip.context()
+    cg.set_lu_ordering(0)
+    cg.load(p)
+    co = oracle.context().load(p)
+    eg = cg.ros23_step(0.0, 5.0, 1e-6, 1e-3)
+    eo = co.ros23_step(0.0, 5.0, 1e-6, 1e-3)
+    assert relmax(cg.get_candidate(), co.get_candidate()) <= 1e-12
+    assert abs(eg - eo) <= 1e-12 * eo
+
+
+def test_adaptive_chem_same_sequence(hip, oracle):
+    p = P.config4(nlon=12, nlat=7, nlev=4)
+    cg, co = both(hip, oracle, p)
+    dg, kg = cg.adaptive_chem(0.0, 900.0, 0.5, pictrl_ros23(), 1e-6, 1e-3, True)
+    do, ko = co.adaptive_chem(0.0, 900.0, 0.5, pictrl_ros23(), 1e-6, 1e-3, True)
+    assert (kg.naccept, kg.nreject) == (ko.naccept, ko.nreject)
+    assert relmax(cg.get_state(), co.get_state()) <= TOL_C
+
+
+def test_percell_chem(hip, oracle):
+    p = P.config3(nlon=16, nlat=10, nlev=2)
+    cg, co = both(hip, oracle, p)
+    dtg = np.full(p.nc, 0.5)
+    dto = dtg.copy()
+    sg = cg.chem_percell(0.0, 1800.0, 0.5, dtg, pictrl_ros23(), 1e-6, 1e-3, True)
+    so = co.chem_percell(0.0, 1800.0, 0.5, dto, pictrl_ros23(), 1e-6, 1e-3, True)
+    assert (sg.attempts_total, sg.accepts_total, sg.rejects_total) == \
+           (so.attempts_total, so.accepts_total, so.rejects_total)
+    assert (sg.attempts_max, sg.attempts_min) == (so.attempts_max, so.attempts_min)
+    assert relmax(cg.get_state(), co.get_state()) <= TOL_C
+    assert relmax(dtg, dto) <= 1e-8
+    # night cells take fewer attempts than day cells
+    assert sg.attempts_max > sg.attempts_min
+
+
+# ---------------------------------------------------------------- coupled
+def test_macro_steps_config1(hip, oracle):
+    p = P.config1(nlon=24, nlat=12, nlev=6, nmacro=3)
+    cg, co = both(hip, oracle, p)
+    opts = split_opts(p)
+    st = [[p.dt0_T, p.dt0_C], [p.dt0_T, p.dt0_C]]
+    for m in range(p.nmacro):
+        t0, t1 = m * p.dt_macro, (m + 1) * p.dt_macro
+        rg = cg.macro_step(t0, t1, st[0][0], st[0][1], opts)
+        ro = co.macro_step(t0, t1, st[1][0], st[1][1], opts)
+        st[0] = [rg[0], rg[1]]
+        st[1] = [ro[0], ro[1]]
+        assert (rg[2].naccept, rg[2].nreject, rg[3].naccept, rg[3].nreject) == \
+               (ro[2].naccept, ro[2].nreject, ro[3].naccept, ro[3].nreject)
+    assert relmax(cg.get_state(), co.get_state()) <= TOL_C
+
+
+def test_repeat_call_bitwise(hip):
+    p = P.config4(nlon=16, nlat=8, nlev=5)
+    outs = []
+    for _ in range(2):
+        with hip.context() as c:
+            c.load(p)
+            c.macro_step(0.0, 900.0, 15.0, 0.5, split_opts(p))
+            outs.append(c.get_state())
+    assert np.array_equal(outs[0], outs[1])
+
+
+# ---------------------------------------------------------------- adjoint
+def test_step_vjps(hip, oracle):
+    p = P.config4(nlon=10, nlat=6, nlev=5)
+    cg, co = both(hip, oracle, p)
+    lam = np.random.default_rng(3).standard_normal(p.ns * p.nc)
+    for c in (cg, co):
+        c.set_lambda(lam)
+        c.ssprk43_vjp(0.0, 120.0)
+    assert relnorm(cg.get_lambda(), co.get_lambda()) <= 1e-12
+    for c in (cg, co):
+        c.set_lambda(lam)
+        c.zero_grad()
+        c.ros23_vjp(0.0, 20.0)
+    assert relnorm(cg.get_lambda(), co.get_lambda()) <= 1e-10
+    assert relnorm(cg.get_grad_emis(), co.get_grad_emis()) <= 1e-10
+
+
+def test_adjoint_run_config4_small(hip, oracle):
+    p = P.config4(nlon=12, nlat=7, nlev=5, nmacro=2)
+    cg, co = both(hip, oracle, p)
+    opts = split_opts(p)
+    Jg, gug, geg, sg = cg.adjoint_run(p.tstops(), p.dt0_T, p.dt0_C, opts)
+    Jo, guo, geo, so = co.adjoint_run(p.tstops(), p.dt0_T, p.dt0_C, opts)
+    assert (sg.naccept_T, sg.naccept_C) == (so.naccept_T, so.naccept_C)
+    assert abs(Jg - Jo) <= TOL_C * Jo
+    assert relnorm(gug, guo) <= TOL_G
+    assert relnorm(geg, geo) <= TOL_G
+    assert sg.replay_maxrel < 1e-12
+
+
+# ---------------------------------------------------------------- full size
+def test_transport_full_grid_mass(hip):
+    """360x180x72 with 4 tracers at Courant 0.8: flux form conserves the
+    volume-weighted tracer mass to round-off (size-independent property)."""
+    p = P.config2(nt=4)
+    dt = p.extra["dt_fixed"]
+    with hip.context() as c:
+        c.load(p)
+        w = p.grid.cell_volume_weights().reshape(-1)
+        m0 = [float(np.dot(p.u0[s * p.nc:(s + 1) * p.nc], w)) for s in range(p.ns)]
+        for _ in range(3):
+            c.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+            c.accept_candidate(False)
+        u = c.get_state()
+    for s in range(p.ns):
+        m1 = float(np.dot(u[s * p.nc:(s + 1) * p.nc], w))
+        assert abs(m1 - m0[s]) <= 1e-13 * m0[s]
