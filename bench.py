@@ -1,0 +1,234 @@
+"""Benchmark of the operator-split step on one MI355X.
+
+Workload (the default, and the `value` line): the coupled forward of the
+144x91x47 grid, 20 species / 40 reactions, split steps of 900 s (SSPRK43
+transport of all 20 species, then Rosenbrock23 chemistry with one global
+error norm), from seeded synthetic inputs already resident in HBM.  A "step"
+is one split step over the whole grid; value = cells * steps / seconds.
+
+Also reported on the same run:
+  * adjoint/forward ratio: checkpointed forward + backward sweep over the
+    same steps, divided by the plain forward time;
+  * transport HBM roofline: the 360x180x72, 20-tracer transport at Courant
+    0.8 (fixed dt), algorithmic 72 B per tracer-cell per step;
+  * chemistry FP64 roofline: the 360x180x72, 50-species / 120-reaction
+    per-cell adaptive chemistry over 1800 s in one persistent launch;
+  * cpu_baseline: the CPU oracle (C++/OpenMP, same algorithm) on a bounded
+    sub-grid of the coupled workload, on this host's cores.
+Kernel durations come from HIP events recorded by the library on its own
+stream around every launch (rs_profile).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+from reseact_esm_amd import hip_library, problems as P, split_opts  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_ms(prof: dict, name: str) -> tuple[int, float]:
+    n, ms = prof.get(name, (0, 0.0))
+    return n, ms
+
+
+def coupled(lib, steps: int, warmup: int):
+    p = P.config4(nmacro=max(steps, 1))
+    opts = split_opts(p)
+    ctx = lib.context().load(p)
+    info = ctx.mechanism_info()
+    log(f"coupled {p.name}: nc={p.nc} mech nnz_LU={info['nnz_LU']} flops/attempt="
+        f"{info['flops_attempt']} regs={info['regs_attempt']} scratch={info['scratch_attempt']}")
+    dtT, dtC = p.dt0_T, p.dt0_C
+    for m in range(warmup):
+        dtT, dtC, _, _ = ctx.macro_step(m * p.dt_macro, (m + 1) * p.dt_macro, dtT, dtC, opts)
+    ctx.set_state(p.u0)
+    ctx.profile(True)
+    dtT, dtC = p.dt0_T, p.dt0_C
+    nT = nC = rT = rC = 0
+    t0 = time.perf_counter()
+    for m in range(steps):
+        dtT, dtC, cT, cC = ctx.macro_step(m * p.dt_macro, (m + 1) * p.dt_macro, dtT, dtC, opts)
+        nT += cT.naccept; rT += cT.nreject; nC += cC.naccept; rC += cC.nreject
+    t1 = time.perf_counter()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    fwd_s = t1 - t0
+    log(f"coupled forward {steps} steps {fwd_s:.3f} s; T {nT}/{rT} C {nC}/{rC}; kernels {prof}")
+    # adjoint over the same steps
+    ctx.set_state(p.u0)
+    J, gu, ge, st = ctx.adjoint_run(p.tstops()[:steps + 1], p.dt0_T, p.dt0_C, opts)
+    ratio = (st.seconds_forward + st.seconds_backward) / fwd_s
+    log(f"adjoint: fwd(ckpt) {st.seconds_forward:.3f} s bwd {st.seconds_backward:.3f} s "
+        f"(replay {st.seconds_replay:.3f}) ratio {ratio:.3f} J={J:.6e} |gu|={np.linalg.norm(gu):.4e} "
+        f"|gE|={np.linalg.norm(ge):.4e} replay_maxrel={st.replay_maxrel:.2e}")
+    n_att, ms_att = kernel_ms(prof, "chem_attempt")
+    flops = info["flops_attempt"] * p.nc
+    achieved = flops / (ms_att / n_att * 1e-3) / 1e12 if n_att else 0.0
+    roof = {"bound": "fp64", "kernel": "chem_attempt", "achieved": round(achieved, 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+            "avg_ms": round(ms_att / max(n_att, 1), 4), "launches": n_att,
+            "flops_per_launch": flops}
+    return dict(p=p, fwd_s=fwd_s, steps=steps, ratio=ratio, adj=st, roof=roof, prof=prof,
+                counts=(nT, rT, nC, rC), info=info, J=J)
+
+
+def transport_roofline(lib, nsteps: int = 60, nt: int = 20):
+    p = P.config2(nt=nt, nsteps=nsteps)
+    dt = p.extra["dt_fixed"]
+    ctx = lib.context().load(p)
+    ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)      # warm-up
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+        ctx.accept_candidate(False)
+    t1 = time.perf_counter()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ns_, ms_s = kernel_ms(prof, "transport_stage")
+    nl_, ms_l = kernel_ms(prof, "transport_last")
+    ms_step = (ms_s + ms_l) / nsteps
+    bytes_step = 72.0 * nt * p.nc
+    achieved = bytes_step / (ms_step * 1e-3) / 1e9
+    log(f"transport {p.name}: {nsteps} steps wall {t1 - t0:.3f} s, kernels {ms_step:.3f} ms/step, "
+        f"{achieved:.0f} GB/s")
+    return {"workload": p.name, "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "ms_per_step": round(ms_step, 4), "bytes_per_step": bytes_step,
+            "tracer_cell_steps_per_s": round(nt * p.nc / (ms_step * 1e-3), 1),
+            "stage_avg_ms": round(ms_s / max(ns_, 1), 4), "last_avg_ms": round(ms_l / max(nl_, 1), 4)}
+
+
+def chemistry_roofline(lib):
+    p = P.config3()
+    ctx = lib.context().load(p)
+    info = ctx.mechanism_info()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    st = ctx.chem_percell(0.0, 1800.0, p.dt0_C, None, None, p.atol_C, p.rtol_C, True)
+    t1 = time.perf_counter()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    n, ms = kernel_ms(prof, "chem_percell")
+    flops = st.attempts_total * info["flops_attempt"]
+    achieved = flops / (ms * 1e-3) / 1e12
+    log(f"chemistry {p.name}: attempts {st.attempts_total} (min {st.attempts_min} max "
+        f"{st.attempts_max}) kernel {ms:.1f} ms wall {t1 - t0:.2f} s {achieved:.2f} TFLOP/s "
+        f"regs {info['regs_percell']} scratch {info['scratch_percell']}")
+    return {"workload": p.name, "bound": "fp64", "achieved": round(achieved, 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "kernel_ms": round(ms, 2),
+            "attempts": st.attempts_total, "attempts_min": st.attempts_min,
+            "attempts_max": st.attempts_max, "flops_per_attempt": info["flops_attempt"],
+            "cell_steps_per_s": round(p.nc / (ms * 1e-3), 1)}
+
+
+def cpu_baseline(sample_dims=(72, 46, 24), steps: int = 1):
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from oracle_lib import oracle_library
+    lib = oracle_library()
+    p = P.config4(nlon=sample_dims[0], nlat=sample_dims[1], nlev=sample_dims[2], nmacro=steps)
+    opts = split_opts(p)
+    ctx = lib.context().load(p)
+    threads = int(lib.lib.rs_oracle_threads())
+    dtT, dtC = p.dt0_T, p.dt0_C
+    t0 = time.perf_counter()
+    for m in range(steps):
+        dtT, dtC, _, _ = ctx.macro_step(m * p.dt_macro, (m + 1) * p.dt_macro, dtT, dtC, opts)
+    t1 = time.perf_counter()
+    rate = p.nc * steps / (t1 - t0)
+    log(f"cpu oracle sample {sample_dims} x {steps} steps: {t1 - t0:.2f} s, {rate:.1f} cell-steps/s, "
+        f"{threads} threads")
+    return {"value": round(rate, 2), "unit": "coupled cell-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"coupled split step of the same workload on a "
+                      f"{sample_dims[0]}x{sample_dims[1]}x{sample_dims[2]} sub-grid, {steps} step(s) "
+                      f"of 900 s, {t1 - t0:.1f} s of CPU time",
+            "cpu_model": cpu_model()}
+
+
+def cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--quick", action="store_true", help="skip the transport/chemistry rooflines")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the split step shards by "
+              "latitude bands across GPUs, which this build does not launch.", file=sys.stderr)
+        return 2
+    lib = hip_library()
+    c = coupled(lib, a.steps, a.warmup)
+    extra = {}
+    if not a.quick:
+        extra["transport_hbm"] = transport_roofline(lib)
+        extra["chemistry_fp64"] = chemistry_roofline(lib)
+    cpu = None if a.no_cpu else cpu_baseline()
+    p = c["p"]
+    value = p.nc * c["steps"] / c["fwd_s"]
+    line = {
+        "metric": "coupled cell-steps/s fwd; adjoint/forward time ratio; % HBM & FP64 roofline",
+        "value": round(value, 1),
+        "unit": "coupled cell-steps/s",
+        "n_gpus": 1,
+        "steps": c["steps"],
+        "warmup": a.warmup,
+        "ms_per_step": round(c["fwd_s"] / c["steps"] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lognormal state, random mass-action mechanism, solid-body wind)",
+        "config": {"workload": p.name, "grid": [p.grid.nlon, p.grid.nlat, p.grid.nlev],
+                   "species": p.ns, "reactions": p.mech.nr, "split_dt_s": p.dt_macro,
+                   "chemistry_norm": "global", "parallelism": "single GPU"},
+        "roofline": c["roof"],
+        "cpu_baseline": cpu,
+        "adjoint_ratio": round(c["ratio"], 3),
+        "adjoint": {"seconds_forward_ckpt": round(c["adj"].seconds_forward, 4),
+                    "seconds_backward": round(c["adj"].seconds_backward, 4),
+                    "seconds_replay": round(c["adj"].seconds_replay, 4),
+                    "nvjp_T": c["adj"].nvjp_T, "nvjp_C": c["adj"].nvjp_C},
+        "inner_steps": {"transport_acc_rej": c["counts"][:2], "chem_acc_rej": c["counts"][2:]},
+        "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        **extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

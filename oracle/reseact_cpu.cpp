@@ -338,7 +338,9 @@ struct CellChem {
 // blocksolve (rx_traced_integrator.jl:110-135): pivot-free Gaussian
 // elimination of W x = b, W row-major NS x NS.  W and b are copied.
 void blocksolve(int ns, const double* Win, const double* bin, double* x) {
-    std::vector<double> W(Win, Win + (size_t)ns * ns), b(bin, bin + ns);
+    thread_local std::vector<double> W, b;   // per-thread copies, reused across calls
+    W.assign(Win, Win + (size_t)ns * ns);
+    b.assign(bin, bin + ns);
     for (int k = 0; k < ns; ++k) {
         double piv = 1.0 / W[(size_t)k * ns + k];
         for (int i = k + 1; i < ns; ++i) {

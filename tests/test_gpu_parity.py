@@ -53,7 +53,9 @@ def test_ssprk43_step(hip, oracle, dims, nt, ns):
         eo = co.ssprk43_step(0.0, dt, 1e-6, 1e-3)
         ug, uo = cg.get_candidate(), co.get_candidate()
         assert relmax(ug, uo) <= TOL_T
-        assert abs(eg - eo) <= TOL_T * abs(eo)
+        # EEst decides accept/reject by comparison with 1; near round-off it is
+        # itself round-off, so the bound is absolute below 1
+        assert abs(eg - eo) <= TOL_T * max(abs(eo), 1.0)
         # untransported species bit-for-bit untouched
         assert np.array_equal(ug[nt * g.nc:], p.u0[nt * g.nc:])
 
@@ -68,7 +70,7 @@ def test_ssprk43_negative_wind(hip, oracle):
     eg = cg.ssprk43_step(0.0, 300.0, 1e-6, 1e-3)
     eo = co.ssprk43_step(0.0, 300.0, 1e-6, 1e-3)
     assert relmax(cg.get_candidate(), co.get_candidate()) <= TOL_T
-    assert abs(eg - eo) <= TOL_T * eo
+    assert abs(eg - eo) <= TOL_T * max(eo, 1.0)
 
 
 def test_adaptive_transport_same_sequence(hip, oracle):
@@ -96,7 +98,7 @@ def test_ros23_step(hip, oracle, cfg):
         eg = cg.ros23_step(0.0, dt, 1e-6, 1e-3)
         eo = co.ros23_step(0.0, dt, 1e-6, 1e-3)
         assert relmax(cg.get_candidate(), co.get_candidate()) <= TOL_C
-        assert abs(eg - eo) <= TOL_C * eo
+        assert abs(eg - eo) <= TOL_C * max(eo, 1.0)
 
 
 def test_ros23_natural_order_matches_reference_elimination(hip, oracle):
@@ -108,7 +110,7 @@ def test_ros23_natural_order_matches_reference_elimination(hip, oracle):
     eg = cg.ros23_step(0.0, 5.0, 1e-6, 1e-3)
     eo = co.ros23_step(0.0, 5.0, 1e-6, 1e-3)
     assert relmax(cg.get_candidate(), co.get_candidate()) <= 1e-12
-    assert abs(eg - eo) <= 1e-12 * eo
+    assert abs(eg - eo) <= 1e-12 * max(eo, 1.0)
 
 
 def test_adaptive_chem_same_sequence(hip, oracle):
