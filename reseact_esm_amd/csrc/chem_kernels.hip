@@ -9,47 +9,50 @@
  */
 
 /* One ROS23 attempt on one cell (ros23_step, rx_traced_integrator.jl:279-349,
- * mass matrix I, dT = 0).  Returns the cell's sum of squared scaled errors. */
+ * mass matrix I, dT = 0).  Returns the cell's sum of squared scaled errors.
+ * E (may be 0) points at this cell's emission of species 0, stride es.
+ * The stage combinations the reference forms at the end -- c = 2(k1 - f0),
+ * a = c32 (k2 - f1), k1 - 2 k2 -- are formed as soon as their inputs exist,
+ * with the same operations, so that f0, f1 and k2 die early: six state
+ * vectors plus the factors are live at the peak instead of ten. */
 RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
-                              const double* __restrict__ ee, const double* __restrict__ u,
-                              double* __restrict__ unew, double dt, double atol, double rtol) {
-    double f0[RS_NS], f1[RS_NS], k1[RS_NS], k2[RS_NS], tmp[RS_NS], dinv[RS_NS];
-    double W[RS_NNZ];
-    rs_rhs(K, jf, ee, u, f0);                                      /* :282 */
+                              const double* __restrict__ E, long long es,
+                              const double* __restrict__ u, double* __restrict__ unew,
+                              double dt, double atol, double rtol) {
+    double W[RS_NNZ], dinv[RS_NS];
+    double f[RS_NS], k1[RS_NS], k2[RS_NS], cc[RS_NS], aa[RS_NS];
+    (void)dinv;
+    rs_rhs(K, jf, E, es, u, f);                                    /* f0, :282 */
     const double hg = dt * RS_ROS23_D;                             /* :291 */
     rs_jac_W(K, jf, u, hg, W);                                     /* :287-296 */
     rs_lu(W, dinv);
-    rs_solve(W, dinv, f0, k1);                                     /* :298 */
+    rs_solve(W, dinv, f, k1);                                      /* :298 */
+#pragma unroll
+    for (int i = 0; i < RS_NS; ++i) { double c = k1[i] - f[i]; cc[i] = 2.0 * c; }   /* :323-324 */
     const double dto2 = dt / 2;
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * k1[i]; tmp[i] = u[i] + du1; }
-    rs_rhs(K, jf, ee, tmp, f1);                                    /* :303 */
+    for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * k1[i]; aa[i] = u[i] + du1; }
+    rs_rhs(K, jf, E, es, aa, f);                                   /* f1, :303 */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) tmp[i] = f1[i] - k1[i];        /* :306-309 */
-    rs_solve(W, dinv, tmp, k2);                                    /* :310 */
+    for (int i = 0; i < RS_NS; ++i) aa[i] = f[i] - k1[i];          /* :306-309 */
+    rs_solve(W, dinv, aa, k2);                                     /* :310 */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) k2[i] = k2[i] + k1[i];         /* :311-313 */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { const double du2 = dt * k2[i]; unew[i] = u[i] + du2; }
-    rs_rhs(K, jf, ee, unew, tmp);                                  /* f2, :318 */
+    for (int i = 0; i < RS_NS; ++i) { double a = k2[i] - f[i]; aa[i] = RS_ROS23_C32 * a; }  /* :321-322 */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) {                              /* :319-327 */
-        double a = k2[i] - f1[i];
-        a = RS_ROS23_C32 * a;
-        double c = k1[i] - f0[i];
-        c = 2.0 * c;
-        const double d3 = tmp[i] - a;
-        tmp[i] = d3 - c;
-    }
-    double k3[RS_NS];
-    rs_solve(W, dinv, tmp, k3);                                    /* :328 */
+    for (int i = 0; i < RS_NS; ++i) { const double du2 = dt * k2[i]; unew[i] = u[i] + du2; }
+#pragma unroll
+    for (int i = 0; i < RS_NS; ++i) { const double w2 = 2.0 * k2[i]; k1[i] = k1[i] - w2; }  /* :333-334 */
+    rs_rhs(K, jf, E, es, unew, f);                                 /* f2, :318 */
+#pragma unroll
+    for (int i = 0; i < RS_NS; ++i) { const double d3 = f[i] - aa[i]; f[i] = d3 - cc[i]; }  /* :325-326 */
+    rs_solve(W, dinv, f, k2);                                      /* k3, :328 */
     const double dto6 = dt / 6;
     double sse = 0.0;
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) {                              /* :330-346 */
-        const double w2 = 2.0 * k2[i];
-        double ut = k1[i] - w2;
-        ut = ut + k3[i];
+    for (int i = 0; i < RS_NS; ++i) {                              /* :335-346 */
+        double ut = k1[i] + k2[i];
         ut = dto6 * ut;
         const double am = rs_max(RS_FABS(u[i]), RS_FABS(unew[i]));
         double den = am * rtol;
@@ -64,49 +67,57 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
  * (f2, k3 and EEst are dead, rx_traced_integrator.jl:540-547):
  *   f0 = f(u); k1 = W\f0; u1 = u + h/2 k1; f1 = f(u1); k2 = W\(f1-k1) + k1;
  *   unew = u + h k2,   W = I - h*gamma*J(u).
- * Given lam = dL/dunew: ubar (returned in lam) and ebar = dL/de (e enters f
- * additively).  The stages are recomputed, not stored. */
-RS_FN void rs_ros23_vjp(const double* __restrict__ K, double jf, const double* __restrict__ ee,
-                        const double* __restrict__ u, double dt, double* __restrict__ lam,
-                        double* __restrict__ ebar) {
-    double f0[RS_NS], k1[RS_NS], k2[RS_NS], u1[RS_NS], tmp[RS_NS], dinv[RS_NS];
-    double W[RS_NNZ];
-    rs_rhs(K, jf, ee, u, f0);
+ * Given lam = dL/dunew, returns dL/du in lam and adds dL/dE to G[i*es] (G
+ * may be 0; E enters f additively on the cells that have it).  Reverse:
+ *   z2 = W^-T (h lam)             = f1bar;  k1bar = h lam - z2
+ *   u1bar = J(u1)^T z2;  ubar = lam + u1bar;  k1bar += h/2 u1bar
+ *   z1 = W^-T k1bar               = f0bar;  ubar += J(u)^T z1
+ *   Jbar = h gamma (z2 (k2-k1)^T + z1 k1^T)  ->  ubar += d/du [z^T J(u) v]
+ *   Ebar = z1 + z2.
+ * The stages are recomputed from u, not stored. */
+RS_FN void rs_ros23_vjp(const double* __restrict__ K, double jf, const double* __restrict__ E,
+                        long long es, const double* __restrict__ u, double dt,
+                        double* __restrict__ lam, double* __restrict__ G) {
+    double W[RS_NNZ], dinv[RS_NS];
+    double f[RS_NS], k1[RS_NS], k2[RS_NS], tmp[RS_NS], z[RS_NS], ub[RS_NS];
+    (void)dinv;
+    rs_rhs(K, jf, E, es, u, f);
     const double hg = dt * RS_ROS23_D;
     rs_jac_W(K, jf, u, hg, W);
     rs_lu(W, dinv);
-    rs_solve(W, dinv, f0, k1);
+    rs_solve(W, dinv, f, k1);
     const double dto2 = dt / 2;
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * k1[i]; u1[i] = u[i] + du1; }
-    rs_rhs(K, jf, ee, u1, tmp);                                    /* f1 */
+    for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * k1[i]; tmp[i] = u[i] + du1; }
+    rs_rhs(K, jf, E, es, tmp, f);                                  /* f1 at u1 = tmp */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) tmp[i] = tmp[i] - k1[i];
-    rs_solve(W, dinv, tmp, k2);
+    for (int i = 0; i < RS_NS; ++i) f[i] = f[i] - k1[i];
+    rs_solve(W, dinv, f, k2);
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) k2[i] = k2[i] + k1[i];
     /* reverse */
-    double z2[RS_NS], z1[RS_NS], ub[RS_NS];
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { tmp[i] = dt * lam[i]; ub[i] = lam[i]; }   /* k2bar */
-    rs_solveT(W, dinv, tmp, z2);                                   /* z2 = W^-T k2bar = f1bar */
+    for (int i = 0; i < RS_NS; ++i) { f[i] = dt * lam[i]; ub[i] = lam[i]; }     /* k2bar */
+    rs_solveT(W, dinv, f, z);                                      /* z2 */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) tmp[i] = tmp[i] - z2[i];       /* k1bar = k2bar - z2 */
-    double u1b[RS_NS];
+    for (int i = 0; i < RS_NS; ++i) f[i] = f[i] - z[i];            /* k1bar */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) u1b[i] = 0.0;
-    rs_jtv(K, jf, u1, z2, u1b);                                    /* u1bar = J(u1)^T f1bar */
+    for (int i = 0; i < RS_NS; ++i) k2[i] = k2[i] - k1[i];         /* v = k2 - k1 */
+    rs_hess(K, jf, z, k2, hg, ub);
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { ub[i] = ub[i] + u1b[i]; tmp[i] = tmp[i] + dto2 * u1b[i]; }
-    rs_solveT(W, dinv, tmp, z1);                                   /* z1 = W^-T k1bar = f0bar */
-    rs_jtv(K, jf, u, z1, ub);                                      /* ubar += J(u)^T f0bar */
-    /* Jbar = h*gamma (z2 (k2-k1)^T + z1 k1^T):  ubar += d/du of z^T J(u) v */
+    for (int i = 0; i < RS_NS; ++i) { k2[i] = z[i]; lam[i] = 0.0; }   /* keep z2 for Ebar */
+    rs_jtv(K, jf, tmp, z, lam);                                    /* u1bar = J(u1)^T z2 */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) tmp[i] = k2[i] - k1[i];
-    rs_hess(K, jf, z2, tmp, hg, ub);
-    rs_hess(K, jf, z1, k1, hg, ub);
+    for (int i = 0; i < RS_NS; ++i) { ub[i] = ub[i] + lam[i]; f[i] = f[i] + dto2 * lam[i]; }
+    rs_solveT(W, dinv, f, z);                                      /* z1 */
+    rs_jtv(K, jf, u, z, ub);
+    rs_hess(K, jf, z, k1, hg, ub);
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { lam[i] = ub[i]; ebar[i] = z1[i] + z2[i]; }
+    for (int i = 0; i < RS_NS; ++i) lam[i] = ub[i];
+    if (G) {
+#pragma unroll
+        for (int i = 0; i < RS_NS; ++i) G[i * es] += z[i] + k2[i];
+    }
 }
 
 /* PI controller update of one attempt (rx_traced_integrator.jl:484-504).
@@ -165,14 +176,12 @@ rs_chem_attempt(RsChemArgs a) {
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double sse = 0.0;
     if (cell < a.nc) {
-        double u[RS_NS], un[RS_NS], ee[RS_NS];
+        double u[RS_NS], un[RS_NS];
 #pragma unroll
         for (int s = 0; s < RS_NS; ++s) u[s] = src[(long long)s * a.nc + cell];
         const double jf = a.jfac ? a.jfac[cell] : 1.0;
-        const bool surf = a.E && cell < a.ncol;
-#pragma unroll
-        for (int s = 0; s < RS_NS; ++s) ee[s] = surf ? a.E[(long long)s * a.ncol + cell] : 0.0;
-        sse = rs_ros23_attempt(a.K, jf, ee, u, un, dt, a.atol, a.rtol);
+        const double* Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;
+        sse = rs_ros23_attempt(a.K, jf, Ec, a.ncol, u, un, dt, a.atol, a.rtol);
 #pragma unroll
         for (int s = 0; s < RS_NS; ++s)
             dst[(long long)s * a.nc + cell] = a.clamp ? rs_max(un[s], 0.0) : un[s];
@@ -188,7 +197,8 @@ extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK)
 rs_chem_percell(RsPercellArgs a) {
     const int lane = threadIdx.x & 63;
     long long cell = -1;     /* -1: needs work, -2: queue drained */
-    double u[RS_NS], ee[RS_NS];
+    double u[RS_NS];
+    const double* Ec = nullptr;
     double t = 0.0, dt = 0.0, qold = 0.0, jf = 1.0;
     long long iters = 0, nacc = 0, nrej = 0;
     long long s_att = 0, s_acc = 0, s_rej = 0, s_max = 0, s_min = -1, s_fail = 0;
@@ -209,10 +219,7 @@ rs_chem_percell(RsPercellArgs a) {
 #pragma unroll
                     for (int s = 0; s < RS_NS; ++s) u[s] = a.u[(long long)s * a.nc + cell];
                     jf = a.jfac ? a.jfac[cell] : 1.0;
-                    const bool surf = a.E && cell < a.ncol;
-#pragma unroll
-                    for (int s = 0; s < RS_NS; ++s)
-                        ee[s] = surf ? a.E[(long long)s * a.ncol + cell] : 0.0;
+                    Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;
                     t = a.t0;
                     dt = a.dtcell ? a.dtcell[cell] : a.dt0;
                     qold = a.qoldinit;
@@ -227,7 +234,7 @@ rs_chem_percell(RsPercellArgs a) {
             if ((t < tlim) && (iters < a.maxiters)) {
                 const double dtc = (dt < a.t1 - t) ? dt : a.t1 - t;
                 double un[RS_NS];
-                const double sse = rs_ros23_attempt(a.K, jf, ee, u, un, dtc, a.atol, a.rtol);
+                const double sse = rs_ros23_attempt(a.K, jf, Ec, a.ncol, u, un, dtc, a.atol, a.rtol);
                 const double eest = RS_SQRT(sse / (double)RS_NS);
                 if (rs_pi_update(a.beta1, a.beta2, a.invqmin, a.invqmax, a.gamma, a.qsmin,
                                  a.qsmax, a.qoldinit, dtc, eest, &t, &dt, &qold, &nacc, &nrej)) {
@@ -275,7 +282,7 @@ extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK)
 rs_chem_vjp(RsChemVjpArgs a) {
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= a.nc) return;
-    double u[RS_NS], lam[RS_NS], ee[RS_NS], eb[RS_NS];
+    double u[RS_NS], lam[RS_NS];
 #pragma unroll
     for (int s = 0; s < RS_NS; ++s) {
         u[s] = a.u[(long long)s * a.nc + cell];
@@ -285,15 +292,11 @@ rs_chem_vjp(RsChemVjpArgs a) {
     }
     const double jf = a.jfac ? a.jfac[cell] : 1.0;
     const bool surf = cell < a.ncol;
-#pragma unroll
-    for (int s = 0; s < RS_NS; ++s) ee[s] = (surf && a.E) ? a.E[(long long)s * a.ncol + cell] : 0.0;
-    rs_ros23_vjp(a.K, jf, ee, u, a.dt, lam, eb);
+    const double* Ec = (surf && a.E) ? a.E + cell : nullptr;
+    double* Gc = (surf && a.gE) ? a.gE + cell : nullptr;
+    rs_ros23_vjp(a.K, jf, Ec, a.ncol, u, a.dt, lam, Gc);
 #pragma unroll
     for (int s = 0; s < RS_NS; ++s) a.lam[(long long)s * a.nc + cell] = lam[s];
-    if (surf && a.gE) {
-#pragma unroll
-        for (int s = 0; s < RS_NS; ++s) a.gE[(long long)s * a.ncol + cell] += eb[s];
-    }
 }
 
 #endif /* RS_HOST_BUILD */

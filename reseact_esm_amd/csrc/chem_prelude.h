@@ -15,12 +15,18 @@
 #define RS_FABS fabs
 #define RS_POW pow
 #endif
-/* Back substitution x_i = acc / U_ii.  The device multiplies by the
- * reciprocal the LU already computed; the exact form divides, as the
- * reference's blocksolve does (rx_traced_integrator.jl:132). */
+/* Pivot reciprocals and back substitution x_i = acc / U_ii.  The device
+ * keeps 1/U_ii in the diagonal slot of the factor (one register per species
+ * fewer) and multiplies; the exact form keeps U_ii, stores the reciprocals
+ * aside and divides, as the reference's blocksolve does
+ * (rx_traced_integrator.jl:114, :132). */
 #ifdef RS_EXACT_DIV
+#define RS_DINV_SET(k, s, v) (dinv[k] = (v))
+#define RS_DINV(k, s) (dinv[k])
 #define RS_DIAG_SOLVE(acc, w, di) ((acc) / (w))
 #else
+#define RS_DINV_SET(k, s, v) (W[s] = (v))
+#define RS_DINV(k, s) (W[s])
 #define RS_DIAG_SOLVE(acc, w, di) ((acc) * (di))
 #endif
 /* Julia's max: propagates a NaN in the first argument like std::max. */

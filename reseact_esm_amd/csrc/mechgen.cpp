@@ -182,9 +182,9 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
     e << "#define RS_NS " << n << "\n#define RS_NR " << m.nr << "\n#define RS_NNZ " << p.nnz
       << "\n";
     // ---- RHS: rates folded into the accumulation, reaction order per species
-    e << "RS_FN void rs_rhs(const double* __restrict__ K, double jf, const double* __restrict__ ee,"
-         " const double* __restrict__ u, double* __restrict__ f) {\n";
-    for (int i = 0; i < n; ++i) e << "  f[" << i << "] = ee[" << i << "];\n";
+    e << "RS_FN void rs_rhs(const double* __restrict__ K, double jf, const double* __restrict__ E,"
+         " long long es, const double* __restrict__ u, double* __restrict__ f) {\n";
+    for (int i = 0; i < n; ++i) e << "  f[" << i << "] = E ? E[" << i << " * es] : 0.0;\n";
     for (int r = 0; r < m.nr; ++r) {
         e << "  { const double rt = ";
         if (m.rb[r] < 0) e << kexpr(m, r) << " * u[" << m.ra[r] << "];";
@@ -226,11 +226,11 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
     // ---- LU without pivoting, in position order
     e << "RS_FN void rs_lu(double* __restrict__ W, double* __restrict__ dinv) {\n";
     for (int k = 0; k < n; ++k) {
-        e << "  dinv[" << k << "] = 1.0 / W[" << p.idx[k][k] << "];\n";
+        e << "  RS_DINV_SET(" << k << ", " << p.idx[k][k] << ", 1.0 / W[" << p.idx[k][k] << "]);\n";
         for (int i = k + 1; i < n; ++i) {
             int sik = p.idx[i][k];
             if (sik < 0) continue;
-            e << "  { const double m = W[" << sik << "] * dinv[" << k << "]; W[" << sik << "] = m;";
+            e << "  { const double m = W[" << sik << "] * RS_DINV(" << k << ", " << p.idx[k][k] << "); W[" << sik << "] = m;";
             for (int j = k + 1; j < n; ++j) {
                 int skj = p.idx[k][j];
                 if (skj < 0) continue;
@@ -254,7 +254,7 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         e << "  { double acc = y" << q << ";";
         for (int j = q + 1; j < n; ++j)
             if (p.idx[q][j] >= 0) e << " acc = acc - W[" << p.idx[q][j] << "] * x[" << p.perm[j] << "];";
-        e << " x[" << p.perm[q] << "] = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << "], dinv[" << q << "]); }\n";
+        e << " x[" << p.perm[q] << "] = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << "], RS_DINV(" << q << ", " << p.idx[q][q] << ")); }\n";
     }
     e << "}\n";
     // ---- W^T x = b: U^T z = b (forward), then L^T x = z (backward)
@@ -264,7 +264,7 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         e << "  double z" << q << "; { double acc = b[" << p.perm[q] << "];";
         for (int k = 0; k < q; ++k)
             if (p.idx[k][q] >= 0) e << " acc = acc - W[" << p.idx[k][q] << "] * z" << k << ";";
-        e << " z" << q << " = acc * dinv[" << q << "]; }\n";
+        e << " z" << q << " = acc * RS_DINV(" << q << ", " << p.idx[q][q] << "); }\n";
     }
     for (int q = n - 1; q >= 0; --q) {
         e << "  { double acc = z" << q << ";";

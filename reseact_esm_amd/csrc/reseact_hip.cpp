@@ -51,6 +51,7 @@ hipError_t rs_launch_clamp(double* x, long long n, hipStream_t st);
 int rs_reduce_blocks(long long n);
 hipError_t rs_launch_relmax(const double* a, const double* b, long long n, double* part, hipStream_t st);
 hipError_t rs_launch_sumsq(const double* q, long long n, double* part, hipStream_t st);
+hipError_t rs_launch_sum(const double* x, long long n, double* part, hipStream_t st);
 hipError_t rs_launch_loss_seed(double* lam, const double* u, long long ntot, long long sbase,
                                long long ncol, hipStream_t st);
 }
@@ -241,8 +242,9 @@ std::string jit_cache_path(const std::string& key) {
     return dir + "/" + key + ".hsaco";
 }
 
-std::vector<char> jit_compile(const std::string& src, std::string& log) {
-    const std::string key = std::to_string(fnv1a(src)) + "_" + std::to_string(src.size());
+std::vector<char> jit_compile(const std::string& src, bool exact, std::string& log) {
+    const std::string key = std::to_string(fnv1a(src)) + "_" + std::to_string(src.size()) +
+                            (exact ? "_x" : "_f");
     {
         std::lock_guard<std::mutex> g(g_jit_mu);
         auto it = g_jit_cache.find(key);
@@ -268,9 +270,15 @@ std::vector<char> jit_compile(const std::string& src, std::string& log) {
         log = "hiprtcCreateProgram failed";
         return {};
     }
-    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
-                          "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    // exact (reference) mode: no FMA contraction and true division, so the
+    // natural-order elimination rounds exactly like the reference's blocksolve
+    std::vector<const char*> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
+                                     "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    if (exact) {
+        opts.push_back("-ffp-contract=off");
+        opts.push_back("-DRS_EXACT_DIV");
+    }
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
     size_t ln = 0;
     hiprtcGetProgramLogSize(prog, &ln);
     if (ln > 1) {
@@ -398,7 +406,14 @@ void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cu
     chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), CHEM_BLOCK, P_CHEM);
 }
 
+// Sum of n device partials: folded on the device to at most 1024 values first.
 double host_sum(rs_ctx* c, const double* dpart, int n) {
+    if (n > 1024) {
+        c->red.ensure(1024);
+        HIPCHK(rs_launch_sum(dpart, n, c->red.p, c->stream));
+        dpart = c->red.p;
+        n = rs_reduce_blocks(n);
+    }
     HIPCHK(hipMemcpyAsync(c->hred, dpart, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     double s = 0.0;
@@ -407,6 +422,7 @@ double host_sum(rs_ctx* c, const double* dpart, int n) {
 }
 
 double host_max(rs_ctx* c, const double* dpart, int n) {
+    if (n > 1024) throw ArgError{"host_max: more than 1024 partials", RS_E_ARG};
     HIPCHK(hipMemcpyAsync(c->hred, dpart, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     double s = 0.0;
@@ -800,7 +816,7 @@ int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const doub
         auto cm = std::make_unique<ChemModule>();
         cm->plan = rs::plan_mechanism(m, c->lu_order);
         std::string log;
-        std::vector<char> code = jit_compile(chem_source(m, cm->plan), log);
+        std::vector<char> code = jit_compile(chem_source(m, cm->plan), c->lu_order == 0, log);
         if (code.empty()) return fail(c, RS_E_JIT, "hipRTC compile failed: " + log.substr(0, 2000));
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipModuleLoadData(&cm->mod, code.data()));

@@ -322,6 +322,22 @@ __global__ void __launch_bounds__(256) rs_relmax(const double* a, const double* 
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// per-block sum of x (fixed order)
+__global__ void __launch_bounds__(256) rs_sum(const double* x, long long n, double* part) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        s += x[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 // J = sum over level-0 cells of q^2 (per block), q = species block of the state
 __global__ void __launch_bounds__(256) rs_sumsq(const double* q, long long n, double* part) {
     __shared__ double sh[256];
@@ -403,6 +419,10 @@ int rs_reduce_blocks(long long n) { return (int)util_grid(n) > 1024 ? 1024 : (in
 hipError_t rs_launch_relmax(const double* a, const double* b, long long n, double* part,
                             hipStream_t st) {
     hipLaunchKernelGGL(rs_relmax, dim3(rs_reduce_blocks(n)), dim3(256), 0, st, a, b, n, part);
+    return hipGetLastError();
+}
+hipError_t rs_launch_sum(const double* x, long long n, double* part, hipStream_t st) {
+    hipLaunchKernelGGL(rs_sum, dim3(rs_reduce_blocks(n)), dim3(256), 0, st, x, n, part);
     return hipGetLastError();
 }
 hipError_t rs_launch_sumsq(const double* q, long long n, double* part, hipStream_t st) {

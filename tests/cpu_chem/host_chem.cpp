@@ -16,13 +16,11 @@ extern "C" double hc_attempt(long long nc, long long ncol, const double* K, cons
                              double atol, double rtol) {
     double tot = 0.0;
     for (long long c = 0; c < nc; ++c) {
-        double uc[RS_NS], un[RS_NS], ee[RS_NS];
-        for (int s = 0; s < RS_NS; ++s) {
-            uc[s] = u[s * nc + c];
-            ee[s] = (E && c < ncol) ? E[s * ncol + c] : 0.0;
-        }
+        double uc[RS_NS], un[RS_NS];
+        for (int s = 0; s < RS_NS; ++s) uc[s] = u[s * nc + c];
         const double jf = jfac ? jfac[c] : 1.0;
-        tot += rs_ros23_attempt(K, jf, ee, uc, un, dt, atol, rtol);
+        const double* Ec = (E && c < ncol) ? E + c : nullptr;
+        tot += rs_ros23_attempt(K, jf, Ec, ncol, uc, un, dt, atol, rtol);
         for (int s = 0; s < RS_NS; ++s) unew[s * nc + c] = un[s];
     }
     return tot;
@@ -31,16 +29,15 @@ extern "C" double hc_attempt(long long nc, long long ncol, const double* K, cons
 extern "C" void hc_vjp(long long nc, long long ncol, const double* K, const double* jfac,
                        const double* E, const double* u, double dt, double* lam, double* gE) {
     for (long long c = 0; c < nc; ++c) {
-        double uc[RS_NS], lc[RS_NS], ee[RS_NS], eb[RS_NS];
+        double uc[RS_NS], lc[RS_NS];
         for (int s = 0; s < RS_NS; ++s) {
             uc[s] = u[s * nc + c];
             lc[s] = lam[s * nc + c];
-            ee[s] = (E && c < ncol) ? E[s * ncol + c] : 0.0;
         }
         const double jf = jfac ? jfac[c] : 1.0;
-        rs_ros23_vjp(K, jf, ee, uc, dt, lc, eb);
+        const double* Ec = (E && c < ncol) ? E + c : nullptr;
+        double* Gc = (c < ncol) ? gE + c : nullptr;
+        rs_ros23_vjp(K, jf, Ec, ncol, uc, dt, lc, Gc);
         for (int s = 0; s < RS_NS; ++s) lam[s * nc + c] = lc[s];
-        if (c < ncol)
-            for (int s = 0; s < RS_NS; ++s) gE[s * ncol + c] += eb[s];
     }
 }

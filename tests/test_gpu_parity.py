@@ -53,6 +53,8 @@ def test_ssprk43_step(hip, oracle, dims, nt, ns):
         eo = co.ssprk43_step(0.0, dt, 1e-6, 1e-3)
         ug, uo = cg.get_candidate(), co.get_candidate()
         assert relmax(ug, uo) <= TOL_T
+        # stages compiled without contraction: bit-for-bit the oracle
+        assert np.array_equal(ug, uo)
         # EEst decides accept/reject by comparison with 1; near round-off it is
         # itself round-off, so the bound is absolute below 1
         assert abs(eg - eo) <= TOL_T * max(abs(eo), 1.0)
@@ -78,6 +80,7 @@ def test_adaptive_transport_same_sequence(hip, oracle):
     cg, co = both(hip, oracle, p)
     dg, kg = cg.adaptive_transport(0.0, 600.0, 15.0, pictrl_ssprk43(), 1e-6, 1e-3, True)
     do, ko = co.adaptive_transport(0.0, 600.0, 15.0, pictrl_ssprk43(), 1e-6, 1e-3, True)
+    assert kg.naccept > 5
     assert (kg.naccept, kg.nreject) == (ko.naccept, ko.nreject)
     assert abs(dg - do) <= 1e-10 * do
     assert relmax(cg.get_state(), co.get_state()) <= TOL_T
@@ -94,7 +97,9 @@ def test_ros23_step(hip, oracle, cfg):
     else:
         p = P.config3(nlon=12, nlat=8, nlev=2)
     cg, co = both(hip, oracle, p)
-    for dt in (0.5, 30.0):
+    # step sizes on the scale the controller accepts (the 50-species mechanism
+    # is stiff up to 1e4 1/s)
+    for dt in ((1e-3, 0.05) if cfg == "c3" else (0.5, 30.0)):
         eg = cg.ros23_step(0.0, dt, 1e-6, 1e-3)
         eo = co.ros23_step(0.0, dt, 1e-6, 1e-3)
         assert relmax(cg.get_candidate(), co.get_candidate()) <= TOL_C
@@ -107,10 +112,12 @@ def test_ros23_natural_order_matches_reference_elimination(hip, oracle):
     cg.set_lu_ordering(0)
     cg.load(p)
     co = oracle.context().load(p)
-    eg = cg.ros23_step(0.0, 5.0, 1e-6, 1e-3)
-    eo = co.ros23_step(0.0, 5.0, 1e-6, 1e-3)
-    assert relmax(cg.get_candidate(), co.get_candidate()) <= 1e-12
-    assert abs(eg - eo) <= 1e-12 * max(eo, 1.0)
+    for dt in (0.5, 5.0):
+        eg = cg.ros23_step(0.0, dt, 1e-6, 1e-3)
+        eo = co.ros23_step(0.0, dt, 1e-6, 1e-3)
+        # reference mode: the candidate is bit-for-bit the oracle's
+        assert np.array_equal(cg.get_candidate(), co.get_candidate())
+        assert abs(eg - eo) <= 1e-13 * max(eo, 1.0)
 
 
 def test_adaptive_chem_same_sequence(hip, oracle):
