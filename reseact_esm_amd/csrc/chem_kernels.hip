@@ -19,14 +19,13 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
                               const double* __restrict__ E, long long es,
                               const double* __restrict__ u, double* __restrict__ unew,
                               double dt, double atol, double rtol) {
-    double W[RS_NNZ], dinv[RS_NS];
+    double W[RS_NNZ];
     double f[RS_NS], k1[RS_NS], k2[RS_NS], cc[RS_NS], aa[RS_NS];
-    (void)dinv;
     rs_rhs(K, jf, E, es, u, f);                                    /* f0, :282 */
     const double hg = dt * RS_ROS23_D;                             /* :291 */
     rs_jac_W(K, jf, u, hg, W);                                     /* :287-296 */
-    rs_lu(W, dinv);
-    rs_solve(W, dinv, f, k1);                                      /* :298 */
+    rs_lu(W);
+    rs_solve(W, f, k1);                                      /* :298 */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) { double c = k1[i] - f[i]; cc[i] = 2.0 * c; }   /* :323-324 */
     const double dto2 = dt / 2;
@@ -35,7 +34,7 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
     rs_rhs(K, jf, E, es, aa, f);                                   /* f1, :303 */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) aa[i] = f[i] - k1[i];          /* :306-309 */
-    rs_solve(W, dinv, aa, k2);                                     /* :310 */
+    rs_solve(W, aa, k2);                                     /* :310 */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) k2[i] = k2[i] + k1[i];         /* :311-313 */
 #pragma unroll
@@ -47,7 +46,7 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
     rs_rhs(K, jf, E, es, unew, f);                                 /* f2, :318 */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) { const double d3 = f[i] - aa[i]; f[i] = d3 - cc[i]; }  /* :325-326 */
-    rs_solve(W, dinv, f, k2);                                      /* k3, :328 */
+    rs_solve(W, f, k2);                                      /* k3, :328 */
     const double dto6 = dt / 6;
     double sse = 0.0;
 #pragma unroll
@@ -78,27 +77,26 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
 RS_FN void rs_ros23_vjp(const double* __restrict__ K, double jf, const double* __restrict__ E,
                         long long es, const double* __restrict__ u, double dt,
                         double* __restrict__ lam, double* __restrict__ G) {
-    double W[RS_NNZ], dinv[RS_NS];
+    double W[RS_NNZ];
     double f[RS_NS], k1[RS_NS], k2[RS_NS], tmp[RS_NS], z[RS_NS], ub[RS_NS];
-    (void)dinv;
     rs_rhs(K, jf, E, es, u, f);
     const double hg = dt * RS_ROS23_D;
     rs_jac_W(K, jf, u, hg, W);
-    rs_lu(W, dinv);
-    rs_solve(W, dinv, f, k1);
+    rs_lu(W);
+    rs_solve(W, f, k1);
     const double dto2 = dt / 2;
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * k1[i]; tmp[i] = u[i] + du1; }
     rs_rhs(K, jf, E, es, tmp, f);                                  /* f1 at u1 = tmp */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) f[i] = f[i] - k1[i];
-    rs_solve(W, dinv, f, k2);
+    rs_solve(W, f, k2);
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) k2[i] = k2[i] + k1[i];
     /* reverse */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) { f[i] = dt * lam[i]; ub[i] = lam[i]; }     /* k2bar */
-    rs_solveT(W, dinv, f, z);                                      /* z2 */
+    rs_solveT(W, f, z);                                      /* z2 */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) f[i] = f[i] - z[i];            /* k1bar */
 #pragma unroll
@@ -109,7 +107,7 @@ RS_FN void rs_ros23_vjp(const double* __restrict__ K, double jf, const double* _
     rs_jtv(K, jf, tmp, z, lam);                                    /* u1bar = J(u1)^T z2 */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) { ub[i] = ub[i] + lam[i]; f[i] = f[i] + dto2 * lam[i]; }
-    rs_solveT(W, dinv, f, z);                                      /* z1 */
+    rs_solveT(W, f, z);                                      /* z1 */
     rs_jtv(K, jf, u, z, ub);
     rs_hess(K, jf, z, k1, hg, ub);
 #pragma unroll

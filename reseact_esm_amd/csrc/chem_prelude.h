@@ -15,19 +15,17 @@
 #define RS_FABS fabs
 #define RS_POW pow
 #endif
-/* Pivot reciprocals and back substitution x_i = acc / U_ii.  The device
- * keeps 1/U_ii in the diagonal slot of the factor (one register per species
- * fewer) and multiplies; the exact form keeps U_ii, stores the reciprocals
- * aside and divides, as the reference's blocksolve does
- * (rx_traced_integrator.jl:114, :132). */
+/* Pivots.  RS_PIVOT(w) gives 1/U_kk for the elimination (the reference's
+ * piv = 1 ./ W[k,k], rx_traced_integrator.jl:114); RS_DIAG_SOLVE(acc, w)
+ * divides by U_kk in the substitutions (:132).  The device build stores the
+ * reciprocal in the diagonal slot and multiplies by it; the exact build keeps
+ * U_kk and divides, operation for operation like the reference. */
 #ifdef RS_EXACT_DIV
-#define RS_DINV_SET(k, s, v) (dinv[k] = (v))
-#define RS_DINV(k, s) (dinv[k])
-#define RS_DIAG_SOLVE(acc, w, di) ((acc) / (w))
+#define RS_PIVOT(w) (1.0 / (w))
+#define RS_DIAG_SOLVE(acc, w) ((acc) / (w))
 #else
-#define RS_DINV_SET(k, s, v) (W[s] = (v))
-#define RS_DINV(k, s) (W[s])
-#define RS_DIAG_SOLVE(acc, w, di) ((acc) * (di))
+#define RS_PIVOT(w) ((w) = 1.0 / (w))
+#define RS_DIAG_SOLVE(acc, w) ((acc) * (w))
 #endif
 /* Julia's max: propagates a NaN in the first argument like std::max. */
 RS_FN double rs_max(double a, double b) { return (a < b) ? b : a; }

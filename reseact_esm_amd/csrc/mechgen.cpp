@@ -224,13 +224,13 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         }
     e << "}\n";
     // ---- LU without pivoting, in position order
-    e << "RS_FN void rs_lu(double* __restrict__ W, double* __restrict__ dinv) {\n";
+    e << "RS_FN void rs_lu(double* __restrict__ W) {\n";
     for (int k = 0; k < n; ++k) {
-        e << "  RS_DINV_SET(" << k << ", " << p.idx[k][k] << ", 1.0 / W[" << p.idx[k][k] << "]);\n";
+        e << "  { const double pv = RS_PIVOT(W[" << p.idx[k][k] << "]);\n";
         for (int i = k + 1; i < n; ++i) {
             int sik = p.idx[i][k];
             if (sik < 0) continue;
-            e << "  { const double m = W[" << sik << "] * RS_DINV(" << k << ", " << p.idx[k][k] << "); W[" << sik << "] = m;";
+            e << "  { const double m = W[" << sik << "] * pv; W[" << sik << "] = m;";
             for (int j = k + 1; j < n; ++j) {
                 int skj = p.idx[k][j];
                 if (skj < 0) continue;
@@ -239,10 +239,11 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
             }
             e << " }\n";
         }
+        e << "  }\n";
     }
     e << "}\n";
     // ---- W x = b  (b, x in species coordinates)
-    e << "RS_FN void rs_solve(const double* __restrict__ W, const double* __restrict__ dinv,"
+    e << "RS_FN void rs_solve(const double* __restrict__ W,"
          " const double* __restrict__ b, double* __restrict__ x) {\n";
     for (int q = 0; q < n; ++q) {
         e << "  double y" << q << " = b[" << p.perm[q] << "];";
@@ -254,17 +255,17 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         e << "  { double acc = y" << q << ";";
         for (int j = q + 1; j < n; ++j)
             if (p.idx[q][j] >= 0) e << " acc = acc - W[" << p.idx[q][j] << "] * x[" << p.perm[j] << "];";
-        e << " x[" << p.perm[q] << "] = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << "], RS_DINV(" << q << ", " << p.idx[q][q] << ")); }\n";
+        e << " x[" << p.perm[q] << "] = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << "]); }\n";
     }
     e << "}\n";
     // ---- W^T x = b: U^T z = b (forward), then L^T x = z (backward)
-    e << "RS_FN void rs_solveT(const double* __restrict__ W, const double* __restrict__ dinv,"
+    e << "RS_FN void rs_solveT(const double* __restrict__ W,"
          " const double* __restrict__ b, double* __restrict__ x) {\n";
     for (int q = 0; q < n; ++q) {
         e << "  double z" << q << "; { double acc = b[" << p.perm[q] << "];";
         for (int k = 0; k < q; ++k)
             if (p.idx[k][q] >= 0) e << " acc = acc - W[" << p.idx[k][q] << "] * z" << k << ";";
-        e << " z" << q << " = acc * RS_DINV(" << q << ", " << p.idx[q][q] << "); }\n";
+        e << " z" << q << " = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << "]); }\n";
     }
     for (int q = n - 1; q >= 0; --q) {
         e << "  { double acc = z" << q << ";";

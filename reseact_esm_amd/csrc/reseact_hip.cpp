@@ -816,7 +816,10 @@ int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const doub
         auto cm = std::make_unique<ChemModule>();
         cm->plan = rs::plan_mechanism(m, c->lu_order);
         std::string log;
-        std::vector<char> code = jit_compile(chem_source(m, cm->plan), c->lu_order == 0, log);
+        // exact arithmetic follows the ordering unless RS_JIT_EXACT=0/1 overrides it
+        bool exact = c->lu_order == 0;
+        if (const char* ev = std::getenv("RS_JIT_EXACT")) exact = (ev[0] == '1');
+        std::vector<char> code = jit_compile(chem_source(m, cm->plan), exact, log);
         if (code.empty()) return fail(c, RS_E_JIT, "hipRTC compile failed: " + log.substr(0, 2000));
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipModuleLoadData(&cm->mod, code.data()));

@@ -1,0 +1,243 @@
+# ReseactHIP.jl -- Julia host side of the MI355X split-step library.
+#
+# Mirrors the operator API of tools/reactant_handoff/rx_traced_integrator.jl
+# (PICtrl, ssprk43_step, ros23_step, adaptive_solve, *_step_vjp) and
+# tools/reactant_handoff/op_split.jl (lie_trotter_solve) on top of the C ABI of
+# include/reseact_hip.h, called with `ccall`.  The RHS closures of the
+# reference are replaced by a `Context` that holds the grid, wind, mechanism
+# and emissions on the GPU.  `rrule(split_loss, ...)` gives ChainRulesCore the
+# discrete adjoint of the whole macro-step loop (checkpoint per split step,
+# fixed-sequence replay, Rosenbrock stages recomputed).
+#
+# Not exercised in this repository's CI (no Julia toolchain in the build
+# image); every call here is a one-to-one wrapper of a function whose
+# behaviour the Python tests pin through the same ABI.
+module ReseactHIP
+
+using ChainRulesCore
+
+const LIB = get(ENV, "RESEACT_HIP_LIB", joinpath(@__DIR__, "..", "libreseact_hip.so"))
+
+# ---- controller (rx_traced_integrator.jl:86-99) -----------------------------
+Base.@kwdef struct PICtrl
+    beta1::Float64
+    beta2::Float64
+    qmin::Float64 = 1 / 5
+    qmax::Float64 = 10.0
+    gamma::Float64 = 9 / 10
+    qsteady_min::Float64 = 1.0
+    qsteady_max::Float64
+    qoldinit::Float64 = 1e-4
+end
+pictrl_ros23() = PICtrl(; beta1 = 7 / 20, beta2 = 1 / 5, qsteady_max = 6 / 5)
+pictrl_ssprk43() = PICtrl(; beta1 = 7 / 30, beta2 = 2 / 15, qsteady_max = 1.0)
+
+struct Counts
+    naccept::Int64
+    nreject::Int64
+    t_end::Float64
+    dt_end::Float64
+end
+Counts() = Counts(0, 0, 0.0, 0.0)
+
+struct SplitOpts
+    ctrl_T::PICtrl
+    ctrl_C::PICtrl
+    atol_T::Float64
+    rtol_T::Float64
+    atol_C::Float64
+    rtol_C::Float64
+    clamp_nonneg::Cint
+    chem_percell::Cint
+    maxiters::Int64
+end
+
+struct AdjointStats
+    naccept_T::Int64; naccept_C::Int64; nreject_T::Int64; nreject_C::Int64
+    nvjp_T::Int64; nvjp_C::Int64
+    seconds_forward::Float64; seconds_backward::Float64; seconds_replay::Float64
+    replay_maxrel::Float64
+end
+AdjointStats() = AdjointStats(0, 0, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0)
+
+# ---- context ----------------------------------------------------------------
+mutable struct Context
+    ptr::Ptr{Cvoid}
+    nlon::Int; nlat::Int; nlev::Int; ns::Int; nt::Int
+    function Context(; device::Integer = 0)
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:rs_create, LIB), Cint, (Ref{Ptr{Cvoid}}, Cint), r, device)
+        rc == 0 || error("rs_create failed ($rc)")
+        c = new(r[], 0, 0, 0, 0, 0)
+        finalizer(x -> ccall((:rs_destroy, LIB), Cint, (Ptr{Cvoid},), x.ptr), c)
+        return c
+    end
+end
+ncells(c::Context) = c.nlon * c.nlat * c.nlev
+
+function check(c::Context, rc::Cint)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:rs_last_error, LIB), Cstring, (Ptr{Cvoid},), c.ptr))
+    rc == 4 && error("maxiters: $msg")              # host_adaptive! raises the same
+    error("reseact_hip error $rc: $msg")
+end
+
+function set_grid!(c::Context, nlon, nlat, nlev, ns, nt)
+    check(c, ccall((:rs_set_grid, LIB), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Cint),
+                   c.ptr, nlon, nlat, nlev, ns, nt))
+    c.nlon, c.nlat, c.nlev, c.ns, c.nt = nlon, nlat, nlev, ns, nt
+    return c
+end
+set_wind!(c::Context, cx::Vector{Float64}, wz::Vector{Float64}, wl::Vector{Float64}) =
+    check(c, ccall((:rs_set_wind, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                   c.ptr, cx, wz, wl))
+# reactants/products: 2 x NR Int32 (column r = reaction r, -1 = none), 0-based species
+set_mechanism!(c::Context, reactants::Matrix{Int32}, products::Matrix{Int32},
+               k::Vector{Float64}, photo::Vector{Int32}) =
+    check(c, ccall((:rs_set_mechanism, LIB), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}, Ptr{Int32}),
+                   c.ptr, length(k), reactants, products, k, photo))
+set_photolysis!(c::Context, jfac::Union{Nothing,Vector{Float64}}) =
+    check(c, ccall((:rs_set_photolysis, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr,
+                   jfac === nothing ? C_NULL : jfac))
+set_emissions!(c::Context, e::Union{Nothing,Vector{Float64}}) =
+    check(c, ccall((:rs_set_emissions, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr,
+                   e === nothing ? C_NULL : e))
+set_state!(c::Context, u::Vector{Float64}) =
+    check(c, ccall((:rs_set_state, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, u))
+function get_state(c::Context)
+    u = Vector{Float64}(undef, c.ns * ncells(c))
+    check(c, ccall((:rs_get_state, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, u))
+    return u
+end
+function candidate(c::Context)
+    u = Vector{Float64}(undef, c.ns * ncells(c))
+    check(c, ccall((:rs_get_candidate, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, u))
+    return u
+end
+
+# ---- step plug-ins (rx_traced_integrator.jl:279-437) --------------------------
+# `f` of the reference is the context; the layout is the same species-major u.
+function ssprk43_step(c::Context, u, t, dt, abstol::Float64, reltol::Float64)
+    set_state!(c, u)
+    e = Ref(0.0)
+    check(c, ccall((:rs_ssprk43_step, LIB), Cint,
+                   (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Ref{Cdouble}),
+                   c.ptr, t, dt, abstol, reltol, e))
+    return candidate(c), e[]
+end
+
+function ros23_step(c::Context, u, t, dt, NS::Int, NC::Int, masks, abstol::Float64,
+                    reltol::Float64; kwargs...)
+    (NS == c.ns && NC == ncells(c)) || error("ros23_step: layout does not match the context")
+    set_state!(c, u)
+    e = Ref(0.0)
+    check(c, ccall((:rs_ros23_step, LIB), Cint,
+                   (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Ref{Cdouble}),
+                   c.ptr, t, dt, abstol, reltol, e))
+    return candidate(c), e[]
+end
+
+# adaptive_solve (:455-507): `stepfn` selects the operator (ssprk43_step or ros23_step)
+function adaptive_solve(stepfn, c::Context, u0, t0, tend, dt0, ctrl::PICtrl;
+                        abstol = 1e-6, reltol = 1e-3, maxiters::Float64 = 2.0e4,
+                        clamp_nonneg::Bool = false)
+    set_state!(c, u0)
+    dt = Ref(Float64(dt0)); cnt = Ref(Counts())
+    sym = stepfn === ssprk43_step ? :rs_adaptive_transport :
+          stepfn === ros23_step ? :rs_adaptive_chem : error("unknown step function")
+    rc = ccall((sym, LIB), Cint,
+               (Ptr{Cvoid}, Cdouble, Cdouble, Ref{Cdouble}, Ref{PICtrl}, Cdouble, Cdouble, Cint,
+                Int64, Ref{Counts}),
+               c.ptr, t0, tend, dt, Ref(ctrl), abstol, reltol, clamp_nonneg, Int64(maxiters), cnt)
+    check(c, rc)
+    return get_state(c), cnt[].t_end, dt[], Float64(cnt[].naccept), Float64(cnt[].nreject)
+end
+
+# ---- Lie-Trotter driver (op_split.jl:100-164) --------------------------------
+function lie_trotter_solve(c::Context, u0, tspan; macro_dt::Real,
+                           reltols = (1e-3, 1e-3), abstols = (1e-6, 1e-6),
+                           clamp_nonneg::Bool = true, dt0 = (15.0, 0.5), maxiters = 20000)
+    t0, tf = Float64(tspan[1]), Float64(tspan[2])
+    stops = sort!(unique!(vcat(collect((t0 + macro_dt):macro_dt:(tf - 1e-9)), [tf])))
+    opts = SplitOpts(pictrl_ssprk43(), pictrl_ros23(), abstols[1], reltols[1], abstols[2],
+                     reltols[2], clamp_nonneg, 0, maxiters)
+    set_state!(c, u0)
+    dtT = Ref(Float64(dt0[1])); dtC = Ref(Float64(dt0[2]))
+    naT = nrT = naC = nrC = 0; nmacro = 0; tcur = t0
+    for tnext in stops
+        tnext <= tcur + 1e-9 && continue
+        cT = Ref(Counts()); cC = Ref(Counts())
+        check(c, ccall((:rs_macro_step, LIB), Cint,
+                       (Ptr{Cvoid}, Cdouble, Cdouble, Ref{Cdouble}, Ref{Cdouble}, Ref{SplitOpts},
+                        Ref{Counts}, Ref{Counts}),
+                       c.ptr, tcur, tnext, dtT, dtC, Ref(opts), cT, cC))
+        naT += cT[].naccept; nrT += cT[].nreject; naC += cC[].naccept; nrC += cC[].nreject
+        tcur = tnext; nmacro += 1
+    end
+    return (; u = get_state(c), retcode = :Success, naT, nrT, naC, nrC, nmacro)
+end
+
+# ---- one-step adjoints (:610-623) ---------------------------------------------
+function ssprk43_step_vjp(c::Context, u, t, dt, lambda)
+    set_state!(c, u)
+    check(c, ccall((:rs_set_lambda, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, lambda))
+    check(c, ccall((:rs_ssprk43_vjp, LIB), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), c.ptr, t, dt))
+    lam = similar(lambda)
+    check(c, ccall((:rs_get_lambda, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, lam))
+    return lam, nothing
+end
+
+function ros23_step_vjp(c::Context, u, t, dt, lambda)
+    set_state!(c, u)
+    check(c, ccall((:rs_set_lambda, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, lambda))
+    check(c, ccall((:rs_zero_grad, LIB), Cint, (Ptr{Cvoid},), c.ptr))
+    check(c, ccall((:rs_ros23_vjp, LIB), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), c.ptr, t, dt))
+    lam = similar(lambda)
+    gE = Vector{Float64}(undef, c.ns * c.nlon * c.nlat)
+    check(c, ccall((:rs_get_lambda, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, lam))
+    check(c, ccall((:rs_get_grad_emis, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, gE))
+    return lam, gE
+end
+
+# ---- the split loop as a differentiable function -----------------------------
+# J = sum over surface cells of u[species](t_end)^2 after the macro steps at
+# `tstops`, as a function of the initial state u0 and the surface emissions E.
+function split_loss(c::Context, u0::Vector{Float64}, E::Vector{Float64}, tstops::Vector{Float64},
+                    opts::SplitOpts; species::Int = 0, dt0 = (15.0, 0.5))
+    set_emissions!(c, E)
+    set_state!(c, u0)
+    dtT, dtC = Float64(dt0[1]), Float64(dt0[2])
+    for m in 1:(length(tstops) - 1)
+        a = Ref(dtT); b = Ref(dtC)
+        check(c, ccall((:rs_macro_step, LIB), Cint,
+                       (Ptr{Cvoid}, Cdouble, Cdouble, Ref{Cdouble}, Ref{Cdouble}, Ref{SplitOpts},
+                        Ref{Counts}, Ref{Counts}),
+                       c.ptr, tstops[m], tstops[m + 1], a, b, Ref(opts), Ref(Counts()), Ref(Counts())))
+        dtT, dtC = a[], b[]
+    end
+    u = get_state(c)
+    ncol = c.nlon * c.nlat
+    off = species * ncells(c)
+    return sum(abs2, view(u, (off + 1):(off + ncol)))
+end
+
+function ChainRulesCore.rrule(::typeof(split_loss), c::Context, u0::Vector{Float64},
+                              E::Vector{Float64}, tstops::Vector{Float64}, opts::SplitOpts;
+                              species::Int = 0, dt0 = (15.0, 0.5))
+    set_emissions!(c, E)
+    set_state!(c, u0)
+    J = Ref(0.0)
+    gu = similar(u0)
+    gE = similar(E)
+    st = Ref(AdjointStats())
+    check(c, ccall((:rs_adjoint_run, LIB), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{Float64}, Cdouble, Cdouble, Ref{SplitOpts}, Cint, Cint,
+                    Ref{Cdouble}, Ptr{Float64}, Ptr{Float64}, Ref{AdjointStats}),
+                   c.ptr, length(tstops) - 1, tstops, dt0[1], dt0[2], Ref(opts), 0, species, J,
+                   gu, gE, st))
+    pullback(Jbar) = (NoTangent(), NoTangent(), Jbar .* gu, Jbar .* gE, NoTangent(), NoTangent())
+    return J[], pullback
+end
+
+end # module
