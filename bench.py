@@ -73,7 +73,10 @@ def coupled(lib, steps: int, warmup: int):
     ctx.profile(False)
     fwd_s = t1 - t0
     log(f"coupled forward {steps} steps {fwd_s:.3f} s; T {nT}/{rT} C {nC}/{rC}; kernels {prof}")
-    # adjoint over the same steps
+    # adjoint over the same steps; one untimed run first sizes the checkpoint
+    # store and the replay tape (device allocations kept by the context)
+    ctx.set_state(p.u0)
+    ctx.adjoint_run(p.tstops()[:steps + 1], p.dt0_T, p.dt0_C, opts)
     ctx.set_state(p.u0)
     J, gu, ge, st = ctx.adjoint_run(p.tstops()[:steps + 1], p.dt0_T, p.dt0_C, opts)
     ratio = (st.seconds_forward + st.seconds_backward) / fwd_s

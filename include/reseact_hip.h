@@ -200,9 +200,10 @@ RS_API int rs_replay_loss(rs_ctx* ctx, int loss_kind, int loss_species, double* 
 
 /* ---- extensions with no counterpart in the reference ----------------------
  * Elimination order of the per-cell sparse LU, set before rs_set_mechanism:
- * 0 = reference mode: natural species order, true division and no FMA
- *     contraction, i.e. the reference blocksolve's operations one for one;
- * 1 = Markowitz fill-reducing order with FMA contraction (default; fewer flops). */
+ * 0 = natural species order, the elimination order of the reference's
+ *     blocksolve (rounding differs only by FMA contraction and reciprocal
+ *     multiplication);
+ * 1 = Markowitz fill-reducing order (default; fewer flops). */
 RS_API int rs_set_lu_ordering(rs_ctx* ctx, int ordering);
 
 /* Static work of the compiled mechanism kernels (roofline accounting). */

@@ -816,8 +816,10 @@ int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const doub
         auto cm = std::make_unique<ChemModule>();
         cm->plan = rs::plan_mechanism(m, c->lu_order);
         std::string log;
-        // exact arithmetic follows the ordering unless RS_JIT_EXACT=0/1 overrides it
-        bool exact = c->lu_order == 0;
+        // Exact arithmetic (no contraction, true division) is opt-in with
+        // RS_JIT_EXACT=1: on gfx950 that build has returned wrong values for
+        // some lanes on repeated launches, so it is experimental and off.
+        bool exact = false;
         if (const char* ev = std::getenv("RS_JIT_EXACT")) exact = (ev[0] == '1');
         std::vector<char> code = jit_compile(chem_source(m, cm->plan), exact, log);
         if (code.empty()) return fail(c, RS_E_JIT, "hipRTC compile failed: " + log.substr(0, 2000));
@@ -1053,6 +1055,11 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
             S.naccept_C += cC.naccept; S.nreject_C += cC.nreject;
         }
         d2d(c->ckpt.p + (size_t)nmacro * N, c->S[c->cur].p, N, c->stream);
+        // size the replay tape once for the longest macro step
+        size_t maxslots = 1;
+        for (int m = 0; m < nmacro; ++m)
+            maxslots = std::max(maxslots, c->seqT[m].size() + c->seqC[m].size() + 1);
+        c->tape.ensure(maxslots * N);
         const double J = loss_on_device(c, c->S[c->cur].p, loss_species);
         if (J_out) *J_out = J;
         HIPCHK(hipStreamSynchronize(c->stream));
