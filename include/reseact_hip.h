@@ -225,6 +225,44 @@ typedef struct rs_kernel_time {
     double total_ms;
 } rs_kernel_time;
 RS_API int rs_profile(rs_ctx* ctx, int enable);
+
+/* Autotuner: times one SSPRK43 attempt for every transport tile height
+ * (4, 8, 16, 32 levels per 64-longitude workgroup) and one ROS23 attempt for
+ * every chemistry workgroup size (64, 128, 256) on this context's own grid
+ * and state, and keeps the fastest of each.  The state is not changed. */
+typedef struct rs_tune_result {
+    int transport_tile_levels;
+    int chem_block;
+    double transport_ms[4];      /* per candidate, 0 if not timed */
+    double chem_ms[3];
+} rs_tune_result;
+RS_API int rs_autotune(rs_ctx* ctx, rs_tune_result* out);
+
+/* Ensembles: N independent contexts (regional domains of different sizes,
+ * each fully set up) advanced together.  Forward: every adaptive phase of
+ * every member runs from one instantiated HIP graph (one branch per member
+ * stream) launched until all controllers are done.  Backward: members are
+ * grouped so their replay tapes fit `tape_budget_gib`, and each group's whole
+ * backward sweep (replays + VJPs over all split steps) is one graph launch.
+ * Each member's gradients stay in its context (rs_get_lambda = dJ/du0,
+ * rs_get_grad_emis).  No counterpart in the reference, which runs one domain
+ * per process. */
+typedef struct rs_ensemble rs_ensemble;
+typedef struct rs_ensemble_stats {
+    double seconds_forward, seconds_backward;
+    long long graph_launches_forward, graph_launches_backward;
+    long long graph_nodes_backward;
+    int backward_groups;
+} rs_ensemble_stats;
+RS_API int rs_ensemble_create(rs_ensemble** out, rs_ctx* const* members, int n);
+RS_API int rs_ensemble_destroy(rs_ensemble* e);
+RS_API const char* rs_ensemble_last_error(const rs_ensemble* e);
+RS_API int rs_ensemble_autotune(rs_ensemble* e, rs_tune_result* out);
+RS_API int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops,
+                                   double dtT0, double dtC0, const rs_split_opts* opts,
+                                   int loss_kind, int loss_species, double tape_budget_gib,
+                                   double* J_out, rs_adjoint_stats* stats,
+                                   rs_ensemble_stats* es);
 RS_API int rs_profile_read(rs_ctx* ctx, rs_kernel_time* out, int max, int* n_out);
 
 #ifdef __cplusplus

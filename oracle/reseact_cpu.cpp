@@ -1173,3 +1173,64 @@ int rs_oracle_threads(void) {
 }
 
 }  // extern "C"
+
+// ---- autotuner and ensembles: the oracle has nothing to tune and runs the
+// members one after the other (the CPU baseline of the ensemble workload) ----
+struct rs_ensemble {
+    std::vector<rs_ctx*> m;
+    std::string err;
+};
+
+extern "C" {
+
+int rs_autotune(rs_ctx* c, rs_tune_result* out) {
+    if (!c) return RS_E_ARG;
+    if (out) { *out = rs_tune_result{}; out->transport_tile_levels = 0; out->chem_block = 1; }
+    return RS_OK;
+}
+
+int rs_ensemble_create(rs_ensemble** out, rs_ctx* const* members, int n) {
+    if (!out || !members || n < 1) return RS_E_ARG;
+    auto* e = new rs_ensemble();
+    for (int d = 0; d < n; ++d) {
+        if (!members[d]) { delete e; return RS_E_ARG; }
+        e->m.push_back(members[d]);
+    }
+    *out = e;
+    return RS_OK;
+}
+
+int rs_ensemble_destroy(rs_ensemble* e) { delete e; return RS_OK; }
+
+const char* rs_ensemble_last_error(const rs_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int rs_ensemble_autotune(rs_ensemble* e, rs_tune_result* out) {
+    if (!e) return RS_E_ARG;
+    for (size_t d = 0; d < e->m.size(); ++d) rs_autotune(e->m[d], out ? out + d : nullptr);
+    return RS_OK;
+}
+
+int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, double dtT0,
+                            double dtC0, const rs_split_opts* o, int loss_kind, int loss_species,
+                            double tape_budget_gib, double* J_out, rs_adjoint_stats* stats,
+                            rs_ensemble_stats* es) {
+    (void)tape_budget_gib;
+    if (!e) return RS_E_ARG;
+    rs_ensemble_stats ES{};
+    for (size_t d = 0; d < e->m.size(); ++d) {
+        rs_adjoint_stats st{};
+        double J = 0.0;
+        int rc = rs_adjoint_run(e->m[d], nmacro, tstops, dtT0, dtC0, o, loss_kind, loss_species,
+                                &J, nullptr, nullptr, &st);
+        if (rc) { e->err = "member " + std::to_string(d) + ": " + e->m[d]->err; return rc; }
+        if (J_out) J_out[d] = J;
+        if (stats) stats[d] = st;
+        ES.seconds_forward += st.seconds_forward;
+        ES.seconds_backward += st.seconds_backward;
+    }
+    ES.backward_groups = (int)e->m.size();
+    if (es) *es = ES;
+    return RS_OK;
+}
+
+}  // extern "C"

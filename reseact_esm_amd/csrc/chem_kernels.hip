@@ -148,7 +148,7 @@ RS_FN int rs_pi_update(double beta1, double beta2, double invqmin, double invqma
 
 #ifndef RS_HOST_BUILD
 
-#define RS_CHEM_BLOCK 64
+#define RS_CHEM_BLOCK 256   /* largest block the launcher uses (64, 128 or 256) */
 
 __device__ inline double rs_block_sum(double v, double* sh) {
     /* deterministic: fixed shuffle tree inside the wave, then wave order */

@@ -41,8 +41,8 @@
 
 // launch wrappers from transport.hip
 extern "C" {
-hipError_t rs_launch_transport(int kind, const RsTransArgs* a, hipStream_t st);
-long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt);
+hipError_t rs_launch_transport(int kind, int tk, const RsTransArgs* a, hipStream_t st);
+long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt, int tk);
 hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog, hipStream_t st);
 hipError_t rs_launch_fill(double* x, long long n, double v, hipStream_t st);
 hipError_t rs_launch_fill_u64(unsigned long long* x, int n, unsigned long long v, hipStream_t st);
@@ -136,6 +136,8 @@ struct rs_ctx {
     bool have_wind = false, have_mech = false, have_state = false, have_cand = false;
     int cur = 0;
     int lu_order = 1;
+    int tk = 16;                 // transport tile height (levels per workgroup)
+    unsigned chem_block = 64;    // chemistry threads per workgroup
     DevArray<double> S[2], X, Y, cx, wz, wl, K, jfac, E, lam, LA, LB, LC, gE, partial, seqlog;
     DevArray<double> red;
     DevArray<unsigned long long> head;
@@ -152,7 +154,7 @@ struct rs_ctx {
     std::vector<std::vector<std::pair<double, double>>> seqT, seqC;
     rs_split_opts last_opts{};
     bool have_record = false;
-    DevArray<double> ckpt, tape;
+    DevArray<double> ckpt, tape, relpart;
     // profiling
     bool prof = false;
     struct EvPair { hipEvent_t a, b; int kind; };
@@ -338,7 +340,7 @@ RsTransArgs trans_args(rs_ctx* c) {
     return a;
 }
 
-long long trans_nblocks(rs_ctx* c) { return rs_transport_nblocks(c->nlon, c->nlat, c->nlev, c->nt); }
+long long trans_nblocks(rs_ctx* c) { return rs_transport_nblocks(c->nlon, c->nlat, c->nlev, c->nt, c->tk); }
 
 // One SSPRK43 attempt: either from the state buffers under a controller (ctrl
 // != null: dt/cur/done from the device), or between explicit buffers.
@@ -357,21 +359,21 @@ void enqueue_ssprk43(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* u
     a.out_is_state = 0;
     a.out = c->X.p;
     a.clamp = 0;
-    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_transport(KIND_STAGE, &a, c->stream)); }
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_transport(KIND_STAGE, c->tk, &a, c->stream)); }
     a.a_is_state = 0;
     a.a = c->X.p;
     a.out = c->Y.p;
-    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_transport(KIND_STAGE, &a, c->stream)); }
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_transport(KIND_STAGE, c->tk, &a, c->stream)); }
     a.a = c->Y.p;
     a.out = c->X.p;
-    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_transport(KIND_STAGE, &a, c->stream)); }
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_transport(KIND_STAGE, c->tk, &a, c->stream)); }
     a.a_is_state = st ? 1 : 0;
     a.a = uin;
     a.b = c->X.p;
     a.out_is_state = st ? 1 : 0;
     a.out = uout;
     a.clamp = clamp;
-    { ProfScope ps(c, P_TLAST); HIPCHK(rs_launch_transport(KIND_LAST, &a, c->stream)); }
+    { ProfScope ps(c, P_TLAST); HIPCHK(rs_launch_transport(KIND_LAST, c->tk, &a, c->stream)); }
 }
 
 void chem_launch(rs_ctx* c, hipFunction_t f, void* argp, size_t argsz, unsigned grid,
@@ -382,9 +384,8 @@ void chem_launch(rs_ctx* c, hipFunction_t f, void* argp, size_t argsz, unsigned 
     HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, c->stream, nullptr, cfg));
 }
 
-const unsigned CHEM_BLOCK = 64;
 
-unsigned chem_grid(rs_ctx* c) { return (unsigned)((c->nc + CHEM_BLOCK - 1) / CHEM_BLOCK); }
+unsigned chem_grid(rs_ctx* c) { return (unsigned)((c->nc + c->chem_block - 1) / c->chem_block); }
 
 void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cur_static,
                    double dt, double atol, double rtol, int clamp) {
@@ -403,7 +404,7 @@ void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cu
     a.rtol = rtol;
     a.cur_static = cur_static;
     a.clamp = clamp;
-    chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), CHEM_BLOCK, P_CHEM);
+    chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), c->chem_block, P_CHEM);
 }
 
 // Sum of n device partials: folded on the device to at most 1024 values first.
@@ -433,14 +434,28 @@ double host_max(rs_ctx* c, const double* dpart, int n) {
 enum class Op { T, C };
 
 // Global-norm adaptive solve on the state buffers (rx_traced_integrator.jl:455-507).
-int run_adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs_pictrl& pc,
-                 double atol, double rtol, int clamp, long long maxiters, rs_counts* out,
-                 std::vector<std::pair<double, double>>* seq) {
+// The global-norm adaptive solve (rx_traced_integrator.jl:455-507) in three
+// pieces so that one context or a graph-captured ensemble can drive it:
+// adaptive_begin fills and uploads the controller block, adaptive_chunk
+// enqueues `n` attempts + controller updates, adaptive_finish reads back the
+// outcome.  Attempts enqueued after the controller set `done` exit at once.
+struct AdaptiveRun {
+    Op op;
+    double atol, rtol;
+    int clamp;
+    long long maxiters;
+    int start_cur;
+    bool log;
+};
+
+AdaptiveRun adaptive_begin(rs_ctx* c, Op op, double t0, double t1, double dt0, const rs_pictrl& pc,
+                           double atol, double rtol, int clamp, long long maxiters, bool log,
+                           hipStream_t st) {
     RsCtrl h{};
     h.t = t0;
     h.t1 = t1;
     h.tlim = t1 - 1.0e-9;
-    h.dt = *dt_io;
+    h.dt = dt0;
     h.qold = pc.qoldinit;
     h.nnorm = (double)c->ns * (double)c->nc;
     h.beta1 = pc.beta1;
@@ -455,32 +470,34 @@ int run_adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs
     h.cur = c->cur;
     h.done = (!(t0 < h.tlim) || maxiters <= 0) ? 1 : 0;
     h.dtc = std::min(h.dt, t1 - t0);
-    long long cap = seq ? std::min<long long>(maxiters, 1 << 22) : 0;
-    if (seq) c->seqlog.ensure((size_t)(2 * std::max<long long>(cap, 1)));
+    long long cap = log ? std::min<long long>(maxiters, 1 << 22) : 0;
+    if (log) c->seqlog.ensure((size_t)(2 * std::max<long long>(cap, 1)));
     h.logcap = (int)cap;
-    const int start_cur = c->cur;
     *c->hctrl = h;
-    HIPCHK(hipMemcpyAsync(c->dctrl, c->hctrl, sizeof(RsCtrl), hipMemcpyHostToDevice, c->stream));
-    const int npart = (op == Op::T) ? (int)trans_nblocks(c) : (int)chem_grid(c);
-    double* logp = seq ? c->seqlog.p : nullptr;
-    int chunk = 2;
-    while (!c->hctrl->done) {
-        for (int q = 0; q < chunk; ++q) {
-            if (op == Op::T) enqueue_ssprk43(c, c->dctrl, nullptr, nullptr, 0.0, atol, rtol, clamp);
-            else enqueue_ros23(c, c->dctrl, c->S[0].p, c->S[1].p, 0, 0.0, atol, rtol, clamp);
-            ProfScope ps(c, P_CTRL);
-            HIPCHK(rs_launch_ctrl(c->dctrl, c->partial.p, npart, logp, c->stream));
-        }
-        HIPCHK(hipMemcpyAsync(c->hctrl, c->dctrl, sizeof(RsCtrl), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        chunk = std::min(chunk * 2, 64);
+    HIPCHK(hipMemcpyAsync(c->dctrl, c->hctrl, sizeof(RsCtrl), hipMemcpyHostToDevice, st));
+    return AdaptiveRun{op, atol, rtol, clamp, maxiters, c->cur, log};
+}
+
+void adaptive_chunk(rs_ctx* c, const AdaptiveRun& ar, int n) {
+    const int npart = (ar.op == Op::T) ? (int)trans_nblocks(c) : (int)chem_grid(c);
+    double* logp = ar.log ? c->seqlog.p : nullptr;
+    for (int q = 0; q < n; ++q) {
+        if (ar.op == Op::T) enqueue_ssprk43(c, c->dctrl, nullptr, nullptr, 0.0, ar.atol, ar.rtol, ar.clamp);
+        else enqueue_ros23(c, c->dctrl, c->S[0].p, c->S[1].p, 0, 0.0, ar.atol, ar.rtol, ar.clamp);
+        ProfScope ps(c, P_CTRL);
+        HIPCHK(rs_launch_ctrl(c->dctrl, c->partial.p, npart, logp, c->stream));
     }
+}
+
+// Requires c->hctrl to hold the final controller block (read back by the caller).
+int adaptive_finish(rs_ctx* c, const AdaptiveRun& ar, double* dt_io, rs_counts* out,
+                    std::vector<std::pair<double, double>>* seq) {
     const RsCtrl r = *c->hctrl;
     c->cur = r.cur;
-    if (op == Op::T && c->nt < c->ns && c->cur != start_cur) {
+    if (ar.op == Op::T && c->nt < c->ns && c->cur != ar.start_cur) {
         // species nt..ns-1 are untouched by transport: carry them over
         const size_t off = (size_t)c->nt * c->nc, n = (size_t)(c->ns - c->nt) * c->nc;
-        d2d(c->S[c->cur].p + off, c->S[start_cur].p + off, n, c->stream);
+        d2d(c->S[c->cur].p + off, c->S[ar.start_cur].p + off, n, c->stream);
     }
     if (seq) {
         seq->clear();
@@ -498,9 +515,24 @@ int run_adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs
         out->t_end = r.t;
         out->dt_end = r.dt;
     }
-    if (r.iters >= maxiters && r.t < r.tlim)
+    if (r.iters >= ar.maxiters && r.t < r.tlim)
         return fail(c, RS_E_MAXITERS, "adaptive loop hit maxiters at t=" + std::to_string(r.t));
     return RS_OK;
+}
+
+int run_adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs_pictrl& pc,
+                 double atol, double rtol, int clamp, long long maxiters, rs_counts* out,
+                 std::vector<std::pair<double, double>>* seq) {
+    const AdaptiveRun ar = adaptive_begin(c, op, t0, t1, *dt_io, pc, atol, rtol, clamp, maxiters,
+                                          seq != nullptr, c->stream);
+    int chunk = 2;
+    while (!c->hctrl->done) {
+        adaptive_chunk(c, ar, chunk);
+        HIPCHK(hipMemcpyAsync(c->hctrl, c->dctrl, sizeof(RsCtrl), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        chunk = std::min(chunk * 2, 64);
+    }
+    return adaptive_finish(c, ar, dt_io, out, seq);
 }
 
 int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_host,
@@ -545,12 +577,12 @@ int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_ho
     HIPCHK(rs_launch_fill_u64(c->head.p, 1, 0ULL, c->stream));
     int per_cu = 0;
     HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c->chem->percell,
-                                                              (int)CHEM_BLOCK, 0));
+                                                              (int)c->chem_block, 0));
     per_cu = std::max(per_cu, 1);
     long long grid = (long long)per_cu * c->ncu;
-    grid = std::min<long long>(grid, (c->nc + CHEM_BLOCK - 1) / CHEM_BLOCK);
+    grid = std::min<long long>(grid, (c->nc + c->chem_block - 1) / c->chem_block);
     grid = std::max<long long>(grid, 1);
-    chem_launch(c, c->chem->percell, &a, sizeof(a), (unsigned)grid, CHEM_BLOCK, P_PERCELL);
+    chem_launch(c, c->chem->percell, &a, sizeof(a), (unsigned)grid, c->chem_block, P_PERCELL);
     HIPCHK(hipMemcpyAsync(c->hstats, c->stats.p, 8 * sizeof(long long), hipMemcpyDeviceToHost,
                           c->stream));
     if (dt_cells_host) d2h(dt_cells_host, c->dtcell.p, (size_t)c->nc, c->stream);
@@ -605,17 +637,17 @@ void enqueue_ssprk43_vjp(rs_ctx* c, double dt) {
     a.out_is_state = 0;
     a.a = c->lam.p;
     a.out = c->LA.p;
-    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_A, &a, c->stream)); }
+    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_A, c->tk, &a, c->stream)); }
     a.a = c->LA.p;
     a.out = c->LB.p;
-    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_B, &a, c->stream)); }
+    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_B, c->tk, &a, c->stream)); }
     a.a = c->LB.p;
     a.out = c->LC.p;
-    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_C, &a, c->stream)); }
+    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_C, c->tk, &a, c->stream)); }
     a.a = c->LC.p;
     a.b = c->LA.p;
     a.out = c->lam.p;
-    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_D, &a, c->stream)); }
+    { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_D, c->tk, &a, c->stream)); }
 }
 
 void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt) {
@@ -630,7 +662,7 @@ void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt
     a.nc = c->nc;
     a.ncol = c->ncol;
     a.dt = dt;
-    chem_launch(c, c->chem->vjp, &a, sizeof(a), chem_grid(c), CHEM_BLOCK, P_CHEMVJP);
+    chem_launch(c, c->chem->vjp, &a, sizeof(a), chem_grid(c), c->chem_block, P_CHEMVJP);
 }
 
 double loss_on_device(rs_ctx* c, const double* u, int species) {
@@ -668,6 +700,72 @@ size_t replay_into_tape(rs_ctx* c, const double* u0, const std::vector<std::pair
         ++j;
     }
     return nslots;
+}
+
+// Size the replay tape once for the longest recorded macro step (before any
+// graph capture: no allocation may happen while a stream is captured).
+void size_tape(rs_ctx* c, int nmacro) {
+    size_t maxslots = 1;
+    for (int m = 0; m < nmacro; ++m)
+        maxslots = std::max(maxslots, c->seqT[m].size() + c->seqC[m].size() + 1);
+    c->tape.ensure(maxslots * c->N());
+}
+
+size_t tape_bytes(rs_ctx* c, int nmacro) {
+    size_t maxslots = 1;
+    for (int m = 0; m < nmacro; ++m)
+        maxslots = std::max(maxslots, c->seqT[m].size() + c->seqC[m].size() + 1);
+    return maxslots * c->N() * sizeof(double);
+}
+
+// lambda = dJ/du(t_end) for J = sum over level-0 cells of u[species]^2, from the
+// final checkpoint; emission gradient cleared.
+void backward_seed(rs_ctx* c, int species) {
+    const size_t N = c->N();
+    const int nm = (int)c->seqT.size();
+    HIPCHK(rs_launch_loss_seed(c->lam.p, c->ckpt.p + (size_t)nm * N, (long long)N,
+                               (long long)species * c->nc, c->ncol, c->stream));
+    HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
+}
+
+// Backward work of macro step m, enqueued on c->stream without host
+// synchronisation (so it can be captured into a graph): fixed-sequence replay
+// from checkpoint m into the tape (replay_fixed, adjoint_gradient.jl:647-661),
+// the replay's deviation from checkpoint m+1 into `relpart` (:802-808), then
+// the chemistry VJPs, last step first, and the transport VJPs
+// (backward_stage!, :731-784; order :821-822), each after the clamp mask of
+// its own step.  Optional events bracket the replay.
+void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* relpart,
+                            hipEvent_t ev0, hipEvent_t ev1) {
+    const size_t N = c->N();
+    const auto& sT = c->seqT[m];
+    const auto& sC = c->seqC[m];
+    if (ev0) HIPCHK(hipEventRecord(ev0, c->stream));
+    const size_t ns = replay_into_tape(c, c->ckpt.p + (size_t)m * N, sT, sC, o);
+    double* tp = c->tape.p;
+    HIPCHK(rs_launch_relmax(tp + (ns - 1) * N, c->ckpt.p + (size_t)(m + 1) * N, (long long)N,
+                            relpart, c->stream));
+    if (ev1) HIPCHK(hipEventRecord(ev1, c->stream));
+    const size_t nT = sT.size();
+    for (size_t q = sC.size(); q-- > 0;) {
+        const size_t slot = nT + q;
+        enqueue_ros23_vjp(c, tp + slot * N, o.clamp_nonneg ? tp + (slot + 1) * N : nullptr,
+                          sC[q].second);
+    }
+    for (size_t q = nT; q-- > 0;) {
+        if (o.clamp_nonneg) {
+            ProfScope ps(c, P_UTIL);
+            HIPCHK(rs_launch_mask(c->lam.p, tp + (q + 1) * N, (long long)N, c->stream));
+        }
+        enqueue_ssprk43_vjp(c, sT[q].second);
+    }
+}
+
+// max over `nsets` groups of `n` device partials (each group <= 1024)
+double host_max_many(rs_ctx* c, const double* dpart, int nsets, int n) {
+    double mx = 0.0;
+    for (int s = 0; s < nsets; ++s) mx = std::max(mx, host_max(c, dpart + (size_t)s * 1024, n));
+    return mx;
 }
 
 }  // namespace
@@ -758,8 +856,8 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
         c->lam.resize(N);
         c->LA.resize(NT); c->LB.resize(NT); c->LC.resize(NT);
         c->gE.resize((size_t)ns * c->ncol);
-        const long long np = std::max<long long>(rs_transport_nblocks(nlon, nlat, nlev, std::max(nt, 1)),
-                                                 (c->nc + CHEM_BLOCK - 1) / CHEM_BLOCK);
+        const long long np = std::max<long long>(rs_transport_nblocks(nlon, nlat, nlev, std::max(nt, 1), 4),
+                                                 (c->nc + 63) / 64);
         c->partial.resize((size_t)np);
         c->red.resize(1024);
         HIPCHK(rs_launch_fill(c->lam.p, (long long)N, 0.0, c->stream));
@@ -1055,11 +1153,7 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
             S.naccept_C += cC.naccept; S.nreject_C += cC.nreject;
         }
         d2d(c->ckpt.p + (size_t)nmacro * N, c->S[c->cur].p, N, c->stream);
-        // size the replay tape once for the longest macro step
-        size_t maxslots = 1;
-        for (int m = 0; m < nmacro; ++m)
-            maxslots = std::max(maxslots, c->seqT[m].size() + c->seqC[m].size() + 1);
-        c->tape.ensure(maxslots * N);
+        size_tape(c, nmacro);
         const double J = loss_on_device(c, c->S[c->cur].p, loss_species);
         if (J_out) *J_out = J;
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1068,42 +1162,27 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         c->have_record = true;
 
         auto tb0 = clk::now();
-        double trep = 0.0;
-        HIPCHK(rs_launch_loss_seed(c->lam.p, c->S[c->cur].p, (long long)N,
-                                   (long long)loss_species * c->nc, c->ncol, c->stream));
-        HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
-        const int nbr = rs_reduce_blocks((long long)N);
-        for (int m = nmacro - 1; m >= 0; --m) {
-            auto tr0 = clk::now();
-            const auto& sT = c->seqT[m];
-            const auto& sC = c->seqC[m];
-            const size_t ns = replay_into_tape(c, c->ckpt.p + (size_t)m * N, sT, sC, *o);
-            double* tp = c->tape.p;
-            HIPCHK(rs_launch_relmax(tp + (ns - 1) * N, c->ckpt.p + (size_t)(m + 1) * N, (long long)N,
-                                    c->red.p, c->stream));
-            S.replay_maxrel = std::max(S.replay_maxrel, host_max(c, c->red.p, nbr));
-            trep += std::chrono::duration<double>(clk::now() - tr0).count();
-            // chemistry VJPs, last step first (adjoint_gradient.jl:821)
-            const size_t nT = sT.size();
-            for (size_t q = sC.size(); q-- > 0;) {
-                const size_t slot = nT + q;
-                enqueue_ros23_vjp(c, tp + slot * N, o->clamp_nonneg ? tp + (slot + 1) * N : nullptr,
-                                  sC[q].second);
-                S.nvjp_C += 1;
-            }
-            // transport VJPs (:822)
-            for (size_t q = nT; q-- > 0;) {
-                if (o->clamp_nonneg) {
-                    ProfScope ps(c, P_UTIL);
-                    HIPCHK(rs_launch_mask(c->lam.p, tp + (q + 1) * N, (long long)N, c->stream));
-                }
-                enqueue_ssprk43_vjp(c, sT[q].second);
-                S.nvjp_T += 1;
-            }
-        }
+        backward_seed(c, loss_species);
+        c->relpart.ensure((size_t)nmacro * 1024);
+        std::vector<hipEvent_t> ev(2 * (size_t)nmacro);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        for (int m = nmacro - 1; m >= 0; --m)
+            enqueue_backward_macro(c, m, *o, c->relpart.p + (size_t)m * 1024, ev[2 * m], ev[2 * m + 1]);
         HIPCHK(hipStreamSynchronize(c->stream));
         S.seconds_backward = std::chrono::duration<double>(clk::now() - tb0).count();
+        double trep = 0.0;
+        for (int m = 0; m < nmacro; ++m) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev[2 * m], ev[2 * m + 1]));
+            trep += ms * 1e-3;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
         S.seconds_replay = trep;
+        S.replay_maxrel = host_max_many(c, c->relpart.p, nmacro, rs_reduce_blocks((long long)N));
+        for (int m = 0; m < nmacro; ++m) {
+            S.nvjp_T += (long long)c->seqT[m].size();
+            S.nvjp_C += (long long)c->seqC[m].size();
+        }
         if (grad_u0) d2h(grad_u0, c->lam.p, N, c->stream);
         if (grad_emis) d2h(grad_emis, c->gE.p, c->gE.n, c->stream);
         if (st) *st = S;
@@ -1193,6 +1272,328 @@ int rs_profile_read(rs_ctx* c, rs_kernel_time* out, int max, int* n_out) {
         if (n_out) *n_out = n;
         return RS_OK;
     });
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Autotuner and ensembles
+// ============================================================================
+namespace {
+
+double time_ms(rs_ctx* c, const std::function<void()>& body, int reps) {
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    body();   // warm-up
+    HIPCHK(hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; ++r) body();
+    HIPCHK(hipEventRecord(b, c->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / reps;
+}
+
+int autotune(rs_ctx* c, rs_tune_result* out) {
+    rs_tune_result r{};
+    r.transport_tile_levels = c->tk;
+    r.chem_block = (int)c->chem_block;
+    const bool prof = c->prof;
+    c->prof = false;
+    double* in = c->S[c->cur].p;
+    double* cand = c->S[c->cur ^ 1].p;
+    if (c->nt > 0 && c->have_wind) {
+        const int tks[4] = {4, 8, 16, 32};
+        double best = 1e300;
+        for (int q = 0; q < 4; ++q) {
+            c->tk = tks[q];
+            r.transport_ms[q] = time_ms(c, [&]() {
+                enqueue_ssprk43(c, nullptr, in, cand, 60.0, 1e-6, 1e-3, 0);
+            }, 3);
+            if (r.transport_ms[q] < best) { best = r.transport_ms[q]; r.transport_tile_levels = tks[q]; }
+        }
+        c->tk = r.transport_tile_levels;
+    }
+    if (c->have_mech) {
+        const unsigned bl[3] = {64, 128, 256};
+        double best = 1e300;
+        for (int q = 0; q < 3; ++q) {
+            c->chem_block = bl[q];
+            r.chem_ms[q] = time_ms(c, [&]() {
+                enqueue_ros23(c, nullptr, c->S[0].p, c->S[1].p, c->cur, 1.0, 1e-6, 1e-3, 0);
+            }, 3);
+            if (r.chem_ms[q] < best) { best = r.chem_ms[q]; r.chem_block = (int)bl[q]; }
+        }
+        c->chem_block = (unsigned)r.chem_block;
+    }
+    c->prof = prof;
+    c->have_cand = false;
+    if (out) *out = r;
+    return RS_OK;
+}
+
+}  // namespace
+
+struct rs_ensemble {
+    std::vector<rs_ctx*> m;
+    int device = 0;
+    hipStream_t origin = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    std::string err;
+};
+
+namespace {
+
+int ens_fail(rs_ensemble* e, int code, const std::string& msg) {
+    if (e) e->err = msg;
+    return code;
+}
+
+// Captures body(d) for every member d in `idx` into one graph: the origin
+// stream forks to each member's stream and joins them again.
+hipGraphExec_t ens_capture(rs_ensemble* e, const std::vector<int>& idx,
+                           const std::function<void(int)>& body, size_t* nnodes) {
+    HIPCHK(hipStreamBeginCapture(e->origin, hipStreamCaptureModeGlobal));
+    hipGraph_t g = nullptr;
+    try {
+        HIPCHK(hipEventRecord(e->fork, e->origin));
+        for (int d : idx) {
+            HIPCHK(hipStreamWaitEvent(e->m[d]->stream, e->fork, 0));
+            body(d);
+            HIPCHK(hipEventRecord(e->join[d], e->m[d]->stream));
+            HIPCHK(hipStreamWaitEvent(e->origin, e->join[d], 0));
+        }
+    } catch (...) {
+        (void)hipStreamEndCapture(e->origin, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(e->origin, &g));
+    if (nnodes) {
+        size_t n = 0;
+        HIPCHK(hipGraphGetNodes(g, nullptr, &n));
+        *nnodes = n;
+    }
+    hipGraphExec_t x = nullptr;
+    HIPCHK(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    return x;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rs_autotune(rs_ctx* c, rs_tune_result* out) {
+    int rc = check_ready(c, false, false);
+    if (rc) return rc;
+    return guarded(c, [&]() { return autotune(c, out); });
+}
+
+int rs_ensemble_create(rs_ensemble** out, rs_ctx* const* members, int n) {
+    if (!out || !members || n < 1) return RS_E_ARG;
+    *out = nullptr;
+    auto* e = new rs_ensemble();
+    for (int d = 0; d < n; ++d) {
+        if (!members[d]) { delete e; return RS_E_ARG; }
+        if (d && members[d]->device != members[0]->device) { delete e; return RS_E_ARG; }
+        e->m.push_back(members[d]);
+    }
+    e->device = members[0]->device;
+    try {
+        HIPCHK(hipSetDevice(e->device));
+        HIPCHK(hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+        e->join.resize(n);
+        for (auto& j : e->join) HIPCHK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+    } catch (const HipError&) {
+        rs_ensemble_destroy(e);
+        return RS_E_HIP;
+    }
+    *out = e;
+    return RS_OK;
+}
+
+int rs_ensemble_destroy(rs_ensemble* e) {
+    if (!e) return RS_OK;
+    if (e->origin) (void)hipStreamSynchronize(e->origin);
+    for (auto& j : e->join) if (j) (void)hipEventDestroy(j);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->origin) (void)hipStreamDestroy(e->origin);
+    delete e;
+    return RS_OK;
+}
+
+const char* rs_ensemble_last_error(const rs_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int rs_ensemble_autotune(rs_ensemble* e, rs_tune_result* out) {
+    if (!e) return RS_E_ARG;
+    for (size_t d = 0; d < e->m.size(); ++d) {
+        int rc = rs_autotune(e->m[d], out ? out + d : nullptr);
+        if (rc) return ens_fail(e, rc, "member " + std::to_string(d) + ": " + e->m[d]->err);
+    }
+    return RS_OK;
+}
+
+int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, double dtT0,
+                            double dtC0, const rs_split_opts* o, int loss_kind, int loss_species,
+                            double tape_budget_gib, double* J_out, rs_adjoint_stats* stats,
+                            rs_ensemble_stats* es) {
+    if (!e || !o || !tstops || nmacro < 1) return ens_fail(e, RS_E_ARG, "bad arguments");
+    if (o->chem_percell) return ens_fail(e, RS_E_ARG, "adjoint needs the global-norm chemistry");
+    if (loss_kind != 0) return ens_fail(e, RS_E_ARG, "bad loss");
+    const int n = (int)e->m.size();
+    for (int d = 0; d < n; ++d) {
+        int rc = check_ready(e->m[d], true, true);
+        if (rc) return ens_fail(e, rc, "member " + std::to_string(d) + ": " + e->m[d]->err);
+        if (loss_species < 0 || loss_species >= e->m[d]->ns) return ens_fail(e, RS_E_ARG, "bad loss species");
+    }
+    using clk = std::chrono::steady_clock;
+    rs_ensemble_stats ES{};
+    std::vector<rs_adjoint_stats> S(n);
+    std::vector<int> profs(n);
+    try {
+        HIPCHK(hipSetDevice(e->device));
+        std::vector<int> all(n), withT;
+        for (int d = 0; d < n; ++d) {
+            rs_ctx* c = e->m[d];
+            all[d] = d;
+            if (c->nt > 0) withT.push_back(d);
+            profs[d] = c->prof;
+            c->prof = false;
+            c->have_cand = false;
+            const size_t N = c->N();
+            c->ckpt.ensure((size_t)(nmacro + 1) * N);
+            c->seqT.assign(nmacro, {});
+            c->seqC.assign(nmacro, {});
+            const long long cap = std::min<long long>(o->maxiters, 1 << 22);
+            c->seqlog.ensure((size_t)(2 * std::max<long long>(cap, 1)));
+            c->relpart.ensure((size_t)nmacro * 1024);
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        const int CH = 8;   // attempts per member per graph launch
+        AdaptiveRun proto_T{Op::T, o->atol_T, o->rtol_T, o->clamp_nonneg, o->maxiters, 0, true};
+        AdaptiveRun proto_C{Op::C, o->atol_C, o->rtol_C, o->clamp_nonneg, o->maxiters, 0, true};
+        hipGraphExec_t gT = withT.empty() ? nullptr
+                                          : ens_capture(e, withT, [&](int d) { adaptive_chunk(e->m[d], proto_T, CH); }, nullptr);
+        hipGraphExec_t gC = ens_capture(e, all, [&](int d) { adaptive_chunk(e->m[d], proto_C, CH); }, nullptr);
+        std::vector<double> dtT(n, dtT0), dtC(n, dtC0);
+        auto tf0 = clk::now();
+        auto phase = [&](Op op, hipGraphExec_t g, const std::vector<int>& idx, int mstep) {
+            std::vector<AdaptiveRun> ar(n);
+            for (int d : idx) {
+                rs_ctx* c = e->m[d];
+                const rs_pictrl& pc = (op == Op::T) ? o->ctrl_T : o->ctrl_C;
+                ar[d] = adaptive_begin(c, op, tstops[mstep], tstops[mstep + 1],
+                                       op == Op::T ? dtT[d] : dtC[d], pc,
+                                       op == Op::T ? o->atol_T : o->atol_C,
+                                       op == Op::T ? o->rtol_T : o->rtol_C, o->clamp_nonneg,
+                                       o->maxiters, true, e->origin);
+            }
+            for (;;) {
+                bool done = true;
+                for (int d : idx) done = done && e->m[d]->hctrl->done;
+                if (done) break;
+                HIPCHK(hipGraphLaunch(g, e->origin));
+                ES.graph_launches_forward += 1;
+                for (int d : idx)
+                    HIPCHK(hipMemcpyAsync(e->m[d]->hctrl, e->m[d]->dctrl, sizeof(RsCtrl),
+                                          hipMemcpyDeviceToHost, e->origin));
+                HIPCHK(hipStreamSynchronize(e->origin));
+            }
+            for (int d : idx) {
+                rs_ctx* c = e->m[d];
+                rs_counts cnt{};
+                auto* seq = (op == Op::T) ? &c->seqT[mstep] : &c->seqC[mstep];
+                int rc = adaptive_finish(c, ar[d], op == Op::T ? &dtT[d] : &dtC[d], &cnt, seq);
+                if (rc) throw ArgError{"member " + std::to_string(d) + ": " + c->err, rc};
+                if (op == Op::T) { S[d].naccept_T += cnt.naccept; S[d].nreject_T += cnt.nreject; }
+                else { S[d].naccept_C += cnt.naccept; S[d].nreject_C += cnt.nreject; }
+            }
+        };
+        for (int mstep = 0; mstep < nmacro; ++mstep) {
+            for (int d = 0; d < n; ++d) {
+                rs_ctx* c = e->m[d];
+                d2d(c->ckpt.p + (size_t)mstep * c->N(), c->S[c->cur].p, c->N(), c->stream);
+                HIPCHK(hipStreamSynchronize(c->stream));
+            }
+            if (gT) phase(Op::T, gT, withT, mstep);
+            phase(Op::C, gC, all, mstep);
+        }
+        for (int d = 0; d < n; ++d) {
+            rs_ctx* c = e->m[d];
+            d2d(c->ckpt.p + (size_t)nmacro * c->N(), c->S[c->cur].p, c->N(), c->stream);
+            const double J = loss_on_device(c, c->S[c->cur].p, loss_species);
+            if (J_out) J_out[d] = J;
+            c->last_opts = *o;
+            c->have_record = true;
+        }
+        ES.seconds_forward = std::chrono::duration<double>(clk::now() - tf0).count();
+        if (gT) (void)hipGraphExecDestroy(gT);
+        (void)hipGraphExecDestroy(gC);
+
+        // backward: groups of members whose tapes fit the budget
+        auto tb0 = clk::now();
+        for (int d = 0; d < n; ++d) {
+            backward_seed(e->m[d], loss_species);
+            HIPCHK(hipStreamSynchronize(e->m[d]->stream));
+        }
+        const double budget = (tape_budget_gib > 0 ? tape_budget_gib : 64.0) * 1073741824.0;
+        std::vector<std::vector<int>> groups;
+        double acc = 0.0;
+        for (int d = 0; d < n; ++d) {
+            const double b = (double)tape_bytes(e->m[d], nmacro);
+            if (groups.empty() || (acc + b > budget && !groups.back().empty())) {
+                groups.push_back({});
+                acc = 0.0;
+            }
+            groups.back().push_back(d);
+            acc += b;
+        }
+        for (auto& grp : groups) {
+            for (int d : grp) size_tape(e->m[d], nmacro);
+            size_t nn = 0;
+            hipGraphExec_t gB = ens_capture(e, grp, [&](int d) {
+                rs_ctx* c = e->m[d];
+                for (int mstep = nmacro - 1; mstep >= 0; --mstep)
+                    enqueue_backward_macro(c, mstep, *o, c->relpart.p + (size_t)mstep * 1024, nullptr, nullptr);
+            }, &nn);
+            HIPCHK(hipGraphLaunch(gB, e->origin));
+            HIPCHK(hipStreamSynchronize(e->origin));
+            (void)hipGraphExecDestroy(gB);
+            ES.graph_launches_backward += 1;
+            ES.graph_nodes_backward += (long long)nn;
+            for (int d : grp) {
+                rs_ctx* c = e->m[d];
+                S[d].replay_maxrel = host_max_many(c, c->relpart.p, nmacro, rs_reduce_blocks((long long)c->N()));
+                for (int mstep = 0; mstep < nmacro; ++mstep) {
+                    S[d].nvjp_T += (long long)c->seqT[mstep].size();
+                    S[d].nvjp_C += (long long)c->seqC[mstep].size();
+                }
+                if (groups.size() > 1) c->tape.release();
+            }
+        }
+        ES.backward_groups = (int)groups.size();
+        ES.seconds_backward = std::chrono::duration<double>(clk::now() - tb0).count();
+    } catch (const HipError& x) {
+        for (int d = 0; d < n; ++d) e->m[d]->prof = profs[d];
+        return ens_fail(e, RS_E_HIP, std::string(x.what) + ": " + hipGetErrorString(x.e));
+    } catch (const ArgError& x) {
+        for (int d = 0; d < n; ++d) e->m[d]->prof = profs[d];
+        return ens_fail(e, x.code, x.msg);
+    }
+    for (int d = 0; d < n; ++d) {
+        e->m[d]->prof = profs[d];
+        S[d].seconds_forward = ES.seconds_forward;
+        S[d].seconds_backward = ES.seconds_backward;
+        if (stats) stats[d] = S[d];
+    }
+    if (es) *es = ES;
+    return RS_OK;
 }
 
 }  // extern "C"

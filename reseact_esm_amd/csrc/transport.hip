@@ -29,9 +29,9 @@
 #include "rs_launch.h"
 
 #define TX 64
-#define TK 16
+#define TK_DEFAULT 16
 #define LXW (TX + 4)
-#define LKW (TK + 4)
+
 #define TB 256
 
 #define KIND_STAGE 0
@@ -133,9 +133,9 @@ __device__ __forceinline__ double tend_adj(const double (*T)[LXW], int kk, int i
 
 }  // namespace
 
-template <int KIND>
+template <int KIND, int TKV>
 __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
-    __shared__ double T[LKW][LXW];
+    __shared__ double T[TKV + 4][LXW];
     __shared__ double red[TB / 64];
     const RsCtrl* ctl = A.ctrl;
     if (ctl && ctl->done) return;
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
     const double dt = ctl ? ctl->dtc : A.dt_static;
     const double h2 = dt / 2;
     const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev;
-    const int i0 = blockIdx.x * TX, k0 = blockIdx.y * TK;
+    const int i0 = blockIdx.x * TX, k0 = blockIdx.y * TKV;
     const int j = blockIdx.z % nlat, s = blockIdx.z / nlat;
     const long long sb = (long long)s * A.nc;
     const long long plane = (long long)nlon * nlat;
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
     double* __restrict__ po = A.out_is_state ? (cur ? A.S0 : A.S1) : A.out;
     const int tid = threadIdx.x;
     // ---- stage the tile + halo
-    for (int x = tid; x < LKW * LXW; x += TB) {
+    for (int x = tid; x < (TKV + 4) * LXW; x += TB) {
         const int kk = x / LXW, ii = x - kk * LXW;
         const int gk = k0 - 2 + kk;
         double v = 0.0;
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
     const double cxj = A.cx[j];
     double sq = 0.0;
 #pragma unroll
-    for (int rr = 0; rr < TK / 4; ++rr) {
+    for (int rr = 0; rr < TKV / 4; ++rr) {
         const int kk = ky + 4 * rr;
         const int k = k0 + kk;
         if (i >= nlon || k >= nlev) continue;
@@ -365,25 +365,37 @@ __global__ void rs_loss_seed(double* lam, const double* u, long long ntot, long 
 }
 
 // ---- host-visible launch wrappers ------------------------------------------
-extern "C" {
-
-hipError_t rs_launch_transport(int kind, const RsTransArgs* a, hipStream_t st) {
-    dim3 grid((a->nlon + TX - 1) / TX, (a->nlev + TK - 1) / TK, (unsigned)(a->nlat * a->nt));
+// Tile heights the launcher (and the autotuner) can choose from.
+template <int TKV>
+static hipError_t launch_tk(int kind, const RsTransArgs* a, hipStream_t st) {
+    dim3 grid((a->nlon + TX - 1) / TX, (a->nlev + TKV - 1) / TKV, (unsigned)(a->nlat * a->nt));
     dim3 block(TB);
     switch (kind) {
-        case KIND_STAGE: hipLaunchKernelGGL(rs_transport_stage<KIND_STAGE>, grid, block, 0, st, *a); break;
-        case KIND_LAST: hipLaunchKernelGGL(rs_transport_stage<KIND_LAST>, grid, block, 0, st, *a); break;
-        case KIND_ADJ_A: hipLaunchKernelGGL(rs_transport_stage<KIND_ADJ_A>, grid, block, 0, st, *a); break;
-        case KIND_ADJ_B: hipLaunchKernelGGL(rs_transport_stage<KIND_ADJ_B>, grid, block, 0, st, *a); break;
-        case KIND_ADJ_C: hipLaunchKernelGGL(rs_transport_stage<KIND_ADJ_C>, grid, block, 0, st, *a); break;
-        case KIND_ADJ_D: hipLaunchKernelGGL(rs_transport_stage<KIND_ADJ_D>, grid, block, 0, st, *a); break;
+        case KIND_STAGE: hipLaunchKernelGGL((rs_transport_stage<KIND_STAGE, TKV>), grid, block, 0, st, *a); break;
+        case KIND_LAST: hipLaunchKernelGGL((rs_transport_stage<KIND_LAST, TKV>), grid, block, 0, st, *a); break;
+        case KIND_ADJ_A: hipLaunchKernelGGL((rs_transport_stage<KIND_ADJ_A, TKV>), grid, block, 0, st, *a); break;
+        case KIND_ADJ_B: hipLaunchKernelGGL((rs_transport_stage<KIND_ADJ_B, TKV>), grid, block, 0, st, *a); break;
+        case KIND_ADJ_C: hipLaunchKernelGGL((rs_transport_stage<KIND_ADJ_C, TKV>), grid, block, 0, st, *a); break;
+        case KIND_ADJ_D: hipLaunchKernelGGL((rs_transport_stage<KIND_ADJ_D, TKV>), grid, block, 0, st, *a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt) {
-    return (long long)((nlon + TX - 1) / TX) * ((nlev + TK - 1) / TK) * (long long)nlat * nt;
+extern "C" {
+
+hipError_t rs_launch_transport(int kind, int tk, const RsTransArgs* a, hipStream_t st) {
+    switch (tk) {
+        case 4: return launch_tk<4>(kind, a, st);
+        case 8: return launch_tk<8>(kind, a, st);
+        case 16: return launch_tk<16>(kind, a, st);
+        case 32: return launch_tk<32>(kind, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt, int tk) {
+    return (long long)((nlon + TX - 1) / TX) * ((nlev + tk - 1) / tk) * (long long)nlat * nt;
 }
 
 hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog,

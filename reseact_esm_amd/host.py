@@ -79,6 +79,17 @@ class MechInfo(C.Structure):
                 ("scratch_vjp", C.c_longlong)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("transport_tile_levels", C.c_int), ("chem_block", C.c_int),
+                ("transport_ms", C.c_double * 4), ("chem_ms", C.c_double * 3)]
+
+
+class EnsembleStats(C.Structure):
+    _fields_ = [("seconds_forward", C.c_double), ("seconds_backward", C.c_double),
+                ("graph_launches_forward", C.c_longlong), ("graph_launches_backward", C.c_longlong),
+                ("graph_nodes_backward", C.c_longlong), ("backward_groups", C.c_int)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_longlong), ("total_ms", C.c_double)]
 
@@ -137,8 +148,16 @@ ABI = {
     "rs_mechanism_info": [_P, C.c_void_p],
     "rs_profile": [_P, C.c_int],
     "rs_profile_read": [_P, C.c_void_p, C.c_int, C.POINTER(C.c_int)],
+    "rs_autotune": [_P, C.c_void_p],
+    "rs_ensemble_create": [C.POINTER(_P), C.POINTER(_P), C.c_int],
+    "rs_ensemble_destroy": [_P],
+    "rs_ensemble_last_error": [_P],
+    "rs_ensemble_autotune": [_P, C.c_void_p],
+    "rs_ensemble_adjoint_run": [_P, C.c_int, _D, C.c_double, C.c_double, C.c_void_p, C.c_int,
+                                C.c_int, C.c_double, _D, C.c_void_p, C.c_void_p],
 }
-_RET = {"rs_last_error": C.c_char_p, "rs_backend": C.c_char_p}
+_RET = {"rs_last_error": C.c_char_p, "rs_backend": C.c_char_p,
+        "rs_ensemble_last_error": C.c_char_p}
 
 
 def _dp(a: np.ndarray | None):
@@ -256,6 +275,11 @@ class Context:
         n = C.c_int()
         self._chk(self.lib.rs_profile_read(self.h, arr, 16, C.byref(n)))
         return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}
+
+    def autotune(self) -> TuneResult:
+        r = TuneResult()
+        self._chk(self.lib.rs_autotune(self.h, C.byref(r)))
+        return r
 
     def set_photolysis(self, jfac):
         a = None if jfac is None else np.ascontiguousarray(jfac, np.float64)
@@ -380,6 +404,51 @@ class Context:
         J = C.c_double()
         self._chk(self.lib.rs_replay_loss(self.h, 0, loss_species, C.byref(J)))
         return J.value
+
+
+class Ensemble:
+    """rs_ensemble over already loaded member contexts."""
+
+    def __init__(self, members: list[Context]):
+        self.members = members
+        self.lib = members[0].lib
+        arr = (_P * len(members))(*[m.h for m in members])
+        self.h = _P()
+        rc = self.lib.rs_ensemble_create(C.byref(self.h), arr, len(members))
+        if rc != RS_OK:
+            raise ReseactError(rc, "rs_ensemble_create failed")
+
+    def _chk(self, rc: int):
+        if rc != RS_OK:
+            raise ReseactError(rc, self.lib.rs_ensemble_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.rs_ensemble_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self) -> list:
+        arr = (TuneResult * len(self.members))()
+        self._chk(self.lib.rs_ensemble_autotune(self.h, arr))
+        return list(arr)
+
+    def adjoint_run(self, tstops, dtT0, dtC0, opts: SplitOpts, loss_species=0,
+                    tape_budget_gib=64.0):
+        ts = np.ascontiguousarray(tstops, np.float64)
+        n = len(self.members)
+        J = np.empty(n)
+        st = (AdjointStats * n)()
+        es = EnsembleStats()
+        self._chk(self.lib.rs_ensemble_adjoint_run(self.h, len(ts) - 1, _dp(ts), dtT0, dtC0,
+                                                   C.byref(opts), 0, loss_species,
+                                                   tape_budget_gib, _dp(J), st, C.byref(es)))
+        return J, list(st), es
 
 
 @dataclass
