@@ -151,6 +151,45 @@ def chemistry_roofline(lib):
             "cell_steps_per_s": round(p.nc / (ms * 1e-3), 1)}
 
 
+def ensemble_run(lib, ndom: int = 32, nmacro: int = 6, tape_budget_gib: float = 96.0):
+    """32 regional domains (64x64x20 .. 256x192x47), 6 split steps forward plus
+    adjoint, autotuned tiles, graph-captured (rs_ensemble_adjoint_run)."""
+    from reseact_esm_amd.host import Ensemble
+    probs = P.config5(ndom=ndom, nmacro=nmacro)
+    members = [lib.context().load(p) for p in probs]
+    ens = Ensemble(members)
+    t0 = time.perf_counter()
+    tunes = ens.autotune()
+    t_tune = time.perf_counter() - t0
+    p0 = probs[0]
+    J, st, es = ens.adjoint_run(p0.tstops(), p0.dt0_T, p0.dt0_C, split_opts(p0), 0,
+                                tape_budget_gib)
+    cells = sum(p.nc for p in probs)
+    hist_t: dict = {}
+    hist_c: dict = {}
+    for r in tunes:
+        hist_t[r.transport_tile_levels] = hist_t.get(r.transport_tile_levels, 0) + 1
+        hist_c[r.chem_block] = hist_c.get(r.chem_block, 0) + 1
+    out = {"workload": f"ensemble-{ndom}-domains-20sp", "cells": cells, "split_steps": nmacro,
+           "forward_cell_steps_per_s": round(cells * nmacro / es.seconds_forward, 1),
+           "seconds_forward": round(es.seconds_forward, 3),
+           "seconds_backward": round(es.seconds_backward, 3),
+           "adjoint_ratio": round((es.seconds_forward + es.seconds_backward) / es.seconds_forward, 3),
+           "graph_launches_forward": es.graph_launches_forward,
+           "graph_launches_backward": es.graph_launches_backward,
+           "graph_nodes_backward": es.graph_nodes_backward,
+           "backward_groups": es.backward_groups,
+           "autotune_seconds": round(t_tune, 2),
+           "tile_levels_chosen": {str(k): v for k, v in sorted(hist_t.items())},
+           "chem_block_chosen": {str(k): v for k, v in sorted(hist_c.items())},
+           "max_replay_maxrel": max(s.replay_maxrel for s in st)}
+    log(f"ensemble: {out}")
+    for m in members:
+        m.close()
+    ens.close()
+    return out
+
+
 def cpu_baseline(sample_dims=(72, 46, 24), steps: int = 1):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from oracle_lib import oracle_library
@@ -193,6 +232,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--quick", action="store_true", help="skip the transport/chemistry rooflines")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-domain ensemble")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the split step shards by "
@@ -204,6 +244,8 @@ def main() -> int:
     if not a.quick:
         extra["transport_hbm"] = transport_roofline(lib)
         extra["chemistry_fp64"] = chemistry_roofline(lib)
+    if not a.quick and not a.no_ensemble:
+        extra["ensemble"] = ensemble_run(lib)
     cpu = None if a.no_cpu else cpu_baseline()
     p = c["p"]
     value = p.nc * c["steps"] / c["fwd_s"]
