@@ -291,6 +291,29 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         e << " }\n";
     }
     e << "}\n";
+    // ---- o1 += J(u)^T z and o2 += c2 * J(u)^T z in one pass (adjoint of u1)
+    e << "RS_FN void rs_jtv2(const double* __restrict__ K, double jf, const double* __restrict__ u,"
+         " const double* __restrict__ z, double* __restrict__ o1, double* __restrict__ o2,"
+         " double c2) {\n";
+    for (int r = 0; r < m.nr; ++r) {
+        if (p.net[r].empty()) continue;
+        int a = m.ra[r], b = m.rb[r];
+        e << "  { double s = 0.0;";
+        for (auto& pr : p.net[r]) e << " " << accum("s", pr.second, "z[" + std::to_string(pr.first) + "]");
+        e << " const double kc = " << kexpr(m, r) << ";";
+        auto both = [&](int x, const std::string& d) {
+            e << " { const double t = " << d << " * s; o1[" << x << "] = o1[" << x << "] + t; o2[" << x
+              << "] = o2[" << x << "] + c2 * t; }";
+        };
+        if (b < 0) both(a, "kc");
+        else if (a == b) both(a, "((2.0 * kc) * u[" + std::to_string(a) + "])");
+        else {
+            both(a, "(kc * u[" + std::to_string(b) + "])");
+            both(b, "(kc * u[" + std::to_string(a) + "])");
+        }
+        e << " }\n";
+    }
+    e << "}\n";
     // ---- ub += coef * d/du [ z^T J(u) v ]   (bimolecular reactions only)
     e << "RS_FN void rs_hess(const double* __restrict__ K, double jf, const double* __restrict__ z,"
          " const double* __restrict__ v, double coef, double* __restrict__ ub) {\n";
