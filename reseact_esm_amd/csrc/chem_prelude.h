@@ -24,8 +24,21 @@
 #define RS_PIVOT(w) (1.0 / (w))
 #define RS_DIAG_SOLVE(acc, w) ((acc) / (w))
 #else
-#define RS_PIVOT(w) ((w) = 1.0 / (w))
+#define RS_PIVOT(w) ((w) = rs_rcp(w))
 #define RS_DIAG_SOLVE(acc, w) ((acc) * (w))
+#endif
+/* 1/x from the hardware reciprocal estimate refined by two Newton steps
+ * (~1 ulp; a zero pivot gives a NaN, which the controller rejects). */
+#ifdef RS_HOST_BUILD
+RS_FN double rs_rcp(double x) { return 1.0 / x; }
+#else
+RS_FN double rs_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
 #endif
 /* Julia's max: propagates a NaN in the first argument like std::max. */
 RS_FN double rs_max(double a, double b) { return (a < b) ? b : a; }
