@@ -165,8 +165,7 @@ __device__ inline double rs_block_sum(double v, double* sh) {
 extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK)
 rs_chem_attempt(RsChemArgs a) {
     __shared__ double sh[RS_CHEM_BLOCK / 64];
-    __shared__ int last_flag;
-    RsCtrl* c = a.ctrl;
+    const RsCtrl* c = a.ctrl;
     if (c && c->done) return;
     const int cur = c ? c->cur : a.cur_static;
     const double dt = c ? c->dtc : a.dt_static;
@@ -186,8 +185,7 @@ rs_chem_attempt(RsChemArgs a) {
             dst[(long long)s * a.nc + cell] = a.clamp ? rs_max(un[s], 0.0) : un[s];
     }
     const double tot = rs_block_sum(sse, sh);
-    if (c) rs_ctrl_arrive(c, a.partial, gridDim.x, blockIdx.x, tot, a.seqlog, &last_flag);
-    else if (threadIdx.x == 0) a.partial[blockIdx.x] = tot;
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = tot;
 }
 
 /* Persistent per-cell adaptive solve with wave-aggregated work stealing: a

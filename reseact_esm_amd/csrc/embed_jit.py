@@ -6,7 +6,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PARTS = [("kJitPrelude", "chem_prelude.h"), ("kJitLaunch", "rs_launch.h"),
-         ("kJitCtrl", "rs_ctrl_dev.h"), ("kJitChemKernels", "chem_kernels.hip")]
+         ("kJitChemKernels", "chem_kernels.hip")]
 DELIM = "RSJIT"
 
 out = ["// generated by embed_jit.py -- do not edit"]

@@ -3,10 +3,10 @@
 //
 // Ownership: a context owns one HIP stream (default priority) and all device
 // memory; every launch goes to that stream.  Adaptive solves never stop for
-// the host per attempt: the PI controller runs on the device, in the last
-// block to finish each attempt (rs_ctrl_arrive, rs_ctrl_dev.h), and the host
-// enqueues attempts in growing chunks, reading the controller block back once
-// per chunk; kernels of a finished solve exit at their first instruction.  Chemistry kernels are compiled at rs_set_mechanism time with
+// the host per attempt: the PI controller runs on the device (rs_ctrl_update)
+// and the host enqueues attempts in growing chunks, reading the controller
+// block back once per chunk; kernels of a finished solve exit at their first
+// instruction.  Chemistry kernels are compiled at rs_set_mechanism time with
 // hipRTC from the mechanism compiler's straight-line code (mechgen.cpp).
 //
 // Reference interfaces replaced (EarthSciML/reseact.esm):
@@ -43,6 +43,7 @@
 extern "C" {
 hipError_t rs_launch_transport(int kind, int tk, const RsTransArgs* a, hipStream_t st);
 long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt, int tk);
+hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog, hipStream_t st);
 hipError_t rs_launch_fill(double* x, long long n, double v, hipStream_t st);
 hipError_t rs_launch_fill_u64(unsigned long long* x, int n, unsigned long long v, hipStream_t st);
 hipError_t rs_launch_mask(double* lam, const double* uout, long long n, hipStream_t st);
@@ -222,7 +223,6 @@ std::string chem_source(const rs::MechSpec& m, const rs::MechPlan& p) {
     std::string src;
     src += kJitPrelude;
     src += kJitLaunch;
-    src += kJitCtrl;
     src += rs::generate_mech_source(m, p);
     src += kJitChemKernels;
     return src;
@@ -344,11 +344,10 @@ long long trans_nblocks(rs_ctx* c) { return rs_transport_nblocks(c->nlon, c->nla
 
 // One SSPRK43 attempt: either from the state buffers under a controller (ctrl
 // != null: dt/cur/done from the device), or between explicit buffers.
-void enqueue_ssprk43(rs_ctx* c, RsCtrl* ctrl, const double* uin, double* uout, double dt,
-                     double atol, double rtol, int clamp, double* seqlog = nullptr) {
+void enqueue_ssprk43(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* uout, double dt,
+                     double atol, double rtol, int clamp) {
     RsTransArgs a = trans_args(c);
     a.ctrl = ctrl;
-    a.seqlog = seqlog;
     a.dt_static = dt;
     a.cur_static = c->cur;
     a.atol = atol;
@@ -388,13 +387,12 @@ void chem_launch(rs_ctx* c, hipFunction_t f, void* argp, size_t argsz, unsigned 
 
 unsigned chem_grid(rs_ctx* c) { return (unsigned)((c->nc + c->chem_block - 1) / c->chem_block); }
 
-void enqueue_ros23(rs_ctx* c, RsCtrl* ctrl, double* S0, double* S1, int cur_static,
-                   double dt, double atol, double rtol, int clamp, double* seqlog = nullptr) {
+void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cur_static,
+                   double dt, double atol, double rtol, int clamp) {
     RsChemArgs a{};
     a.S0 = S0;
     a.S1 = S1;
     a.ctrl = ctrl;
-    a.seqlog = seqlog;
     a.K = c->K.p;
     a.jfac = c->has_jfac ? c->jfac.p : nullptr;
     a.E = c->has_E ? c->E.p : nullptr;
@@ -480,15 +478,14 @@ AdaptiveRun adaptive_begin(rs_ctx* c, Op op, double t0, double t1, double dt0, c
     return AdaptiveRun{op, atol, rtol, clamp, maxiters, c->cur, log};
 }
 
-// One attempt = the operator's kernels; the last block of the last kernel
-// runs the controller (rs_ctrl_arrive), so there is no separate launch.
 void adaptive_chunk(rs_ctx* c, const AdaptiveRun& ar, int n) {
+    const int npart = (ar.op == Op::T) ? (int)trans_nblocks(c) : (int)chem_grid(c);
     double* logp = ar.log ? c->seqlog.p : nullptr;
     for (int q = 0; q < n; ++q) {
-        if (ar.op == Op::T)
-            enqueue_ssprk43(c, c->dctrl, nullptr, nullptr, 0.0, ar.atol, ar.rtol, ar.clamp, logp);
-        else
-            enqueue_ros23(c, c->dctrl, c->S[0].p, c->S[1].p, 0, 0.0, ar.atol, ar.rtol, ar.clamp, logp);
+        if (ar.op == Op::T) enqueue_ssprk43(c, c->dctrl, nullptr, nullptr, 0.0, ar.atol, ar.rtol, ar.clamp);
+        else enqueue_ros23(c, c->dctrl, c->S[0].p, c->S[1].p, 0, 0.0, ar.atol, ar.rtol, ar.clamp);
+        ProfScope ps(c, P_CTRL);
+        HIPCHK(rs_launch_ctrl(c->dctrl, c->partial.p, npart, logp, c->stream));
     }
 }
 

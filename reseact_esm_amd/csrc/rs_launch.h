@@ -11,8 +11,7 @@ typedef struct RsCtrl {
     double t, t1, tlim, dt, dtc, qold, eest, nnorm;
     double beta1, beta2, invqmin, invqmax, gamma, qsmin, qsmax, qoldinit;
     long long nacc, nrej, iters, maxiters;
-    int done, accept, cur, nlog, logcap;
-    unsigned arrive;        /* blocks of the running attempt that have finished */
+    int done, accept, cur, nlog, logcap, pad;
 } RsCtrl;
 
 /* One ROS23 attempt over all cells.  With ctrl != 0 the step size, the
@@ -20,8 +19,7 @@ typedef struct RsCtrl {
 typedef struct RsChemArgs {
     double* S0;
     double* S1;
-    RsCtrl* ctrl;
-    double* seqlog;         /* accepted (t, dt) log of the controller, or 0 */
+    const RsCtrl* ctrl;
     const double* K;        /* rate constants, nr          */
     const double* jfac;     /* photolysis factor, nc or 0  */
     const double* E;        /* surface emissions, ns*ncol or 0 */
@@ -69,8 +67,7 @@ typedef struct RsTransArgs {
     const double* cx;       /* nlat   */
     const double* wz;       /* nlev+1 */
     const double* wl;       /* nlon   */
-    RsCtrl* ctrl;
-    double* seqlog;
+    const RsCtrl* ctrl;
     double* partial;
     long long nc;
     double dt_static, atol, rtol;

@@ -27,7 +27,6 @@
 #include <hip/hip_runtime.h>
 
 #include "rs_launch.h"
-#include "rs_ctrl_dev.h"
 
 #define TX 64
 #define TK_DEFAULT 16
@@ -138,8 +137,7 @@ template <int KIND, int TKV>
 __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
     __shared__ double T[TKV + 4][LXW];
     __shared__ double red[TB / 64];
-    __shared__ int last_flag;
-    RsCtrl* ctl = A.ctrl;
+    const RsCtrl* ctl = A.ctrl;
     if (ctl && ctl->done) return;
     const int cur = ctl ? ctl->cur : A.cur_static;
     const double dt = ctl ? ctl->dtc : A.dt_static;
@@ -216,17 +214,78 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
         for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
         if ((tid & 63) == 0) red[tid >> 6] = sq;
         __syncthreads();
-        double t = 0.0;
-        if (tid == 0)
+        if (tid == 0) {
+            double t = 0.0;
             for (int w = 0; w < TB / 64; ++w) t += red[w];
-        const unsigned bid = ((unsigned)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        if (ctl) {
-            // adaptive mode: the last block to finish runs the PI controller
-            rs_ctrl_arrive(ctl, A.partial, gridDim.x * gridDim.y * gridDim.z, bid, t, A.seqlog,
-                           &last_flag);
-        } else if (tid == 0) {
+            const long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
             A.partial[bid] = t;
         }
+    }
+}
+
+// ---- device-side PI controller (rx_traced_integrator.jl:480-505) ----------
+// One 1024-lane block: each lane folds its strided share of the block partials
+// into four independent accumulators (so the loads overlap instead of
+// forming one dependent chain), then a fixed tree sums the lanes.  The order
+// depends only on npart, so EEst is reproducible run to run.
+#define CTRL_THREADS 1024
+__global__ void __launch_bounds__(CTRL_THREADS) rs_ctrl_update(RsCtrl* c, const double* partial,
+                                                               int npart, double* seqlog) {
+    __shared__ double sh[CTRL_THREADS];
+    if (c->done) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int x = threadIdx.x;
+    for (; x + 3 * CTRL_THREADS < npart; x += 4 * CTRL_THREADS) {
+        s0 += partial[x];
+        s1 += partial[x + CTRL_THREADS];
+        s2 += partial[x + 2 * CTRL_THREADS];
+        s3 += partial[x + 3 * CTRL_THREADS];
+    }
+    for (; x < npart; x += CTRL_THREADS) s0 += partial[x];
+    sh[threadIdx.x] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    for (int w = CTRL_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double sse = sh[0];
+    const double raw = sqrt(sse / c->nnorm);
+    const double EEst = (raw != raw) ? 1.0e10 : raw;
+    const double q11 = pow(dmax(EEst, 1.0e-35), c->beta1);
+    const double qden = pow(c->qold, c->beta2);
+    double q = q11 / qden;
+    const double qg = q / c->gamma;
+    q = dmax(c->invqmax, (c->invqmin < qg) ? c->invqmin : qg);
+    const int accept = EEst <= 1.0;
+    const int insteady = (c->qsmin <= q) && (q <= c->qsmax);
+    const double qa = insteady ? 1.0 : q;
+    const double dtc = c->dtc;
+    c->eest = EEst;
+    c->accept = accept;
+    if (accept) {
+        if (seqlog && c->nlog < c->logcap) {
+            seqlog[2 * c->nlog] = c->t;
+            seqlog[2 * c->nlog + 1] = dtc;
+            c->nlog += 1;
+        }
+        c->t = c->t + dtc;
+        c->dt = dtc / qa;
+        c->qold = dmax(EEst, c->qoldinit);
+        c->nacc += 1;
+        c->cur ^= 1;
+    } else {
+        const double q11g = q11 / c->gamma;
+        const double qr = (c->invqmin < q11g) ? c->invqmin : q11g;
+        c->dt = dtc / qr;
+        c->nrej += 1;
+    }
+    c->iters += 1;
+    if (!(c->t < c->tlim) || c->iters >= c->maxiters) {
+        c->done = 1;
+    } else {
+        const double rem = c->t1 - c->t;
+        c->dtc = (c->dt < rem) ? c->dt : rem;
     }
 }
 
@@ -349,6 +408,12 @@ hipError_t rs_launch_transport(int kind, int tk, const RsTransArgs* a, hipStream
 
 long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt, int tk) {
     return (long long)((nlon + TX - 1) / TX) * ((nlev + tk - 1) / tk) * (long long)nlat * nt;
+}
+
+hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(rs_ctrl_update, dim3(1), dim3(CTRL_THREADS), 0, st, c, partial, npart, seqlog);
+    return hipGetLastError();
 }
 
 static unsigned util_grid(long long n) {
