@@ -206,6 +206,13 @@ RS_API int rs_replay_loss(rs_ctx* ctx, int loss_kind, int loss_species, double* 
  * 1 = Markowitz fill-reducing order (default; fewer flops). */
 RS_API int rs_set_lu_ordering(rs_ctx* ctx, int ordering);
 
+/* Transport kernels for each SSPRK43 attempt and its VJP:
+ * 1 = one fused launch per attempt (all four stages in LDS; default, used
+ *     where the grid allows: nlon >= 4 and a longitude segment with an
+ *     8-column halo fits in LDS at this nlev);
+ * 0 = one launch per stage.  Both give the same bits. */
+RS_API int rs_set_transport_kernel(rs_ctx* ctx, int mode);
+
 /* Static work of the compiled mechanism kernels (roofline accounting). */
 typedef struct rs_mech_info {
     int ns, nr, nnz_J, nnz_LU, ordering;
@@ -227,14 +234,18 @@ typedef struct rs_kernel_time {
 RS_API int rs_profile(rs_ctx* ctx, int enable);
 
 /* Autotuner: times one SSPRK43 attempt for every transport tile height
- * (4, 8, 16, 32 levels per 64-longitude workgroup) and one ROS23 attempt for
- * every chemistry workgroup size (64, 128, 256) on this context's own grid
- * and state, and keeps the fastest of each.  The state is not changed. */
+ * (4, 8, 16, 32 levels per 64-longitude workgroup) and for the fused
+ * kernel, and one ROS23 attempt for every chemistry workgroup size (64, 128,
+ * 256) on this context's own grid and state, and keeps the fastest of each.
+ * The state is not changed. */
 typedef struct rs_tune_result {
     int transport_tile_levels;
     int chem_block;
     double transport_ms[4];      /* per candidate, 0 if not timed */
     double chem_ms[3];
+    int transport_fused;         /* 1 if the fused kernel was kept */
+    int pad;
+    double transport_fused_ms;   /* 0 if the grid does not allow it */
 } rs_tune_result;
 RS_API int rs_autotune(rs_ctx* ctx, rs_tune_result* out);
 

@@ -1071,6 +1071,12 @@ int rs_set_lu_ordering(rs_ctx* c, int ordering) {
     return RS_OK;   // the oracle always eliminates densely in natural order
 }
 
+int rs_set_transport_kernel(rs_ctx* c, int mode) {
+    if (!c) return RS_E_ARG;
+    if (mode != 0 && mode != 1) return fail(c, RS_E_ARG, "mode must be 0 or 1");
+    return RS_OK;   // one transport path on the CPU
+}
+
 int rs_mechanism_info(rs_ctx* c, rs_mech_info* out) {
     if (!c || !c->have_mech || !out) return fail(c, RS_E_STATE, "mechanism not set");
     rs_mech_info m{};

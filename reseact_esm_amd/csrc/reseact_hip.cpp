@@ -43,6 +43,9 @@
 extern "C" {
 hipError_t rs_launch_transport(int kind, int tk, const RsTransArgs* a, hipStream_t st);
 long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt, int tk);
+int rs_transport_fused_plan(int nlon, int nlev, int* plan);
+long long rs_transport_fused_nblocks(int nlat, int nt, int nseg);
+hipError_t rs_launch_transport_fused(int adj, const RsTransArgs* a, hipStream_t st);
 hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog, hipStream_t st);
 hipError_t rs_launch_fill(double* x, long long n, double v, hipStream_t st);
 hipError_t rs_launch_fill_u64(unsigned long long* x, int n, unsigned long long v, hipStream_t st);
@@ -68,10 +71,12 @@ hipError_t rs_launch_loss_seed(double* lam, const double* u, long long ntot, lon
 
 namespace {
 
-enum ProfKind { P_TSTAGE, P_TLAST, P_CTRL, P_CHEM, P_PERCELL, P_CHEMVJP, P_TADJ, P_UTIL, P_COUNT };
+enum ProfKind { P_TSTAGE, P_TLAST, P_CTRL, P_CHEM, P_PERCELL, P_CHEMVJP, P_TADJ, P_UTIL,
+                P_TFUSED, P_TFUSEDADJ, P_COUNT };
 const char* kProfNames[P_COUNT] = {"transport_stage", "transport_last", "ctrl_update",
                                    "chem_attempt",    "chem_percell",   "chem_vjp",
-                                   "transport_adj",   "util"};
+                                   "transport_adj",   "util",           "transport_fused",
+                                   "transport_fused_adj"};
 
 struct HipError {
     hipError_t e;
@@ -137,6 +142,9 @@ struct rs_ctx {
     int cur = 0;
     int lu_order = 1;
     int tk = 16;                 // transport tile height (levels per workgroup)
+    int tmode = 1;               // 1: fused SSPRK43 kernels where the grid allows, 0: per stage
+    int fplan[4] = {0, 0, 0, 0}; // fused plan: seg_w, seg_in, seg_halo, nseg
+    bool fplan_ok = false;
     unsigned chem_block = 64;    // chemistry threads per workgroup
     DevArray<double> S[2], X, Y, cx, wz, wl, K, jfac, E, lam, LA, LB, LC, gE, partial, seqlog;
     DevArray<double> red;
@@ -340,7 +348,19 @@ RsTransArgs trans_args(rs_ctx* c) {
     return a;
 }
 
-long long trans_nblocks(rs_ctx* c) { return rs_transport_nblocks(c->nlon, c->nlat, c->nlev, c->nt, c->tk); }
+bool trans_fused(const rs_ctx* c) { return c->tmode == 1 && c->fplan_ok; }
+
+long long trans_nblocks(rs_ctx* c) {
+    if (trans_fused(c)) return rs_transport_fused_nblocks(c->nlat, c->nt, c->fplan[3]);
+    return rs_transport_nblocks(c->nlon, c->nlat, c->nlev, c->nt, c->tk);
+}
+
+void set_fused_plan(RsTransArgs& a, const rs_ctx* c) {
+    a.seg_w = c->fplan[0];
+    a.seg_in = c->fplan[1];
+    a.seg_halo = c->fplan[2];
+    a.nseg = c->fplan[3];
+}
 
 // One SSPRK43 attempt: either from the state buffers under a controller (ctrl
 // != null: dt/cur/done from the device), or between explicit buffers.
@@ -353,6 +373,20 @@ void enqueue_ssprk43(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* u
     a.atol = atol;
     a.rtol = rtol;
     const bool st = (uin == nullptr);
+    if (trans_fused(c) && (st || uin != uout)) {
+        // all four stages in one launch; a segmented launch reads halo columns
+        // other workgroups write, so input and output must differ (they do:
+        // S[cur] -> S[cur^1], or two explicit buffers)
+        set_fused_plan(a, c);
+        a.a_is_state = st ? 1 : 0;
+        a.a = uin;
+        a.out_is_state = st ? 1 : 0;
+        a.out = uout;
+        a.clamp = clamp;
+        ProfScope ps(c, P_TFUSED);
+        HIPCHK(rs_launch_transport_fused(0, &a, c->stream));
+        return;
+    }
     // stage 1: X = u + h/2 L u
     a.a_is_state = st ? 1 : 0;
     a.a = uin;
@@ -635,6 +669,22 @@ void enqueue_ssprk43_vjp(rs_ctx* c, double dt) {
     a.dt_static = dt;
     a.a_is_state = 0;
     a.out_is_state = 0;
+    if (trans_fused(c)) {
+        set_fused_plan(a, c);
+        a.a = c->lam.p;
+        if (c->fplan[2] == 0) {
+            // whole periodic rows: each workgroup reads its slab before it
+            // writes, and no other workgroup touches it, so in place is safe
+            a.out = c->lam.p;
+            ProfScope ps(c, P_TFUSEDADJ);
+            HIPCHK(rs_launch_transport_fused(1, &a, c->stream));
+        } else {
+            a.out = c->LA.p;
+            { ProfScope ps(c, P_TFUSEDADJ); HIPCHK(rs_launch_transport_fused(1, &a, c->stream)); }
+            d2d(c->lam.p, c->LA.p, (size_t)c->nt * c->nc, c->stream);
+        }
+        return;
+    }
     a.a = c->lam.p;
     a.out = c->LA.p;
     { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_A, c->tk, &a, c->stream)); }
@@ -856,8 +906,11 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
         c->lam.resize(N);
         c->LA.resize(NT); c->LB.resize(NT); c->LC.resize(NT);
         c->gE.resize((size_t)ns * c->ncol);
-        const long long np = std::max<long long>(rs_transport_nblocks(nlon, nlat, nlev, std::max(nt, 1), 4),
-                                                 (c->nc + 63) / 64);
+        c->fplan_ok = rs_transport_fused_plan(nlon, nlev, c->fplan) != 0;
+        long long np = std::max<long long>(rs_transport_nblocks(nlon, nlat, nlev, std::max(nt, 1), 4),
+                                           (c->nc + 63) / 64);
+        if (c->fplan_ok)
+            np = std::max<long long>(np, rs_transport_fused_nblocks(nlat, std::max(nt, 1), c->fplan[3]));
         c->partial.resize((size_t)np);
         c->red.resize(1024);
         HIPCHK(rs_launch_fill(c->lam.p, (long long)N, 0.0, c->stream));
@@ -886,6 +939,13 @@ int rs_set_wind(rs_ctx* c, const double* cx, const double* wz, const double* wl)
         c->have_wind = true;
         return RS_OK;
     });
+}
+
+int rs_set_transport_kernel(rs_ctx* c, int mode) {
+    if (!c) return RS_E_ARG;
+    if (mode != 0 && mode != 1) return fail(c, RS_E_ARG, "mode must be 0 or 1");
+    c->tmode = mode;
+    return RS_OK;
 }
 
 int rs_set_lu_ordering(rs_ctx* c, int ordering) {
@@ -1308,6 +1368,7 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
     if (c->nt > 0 && c->have_wind) {
         const int tks[4] = {4, 8, 16, 32};
         double best = 1e300;
+        c->tmode = 0;
         for (int q = 0; q < 4; ++q) {
             c->tk = tks[q];
             r.transport_ms[q] = time_ms(c, [&]() {
@@ -1316,6 +1377,14 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
             if (r.transport_ms[q] < best) { best = r.transport_ms[q]; r.transport_tile_levels = tks[q]; }
         }
         c->tk = r.transport_tile_levels;
+        if (c->fplan_ok) {
+            c->tmode = 1;
+            r.transport_fused_ms = time_ms(c, [&]() {
+                enqueue_ssprk43(c, nullptr, in, cand, 60.0, 1e-6, 1e-3, 0);
+            }, 3);
+            if (r.transport_fused_ms >= best) c->tmode = 0;
+        }
+        r.transport_fused = trans_fused(c) ? 1 : 0;
     }
     if (c->have_mech) {
         const unsigned bl[3] = {64, 128, 256};

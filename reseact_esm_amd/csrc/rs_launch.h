@@ -73,6 +73,9 @@ typedef struct RsTransArgs {
     double dt_static, atol, rtol;
     int nlon, nlat, nlev, nt;
     int cur_static, a_is_state, out_is_state, clamp;
+    /* fused kernels: longitude segment of seg_w columns (seg_in interior plus
+     * seg_halo on each side; seg_halo = 0 means the whole periodic row) */
+    int seg_w, seg_in, seg_halo, nseg;
 } RsTransArgs;
 
 #endif

@@ -26,6 +26,8 @@
 //   KIND_ADJ_D : out = 2/3 b + a + dt/2 L^T a             (lambda_in; b = lambda4)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rs_launch.h"
 
 #define TX 64
@@ -219,6 +221,242 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
             for (int w = 0; w < TB / 64; ++w) t += red[w];
             const long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
             A.partial[bid] = t;
+        }
+    }
+}
+
+// ---- fused SSPRK43 attempt / adjoint: all four stages in one launch --------
+// A 1024-lane workgroup owns one (tracer, latitude) slab, or a longitude
+// segment of it, with all levels: seg_w columns x nlev levels live in LDS
+// (plus the segment's wl and the wz column).  With seg_halo = 0 the segment is
+// the whole periodic row and the zonal stencil wraps inside LDS; otherwise
+// FHALO = 8 columns on each side (2 per stage x 4 stages) are recomputed and
+// only the seg_in interior columns are written.  Between stages each lane
+// holds its new values in registers until every lane has read the old ones.
+// HBM sees one read of the input and one write of the output per tracer-cell
+// (times seg_w / seg_in for the read when segmented).  The arithmetic per cell
+// is that of the four stage kernels above, operation for operation, so both
+// paths give the same bits.
+#define FT 1024
+#define FCPT 8
+#define FHALO 8
+
+namespace {
+
+// LDS column of local column il + d, |d| <= 2: wraps around the row when the
+// segment is the whole periodic row, clamps at a halo edge otherwise (those
+// columns hold values that never reach a kept interior cell).
+__device__ __forceinline__ int fcol(int i, int w, int periodic) {
+    if (periodic) {
+        i = i < 0 ? i + w : i;
+        return i >= w ? i - w : i;
+    }
+    return i < 0 ? 0 : (i >= w ? w - 1 : i);
+}
+
+__device__ __forceinline__ double ftend_fwd(const double* __restrict__ A, int w, int periodic,
+                                            int k, int il, int nlev, double cxj,
+                                            const double* __restrict__ wz, double wli) {
+    const double* row = A + k * w;
+    const double qm2 = row[fcol(il - 2, w, periodic)], qm1 = row[fcol(il - 1, w, periodic)];
+    const double q0 = row[il];
+    const double qp1 = row[fcol(il + 1, w, periodic)], qp2 = row[fcol(il + 2, w, periodic)];
+    double fe, fw;
+    if (cxj >= 0.0) {
+        fe = cxj * face3(qm1, q0, qp1);
+        fw = cxj * face3(qm2, qm1, q0);
+    } else {
+        fe = cxj * face3(qp2, qp1, q0);
+        fw = cxj * face3(qp1, q0, qm1);
+    }
+    auto V = [&](int kk) { return A[kk * w + il]; };
+    auto G = [&](int kf) -> double {
+        if (kf <= 0 || kf >= nlev) return 0.0;
+        const double wv = wz[kf] * wli;
+        if (wv >= 0.0) {
+            const int mm = kf - 2 < 0 ? 0 : kf - 2;
+            return wv * face3(V(mm), V(kf - 1), V(kf));
+        } else {
+            const int mm = kf + 1 > nlev - 1 ? nlev - 1 : kf + 1;
+            return wv * face3(V(mm), V(kf), V(kf - 1));
+        }
+    };
+    const double gt = G(k + 1), gb = G(k);
+    return -(fe - fw) - (gt - gb);
+}
+
+__device__ __forceinline__ double ftend_adj(const double* __restrict__ A, int w, int periodic,
+                                            int k, int il, int nlev, double cxj,
+                                            const double* __restrict__ wz, double wli) {
+    const double* row = A + k * w;
+    double q[5];
+#pragma unroll
+    for (int d = -2; d <= 2; ++d) q[d + 2] = d == 0 ? row[il] : row[fcol(il + d, w, periodic)];
+    double acc = 0.0;
+#pragma unroll
+    for (int d = -2; d <= 1; ++d) {
+        const double Fb = q[d + 3] - q[d + 2];
+        int omm, om, op;
+        if (cxj >= 0.0) { omm = d - 1; om = d; op = d + 1; }
+        else { omm = d + 2; om = d + 1; op = d; }
+        double wgt = 0.0;
+        if (omm == 0) wgt += -1.0;
+        if (om == 0) wgt += 5.0;
+        if (op == 0) wgt += 2.0;
+        acc += (cxj * Fb * C16) * wgt;
+    }
+#pragma unroll
+    for (int kf = k - 1; kf <= k + 2; ++kf) {
+        if (kf <= 0 || kf >= nlev) continue;
+        const double wv = wz[kf] * wli;
+        const double Gb = A[kf * w + il] - A[(kf - 1) * w + il];
+        int mm, m, p;
+        if (wv >= 0.0) { mm = kf - 2 < 0 ? 0 : kf - 2; m = kf - 1; p = kf; }
+        else { mm = kf + 1 > nlev - 1 ? nlev - 1 : kf + 1; m = kf; p = kf - 1; }
+        double wgt = 0.0;
+        if (mm == k) wgt += -1.0;
+        if (m == k) wgt += 5.0;
+        if (p == k) wgt += 2.0;
+        acc += (wv * Gb * C16) * wgt;
+    }
+    return acc;
+}
+
+}  // namespace
+
+template <int ADJ>
+__global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
+    extern __shared__ double lds[];
+    __shared__ double red[FT / 64];
+    const RsCtrl* ctl = P.ctrl;
+    if (ctl && ctl->done) return;
+    const int cur = ctl ? ctl->cur : P.cur_static;
+    const double dt = ctl ? ctl->dtc : P.dt_static;
+    const double h2 = dt / 2;
+    const int nlon = P.nlon, nlat = P.nlat, nlev = P.nlev, w = P.seg_w;
+    const int periodic = P.seg_halo == 0;
+    const int seg = blockIdx.x, j = blockIdx.y % nlat, s = blockIdx.y / nlat;
+    const int i_start = periodic ? 0 : seg * P.seg_in - FHALO;
+    const long long base = (long long)s * P.nc + (long long)j * nlon;
+    const long long plane = (long long)nlon * nlat;
+    // pa and po may be the same buffer (in-place adjoint on a whole row): every
+    // read of pa happens before the first barrier, every write after the last
+    const double* pa = P.a_is_state ? (cur ? P.S1 : P.S0) : P.a;
+    double* po = P.out_is_state ? (cur ? P.S0 : P.S1) : P.out;
+    const int ncell = w * nlev;
+    double* __restrict__ A = lds;
+    double* __restrict__ Wl = lds + ncell;
+    double* __restrict__ Wz = Wl + w;
+    const double cxj = P.cx[j];
+    const int tid = threadIdx.x;
+    // global column of local column il (the segment never spans more than one wrap)
+    auto gcol = [&](int il) {
+        int gi = i_start + il;
+        gi = gi < 0 ? gi + nlon : gi;
+        return gi >= nlon ? gi - nlon : gi;
+    };
+    for (int x = tid; x < w; x += FT) Wl[x] = P.wl[gcol(x)];
+    for (int x = tid; x <= nlev; x += FT) Wz[x] = P.wz[x];
+    int kk[FCPT], il[FCPT];
+    double v[FCPT], y[FCPT];
+#pragma unroll
+    for (int r = 0; r < FCPT; ++r) {
+        const int c = tid + r * FT;
+        kk[r] = c / w;
+        il[r] = c - kk[r] * w;
+        v[r] = 0.0;
+        if (c < ncell) {
+            v[r] = pa[base + kk[r] * plane + gcol(il[r])];
+            A[c] = v[r];
+        }
+    }
+    __syncthreads();
+    // stages 1-3 forward (x <- x + dt/2 L x), or the first three adjoint stages
+    for (int st = 0; st < 3; ++st) {
+#pragma unroll
+        for (int r = 0; r < FCPT; ++r) {
+            const int c = tid + r * FT;
+            y[r] = 0.0;
+            if (c < ncell) {
+                const double xc = A[c];
+                if (ADJ) {
+                    const double g2 = ftend_adj(A, w, periodic, kk[r], il[r], nlev, cxj, Wz, Wl[il[r]]);
+                    y[r] = xc + h2 * g2;
+                } else {
+                    const double f = ftend_fwd(A, w, periodic, kk[r], il[r], nlev, cxj, Wz, Wl[il[r]]);
+                    const double du = h2 * f;
+                    y[r] = xc + du;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < FCPT; ++r) {
+            const int c = tid + r * FT;
+            if (c >= ncell) continue;
+            if (ADJ && st == 0) {
+                // v <- lambda4; the next stage starts from lambda4 / 3
+                v[r] = y[r];
+                A[c] = C13 * y[r];
+            } else {
+                A[c] = y[r];
+            }
+        }
+        __syncthreads();
+    }
+    if (!ADJ) {
+        // u4 = 2/3 u + 1/3 u3 (v = u, y = u3)
+#pragma unroll
+        for (int r = 0; r < FCPT; ++r) {
+            const int c = tid + r * FT;
+            if (c >= ncell) continue;
+            const double a = C23 * v[r];
+            const double b = C13 * y[r];
+            A[c] = a + b;
+        }
+        __syncthreads();
+    }
+    double sq = 0.0;
+#pragma unroll
+    for (int r = 0; r < FCPT; ++r) {
+        const int c = tid + r * FT;
+        if (c >= ncell) continue;
+        if (!periodic && (il[r] < FHALO || il[r] >= FHALO + P.seg_in ||
+                          seg * P.seg_in + il[r] - FHALO >= nlon))
+            continue;
+        const long long g = base + kk[r] * plane + gcol(il[r]);
+        const double xc = A[c];
+        if (ADJ) {
+            // out = 2/3 lambda4 + (x + dt/2 L^T x)
+            const double g2 = ftend_adj(A, w, periodic, kk[r], il[r], nlev, cxj, Wz, Wl[il[r]]);
+            const double t = xc + h2 * g2;
+            po[g] = C23 * v[r] + t;
+        } else {
+            const double f = ftend_fwd(A, w, periodic, kk[r], il[r], nlev, cxj, Wz, Wl[il[r]]);
+            const double du = h2 * f;
+            const double un = xc + du;
+            const double u = v[r], u3 = y[r];
+            const double a = C13 * u;
+            const double b = C23 * u3;
+            double ut = a + b;
+            const double ud = ut - un;
+            ut = 0.5 * ud;
+            const double am = dmax(fabs(u), fabs(un));
+            double den = am * P.rtol;
+            den = den + P.atol;
+            const double at = ut / den;
+            sq += at * at;
+            po[g] = P.clamp ? dmax(un, 0.0) : un;
+        }
+    }
+    if (!ADJ) {
+        for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = sq;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int q = 0; q < FT / 64; ++q) t += red[q];
+            P.partial[blockIdx.y * gridDim.x + blockIdx.x] = t;
         }
     }
 }
@@ -457,6 +695,48 @@ hipError_t rs_launch_loss_seed(double* lam, const double* u, long long ntot, lon
                                long long ncol, hipStream_t st) {
     hipLaunchKernelGGL(rs_loss_seed, dim3(util_grid(ntot)), dim3(256), 0, st, lam, u, ntot, sbase,
                        ncol);
+    return hipGetLastError();
+}
+
+}  // extern "C"
+
+// ---- fused launch wrappers ---------------------------------------------------
+extern "C" {
+
+// Segment plan for the fused kernels on an nlon x nlev slab: plan[0..3] =
+// seg_w, seg_in, seg_halo, nseg.  Returns 0 if the fused kernels cannot take
+// this grid (nlon < 4, or too many levels for a useful segment in LDS).
+int rs_transport_fused_plan(int nlon, int nlev, int* plan) {
+    const long long lds_doubles = (65536 - 256) / 8;   // dynamic LDS per workgroup
+    auto fits = [&](long long w) {
+        return w * nlev <= (long long)FT * FCPT && w * nlev + w + nlev + 1 <= lds_doubles;
+    };
+    if (nlon < 4 || nlev < 1) return 0;
+    if (fits(nlon)) {
+        plan[0] = nlon; plan[1] = nlon; plan[2] = 0; plan[3] = 1;
+        return 1;
+    }
+    long long wmax = std::min<long long>((long long)FT * FCPT / nlev,
+                                         (lds_doubles - nlev - 1) / (nlev + 1));
+    if (wmax - 2 * FHALO < 16) return 0;
+    const int nseg = (int)((nlon + (wmax - 2 * FHALO) - 1) / (wmax - 2 * FHALO));
+    const int seg_in = (nlon + nseg - 1) / nseg;
+    // every local column maps into [-nlon, 2 nlon) (one wrap at most)
+    if (seg_in + FHALO > nlon || FHALO > nlon) return 0;
+    plan[0] = seg_in + 2 * FHALO; plan[1] = seg_in; plan[2] = FHALO; plan[3] = nseg;
+    return 1;
+}
+
+long long rs_transport_fused_nblocks(int nlat, int nt, int nseg) {
+    return (long long)nseg * nlat * nt;
+}
+
+hipError_t rs_launch_transport_fused(int adj, const RsTransArgs* a, hipStream_t st) {
+    if (a->seg_w < 4 || a->nseg < 1) return hipErrorInvalidValue;
+    const size_t lds = ((size_t)a->seg_w * a->nlev + a->seg_w + a->nlev + 1) * sizeof(double);
+    dim3 grid((unsigned)a->nseg, (unsigned)(a->nlat * a->nt));
+    if (adj) hipLaunchKernelGGL(rs_transport_fused<1>, grid, dim3(FT), lds, st, *a);
+    else hipLaunchKernelGGL(rs_transport_fused<0>, grid, dim3(FT), lds, st, *a);
     return hipGetLastError();
 }
 

@@ -81,7 +81,8 @@ class MechInfo(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("transport_tile_levels", C.c_int), ("chem_block", C.c_int),
-                ("transport_ms", C.c_double * 4), ("chem_ms", C.c_double * 3)]
+                ("transport_ms", C.c_double * 4), ("chem_ms", C.c_double * 3),
+                ("transport_fused", C.c_int), ("pad", C.c_int), ("transport_fused_ms", C.c_double)]
 
 
 class EnsembleStats(C.Structure):
@@ -145,6 +146,7 @@ ABI = {
                        C.c_int, _D, _D, _D, C.POINTER(AdjointStats)],
     "rs_replay_loss": [_P, C.c_int, C.c_int, _D],
     "rs_set_lu_ordering": [_P, C.c_int],
+    "rs_set_transport_kernel": [_P, C.c_int],
     "rs_mechanism_info": [_P, C.c_void_p],
     "rs_profile": [_P, C.c_int],
     "rs_profile_read": [_P, C.c_void_p, C.c_int, C.POINTER(C.c_int)],
@@ -261,6 +263,10 @@ class Context:
 
     def set_lu_ordering(self, ordering: int):
         self._chk(self.lib.rs_set_lu_ordering(self.h, ordering))
+
+    def set_transport_kernel(self, mode: int):
+        """1: fused SSPRK43 launch per attempt (default where the grid allows), 0: per stage."""
+        self._chk(self.lib.rs_set_transport_kernel(self.h, mode))
 
     def mechanism_info(self) -> dict:
         m = MechInfo()

@@ -150,13 +150,36 @@ def test_ros23_exact_build_is_bitwise(hip, oracle):
 
 
 def test_ros23_larger_grid(hip, oracle):
-    """Chemistry parity on a 23k-cell grid with photolysis and emissions."""
+    """Chemistry parity on a 23k-cell grid with photolysis and emissions.
+
+    Steps the controller takes are held entrywise: from the initial state at
+    1e-4 and 1e-3 s (EEst 0.02 and 3), an adaptive run over 120 s (168
+    accepted, 10 rejected steps), then single steps from the oracle's state at
+    the controller's final step size and 3x it.  Candidates far outside what
+    the controller accepts (0.5 - 60 s from the initial state, EEst ~ 7e3,
+    where W = I - gamma h J is ill-conditioned and the elimination order shows
+    in single entries) are held normwise."""
     p = P.config4(nlon=48, nlat=30, nlev=16)
     p.jfac = np.random.default_rng(8).uniform(0.0, 1.0, p.nc)
     cg, co = both(hip, oracle, p)
-    for dt in (0.5, 5.0, 60.0):
+    for dt in (1e-4, 1e-3, 0.5, 5.0, 60.0):
         eg = cg.ros23_step(0.0, dt, 1e-6, 1e-3)
         eo = co.ros23_step(0.0, dt, 1e-6, 1e-3)
+        if eo < 10:
+            assert relchem(cg.get_candidate(), co.get_candidate(), p.nc) <= TOL_C
+        else:
+            assert relnorm(cg.get_candidate(), co.get_candidate()) <= TOL_C
+        assert abs(eg - eo) <= TOL_C * max(eo, 1.0)
+    dg, kg = cg.adaptive_chem(0.0, 120.0, 0.5, pictrl_ros23(), 1e-6, 1e-3, True)
+    do, ko = co.adaptive_chem(0.0, 120.0, 0.5, pictrl_ros23(), 1e-6, 1e-3, True)
+    assert (kg.naccept, kg.nreject) == (ko.naccept, ko.nreject)
+    assert abs(dg - do) <= TOL_C * do
+    uo = co.get_state()
+    assert relchem(cg.get_state(), uo, p.nc) <= TOL_C
+    cg.set_state(uo)
+    for dt in (do, 3.0 * do):
+        eg = cg.ros23_step(120.0, dt, 1e-6, 1e-3)
+        eo = co.ros23_step(120.0, dt, 1e-6, 1e-3)
         assert relchem(cg.get_candidate(), co.get_candidate(), p.nc) <= TOL_C
         assert abs(eg - eo) <= TOL_C * max(eo, 1.0)
 

@@ -1,0 +1,98 @@
+"""GPU: the fused SSPRK43 transport kernels (all four stages of an attempt, or
+of its VJP, in one launch) against the per-stage kernels and the oracle.
+
+Grids cover both plans: a whole periodic row per workgroup (nlon * nlev small
+enough for LDS) and longitude segments with an 8-column halo (360x72 and
+200x90 split into 4 segments, 97x90 into 2 uneven ones), winds of both signs.
+The arithmetic per cell is the same in both kernels, so the bar is bit-for-bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from reseact_esm_amd import problems as P
+from reseact_esm_amd.host import pictrl_ssprk43
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [(144, 3, 47), (360, 2, 72), (200, 3, 90), (97, 2, 90), (4, 3, 1), (23, 2, 5)]
+
+
+def problem(dims, flip=False, nt=2, ns=3, seed=5):
+    g = P.Grid(*dims)
+    w = P.solid_body_wind(g, 600.0, 0.8)
+    if flip:
+        w.cx = -w.cx
+        w.wl = np.roll(w.wl, 3)
+    return P.Problem(name="t", grid=g, ns=ns, nt=nt, wind=w, mech=None,
+                     u0=P.blob_state(g, ns, seed))
+
+
+def contexts(hip, p):
+    cf = hip.context()
+    cf.set_transport_kernel(1)
+    cf.load(p)
+    cs = hip.context()
+    cs.set_transport_kernel(0)
+    cs.load(p)
+    return cf, cs
+
+
+@pytest.mark.parametrize("flip", [False, True])
+@pytest.mark.parametrize("dims", GRIDS)
+def test_fused_step_bitwise(hip, oracle, dims, flip):
+    print(f"fused step dims={dims} flip={flip}", flush=True)
+    p = problem(dims, flip)
+    cf, cs = contexts(hip, p)
+    co = oracle.context().load(p)
+    for dt in (60.0, 600.0):
+        ef = cf.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+        es = cs.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+        eo = co.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+        uf, us, uo = cf.get_candidate(), cs.get_candidate(), co.get_candidate()
+        assert np.array_equal(uf, us)
+        assert np.array_equal(uf, uo)
+        # the error norm is summed per workgroup: same terms, other order
+        assert abs(ef - es) <= 1e-13 * max(es, 1e-300)
+        assert abs(ef - eo) <= 1e-12 * max(eo, 1.0)
+
+
+@pytest.mark.parametrize("dims", GRIDS)
+def test_fused_vjp_bitwise(hip, oracle, dims):
+    print(f"fused vjp dims={dims}", flush=True)
+    p = problem(dims, flip=dims[0] % 2 == 1)
+    cf, cs = contexts(hip, p)
+    co = oracle.context().load(p)
+    lam = np.random.default_rng(11).standard_normal(p.ns * p.nc)
+    for c in (cf, cs, co):
+        c.set_lambda(lam)
+        c.ssprk43_vjp(0.0, 300.0)
+    lf, ls, lo = cf.get_lambda(), cs.get_lambda(), co.get_lambda()
+    assert np.array_equal(lf, ls)
+    assert float(np.linalg.norm(lf - lo) / np.linalg.norm(lo)) <= 1e-12
+    # species beyond the transported ones pass through
+    assert np.array_equal(lf[p.nt * p.nc:], lam[p.nt * p.nc:])
+
+
+@pytest.mark.parametrize("dims", [(360, 2, 72), (36, 4, 10)])
+def test_fused_adaptive_same_sequence(hip, oracle, dims):
+    print(f"fused adaptive dims={dims}", flush=True)
+    p = problem(dims, nt=3, ns=3)
+    cf, cs = contexts(hip, p)
+    co = oracle.context().load(p)
+    runs = [c.adaptive_transport(0.0, 1800.0, 15.0, pictrl_ssprk43(), 1e-6, 1e-3, True)
+            for c in (cf, cs, co)]
+    (df, kf), (ds, ks), (do, ko) = runs
+    assert kf.naccept > 3
+    assert (kf.naccept, kf.nreject) == (ko.naccept, ko.nreject) == (ks.naccept, ks.nreject)
+    assert abs(df - do) <= 1e-10 * do
+    b = co.get_state()
+    # the error norm is summed per workgroup (fused: per slab or segment,
+    # per stage: per tile; the oracle: in runs of 4096 cells), so step sizes differ in
+    # the last bits; on this grid a 1-ulp change of the first step alone moves
+    # the oracle's own state by 7e-13 entrywise, so the 1e-12 bar is taken
+    # relative to the largest entry
+    for a in (cf.get_state(), cs.get_state()):
+        assert float(np.max(np.abs(a - b)) / np.max(np.abs(b))) <= 1e-12
+        assert float(np.linalg.norm(a - b) / np.linalg.norm(b)) <= 1e-12
