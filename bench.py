@@ -10,7 +10,8 @@ Also reported on the same run:
   * adjoint/forward ratio: checkpointed forward + backward sweep over the
     same steps, divided by the plain forward time;
   * transport HBM roofline: the 360x180x72, 20-tracer transport at Courant
-    0.8 (fixed dt), algorithmic 72 B per tracer-cell per step;
+    0.8 (fixed dt); the fused kernel (one launch per attempt) moves 16 B per
+    tracer-cell algorithmically (u in, unew out), the per-stage chain 72 B;
   * chemistry FP64 roofline: the 360x180x72, 50-species / 120-reaction
     per-cell adaptive chemistry over 1800 s in one persistent launch;
   * cpu_baseline: the CPU oracle (C++/OpenMP, same algorithm) on a bounded
@@ -18,7 +19,8 @@ Also reported on the same run:
 Kernel durations come from HIP events recorded by the library on its own
 stream around every launch (rs_profile).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--no-cpu]
+                       [--no-ensemble] [--probe]
 """
 from __future__ import annotations
 
@@ -100,31 +102,47 @@ def coupled(lib, steps: int, warmup: int):
 
 
 def transport_roofline(lib, nsteps: int = 60, nt: int = 20):
+    """C2 transport at Courant 0.8 with a fixed dt, fused kernel (one launch per
+    attempt) and per-stage kernels (four launches) on the same inputs."""
     p = P.config2(nt=nt, nsteps=nsteps)
     dt = p.extra["dt_fixed"]
-    ctx = lib.context().load(p)
-    ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)      # warm-up
-    ctx.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(nsteps):
-        ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)
-        ctx.accept_candidate(False)
-    t1 = time.perf_counter()
-    prof = ctx.profile_read()
-    ctx.profile(False)
-    ns_, ms_s = kernel_ms(prof, "transport_stage")
-    nl_, ms_l = kernel_ms(prof, "transport_last")
-    # with a fixed dt every launch does work: 3 stage + 1 last launch per step
-    ms_step = (ms_s + ms_l) / nsteps
-    bytes_step = 72.0 * nt * p.nc
-    achieved = bytes_step / (ms_step * 1e-3) / 1e9
-    log(f"transport {p.name}: {nsteps} steps wall {t1 - t0:.3f} s, kernels {ms_step:.3f} ms/step, "
-        f"{achieved:.0f} GB/s")
-    return {"workload": p.name, "bound": "hbm", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "ms_per_step": round(ms_step, 4), "bytes_per_step": bytes_step,
-            "tracer_cell_steps_per_s": round(nt * p.nc / (ms_step * 1e-3), 1),
-            "stage_avg_ms": round(ms_s / max(ns_, 1), 4), "last_avg_ms": round(ms_l / max(nl_, 1), 4)}
+    res = {}
+    for mode, kernels in ((1, ("transport_fused",)), (0, ("transport_stage", "transport_last"))):
+        ctx = lib.context()
+        ctx.set_transport_kernel(mode)
+        ctx.load(p)
+        ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)      # warm-up
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+            ctx.accept_candidate(False)
+        t1 = time.perf_counter()
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        ctx.close()
+        launches = sum(kernel_ms(prof, k)[0] for k in kernels)
+        ms = sum(kernel_ms(prof, k)[1] for k in kernels)
+        res[mode] = dict(ms_step=ms / nsteps, wall=t1 - t0, launches=launches,
+                         avg_launch_ms=ms / max(launches, 1))
+    cells = nt * p.nc
+    f, s = res[1], res[0]
+    # algorithmic bytes: the fused kernel must read u and write unew (16 B per
+    # tracer-cell); the per-stage chain moves 72 B (stages 1-3: 8 + 8, last
+    # stage: u, u3 in, unew out)
+    achieved = 16.0 * cells / (f["avg_launch_ms"] * 1e-3) / 1e9
+    log(f"transport {p.name}: fused {f['ms_step']:.3f} ms/step ({achieved:.0f} GB/s at 16 B), "
+        f"per-stage {s['ms_step']:.3f} ms/step ({72.0 * cells / (s['ms_step'] * 1e-3) / 1e9:.0f} GB/s "
+        f"at 72 B)")
+    return {"workload": p.name, "kernel": "transport_fused", "bound": "hbm",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": 16.0 * cells,
+            "avg_launch_ms": round(f["avg_launch_ms"], 4), "ms_per_step": round(f["ms_step"], 4),
+            "tracer_cell_steps_per_s": round(cells / (f["ms_step"] * 1e-3), 1),
+            "per_stage": {"ms_per_step": round(s["ms_step"], 4), "bytes_per_step": 72.0 * cells,
+                          "achieved_GBs": round(72.0 * cells / (s["ms_step"] * 1e-3) / 1e9, 1),
+                          "launches_per_step": round(s["launches"] / nsteps, 2)},
+            "fused_speedup": round(s["ms_step"] / f["ms_step"], 3)}
 
 
 def chemistry_roofline(lib):
@@ -225,6 +243,33 @@ def cpu_model() -> str:
     return platform.processor()
 
 
+def probe(lib) -> dict:
+    """Short run for hardware-counter passes (FETCH_SIZE, WRITE_SIZE): a
+    1 GiB device-to-device copy (torch) that calibrates the counters, then two
+    fused transport attempts on the 360x180x72 20-tracer grid and two
+    chemistry attempts on the 144x91x47 coupled grid."""
+    import torch
+    n = 1 << 27
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    p = P.config2(nt=20, nsteps=1)
+    ctx = lib.context().load(p)
+    for _ in range(2):
+        ctx.ssprk43_step(0.0, p.extra["dt_fixed"], 1e-6, 1e-3)
+    ctx.close()
+    p4 = P.config4(nmacro=1)
+    ctx = lib.context().load(p4)
+    for _ in range(2):
+        ctx.ros23_step(0.0, 1.0, p4.atol_C, p4.rtol_C)
+    ctx.close()
+    return {"copy_bytes": 8 * n, "transport": {"workload": p.name, "tracer_cells": 20 * p.nc},
+            "chemistry": {"workload": p4.name, "cells": p4.nc}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,12 +278,17 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="skip the transport/chemistry rooflines")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-domain ensemble")
+    ap.add_argument("--probe", action="store_true",
+                    help="only the short hardware-counter run (copy + one transport and chemistry)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the split step shards by "
               "latitude bands across GPUs, which this build does not launch.", file=sys.stderr)
         return 2
     lib = hip_library()
+    if a.probe:
+        print(json.dumps({"probe": probe(lib)}), flush=True)
+        return 0
     c = coupled(lib, a.steps, a.warmup)
     extra = {}
     if not a.quick:

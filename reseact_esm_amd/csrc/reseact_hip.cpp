@@ -143,7 +143,7 @@ struct rs_ctx {
     int lu_order = 1;
     int tk = 16;                 // transport tile height (levels per workgroup)
     int tmode = 1;               // 1: fused SSPRK43 kernels where the grid allows, 0: per stage
-    int fplan[4] = {0, 0, 0, 0}; // fused plan: seg_w, seg_in, seg_halo, nseg
+    int fplan[6] = {0, 0, 0, 0, 0, 0};  // fused plan: seg_w, seg_in, seg_halo, nseg, seg_lev, seg_chunks
     bool fplan_ok = false;
     unsigned chem_block = 64;    // chemistry threads per workgroup
     DevArray<double> S[2], X, Y, cx, wz, wl, K, jfac, E, lam, LA, LB, LC, gE, partial, seqlog;
@@ -360,6 +360,8 @@ void set_fused_plan(RsTransArgs& a, const rs_ctx* c) {
     a.seg_in = c->fplan[1];
     a.seg_halo = c->fplan[2];
     a.nseg = c->fplan[3];
+    a.seg_lev = c->fplan[4];
+    a.seg_chunks = c->fplan[5];
 }
 
 // One SSPRK43 attempt: either from the state buffers under a controller (ctrl

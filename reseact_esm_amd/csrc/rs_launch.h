@@ -74,8 +74,10 @@ typedef struct RsTransArgs {
     int nlon, nlat, nlev, nt;
     int cur_static, a_is_state, out_is_state, clamp;
     /* fused kernels: longitude segment of seg_w columns (seg_in interior plus
-     * seg_halo on each side; seg_halo = 0 means the whole periodic row) */
-    int seg_w, seg_in, seg_halo, nseg;
+     * seg_halo on each side; seg_halo = 0 means the whole periodic row); each
+     * lane owns one column and seg_lev consecutive levels, seg_chunks lanes
+     * per column */
+    int seg_w, seg_in, seg_halo, nseg, seg_lev, seg_chunks;
 } RsTransArgs;
 
 #endif

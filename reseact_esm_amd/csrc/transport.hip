@@ -227,24 +227,31 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
 
 // ---- fused SSPRK43 attempt / adjoint: all four stages in one launch --------
 // A 1024-lane workgroup owns one (tracer, latitude) slab, or a longitude
-// segment of it, with all levels: seg_w columns x nlev levels live in LDS
-// (plus the segment's wl and the wz column).  With seg_halo = 0 the segment is
+// segment of it, with all levels, in LDS.  With seg_halo = 0 the segment is
 // the whole periodic row and the zonal stencil wraps inside LDS; otherwise
 // FHALO = 8 columns on each side (2 per stage x 4 stages) are recomputed and
-// only the seg_in interior columns are written.  Between stages each lane
-// holds its new values in registers until every lane has read the old ones.
-// HBM sees one read of the input and one write of the output per tracer-cell
-// (times seg_w / seg_in for the read when segmented).  The arithmetic per cell
-// is that of the four stage kernels above, operation for operation, so both
-// paths give the same bits.
+// only the seg_in interior columns are written.
+//
+// LDS layout: column-major, pitch P = (nlev + 4) | 1 doubles per column, with
+// two ghost levels below level 0 and above level nlev-1 holding copies of the
+// boundary value (the vertical stencil's index clamp becomes a plain read;
+// the odd pitch keeps the zonal reads of a wave on distinct banks).
+// Lane t owns column t % seg_w and the seg_lev consecutive levels of chunk
+// t / seg_w.  It walks its levels upwards with the column values in a sliding
+// register window, reuses each vertical face flux for the two cells it
+// separates, and keeps its new values in registers until every lane has read
+// the old ones.  HBM sees one read of the input and one write of the output
+// per tracer-cell (times seg_w / seg_in for the read when segmented).  Every
+// face flux and every cell update is the same expression, in the same order,
+// as in the stage kernels above, so both paths give the same bits.
 #define FT 1024
-#define FCPT 8
+#define FL 8
 #define FHALO 8
 
 namespace {
 
-// LDS column of local column il + d, |d| <= 2: wraps around the row when the
-// segment is the whole periodic row, clamps at a halo edge otherwise (those
+// LDS column of local column i = il + d, |d| <= 2: wraps around the row when
+// the segment is the whole periodic row, clamps at a halo edge otherwise (those
 // columns hold values that never reach a kept interior cell).
 __device__ __forceinline__ int fcol(int i, int w, int periodic) {
     if (periodic) {
@@ -254,72 +261,111 @@ __device__ __forceinline__ int fcol(int i, int w, int periodic) {
     return i < 0 ? 0 : (i >= w ? w - 1 : i);
 }
 
-__device__ __forceinline__ double ftend_fwd(const double* __restrict__ A, int w, int periodic,
-                                            int k, int il, int nlev, double cxj,
-                                            const double* __restrict__ wz, double wli) {
-    const double* row = A + k * w;
-    const double qm2 = row[fcol(il - 2, w, periodic)], qm1 = row[fcol(il - 1, w, periodic)];
-    const double q0 = row[il];
-    const double qp1 = row[fcol(il + 1, w, periodic)], qp2 = row[fcol(il + 2, w, periodic)];
-    double fe, fw;
-    if (cxj >= 0.0) {
-        fe = cxj * face3(qm1, q0, qp1);
-        fw = cxj * face3(qm2, qm1, q0);
-    } else {
-        fe = cxj * face3(qp2, qp1, q0);
-        fw = cxj * face3(qp1, q0, qm1);
-    }
-    auto V = [&](int kk) { return A[kk * w + il]; };
-    auto G = [&](int kf) -> double {
+__host__ __device__ constexpr int fpitch(int nlev) { return (nlev + 4) | 1; }
+
+// One lane's view: pointers at level k0 of its own column and of the four
+// zonal neighbours, and at wz[k0]; x[j] is level k0 + j.
+struct FCol {
+    const double* pc;
+    const double* pm2;
+    const double* pm1;
+    const double* pp1;
+    const double* pp2;
+    const double* pwz;
+    int k0, nlev;
+    double wli, cxj;
+    // vertical flux at face kf = k0 + jf from a = V(kf-2), b = V(kf-1),
+    // c = V(kf), d = V(kf+1) (clamped): zero at the bottom and top faces
+    __device__ __forceinline__ double G(int jf, double a, double b, double c, double d) const {
+        const int kf = k0 + jf;
         if (kf <= 0 || kf >= nlev) return 0.0;
-        const double wv = wz[kf] * wli;
-        if (wv >= 0.0) {
-            const int mm = kf - 2 < 0 ? 0 : kf - 2;
-            return wv * face3(V(mm), V(kf - 1), V(kf));
+        const double wk = pwz[jf] * wli;
+        if (wk >= 0.0) return wk * face3(a, b, c);
+        return wk * face3(d, c, b);
+    }
+    // fe - fw at level k0 + j, q0 = the cell's own value
+    __device__ __forceinline__ double zonal(int j, double q0) const {
+        double fe, fw;
+        if (cxj >= 0.0) {
+            const double qm2 = pm2[j], qm1 = pm1[j], qp1 = pp1[j];
+            fe = cxj * face3(qm1, q0, qp1);
+            fw = cxj * face3(qm2, qm1, q0);
         } else {
-            const int mm = kf + 1 > nlev - 1 ? nlev - 1 : kf + 1;
-            return wv * face3(V(mm), V(kf), V(kf - 1));
+            const double qm1 = pm1[j], qp1 = pp1[j], qp2 = pp2[j];
+            fe = cxj * face3(qp2, qp1, q0);
+            fw = cxj * face3(qp1, q0, qm1);
         }
-    };
-    const double gt = G(k + 1), gb = G(k);
-    return -(fe - fw) - (gt - gb);
+        return fe - fw;
+    }
+};
+
+// out(j, x + dt/2 L x) for the lane's levels k0 + j, j < nk
+template <class Out>
+__device__ __forceinline__ void fstage_fwd(const FCol& C, int nk, double h2, Out out) {
+    double a = C.pc[-2], b = C.pc[-1], c = C.pc[0], d = C.pc[1];
+    double gb = C.G(0, a, b, c, d);
+#pragma unroll
+    for (int j = 0; j < FL; ++j) {
+        if (j >= nk) break;
+        const double e = C.pc[j + 2];
+        const double gt = C.G(j + 1, b, c, d, e);
+        const double fz = C.zonal(j, c);
+        const double f = -fz - (gt - gb);
+        const double du = h2 * f;
+        out(j, c + du);
+        a = b; b = c; c = d; d = e; gb = gt;
+    }
 }
 
-__device__ __forceinline__ double ftend_adj(const double* __restrict__ A, int w, int periodic,
-                                            int k, int il, int nlev, double cxj,
-                                            const double* __restrict__ wz, double wli) {
-    const double* row = A + k * w;
-    double q[5];
+// out(j, x + dt/2 L^T x); L^T gathers, for cell k, the faces whose 3-point
+// stencil contains it: zonal faces i-2..i+1, vertical faces k-1..k+2
+template <class Out>
+__device__ __forceinline__ void fstage_adj(const FCol& C, int nk, double h2, Out out) {
+    const int nlev = C.nlev;
+    double a = C.pc[-2], b = C.pc[-1], c = C.pc[0], d = C.pc[1];
+    double w0 = C.pwz[-1] * C.wli, w1 = C.pwz[0] * C.wli, w2 = C.pwz[1] * C.wli;
 #pragma unroll
-    for (int d = -2; d <= 2; ++d) q[d + 2] = d == 0 ? row[il] : row[fcol(il + d, w, periodic)];
-    double acc = 0.0;
+    for (int j = 0; j < FL; ++j) {
+        if (j >= nk) break;
+        const int k = C.k0 + j;
+        const double e = C.pc[j + 2];
+        const double w3 = C.pwz[j + 2] * C.wli;
+        double q[5];
+        q[0] = C.pm2[j]; q[1] = C.pm1[j]; q[2] = c; q[3] = C.pp1[j]; q[4] = C.pp2[j];
+        double acc = 0.0;
 #pragma unroll
-    for (int d = -2; d <= 1; ++d) {
-        const double Fb = q[d + 3] - q[d + 2];
-        int omm, om, op;
-        if (cxj >= 0.0) { omm = d - 1; om = d; op = d + 1; }
-        else { omm = d + 2; om = d + 1; op = d; }
-        double wgt = 0.0;
-        if (omm == 0) wgt += -1.0;
-        if (om == 0) wgt += 5.0;
-        if (op == 0) wgt += 2.0;
-        acc += (cxj * Fb * C16) * wgt;
+        for (int dd = -2; dd <= 1; ++dd) {
+            const double Fb = q[dd + 3] - q[dd + 2];
+            int omm, om, op;
+            if (C.cxj >= 0.0) { omm = dd - 1; om = dd; op = dd + 1; }
+            else { omm = dd + 2; om = dd + 1; op = dd; }
+            double wgt = 0.0;
+            if (omm == 0) wgt += -1.0;
+            if (om == 0) wgt += 5.0;
+            if (op == 0) wgt += 2.0;
+            acc += (C.cxj * Fb * C16) * wgt;
+        }
+        // faces kf = k-1 .. k+2 and the values below / above each
+        const double lo[4] = {a, b, c, d}, hi[4] = {b, c, d, e}, wvs[4] = {w0, w1, w2, w3};
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
+            const int kf = k - 1 + q4;
+            if (kf <= 0 || kf >= nlev) continue;
+            const double wk = wvs[q4];
+            const double Gb = hi[q4] - lo[q4];
+            int mm, m, p;
+            if (wk >= 0.0) { mm = kf - 2 < 0 ? 0 : kf - 2; m = kf - 1; p = kf; }
+            else { mm = kf + 1 > nlev - 1 ? nlev - 1 : kf + 1; m = kf; p = kf - 1; }
+            double wgt = 0.0;
+            if (mm == k) wgt += -1.0;
+            if (m == k) wgt += 5.0;
+            if (p == k) wgt += 2.0;
+            acc += (wk * Gb * C16) * wgt;
+        }
+        out(j, c + h2 * acc);
+        a = b; b = c; c = d; d = e;
+        w0 = w1; w1 = w2; w2 = w3;
     }
-#pragma unroll
-    for (int kf = k - 1; kf <= k + 2; ++kf) {
-        if (kf <= 0 || kf >= nlev) continue;
-        const double wv = wz[kf] * wli;
-        const double Gb = A[kf * w + il] - A[(kf - 1) * w + il];
-        int mm, m, p;
-        if (wv >= 0.0) { mm = kf - 2 < 0 ? 0 : kf - 2; m = kf - 1; p = kf; }
-        else { mm = kf + 1 > nlev - 1 ? nlev - 1 : kf + 1; m = kf; p = kf - 1; }
-        double wgt = 0.0;
-        if (mm == k) wgt += -1.0;
-        if (m == k) wgt += 5.0;
-        if (p == k) wgt += 2.0;
-        acc += (wv * Gb * C16) * wgt;
-    }
-    return acc;
 }
 
 }  // namespace
@@ -333,109 +379,105 @@ __global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
     const int cur = ctl ? ctl->cur : P.cur_static;
     const double dt = ctl ? ctl->dtc : P.dt_static;
     const double h2 = dt / 2;
-    const int nlon = P.nlon, nlat = P.nlat, nlev = P.nlev, w = P.seg_w;
+    const int nlon = P.nlon, nlat = P.nlat, nlev = P.nlev, w = P.seg_w, L = P.seg_lev;
+    const int pitch = fpitch(nlev);
     const int periodic = P.seg_halo == 0;
     const int seg = blockIdx.x, j = blockIdx.y % nlat, s = blockIdx.y / nlat;
     const int i_start = periodic ? 0 : seg * P.seg_in - FHALO;
     const long long base = (long long)s * P.nc + (long long)j * nlon;
     const long long plane = (long long)nlon * nlat;
-    // pa and po may be the same buffer (in-place adjoint on a whole row): every
+    // pa and po may be the same buffer (in-place adjoint on whole rows): every
     // read of pa happens before the first barrier, every write after the last
     const double* pa = P.a_is_state ? (cur ? P.S1 : P.S0) : P.a;
     double* po = P.out_is_state ? (cur ? P.S0 : P.S1) : P.out;
-    const int ncell = w * nlev;
-    double* __restrict__ A = lds;
-    double* __restrict__ Wl = lds + ncell;
-    double* __restrict__ Wz = Wl + w;
-    const double cxj = P.cx[j];
+    double* A = lds;                  // w columns of pitch doubles
+    double* Wl = lds + w * pitch;     // w
+    double* Wz = Wl + w;              // wz[-1 .. nlev+2], zero outside 0..nlev
     const int tid = threadIdx.x;
-    // global column of local column il (the segment never spans more than one wrap)
-    auto gcol = [&](int il) {
-        int gi = i_start + il;
+    // global column of local column x (a segment wraps at most once)
+    auto gcol = [&](int x) {
+        int gi = i_start + x;
         gi = gi < 0 ? gi + nlon : gi;
         return gi >= nlon ? gi - nlon : gi;
     };
     for (int x = tid; x < w; x += FT) Wl[x] = P.wl[gcol(x)];
-    for (int x = tid; x <= nlev; x += FT) Wz[x] = P.wz[x];
-    int kk[FCPT], il[FCPT];
-    double v[FCPT], y[FCPT];
+    for (int x = tid; x < nlev + 4; x += FT) Wz[x] = (x >= 1 && x <= nlev + 1) ? P.wz[x - 1] : 0.0;
+    const int il = tid % w, kc = tid / w;
+    const int nk = kc < P.seg_chunks ? min(L, nlev - kc * L) : 0;
+    const int k0 = nk > 0 ? kc * L : 0;
+    const long long g0 = base + gcol(il);
+    double* col = A + il * pitch + 2 + k0;   // col[j] = level k0 + j of this column
+    // write x[j] for the lane's levels, and the ghost levels at either end
+    auto put = [&](auto&& val) {
 #pragma unroll
-    for (int r = 0; r < FCPT; ++r) {
-        const int c = tid + r * FT;
-        kk[r] = c / w;
-        il[r] = c - kk[r] * w;
-        v[r] = 0.0;
-        if (c < ncell) {
-            v[r] = pa[base + kk[r] * plane + gcol(il[r])];
-            A[c] = v[r];
+        for (int jj = 0; jj < FL; ++jj) {
+            if (jj >= nk) break;
+            col[jj] = val(jj);
         }
+        if (nk > 0 && k0 == 0) { const double x0 = val(0); col[-1] = x0; col[-2] = x0; }
+        if (nk > 0 && k0 + nk == nlev) {
+            const double xt = val(nk - 1);
+            col[nk] = xt;
+            col[nk + 1] = xt;
+        }
+    };
+    double v[FL], y[FL];
+#pragma unroll
+    for (int jj = 0; jj < FL; ++jj) {
+        v[jj] = 0.0;
+        y[jj] = 0.0;
+        if (jj < nk) v[jj] = pa[g0 + (k0 + jj) * plane];
     }
+    put([&](int jj) { return v[jj]; });
     __syncthreads();
+    FCol C;
+    C.pc = col;
+    C.pm2 = A + fcol(il - 2, w, periodic) * pitch + 2 + k0;
+    C.pm1 = A + fcol(il - 1, w, periodic) * pitch + 2 + k0;
+    C.pp1 = A + fcol(il + 1, w, periodic) * pitch + 2 + k0;
+    C.pp2 = A + fcol(il + 2, w, periodic) * pitch + 2 + k0;
+    C.pwz = Wz + 1 + k0;
+    C.k0 = k0;
+    C.nlev = nlev;
+    C.wli = Wl[il];
+    C.cxj = P.cx[j];
     // stages 1-3 forward (x <- x + dt/2 L x), or the first three adjoint stages
+    auto to_y = [&](int jj, double x) { y[jj] = x; };
     for (int st = 0; st < 3; ++st) {
-#pragma unroll
-        for (int r = 0; r < FCPT; ++r) {
-            const int c = tid + r * FT;
-            y[r] = 0.0;
-            if (c < ncell) {
-                const double xc = A[c];
-                if (ADJ) {
-                    const double g2 = ftend_adj(A, w, periodic, kk[r], il[r], nlev, cxj, Wz, Wl[il[r]]);
-                    y[r] = xc + h2 * g2;
-                } else {
-                    const double f = ftend_fwd(A, w, periodic, kk[r], il[r], nlev, cxj, Wz, Wl[il[r]]);
-                    const double du = h2 * f;
-                    y[r] = xc + du;
-                }
-            }
-        }
+        if (ADJ) fstage_adj(C, nk, h2, to_y);
+        else fstage_fwd(C, nk, h2, to_y);
         __syncthreads();
+        if (ADJ && st == 0) {
+            // v <- lambda4; the next stage starts from lambda4 / 3
 #pragma unroll
-        for (int r = 0; r < FCPT; ++r) {
-            const int c = tid + r * FT;
-            if (c >= ncell) continue;
-            if (ADJ && st == 0) {
-                // v <- lambda4; the next stage starts from lambda4 / 3
-                v[r] = y[r];
-                A[c] = C13 * y[r];
-            } else {
-                A[c] = y[r];
-            }
+            for (int jj = 0; jj < FL; ++jj) v[jj] = y[jj];
+            put([&](int jj) { return C13 * y[jj]; });
+        } else {
+            put([&](int jj) { return y[jj]; });
         }
         __syncthreads();
     }
     if (!ADJ) {
         // u4 = 2/3 u + 1/3 u3 (v = u, y = u3)
-#pragma unroll
-        for (int r = 0; r < FCPT; ++r) {
-            const int c = tid + r * FT;
-            if (c >= ncell) continue;
-            const double a = C23 * v[r];
-            const double b = C13 * y[r];
-            A[c] = a + b;
-        }
+        put([&](int jj) {
+            const double a = C23 * v[jj];
+            const double b = C13 * y[jj];
+            return a + b;
+        });
         __syncthreads();
     }
+    const bool keep = periodic || (il >= FHALO && il < FHALO + P.seg_in &&
+                                   seg * P.seg_in + il - FHALO < nlon);
     double sq = 0.0;
-#pragma unroll
-    for (int r = 0; r < FCPT; ++r) {
-        const int c = tid + r * FT;
-        if (c >= ncell) continue;
-        if (!periodic && (il[r] < FHALO || il[r] >= FHALO + P.seg_in ||
-                          seg * P.seg_in + il[r] - FHALO >= nlon))
-            continue;
-        const long long g = base + kk[r] * plane + gcol(il[r]);
-        const double xc = A[c];
-        if (ADJ) {
-            // out = 2/3 lambda4 + (x + dt/2 L^T x)
-            const double g2 = ftend_adj(A, w, periodic, kk[r], il[r], nlev, cxj, Wz, Wl[il[r]]);
-            const double t = xc + h2 * g2;
-            po[g] = C23 * v[r] + t;
-        } else {
-            const double f = ftend_fwd(A, w, periodic, kk[r], il[r], nlev, cxj, Wz, Wl[il[r]]);
-            const double du = h2 * f;
-            const double un = xc + du;
-            const double u = v[r], u3 = y[r];
+    if (ADJ) {
+        // out = 2/3 lambda4 + (x + dt/2 L^T x)
+        fstage_adj(C, nk, h2, [&](int jj, double z) {
+            if (keep) po[g0 + (k0 + jj) * plane] = C23 * v[jj] + z;
+        });
+    } else {
+        fstage_fwd(C, nk, h2, [&](int jj, double un) {
+            if (!keep) return;
+            const double u = v[jj], u3 = y[jj];
             const double a = C13 * u;
             const double b = C23 * u3;
             double ut = a + b;
@@ -446,8 +488,8 @@ __global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
             den = den + P.atol;
             const double at = ut / den;
             sq += at * at;
-            po[g] = P.clamp ? dmax(un, 0.0) : un;
-        }
+            po[g0 + (k0 + jj) * plane] = P.clamp ? dmax(un, 0.0) : un;
+        });
     }
     if (!ADJ) {
         for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
@@ -703,27 +745,38 @@ hipError_t rs_launch_loss_seed(double* lam, const double* u, long long ntot, lon
 // ---- fused launch wrappers ---------------------------------------------------
 extern "C" {
 
-// Segment plan for the fused kernels on an nlon x nlev slab: plan[0..3] =
-// seg_w, seg_in, seg_halo, nseg.  Returns 0 if the fused kernels cannot take
-// this grid (nlon < 4, or too many levels for a useful segment in LDS).
+// Segment plan for the fused kernels on an nlon x nlev slab: plan[0..5] =
+// seg_w, seg_in, seg_halo, nseg, seg_lev, seg_chunks.  Returns 0 if the fused
+// kernels cannot take this grid (nlon < 4, or too many levels for a useful
+// segment: at most FL levels per lane and FT lanes per workgroup).
 int rs_transport_fused_plan(int nlon, int nlev, int* plan) {
     const long long lds_doubles = (65536 - 256) / 8;   // dynamic LDS per workgroup
-    auto fits = [&](long long w) {
-        return w * nlev <= (long long)FT * FCPT && w * nlev + w + nlev + 1 <= lds_doubles;
+    auto fits = [&](long long w, int* lev, int* chunks) {
+        if (w < 4 || w > FT) return false;
+        if (w * fpitch(nlev) + w + nlev + 4 > lds_doubles) return false;
+        const int per_col = (int)(FT / w);
+        const int L = (nlev + per_col - 1) / per_col;
+        if (L > FL) return false;
+        *lev = L;
+        *chunks = (nlev + L - 1) / L;
+        return true;
     };
     if (nlon < 4 || nlev < 1) return 0;
-    if (fits(nlon)) {
-        plan[0] = nlon; plan[1] = nlon; plan[2] = 0; plan[3] = 1;
+    int L = 0, ch = 0;
+    if (fits(nlon, &L, &ch)) {
+        plan[0] = nlon; plan[1] = nlon; plan[2] = 0; plan[3] = 1; plan[4] = L; plan[5] = ch;
         return 1;
     }
-    long long wmax = std::min<long long>((long long)FT * FCPT / nlev,
-                                         (lds_doubles - nlev - 1) / (nlev + 1));
-    if (wmax - 2 * FHALO < 16) return 0;
+    long long wmax = std::min<long long>(FT, nlon + 2 * FHALO);
+    while (wmax >= 2 * FHALO + 16 && !fits(wmax, &L, &ch)) --wmax;
+    if (wmax < 2 * FHALO + 16) return 0;
     const int nseg = (int)((nlon + (wmax - 2 * FHALO) - 1) / (wmax - 2 * FHALO));
     const int seg_in = (nlon + nseg - 1) / nseg;
     // every local column maps into [-nlon, 2 nlon) (one wrap at most)
     if (seg_in + FHALO > nlon || FHALO > nlon) return 0;
+    if (!fits(seg_in + 2 * FHALO, &L, &ch)) return 0;
     plan[0] = seg_in + 2 * FHALO; plan[1] = seg_in; plan[2] = FHALO; plan[3] = nseg;
+    plan[4] = L; plan[5] = ch;
     return 1;
 }
 
@@ -732,8 +785,11 @@ long long rs_transport_fused_nblocks(int nlat, int nt, int nseg) {
 }
 
 hipError_t rs_launch_transport_fused(int adj, const RsTransArgs* a, hipStream_t st) {
-    if (a->seg_w < 4 || a->nseg < 1) return hipErrorInvalidValue;
-    const size_t lds = ((size_t)a->seg_w * a->nlev + a->seg_w + a->nlev + 1) * sizeof(double);
+    if (a->seg_w < 4 || a->seg_w > FT || a->nseg < 1 || a->seg_lev < 1 || a->seg_lev > FL ||
+        (long long)a->seg_w * a->seg_chunks > FT || (long long)a->seg_lev * a->seg_chunks < a->nlev)
+        return hipErrorInvalidValue;
+    const size_t lds = ((size_t)a->seg_w * fpitch(a->nlev) + a->seg_w + a->nlev + 4) * sizeof(double);
+    if (lds > 65536 - 256) return hipErrorInvalidValue;
     dim3 grid((unsigned)a->nseg, (unsigned)(a->nlat * a->nt));
     if (adj) hipLaunchKernelGGL(rs_transport_fused<1>, grid, dim3(FT), lds, st, *a);
     else hipLaunchKernelGGL(rs_transport_fused<0>, grid, dim3(FT), lds, st, *a);
