@@ -245,17 +245,25 @@ def cpu_model() -> str:
 
 def probe(lib) -> dict:
     """Short run for hardware-counter passes (FETCH_SIZE, WRITE_SIZE): a
-    1 GiB device-to-device copy (torch) that calibrates the counters, then two
-    fused transport attempts on the 360x180x72 20-tracer grid and two
-    chemistry attempts on the 144x91x47 coupled grid."""
-    import torch
+    1 GiB device-to-device hipMemcpy (the runtime's copy kernel) that
+    calibrates the counters, then two fused transport attempts on the
+    360x180x72 20-tracer grid and two chemistry attempts on the 144x91x47
+    coupled grid.  The copy goes through the HIP runtime the library itself
+    links (one runtime per process)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
     n = 1 << 27
-    x = torch.ones(n, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    y.copy_(x)
-    torch.cuda.synchronize()
-    del x, y
+    a, b = C.c_void_p(), C.c_void_p()
+    for q in (a, b):
+        if hip.hipMalloc(C.byref(q), C.c_size_t(8 * n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 0, C.c_size_t(8 * n))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(b, a, C.c_size_t(8 * n), 3) != 0:      # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    for q in (a, b):
+        hip.hipFree(q)
     p = P.config2(nt=20, nsteps=1)
     ctx = lib.context().load(p)
     for _ in range(2):

@@ -11,56 +11,109 @@
 /* One ROS23 attempt on one cell (ros23_step, rx_traced_integrator.jl:279-349,
  * mass matrix I, dT = 0).  Returns the cell's sum of squared scaled errors.
  * E (may be 0) points at this cell's emission of species 0, stride es.
- * The stage combinations the reference forms at the end -- c = 2(k1 - f0),
- * a = c32 (k2 - f1), k1 - 2 k2 -- are formed as soon as their inputs exist,
- * with the same operations, so that f0, f1 and k2 die early: six state
- * vectors plus the factors are live at the peak instead of ten. */
+ *
+ * Storage: three long-lived vectors sit in a slab sv with stride ST between
+ * species (the lane's LDS slab on the device, ST = 64; a local array on the
+ * host, ST = 1), reused as values die:
+ *   slot A: 2(k1 - f0)  ->  + c32 (k2 - f1)       (the two terms of k3's rhs)
+ *   slot B: k1          ->  k1 - 2 k2             (the error's first term)
+ *   slot C: u on entry  ->  atol + rtol max(|u|, |unew|) (unless KEEP_U)
+ * The factors of W and three short-lived vectors stay in registers.  unew is
+ * handed to out(i, unew_i) as soon as it exists.  The exact build
+ * (RS_EXACT_DIV) keeps the reference's evaluation order (f2 - a) - c for k3's
+ * rhs; the fast build adds a + c first. */
+/* Phase boundary for the compiler only (no instruction): values of the slab
+ * are read again from LDS after it instead of being held in registers across
+ * the phases, which is what the slab is for. */
+#ifdef RS_HOST_BUILD
+#define RS_PHASE()
+#else
+#define RS_PHASE() __asm__ volatile("" ::: "memory")
+#endif
+
+#define RS_SA(i) sv[(i) * ST]
+#define RS_SB(i) sv[(RS_NS + (i)) * ST]
+#define RS_SC(i) sv[(2 * RS_NS + (i)) * ST]
+template <int ST, bool KEEP_U, class Out>
 RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
                               const double* __restrict__ E, long long es,
-                              const double* __restrict__ u, double* __restrict__ unew,
-                              double dt, double atol, double rtol) {
+                              double* __restrict__ sv, double dt, double atol, double rtol,
+                              Out out) {
     double W[RS_NNZ];
-    double f[RS_NS], k1[RS_NS], k2[RS_NS], cc[RS_NS], aa[RS_NS];
-    rs_rhs(K, jf, E, es, u, f);                                    /* f0, :282 */
+    double f[RS_NS], x[RS_NS], y[RS_NS];
+    rs_rhs<ST>(K, jf, E, es, &RS_SC(0), f);                        /* f0, :282 */
     const double hg = dt * RS_ROS23_D;                             /* :291 */
-    rs_jac_W(K, jf, u, hg, W);                                     /* :287-296 */
+    rs_jac_W<ST>(K, jf, &RS_SC(0), hg, W);                         /* :287-296 */
     rs_lu(W);
-    rs_solve(W, f, k1);                                      /* :298 */
+    rs_solve(W, f, x);                                             /* k1, :298 */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { double c = k1[i] - f[i]; cc[i] = 2.0 * c; }   /* :323-324 */
+    for (int i = 0; i < RS_NS; ++i) {                              /* :323-324 */
+        const double c = x[i] - f[i];
+        RS_SA(i) = 2.0 * c;
+        RS_SB(i) = x[i];
+    }
     const double dto2 = dt / 2;
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * k1[i]; aa[i] = u[i] + du1; }
-    rs_rhs(K, jf, E, es, aa, f);                                   /* f1, :303 */
+    for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * x[i]; y[i] = RS_SC(i) + du1; }
+    rs_rhs(K, jf, E, es, y, f);                                    /* f1, :303 */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) aa[i] = f[i] - k1[i];          /* :306-309 */
-    rs_solve(W, aa, k2);                                     /* :310 */
+    for (int i = 0; i < RS_NS; ++i) y[i] = f[i] - x[i];            /* :306-309 */
+    rs_solve(W, y, x);                                             /* :310 */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) k2[i] = k2[i] + k1[i];         /* :311-313 */
+    for (int i = 0; i < RS_NS; ++i) x[i] = x[i] + RS_SB(i);        /* k2, :311-313 */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { double a = k2[i] - f[i]; aa[i] = RS_ROS23_C32 * a; }  /* :321-322 */
+    for (int i = 0; i < RS_NS; ++i) {                              /* :321-322 */
+        const double a = x[i] - f[i];
+#ifdef RS_EXACT_DIV
+        y[i] = RS_ROS23_C32 * a;
+#else
+        const double ca = RS_ROS23_C32 * a;
+        RS_SA(i) = ca + RS_SA(i);
+#endif
+    }
+    double den[KEEP_U ? RS_NS : 1];
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { const double du2 = dt * k2[i]; unew[i] = u[i] + du2; }
+    for (int i = 0; i < RS_NS; ++i) {
+        const double u = RS_SC(i);
+        const double du2 = dt * x[i];
+        const double un = u + du2;
+        f[i] = un;
+        out(i, un);
+        const double am = rs_max(RS_FABS(u), RS_FABS(un));        /* :335-346 */
+        double d = am * rtol;
+        d = d + atol;
+        if (KEEP_U) den[KEEP_U ? i : 0] = d;
+        else RS_SC(i) = d;
+        const double w2 = 2.0 * x[i];                              /* :333-334 */
+        RS_SB(i) = RS_SB(i) - w2;
+    }
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { const double w2 = 2.0 * k2[i]; k1[i] = k1[i] - w2; }  /* :333-334 */
-    rs_rhs(K, jf, E, es, unew, f);                                 /* f2, :318 */
+    for (int i = 0; i < RS_NS; ++i) x[i] = f[i];
+    rs_rhs(K, jf, E, es, x, f);                                    /* f2, :318 */
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { const double d3 = f[i] - aa[i]; f[i] = d3 - cc[i]; }  /* :325-326 */
-    rs_solve(W, f, k2);                                      /* k3, :328 */
+    for (int i = 0; i < RS_NS; ++i) {                              /* :325-326 */
+#ifdef RS_EXACT_DIV
+        const double d3 = f[i] - y[i];
+        f[i] = d3 - RS_SA(i);
+#else
+        f[i] = f[i] - RS_SA(i);
+#endif
+    }
+    rs_solve(W, f, x);                                             /* k3, :328 */
     const double dto6 = dt / 6;
     double sse = 0.0;
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) {                              /* :335-346 */
-        double ut = k1[i] + k2[i];
+    for (int i = 0; i < RS_NS; ++i) {
+        double ut = RS_SB(i) + x[i];
         ut = dto6 * ut;
-        const double am = rs_max(RS_FABS(u[i]), RS_FABS(unew[i]));
-        double den = am * rtol;
-        den = den + atol;
-        const double at = ut / den;
+        const double at = ut / (KEEP_U ? den[KEEP_U ? i : 0] : RS_SC(i));
         sse += at * at;
     }
     return sse;
 }
+#undef RS_SA
+#undef RS_SB
+#undef RS_SC
 
 /* Hand-derived adjoint of the part of one ROS23 step that reaches unew
  * (f2, k3 and EEst are dead, rx_traced_integrator.jl:540-547):
@@ -73,50 +126,71 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
  *   z1 = W^-T k1bar               = f0bar;  ubar += J(u)^T z1
  *   Jbar = h gamma (z2 (k2-k1)^T + z1 k1^T)  ->  ubar += d/du [z^T J(u) v]
  *   Ebar = z1 + z2.
- * The stages are recomputed from u, not stored. */
+ * The stages are recomputed from u, not stored.  Slab sv (stride ST, as for
+ * the attempt): slot A holds u on entry, slot B gets u1, slot C gets k1.
+ * lam is read through lam_in(i) when the reverse sweep starts and written
+ * through lam_out(i, v), so it is not held across the forward recompute. */
+#define RS_SA(i) sv[(i) * ST]
+#define RS_SB(i) sv[(RS_NS + (i)) * ST]
+#define RS_SC(i) sv[(2 * RS_NS + (i)) * ST]
+template <int ST, class LIn, class LOut>
 RS_FN void rs_ros23_vjp(const double* __restrict__ K, double jf, const double* __restrict__ E,
-                        long long es, const double* __restrict__ u, double dt,
-                        double* __restrict__ lam, double* __restrict__ G) {
+                        long long es, double* __restrict__ sv, double dt,
+                        LIn lam_in, LOut lam_out, double* __restrict__ G) {
     double W[RS_NNZ];
-    double f[RS_NS], k1[RS_NS], k2[RS_NS], tmp[RS_NS], z[RS_NS], ub[RS_NS];
-    rs_rhs(K, jf, E, es, u, f);
+    double f[RS_NS], x[RS_NS], y[RS_NS];
+    rs_rhs<ST>(K, jf, E, es, &RS_SA(0), f);
     const double hg = dt * RS_ROS23_D;
-    rs_jac_W(K, jf, u, hg, W);
+    RS_PHASE();
+    rs_jac_W<ST>(K, jf, &RS_SA(0), hg, W);
     rs_lu(W);
-    rs_solve(W, f, k1);
+    rs_solve(W, f, x);                                             /* k1 */
+    RS_PHASE();
     const double dto2 = dt / 2;
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * k1[i]; tmp[i] = u[i] + du1; }
-    rs_rhs(K, jf, E, es, tmp, f);                                  /* f1 at u1 = tmp */
+    for (int i = 0; i < RS_NS; ++i) {
+        RS_SC(i) = x[i];
+        const double du1 = dto2 * x[i];
+        RS_SB(i) = RS_SA(i) + du1;                                 /* u1 */
+    }
+    RS_PHASE();
+    rs_rhs<ST>(K, jf, E, es, &RS_SB(0), f);                        /* f1 */
+    RS_PHASE();
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) f[i] = f[i] - k1[i];
-    rs_solve(W, f, k2);
+    for (int i = 0; i < RS_NS; ++i) f[i] = f[i] - x[i];
+    rs_solve(W, f, y);
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) k2[i] = k2[i] + k1[i];
+    for (int i = 0; i < RS_NS; ++i) y[i] = y[i] + x[i];            /* k2 */
     /* reverse */
+    double ub[RS_NS];
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { f[i] = dt * lam[i]; ub[i] = lam[i]; }     /* k2bar */
-    rs_solveT(W, f, z);                                      /* z2 */
+    for (int i = 0; i < RS_NS; ++i) { const double l = lam_in(i); f[i] = dt * l; ub[i] = l; }  /* k2bar */
+    rs_solveT(W, f, x);                                            /* z2 */
+    RS_PHASE();
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) f[i] = f[i] - z[i];            /* k1bar */
+    for (int i = 0; i < RS_NS; ++i) { f[i] = f[i] - x[i]; y[i] = y[i] - RS_SC(i); }  /* k1bar; v = k2 - k1 */
+    rs_hess(K, jf, x, y, hg, ub);
+    RS_PHASE();
+    /* ubar += J(u1)^T z2, k1bar += h/2 J(u1)^T z2 */
+    rs_jtv2<ST>(K, jf, &RS_SB(0), x, ub, f, dto2);
+    RS_PHASE();
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) k2[i] = k2[i] - k1[i];         /* v = k2 - k1 */
-    rs_hess(K, jf, z, k2, hg, ub);
+    for (int i = 0; i < RS_NS; ++i) y[i] = x[i];                   /* keep z2 for Ebar */
+    rs_solveT(W, f, x);                                            /* z1 */
+    RS_PHASE();
+    rs_jtv<ST>(K, jf, &RS_SA(0), x, ub);
+    RS_PHASE();
+    rs_hess<ST>(K, jf, x, &RS_SC(0), hg, ub);
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { k2[i] = z[i]; lam[i] = 0.0; }   /* keep z2 for Ebar */
-    rs_jtv(K, jf, tmp, z, lam);                                    /* u1bar = J(u1)^T z2 */
-#pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { ub[i] = ub[i] + lam[i]; f[i] = f[i] + dto2 * lam[i]; }
-    rs_solveT(W, f, z);                                      /* z1 */
-    rs_jtv(K, jf, u, z, ub);
-    rs_hess(K, jf, z, k1, hg, ub);
-#pragma unroll
-    for (int i = 0; i < RS_NS; ++i) lam[i] = ub[i];
+    for (int i = 0; i < RS_NS; ++i) lam_out(i, ub[i]);
     if (G) {
 #pragma unroll
-        for (int i = 0; i < RS_NS; ++i) G[i * es] += z[i] + k2[i];
+        for (int i = 0; i < RS_NS; ++i) G[i * es] += x[i] + y[i];
     }
 }
+#undef RS_SA
+#undef RS_SB
+#undef RS_SC
 
 /* PI controller update of one attempt (rx_traced_integrator.jl:484-504).
  * Returns 1 on accept.  t, dt, qold, nacc, nrej updated in place. */
@@ -162,9 +236,33 @@ __device__ inline double rs_block_sum(double v, double* sh) {
     return s;
 }
 
+/* Rate constants: a kernel argument (scalar loads from the argument segment,
+ * uniform across the wave) for mechanisms up to RS_K_INLINE_MAX reactions,
+ * otherwise a pointer. */
+#if RS_NR > 0 && RS_NR <= RS_K_INLINE_MAX
+typedef struct { double k[RS_NR]; } RsKTab;
+#define RS_KPARAM , RsKTab kt
+#define RS_KPTR(a) (kt.k)
+#else
+#define RS_KPARAM
+#define RS_KPTR(a) ((a).K)
+#endif
+
+/* The lane's slab of three vectors: in LDS ([3][NS][64] per wave, sized by
+ * the launcher) when it fits, otherwise a local array. */
+#if RS_SLOW_DOUBLES(RS_NS) > 0
+#define RS_SLAB_ST 64
+#define RS_SLAB(name) extern __shared__ double rs_slab_mem[]; \
+    double* name = rs_slab_mem + (threadIdx.x >> 6) * (3 * RS_NS * 64) + (threadIdx.x & 63)
+#else
+#define RS_SLAB_ST 1
+#define RS_SLAB(name) double name##_arr[3 * RS_NS]; double* name = name##_arr
+#endif
+
 extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK)
-rs_chem_attempt(RsChemArgs a) {
+rs_chem_attempt(RsChemArgs a RS_KPARAM) {
     __shared__ double sh[RS_CHEM_BLOCK / 64];
+    RS_SLAB(sv);
     const RsCtrl* c = a.ctrl;
     if (c && c->done) return;
     const int cur = c ? c->cur : a.cur_static;
@@ -174,15 +272,15 @@ rs_chem_attempt(RsChemArgs a) {
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double sse = 0.0;
     if (cell < a.nc) {
-        double u[RS_NS], un[RS_NS];
 #pragma unroll
-        for (int s = 0; s < RS_NS; ++s) u[s] = src[(long long)s * a.nc + cell];
+        for (int s = 0; s < RS_NS; ++s) sv[(2 * RS_NS + s) * RS_SLAB_ST] = src[(long long)s * a.nc + cell];
         const double jf = a.jfac ? a.jfac[cell] : 1.0;
         const double* Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;
-        sse = rs_ros23_attempt(a.K, jf, Ec, a.ncol, u, un, dt, a.atol, a.rtol);
-#pragma unroll
-        for (int s = 0; s < RS_NS; ++s)
-            dst[(long long)s * a.nc + cell] = a.clamp ? rs_max(un[s], 0.0) : un[s];
+        const int clamp = a.clamp;
+        sse = rs_ros23_attempt<RS_SLAB_ST, false>(
+            RS_KPTR(a), jf, Ec, a.ncol, sv, dt, a.atol, a.rtol, [&](int s, double un) {
+                dst[(long long)s * a.nc + cell] = clamp ? rs_max(un, 0.0) : un;
+            });
     }
     const double tot = rs_block_sum(sse, sh);
     if (threadIdx.x == 0) a.partial[blockIdx.x] = tot;
@@ -192,10 +290,11 @@ rs_chem_attempt(RsChemArgs a) {
  * lane that finishes its cell takes the next index from the queue head; all
  * lanes run the same attempt code, each on its own cell and its own dt. */
 extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK)
-rs_chem_percell(RsPercellArgs a) {
+rs_chem_percell(RsPercellArgs a RS_KPARAM) {
     const int lane = threadIdx.x & 63;
     long long cell = -1;     /* -1: needs work, -2: queue drained */
-    double u[RS_NS];
+    RS_SLAB(sv);
+#define RS_U(s) sv[(2 * RS_NS + (s)) * RS_SLAB_ST]
     const double* Ec = nullptr;
     double t = 0.0, dt = 0.0, qold = 0.0, jf = 1.0;
     long long iters = 0, nacc = 0, nrej = 0;
@@ -215,7 +314,7 @@ rs_chem_percell(RsPercellArgs a) {
                 if (idx < (unsigned long long)a.nc) {
                     cell = (long long)idx;
 #pragma unroll
-                    for (int s = 0; s < RS_NS; ++s) u[s] = a.u[(long long)s * a.nc + cell];
+                    for (int s = 0; s < RS_NS; ++s) RS_U(s) = a.u[(long long)s * a.nc + cell];
                     jf = a.jfac ? a.jfac[cell] : 1.0;
                     Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;
                     t = a.t0;
@@ -232,18 +331,20 @@ rs_chem_percell(RsPercellArgs a) {
             if ((t < tlim) && (iters < a.maxiters)) {
                 const double dtc = (dt < a.t1 - t) ? dt : a.t1 - t;
                 double un[RS_NS];
-                const double sse = rs_ros23_attempt(a.K, jf, Ec, a.ncol, u, un, dtc, a.atol, a.rtol);
+                const double sse = rs_ros23_attempt<RS_SLAB_ST, true>(
+                    RS_KPTR(a), jf, Ec, a.ncol, sv, dtc, a.atol, a.rtol,
+                    [&](int s, double v) { un[s] = v; });
                 const double eest = RS_SQRT(sse / (double)RS_NS);
                 if (rs_pi_update(a.beta1, a.beta2, a.invqmin, a.invqmax, a.gamma, a.qsmin,
                                  a.qsmax, a.qoldinit, dtc, eest, &t, &dt, &qold, &nacc, &nrej)) {
 #pragma unroll
-                    for (int s = 0; s < RS_NS; ++s) u[s] = a.clamp ? rs_max(un[s], 0.0) : un[s];
+                    for (int s = 0; s < RS_NS; ++s) RS_U(s) = a.clamp ? rs_max(un[s], 0.0) : un[s];
                 }
                 iters += 1;
             }
             if (!((t < tlim) && (iters < a.maxiters))) {
 #pragma unroll
-                for (int s = 0; s < RS_NS; ++s) a.u[(long long)s * a.nc + cell] = u[s];
+                for (int s = 0; s < RS_NS; ++s) a.u[(long long)s * a.nc + cell] = RS_U(s);
                 if (a.dtcell) a.dtcell[cell] = dt;
                 s_att += iters; s_acc += nacc; s_rej += nrej;
                 s_max = iters > s_max ? iters : s_max;
@@ -274,27 +375,29 @@ rs_chem_percell(RsPercellArgs a) {
     }
 }
 
+#undef RS_U
+
 /* One ROS23 step VJP for every cell, with the clamp mask of that step
  * (lam *= 1[uout > 0], adjoint_gradient.jl:736) applied first. */
 extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK)
-rs_chem_vjp(RsChemVjpArgs a) {
+rs_chem_vjp(RsChemVjpArgs a RS_KPARAM) {
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= a.nc) return;
-    double u[RS_NS], lam[RS_NS];
+    RS_SLAB(sv);
 #pragma unroll
-    for (int s = 0; s < RS_NS; ++s) {
-        u[s] = a.u[(long long)s * a.nc + cell];
-        double l = a.lam[(long long)s * a.nc + cell];
-        if (a.uout && !(a.uout[(long long)s * a.nc + cell] > 0.0)) l = 0.0;
-        lam[s] = l;
-    }
+    for (int s = 0; s < RS_NS; ++s) sv[s * RS_SLAB_ST] = a.u[(long long)s * a.nc + cell];
     const double jf = a.jfac ? a.jfac[cell] : 1.0;
     const bool surf = cell < a.ncol;
     const double* Ec = (surf && a.E) ? a.E + cell : nullptr;
     double* Gc = (surf && a.gE) ? a.gE + cell : nullptr;
-    rs_ros23_vjp(a.K, jf, Ec, a.ncol, u, a.dt, lam, Gc);
-#pragma unroll
-    for (int s = 0; s < RS_NS; ++s) a.lam[(long long)s * a.nc + cell] = lam[s];
+    rs_ros23_vjp<RS_SLAB_ST>(
+        RS_KPTR(a), jf, Ec, a.ncol, sv, a.dt,
+        [&](int s) {
+            double l = a.lam[(long long)s * a.nc + cell];
+            if (a.uout && !(a.uout[(long long)s * a.nc + cell] > 0.0)) l = 0.0;
+            return l;
+        },
+        [&](int s, double v) { a.lam[(long long)s * a.nc + cell] = v; }, Gc);
 }
 
 #endif /* RS_HOST_BUILD */

@@ -182,19 +182,21 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
     e << "#define RS_NS " << n << "\n#define RS_NR " << m.nr << "\n#define RS_NNZ " << p.nnz
       << "\n";
     // ---- RHS: rates folded into the accumulation, reaction order per species
+    e << "template <int US = 1>\n";
     e << "RS_FN void rs_rhs(const double* __restrict__ K, double jf, const double* __restrict__ E,"
          " long long es, const double* __restrict__ u, double* __restrict__ f) {\n";
     for (int i = 0; i < n; ++i) e << "  f[" << i << "] = E ? E[" << i << " * es] : 0.0;\n";
     for (int r = 0; r < m.nr; ++r) {
         e << "  { const double rt = ";
-        if (m.rb[r] < 0) e << kexpr(m, r) << " * u[" << m.ra[r] << "];";
-        else e << "(" << kexpr(m, r) << " * u[" << m.ra[r] << "]) * u[" << m.rb[r] << "];";
+        if (m.rb[r] < 0) e << kexpr(m, r) << " * u[" << m.ra[r] << " * US];";
+        else e << "(" << kexpr(m, r) << " * u[" << m.ra[r] << " * US]) * u[" << m.rb[r] << " * US];";
         for (auto& pr : p.net[r]) e << " " << accum("f[" + std::to_string(pr.first) + "]", pr.second, "rt");
         e << " }\n";
     }
     e << "}\n";
     // ---- W = I - hg*J directly in LU slot storage
     auto jname = [](int i, int a) { return "J" + std::to_string(i) + "_" + std::to_string(a); };
+    e << "template <int US = 1>\n";
     e << "RS_FN void rs_jac_W(const double* __restrict__ K, double jf, const double* __restrict__ u,"
          " double hg, double* __restrict__ W) {\n";
     for (int i = 0; i < n; ++i)
@@ -204,8 +206,8 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         int a = m.ra[r], b = m.rb[r];
         e << "  { const double kc = " << kexpr(m, r) << ";";
         if (b < 0) e << " const double da = kc;";
-        else if (a == b) e << " const double da = (2.0 * kc) * u[" << a << "];";
-        else e << " const double da = kc * u[" << b << "]; const double db = kc * u[" << a << "];";
+        else if (a == b) e << " const double da = (2.0 * kc) * u[" << a << " * US];";
+        else e << " const double da = kc * u[" << b << " * US]; const double db = kc * u[" << a << " * US];";
         for (auto& pr : p.net[r]) {
             e << " " << accum(jname(pr.first, a), pr.second, "da");
             if (b >= 0 && b != a) e << " " << accum(jname(pr.first, b), pr.second, "db");
@@ -275,6 +277,7 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
     }
     e << "}\n";
     // ---- out += J(u)^T z
+    e << "template <int US = 1>\n";
     e << "RS_FN void rs_jtv(const double* __restrict__ K, double jf, const double* __restrict__ u,"
          " const double* __restrict__ z, double* __restrict__ out) {\n";
     for (int r = 0; r < m.nr; ++r) {
@@ -284,14 +287,15 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         for (auto& pr : p.net[r]) e << " " << accum("s", pr.second, "z[" + std::to_string(pr.first) + "]");
         e << " const double kc = " << kexpr(m, r) << ";";
         if (b < 0) e << " out[" << a << "] = out[" << a << "] + kc * s;";
-        else if (a == b) e << " out[" << a << "] = out[" << a << "] + ((2.0 * kc) * u[" << a << "]) * s;";
+        else if (a == b) e << " out[" << a << "] = out[" << a << "] + ((2.0 * kc) * u[" << a << " * US]) * s;";
         else
-            e << " out[" << a << "] = out[" << a << "] + (kc * u[" << b << "]) * s; out[" << b
-              << "] = out[" << b << "] + (kc * u[" << a << "]) * s;";
+            e << " out[" << a << "] = out[" << a << "] + (kc * u[" << b << " * US]) * s; out[" << b
+              << "] = out[" << b << "] + (kc * u[" << a << " * US]) * s;";
         e << " }\n";
     }
     e << "}\n";
     // ---- o1 += J(u)^T z and o2 += c2 * J(u)^T z in one pass (adjoint of u1)
+    e << "template <int US = 1>\n";
     e << "RS_FN void rs_jtv2(const double* __restrict__ K, double jf, const double* __restrict__ u,"
          " const double* __restrict__ z, double* __restrict__ o1, double* __restrict__ o2,"
          " double c2) {\n";
@@ -306,15 +310,16 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
               << "] = o2[" << x << "] + c2 * t; }";
         };
         if (b < 0) both(a, "kc");
-        else if (a == b) both(a, "((2.0 * kc) * u[" + std::to_string(a) + "])");
+        else if (a == b) both(a, "((2.0 * kc) * u[" + std::to_string(a) + " * US])");
         else {
-            both(a, "(kc * u[" + std::to_string(b) + "])");
-            both(b, "(kc * u[" + std::to_string(a) + "])");
+            both(a, "(kc * u[" + std::to_string(b) + " * US])");
+            both(b, "(kc * u[" + std::to_string(a) + " * US])");
         }
         e << " }\n";
     }
     e << "}\n";
     // ---- ub += coef * d/du [ z^T J(u) v ]   (bimolecular reactions only)
+    e << "template <int VS = 1>\n";
     e << "RS_FN void rs_hess(const double* __restrict__ K, double jf, const double* __restrict__ z,"
          " const double* __restrict__ v, double coef, double* __restrict__ ub) {\n";
     for (int r = 0; r < m.nr; ++r) {
@@ -323,10 +328,10 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         e << "  { double s = 0.0;";
         for (auto& pr : p.net[r]) e << " " << accum("s", pr.second, "z[" + std::to_string(pr.first) + "]");
         e << " const double cs = (coef * " << kexpr(m, r) << ") * s;";
-        if (a == b) e << " ub[" << a << "] = ub[" << a << "] + (2.0 * cs) * v[" << a << "];";
+        if (a == b) e << " ub[" << a << "] = ub[" << a << "] + (2.0 * cs) * v[" << a << " * VS];";
         else
-            e << " ub[" << a << "] = ub[" << a << "] + cs * v[" << b << "]; ub[" << b << "] = ub["
-              << b << "] + cs * v[" << a << "];";
+            e << " ub[" << a << "] = ub[" << a << "] + cs * v[" << b << " * VS]; ub[" << b << "] = ub["
+              << b << "] + cs * v[" << a << " * VS];";
         e << " }\n";
     }
     e << "}\n";

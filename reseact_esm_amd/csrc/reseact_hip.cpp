@@ -136,6 +136,9 @@ struct rs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int ncu = 0;
+    size_t lds_max = 65536;      // dynamic LDS a workgroup may take on this device
+    std::vector<double> hK;      // rate constants (host copy: inline kernel argument)
+    std::vector<char> argbuf;    // chemistry kernel argument block
     int nlon = 0, nlat = 0, nlev = 0, ns = 0, nt = 0;
     long long nc = 0, ncol = 0;
     bool have_wind = false, have_mech = false, have_state = false, have_cand = false;
@@ -412,13 +415,34 @@ void enqueue_ssprk43(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* u
     { ProfScope ps(c, P_TLAST); HIPCHK(rs_launch_transport(KIND_LAST, c->tk, &a, c->stream)); }
 }
 
-void chem_launch(rs_ctx* c, hipFunction_t f, void* argp, size_t argsz, unsigned grid,
-                 unsigned block, int kind) {
-    void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, argp, HIP_LAUNCH_PARAM_BUFFER_SIZE, &argsz,
-                   HIP_LAUNCH_PARAM_END};
-    ProfScope ps(c, kind);
-    HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, c->stream, nullptr, cfg));
+// LDS slab of every runtime-compiled chemistry kernel for a workgroup of `block` lanes
+size_t chem_slab_bytes(const rs_ctx* c, unsigned block) {
+    return (size_t)block * RS_SLOW_DOUBLES(c->ns) * sizeof(double);
 }
+bool chem_block_fits(const rs_ctx* c, unsigned block) { return chem_slab_bytes(c, block) <= c->lds_max; }
+
+// Launch of a runtime-compiled chemistry kernel with the LDS slab its
+// workgroup size needs.  The argument block is the
+// kernel's argument struct followed, for mechanisms of at most
+// RS_K_INLINE_MAX reactions, by the rate constants (the kernels' second
+// parameter; both are 8-byte aligned).
+void chem_launch(rs_ctx* c, hipFunction_t f, const void* argp, size_t argsz, unsigned grid,
+                 unsigned block, int kind) {
+    const size_t lds = chem_slab_bytes(c, block);
+    const bool kin = c->nr > 0 && c->nr <= RS_K_INLINE_MAX;
+    const size_t off = (argsz + 7) & ~(size_t)7;
+    size_t total = kin ? off + (size_t)c->nr * sizeof(double) : argsz;
+    c->argbuf.assign(total, 0);
+    std::memcpy(c->argbuf.data(), argp, argsz);
+    if (kin) std::memcpy(c->argbuf.data() + off, c->hK.data(), (size_t)c->nr * sizeof(double));
+    void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, c->argbuf.data(), HIP_LAUNCH_PARAM_BUFFER_SIZE,
+                   &total, HIP_LAUNCH_PARAM_END};
+    ProfScope ps(c, kind);
+    HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, (unsigned)lds, c->stream, nullptr,
+                                 cfg));
+}
+
+
 
 
 unsigned chem_grid(rs_ctx* c) { return (unsigned)((c->nc + c->chem_block - 1) / c->chem_block); }
@@ -612,8 +636,9 @@ int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_ho
     HIPCHK(hipMemcpyAsync(c->stats.p, c->hstats, sizeof(init), hipMemcpyHostToDevice, c->stream));
     HIPCHK(rs_launch_fill_u64(c->head.p, 1, 0ULL, c->stream));
     int per_cu = 0;
+    const size_t slab = chem_slab_bytes(c, c->chem_block);
     HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c->chem->percell,
-                                                              (int)c->chem_block, 0));
+                                                              (int)c->chem_block, slab));
     per_cu = std::max(per_cu, 1);
     long long grid = (long long)per_cu * c->ncu;
     grid = std::min<long long>(grid, (c->nc + c->chem_block - 1) / c->chem_block);
@@ -856,6 +881,7 @@ int rs_create(rs_ctx** out, int device) {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
         c->ncu = prop.multiProcessorCount;
+        c->lds_max = prop.sharedMemPerBlock;
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCHK(hipMalloc((void**)&c->dctrl, sizeof(RsCtrl)));
         HIPCHK(hipHostMalloc((void**)&c->hctrl, sizeof(RsCtrl), hipHostMallocDefault));
@@ -990,6 +1016,7 @@ int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const doub
         HIPCHK(hipModuleGetFunction(&cm->vjp, cm->mod, "rs_chem_vjp"));
         c->K.resize(std::max(nr, 1));
         if (nr) h2d(c->K.p, k, nr, c->stream);
+        c->hK.assign(k, k + nr);
         HIPCHK(hipStreamSynchronize(c->stream));
         c->nr = nr;
         c->chem = std::move(cm);
@@ -1392,6 +1419,7 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
         const unsigned bl[3] = {64, 128, 256};
         double best = 1e300;
         for (int q = 0; q < 3; ++q) {
+            if (!chem_block_fits(c, bl[q])) continue;
             c->chem_block = bl[q];
             r.chem_ms[q] = time_ms(c, [&]() {
                 enqueue_ros23(c, nullptr, c->S[0].p, c->S[1].p, c->cur, 1.0, 1e-6, 1e-3, 0);

@@ -14,6 +14,16 @@ typedef struct RsCtrl {
     int done, accept, cur, nlog, logcap, pad;
 } RsCtrl;
 
+/* LDS slab of the chemistry kernels, in doubles per lane (0 = none): three
+ * state vectors per lane, laid out per wave as [3][ns][64] so that a wave's
+ * accesses to one species are 64 consecutive doubles; used while four waves
+ * (one per SIMD) still fit in the 160 KiB of LDS of a CU. */
+#define RS_SLOW_DOUBLES(ns) ((3 * (ns) * 8 * 256 <= 160 * 1024) ? 3 * (ns) : 0)
+
+/* Mechanisms with at most this many reactions get their rate constants as a
+ * kernel argument (read with scalar loads) instead of through a pointer. */
+#define RS_K_INLINE_MAX 448
+
 /* One ROS23 attempt over all cells.  With ctrl != 0 the step size, the
  * current state buffer and the done flag are read from it (adaptive mode). */
 typedef struct RsChemArgs {

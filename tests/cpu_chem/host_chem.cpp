@@ -16,12 +16,12 @@ extern "C" double hc_attempt(long long nc, long long ncol, const double* K, cons
                              double atol, double rtol) {
     double tot = 0.0;
     for (long long c = 0; c < nc; ++c) {
-        double uc[RS_NS], un[RS_NS];
-        for (int s = 0; s < RS_NS; ++s) uc[s] = u[s * nc + c];
+        double sv[3 * RS_NS];
+        for (int s = 0; s < RS_NS; ++s) sv[2 * RS_NS + s] = u[s * nc + c];
         const double jf = jfac ? jfac[c] : 1.0;
         const double* Ec = (E && c < ncol) ? E + c : nullptr;
-        tot += rs_ros23_attempt(K, jf, Ec, ncol, uc, un, dt, atol, rtol);
-        for (int s = 0; s < RS_NS; ++s) unew[s * nc + c] = un[s];
+        tot += rs_ros23_attempt<1, false>(K, jf, Ec, ncol, sv, dt, atol, rtol,
+                                          [&](int s, double un) { unew[s * nc + c] = un; });
     }
     return tot;
 }
@@ -29,15 +29,16 @@ extern "C" double hc_attempt(long long nc, long long ncol, const double* K, cons
 extern "C" void hc_vjp(long long nc, long long ncol, const double* K, const double* jfac,
                        const double* E, const double* u, double dt, double* lam, double* gE) {
     for (long long c = 0; c < nc; ++c) {
-        double uc[RS_NS], lc[RS_NS];
+        double sv[3 * RS_NS], lc[RS_NS];
         for (int s = 0; s < RS_NS; ++s) {
-            uc[s] = u[s * nc + c];
+            sv[s] = u[s * nc + c];
             lc[s] = lam[s * nc + c];
         }
         const double jf = jfac ? jfac[c] : 1.0;
         const double* Ec = (E && c < ncol) ? E + c : nullptr;
         double* Gc = (c < ncol) ? gE + c : nullptr;
-        rs_ros23_vjp(K, jf, Ec, ncol, uc, dt, lc, Gc);
+        rs_ros23_vjp<1>(K, jf, Ec, ncol, sv, dt, [&](int s) { return lc[s]; },
+                        [&](int s, double v) { lc[s] = v; }, Gc);
         for (int s = 0; s < RS_NS; ++s) lam[s * nc + c] = lc[s];
     }
 }

@@ -42,7 +42,10 @@ class HostChem:
     def __init__(self, mech, ordering: int = 1, exact_div: bool = False):
         src, info = generate(mech, ordering)
         self.info = dict(zip(info.split()[0::2], map(int, info.split()[1::2])))
-        key = hashlib.sha1((src + str(exact_div)).encode()).hexdigest()[:16]
+        deps = "".join(open(os.path.join(d_, f)).read() for d_, f in
+                       ((HERE, "host_chem.cpp"), (CSRC, "chem_kernels.hip"),
+                        (CSRC, "chem_prelude.h"), (CSRC, "rs_launch.h")))
+        key = hashlib.sha1((src + deps + str(exact_div)).encode()).hexdigest()[:16]
         d = os.path.join(WORK, key)
         os.makedirs(d, exist_ok=True)
         so = os.path.join(d, "hostchem.so")

@@ -25,7 +25,9 @@ def test_autotune_keeps_results(hip, oracle):
     r = cg.autotune()
     assert r.transport_tile_levels in (4, 8, 16, 32)
     assert r.chem_block in (64, 128, 256)
-    assert all(t > 0 for t in r.transport_ms) and all(t > 0 for t in r.chem_ms)
+    assert all(t > 0 for t in r.transport_ms)
+    # a workgroup size whose LDS slab does not fit the device is skipped (0)
+    assert r.chem_ms[0] > 0 and all(t >= 0 for t in r.chem_ms)
     co = oracle.context().load(p)
     opts = split_opts(p)
     rg = cg.macro_step(0.0, 900.0, 15.0, 0.5, opts)
