@@ -92,7 +92,10 @@ def coupled(lib, steps: int, warmup: int):
     flops = info["flops_attempt"] * p.nc
     avg_ms = ms_att / max(n_att, 1)
     achieved = flops / (avg_ms * 1e-3) / 1e12 if n_att else 0.0
-    roof = {"bound": "fp64", "kernel": "chem_attempt", "achieved": round(achieved, 3),
+    # compute roof: FP64 vector (VALU) issue; on MI355X the FP64 vector and
+    # FP64 matrix peaks are both 78.6 TFLOP/s
+    roof = {"bound": "mfma", "compute": "fp64 valu", "kernel": "chem_attempt",
+            "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
             "avg_ms": round(avg_ms, 4), "attempts": n_att, "launches": n_launch,
@@ -161,7 +164,8 @@ def chemistry_roofline(lib):
     log(f"chemistry {p.name}: attempts {st.attempts_total} (min {st.attempts_min} max "
         f"{st.attempts_max}) kernel {ms:.1f} ms wall {t1 - t0:.2f} s {achieved:.2f} TFLOP/s "
         f"regs {info['regs_percell']} scratch {info['scratch_percell']}")
-    return {"workload": p.name, "bound": "fp64", "achieved": round(achieved, 3),
+    return {"workload": p.name, "bound": "mfma", "compute": "fp64 valu", "kernel": "chem_percell",
+            "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "kernel_ms": round(ms, 2),
             "attempts": st.attempts_total, "attempts_min": st.attempts_min,
@@ -278,6 +282,51 @@ def probe(lib) -> dict:
             "chemistry": {"workload": p4.name, "cells": p4.nc}}
 
 
+def _pmc_pass(counter: str, outdir: str, timeout_s: int = 150) -> dict:
+    """One rocprofv3 counter pass over `bench.py --probe` in a child process;
+    returns {kernel name: [per-dispatch KiB]}."""
+    import csv
+    import shutil
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        raise RuntimeError("rocprofv3 not found")
+    os.makedirs(outdir, exist_ok=True)
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", counter.lower(),
+           "--", sys.executable, os.path.join(REPO, "bench.py"), "--probe"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s,
+                       env=dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp")))
+    if r.returncode != 0:
+        raise RuntimeError(f"rocprofv3 {counter} exited {r.returncode}")
+    fn = os.path.join(outdir, f"{counter.lower()}_counter_collection.csv")
+    out: dict = {}
+    with open(fn) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] == counter:
+                out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return out
+
+
+def traffic(outdir: str) -> dict:
+    """HBM bytes per launch of the two roofline kernels from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for
+    the probe's 1 GiB device-to-device copy (bytes moved / counter bytes)."""
+    fe = _pmc_pass("FETCH_SIZE", outdir)
+    wr = _pmc_pass("WRITE_SIZE", outdir)
+    copy = 8.0 * (1 << 27)
+    cname = next(k for k in fe if "copyBuffer" in k)
+    f_fetch = copy / (max(fe[cname]) * 1024.0)
+    f_write = copy / (max(wr[cname]) * 1024.0)
+    res = {"calibration": {"kernel": cname, "bytes": copy, "fetch_factor": round(f_fetch, 4),
+                           "write_factor": round(f_write, 4)}}
+    for key, pat in (("transport_fused", "rs_transport_fused<0>"), ("chem_attempt", "rs_chem_attempt")):
+        kf = next(k for k in fe if pat in k)
+        kw = next(k for k in wr if pat in k)
+        rd = sum(fe[kf]) / len(fe[kf]) * 1024.0 * f_fetch
+        wt = sum(wr[kw]) / len(wr[kw]) * 1024.0 * f_write
+        res[key] = {"read_bytes": round(rd), "write_bytes": round(wt), "bytes": round(rd + wt)}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,6 +337,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-domain ensemble")
     ap.add_argument("--probe", action="store_true",
                     help="only the short hardware-counter run (copy + one transport and chemistry)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure HBM traffic")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the split step shards by "
@@ -305,7 +356,21 @@ def main() -> int:
     if not a.quick and not a.no_ensemble:
         extra["ensemble"] = ensemble_run(lib)
     cpu = None if a.no_cpu else cpu_baseline()
+    tr = None
+    if not a.no_traffic:
+        try:
+            tr = traffic(os.path.join(REPO, "gpurun_out", "bench_pmc"))
+            log(f"traffic: {tr}")
+        except Exception as e:  # the counters are a report, not the measurement
+            log(f"traffic counters unavailable: {e}")
     p = c["p"]
+    if tr:
+        c["roof"]["traffic"] = tr["chem_attempt"]["bytes"]
+        c["roof"]["traffic_read_write"] = [tr["chem_attempt"]["read_bytes"],
+                                           tr["chem_attempt"]["write_bytes"]]
+        if "transport_hbm" in extra:
+            extra["transport_hbm"]["traffic"] = tr["transport_fused"]["bytes"]
+        extra["traffic_calibration"] = tr["calibration"]
     value = p.nc * c["steps"] / c["fwd_s"]
     line = {
         "metric": "coupled cell-steps/s fwd; adjoint/forward time ratio; % HBM & FP64 roofline",
