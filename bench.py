@@ -196,6 +196,8 @@ def ensemble_run(lib, ndom: int = 32, nmacro: int = 6, tape_budget_gib: float = 
            "forward_cell_steps_per_s": round(cells * nmacro / es.seconds_forward, 1),
            "seconds_forward": round(es.seconds_forward, 3),
            "seconds_backward": round(es.seconds_backward, 3),
+           "seconds_backward_setup": round(es.seconds_backward_setup, 3),
+           "seconds_backward_run": round(es.seconds_backward_run, 3),
            "adjoint_ratio": round((es.seconds_forward + es.seconds_backward) / es.seconds_forward, 3),
            "graph_launches_forward": es.graph_launches_forward,
            "graph_launches_backward": es.graph_launches_backward,

@@ -255,6 +255,8 @@ RS_API int rs_autotune(rs_ctx* ctx, rs_tune_result* out);
  * stream) launched until all controllers are done.  Backward: members are
  * grouped so their replay tapes fit `tape_budget_gib`, and each group's whole
  * backward sweep (replays + VJPs over all split steps) is one graph launch.
+ * The groups' tapes are slices of one device pool, sized once for the
+ * largest group and kept by the ensemble.
  * Each member's gradients stay in its context (rs_get_lambda = dJ/du0,
  * rs_get_grad_emis).  No counterpart in the reference, which runs one domain
  * per process. */
@@ -264,6 +266,9 @@ typedef struct rs_ensemble_stats {
     long long graph_launches_forward, graph_launches_backward;
     long long graph_nodes_backward;
     int backward_groups;
+    int pad;
+    double seconds_backward_setup;   /* tape pool, capture and instantiation */
+    double seconds_backward_run;     /* graph launches until done */
 } rs_ensemble_stats;
 RS_API int rs_ensemble_create(rs_ensemble** out, rs_ctx* const* members, int n);
 RS_API int rs_ensemble_destroy(rs_ensemble* e);

@@ -138,6 +138,8 @@ struct rs_ctx {
     int ncu = 0;
     size_t lds_max = 65536;      // dynamic LDS a workgroup may take on this device
     std::vector<double> hK;      // rate constants (host copy: inline kernel argument)
+    double* tape_view = nullptr; // replay tape lent by an ensemble (else `tape`)
+    size_t tape_view_n = 0;
     std::vector<char> argbuf;    // chemistry kernel argument block
     int nlon = 0, nlat = 0, nlev = 0, ns = 0, nt = 0;
     long long nc = 0, ncol = 0;
@@ -750,6 +752,17 @@ double loss_on_device(rs_ctx* c, const double* u, int species) {
     return host_sum(c, c->red.p, nb);
 }
 
+// The replay tape: the ensemble's pool slice when one is lent, else the
+// context's own buffer, grown to `n` doubles.
+double* tape_ptr(rs_ctx* c, size_t n) {
+    if (c->tape_view) {
+        if (n > c->tape_view_n) throw ArgError{"replay tape slice too small", RS_E_STATE};
+        return c->tape_view;
+    }
+    c->tape.ensure(n);
+    return c->tape.p;
+}
+
 // Fixed-sequence replay of one macro step from `u0` into the tape; tape slot j
 // holds the input of step j (transport steps first, then chemistry), the last
 // slot the end state.  Returns the number of slots written.
@@ -757,8 +770,7 @@ size_t replay_into_tape(rs_ctx* c, const double* u0, const std::vector<std::pair
                         const std::vector<std::pair<double, double>>& sC, const rs_split_opts& o) {
     const size_t N = c->N();
     const size_t nslots = sT.size() + sC.size() + 1;
-    c->tape.ensure(nslots * N);
-    double* tp = c->tape.p;
+    double* tp = tape_ptr(c, nslots * N);
     d2d(tp, u0, N, c->stream);
     size_t j = 0;
     for (auto& p : sT) {
@@ -785,7 +797,7 @@ void size_tape(rs_ctx* c, int nmacro) {
     size_t maxslots = 1;
     for (int m = 0; m < nmacro; ++m)
         maxslots = std::max(maxslots, c->seqT[m].size() + c->seqC[m].size() + 1);
-    c->tape.ensure(maxslots * c->N());
+    if (!c->tape_view) c->tape.ensure(maxslots * c->N());
 }
 
 size_t tape_bytes(rs_ctx* c, int nmacro) {
@@ -819,7 +831,7 @@ void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* re
     const auto& sC = c->seqC[m];
     if (ev0) HIPCHK(hipEventRecord(ev0, c->stream));
     const size_t ns = replay_into_tape(c, c->ckpt.p + (size_t)m * N, sT, sC, o);
-    double* tp = c->tape.p;
+    double* tp = tape_ptr(c, ns * N);
     HIPCHK(rs_launch_relmax(tp + (ns - 1) * N, c->ckpt.p + (size_t)(m + 1) * N, (long long)N,
                             relpart, c->stream));
     if (ev1) HIPCHK(hipEventRecord(ev1, c->stream));
@@ -1294,7 +1306,7 @@ int rs_replay_loss(rs_ctx* c, int loss_kind, int loss_species, double* J_out) {
         d2d(tmp.p, u, N, c->stream);
         for (size_t m = 0; m < c->seqT.size(); ++m) {
             const size_t ns = replay_into_tape(c, tmp.p, c->seqT[m], c->seqC[m], c->last_opts);
-            d2d(tmp.p, c->tape.p + (ns - 1) * N, N, c->stream);
+            d2d(tmp.p, tape_ptr(c, ns * N) + (ns - 1) * N, N, c->stream);
         }
         const double J = loss_on_device(c, tmp.p, loss_species);
         if (J_out) *J_out = J;
@@ -1438,6 +1450,7 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
 
 struct rs_ensemble {
     std::vector<rs_ctx*> m;
+    DevArray<double> pool;       // replay tapes of one backward group, reused by every group
     int device = 0;
     hipStream_t origin = nullptr;
     hipEvent_t fork = nullptr;
@@ -1653,16 +1666,38 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
             groups.back().push_back(d);
             acc += b;
         }
+        // one pool, sized for the largest group, lent to the members of each
+        // group in turn (no allocation between groups)
+        size_t pool_n = 0;
         for (auto& grp : groups) {
-            for (int d : grp) size_tape(e->m[d], nmacro);
+            size_t g = 0;
+            for (int d : grp) g += tape_bytes(e->m[d], nmacro) / sizeof(double);
+            pool_n = std::max(pool_n, g);
+        }
+        auto ts0 = clk::now();
+        e->pool.ensure(pool_n);
+        ES.seconds_backward_setup += std::chrono::duration<double>(clk::now() - ts0).count();
+        for (auto& grp : groups) {
+            auto tg0 = clk::now();
+            size_t off = 0;
+            for (int d : grp) {
+                rs_ctx* c = e->m[d];
+                const size_t need = tape_bytes(c, nmacro) / sizeof(double);
+                c->tape_view = e->pool.p + off;
+                c->tape_view_n = need;
+                off += need;
+            }
             size_t nn = 0;
             hipGraphExec_t gB = ens_capture(e, grp, [&](int d) {
                 rs_ctx* c = e->m[d];
                 for (int mstep = nmacro - 1; mstep >= 0; --mstep)
                     enqueue_backward_macro(c, mstep, *o, c->relpart.p + (size_t)mstep * 1024, nullptr, nullptr);
             }, &nn);
+            auto tg1 = clk::now();
+            ES.seconds_backward_setup += std::chrono::duration<double>(tg1 - tg0).count();
             HIPCHK(hipGraphLaunch(gB, e->origin));
             HIPCHK(hipStreamSynchronize(e->origin));
+            ES.seconds_backward_run += std::chrono::duration<double>(clk::now() - tg1).count();
             (void)hipGraphExecDestroy(gB);
             ES.graph_launches_backward += 1;
             ES.graph_nodes_backward += (long long)nn;
@@ -1673,16 +1708,17 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
                     S[d].nvjp_T += (long long)c->seqT[mstep].size();
                     S[d].nvjp_C += (long long)c->seqC[mstep].size();
                 }
-                if (groups.size() > 1) c->tape.release();
+                c->tape_view = nullptr;
+                c->tape_view_n = 0;
             }
         }
         ES.backward_groups = (int)groups.size();
         ES.seconds_backward = std::chrono::duration<double>(clk::now() - tb0).count();
     } catch (const HipError& x) {
-        for (int d = 0; d < n; ++d) e->m[d]->prof = profs[d];
+        for (int d = 0; d < n; ++d) { e->m[d]->prof = profs[d]; e->m[d]->tape_view = nullptr; }
         return ens_fail(e, RS_E_HIP, std::string(x.what) + ": " + hipGetErrorString(x.e));
     } catch (const ArgError& x) {
-        for (int d = 0; d < n; ++d) e->m[d]->prof = profs[d];
+        for (int d = 0; d < n; ++d) { e->m[d]->prof = profs[d]; e->m[d]->tape_view = nullptr; }
         return ens_fail(e, x.code, x.msg);
     }
     for (int d = 0; d < n; ++d) {

@@ -88,7 +88,9 @@ class TuneResult(C.Structure):
 class EnsembleStats(C.Structure):
     _fields_ = [("seconds_forward", C.c_double), ("seconds_backward", C.c_double),
                 ("graph_launches_forward", C.c_longlong), ("graph_launches_backward", C.c_longlong),
-                ("graph_nodes_backward", C.c_longlong), ("backward_groups", C.c_int)]
+                ("graph_nodes_backward", C.c_longlong), ("backward_groups", C.c_int),
+                ("pad", C.c_int), ("seconds_backward_setup", C.c_double),
+                ("seconds_backward_run", C.c_double)]
 
 
 class KernelTime(C.Structure):
