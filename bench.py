@@ -56,25 +56,39 @@ def coupled(lib, steps: int, warmup: int):
     p = P.config4(nmacro=max(steps, 1))
     opts = split_opts(p)
     ctx = lib.context().load(p)
+    # setup, untimed: pick the transport kernel and chemistry launch shape on
+    # this grid (rs_autotune; the state is not changed)
+    tune = ctx.autotune()
+    log(f"autotune: transport fused={tune.transport_fused} ({tune.transport_fused_ms:.4f} ms) tile "
+        f"{tune.transport_tile_levels}; chemistry block {tune.chem_block} "
+        f"{[round(x, 4) for x in tune.chem_ms]} ms, rate constants via pointer={tune.chem_kpointer}")
     info = ctx.mechanism_info()
     log(f"coupled {p.name}: nc={p.nc} mech nnz_LU={info['nnz_LU']} flops/attempt="
         f"{info['flops_attempt']} regs={info['regs_attempt']} scratch={info['scratch_attempt']}")
     dtT, dtC = p.dt0_T, p.dt0_C
     for m in range(warmup):
         dtT, dtC, _, _ = ctx.macro_step(m * p.dt_macro, (m + 1) * p.dt_macro, dtT, dtC, opts)
-    ctx.set_state(p.u0)
+    def forward():
+        ctx.set_state(p.u0)
+        dtT, dtC = p.dt0_T, p.dt0_C
+        nT = nC = rT = rC = 0
+        t0 = time.perf_counter()
+        for m in range(steps):
+            dtT, dtC, cT, cC = ctx.macro_step(m * p.dt_macro, (m + 1) * p.dt_macro, dtT, dtC, opts)
+            nT += cT.naccept; rT += cT.nreject; nC += cC.naccept; rC += cC.nreject
+        return time.perf_counter() - t0, (nT, rT, nC, rC)
+
+    # the timed run records no per-launch events; a second, identical run
+    # with the library's HIP-event profiling gives the per-kernel times
+    fwd_s, (nT, rT, nC, rC) = forward()
     ctx.profile(True)
-    dtT, dtC = p.dt0_T, p.dt0_C
-    nT = nC = rT = rC = 0
-    t0 = time.perf_counter()
-    for m in range(steps):
-        dtT, dtC, cT, cC = ctx.macro_step(m * p.dt_macro, (m + 1) * p.dt_macro, dtT, dtC, opts)
-        nT += cT.naccept; rT += cT.nreject; nC += cC.naccept; rC += cC.nreject
-    t1 = time.perf_counter()
+    fwd_prof_s, counts2 = forward()
     prof = ctx.profile_read()
     ctx.profile(False)
-    fwd_s = t1 - t0
-    log(f"coupled forward {steps} steps {fwd_s:.3f} s; T {nT}/{rT} C {nC}/{rC}; kernels {prof}")
+    if counts2 != (nT, rT, nC, rC):
+        raise RuntimeError(f"profiled forward took another step sequence: {counts2}")
+    log(f"coupled forward {steps} steps {fwd_s:.3f} s (with per-launch events {fwd_prof_s:.3f} s); "
+        f"T {nT}/{rT} C {nC}/{rC}; kernels {prof}")
     # adjoint over the same steps; one untimed run first sizes the checkpoint
     # store and the replay tape (device allocations kept by the context)
     ctx.set_state(p.u0)
@@ -100,6 +114,9 @@ def coupled(lib, steps: int, warmup: int):
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
             "avg_ms": round(avg_ms, 4), "attempts": n_att, "launches": n_launch,
             "flops_per_launch": flops}
+    roof["launch"] = {"chem_block": tune.chem_block, "rate_constants": "pointer" if tune.chem_kpointer
+                      else "kernel argument", "transport": "fused" if tune.transport_fused
+                      else f"per-stage, {tune.transport_tile_levels} levels"}
     return dict(p=p, fwd_s=fwd_s, steps=steps, ratio=ratio, adj=st, roof=roof, prof=prof,
                 counts=(nT, rT, nC, rC), info=info, J=J)
 

@@ -235,8 +235,10 @@ RS_API int rs_profile(rs_ctx* ctx, int enable);
 
 /* Autotuner: times one SSPRK43 attempt for every transport tile height
  * (4, 8, 16, 32 levels per 64-longitude workgroup) and for the fused
- * kernel, and one ROS23 attempt for every chemistry workgroup size (64, 128,
- * 256) on this context's own grid and state, and keeps the fastest of each.
+ * kernel, one ROS23 attempt for every chemistry workgroup size (64, 128,
+ * 256) and, at the fastest size, with the rate constants passed as a kernel
+ * argument or through a pointer, on this context's own grid and state, and
+ * keeps the fastest of each.
  * The state is not changed. */
 typedef struct rs_tune_result {
     int transport_tile_levels;
@@ -246,6 +248,9 @@ typedef struct rs_tune_result {
     int transport_fused;         /* 1 if the fused kernel was kept */
     int pad;
     double transport_fused_ms;   /* 0 if the grid does not allow it */
+    int chem_kpointer;           /* 1: attempt reads rate constants through a pointer */
+    int pad2;
+    double chem_kp_ms;           /* attempt time of the pointer variant at chem_block */
 } rs_tune_result;
 RS_API int rs_autotune(rs_ctx* ctx, rs_tune_result* out);
 

@@ -259,32 +259,41 @@ typedef struct { double k[RS_NR]; } RsKTab;
 #define RS_SLAB(name) double name##_arr[3 * RS_NS]; double* name = name##_arr
 #endif
 
-extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK)
-rs_chem_attempt(RsChemArgs a RS_KPARAM) {
-    __shared__ double sh[RS_CHEM_BLOCK / 64];
-    RS_SLAB(sv);
-    const RsCtrl* c = a.ctrl;
-    if (c && c->done) return;
-    const int cur = c ? c->cur : a.cur_static;
-    const double dt = c ? c->dtc : a.dt_static;
-    const double* __restrict__ src = cur ? a.S1 : a.S0;
-    double* __restrict__ dst = cur ? a.S0 : a.S1;
-    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    double sse = 0.0;
-    if (cell < a.nc) {
-#pragma unroll
-        for (int s = 0; s < RS_NS; ++s) sv[(2 * RS_NS + s) * RS_SLAB_ST] = src[(long long)s * a.nc + cell];
-        const double jf = a.jfac ? a.jfac[cell] : 1.0;
-        const double* Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;
-        const int clamp = a.clamp;
-        sse = rs_ros23_attempt<RS_SLAB_ST, false>(
-            RS_KPTR(a), jf, Ec, a.ncol, sv, dt, a.atol, a.rtol, [&](int s, double un) {
-                dst[(long long)s * a.nc + cell] = clamp ? rs_max(un, 0.0) : un;
-            });
+/* One attempt over all cells, in two entry points: the rate constants as a
+ * kernel argument (scalar loads; RS_KPARAM) or through the pointer in the
+ * argument block (vector loads, fewer SGPRs held); the autotuner keeps the
+ * faster on the context's grid.  The body is spelled out in each kernel (a
+ * macro): shared through a device function, the compiler spilled 100-140
+ * B/lane more. */
+#define RS_CHEM_ATTEMPT_KERNEL(NAME, KPARAM, KEXPR)                                         \
+    extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK) NAME(RsChemArgs a KPARAM) {  \
+        __shared__ double sh[RS_CHEM_BLOCK / 64];                                           \
+        RS_SLAB(sv);                                                                        \
+        const RsCtrl* c = a.ctrl;                                                           \
+        if (c && c->done) return;                                                           \
+        const int cur = c ? c->cur : a.cur_static;                                          \
+        const double dt = c ? c->dtc : a.dt_static;                                         \
+        const double* __restrict__ src = cur ? a.S1 : a.S0;                                 \
+        double* __restrict__ dst = cur ? a.S0 : a.S1;                                       \
+        const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;            \
+        double sse = 0.0;                                                                   \
+        if (cell < a.nc) {                                                                  \
+            _Pragma("unroll") for (int s = 0; s < RS_NS; ++s)                               \
+                sv[(2 * RS_NS + s) * RS_SLAB_ST] = src[(long long)s * a.nc + cell];         \
+            const double jf = a.jfac ? a.jfac[cell] : 1.0;                                  \
+            const double* Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;               \
+            const int clamp = a.clamp;                                                      \
+            sse = rs_ros23_attempt<RS_SLAB_ST, false>(                                      \
+                KEXPR, jf, Ec, a.ncol, sv, dt, a.atol, a.rtol, [&](int s, double un) {      \
+                    dst[(long long)s * a.nc + cell] = clamp ? rs_max(un, 0.0) : un;         \
+                });                                                                         \
+        }                                                                                   \
+        const double tot = rs_block_sum(sse, sh);                                           \
+        if (threadIdx.x == 0) a.partial[blockIdx.x] = tot;                                  \
     }
-    const double tot = rs_block_sum(sse, sh);
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = tot;
-}
+
+RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt, RS_KPARAM, RS_KPTR(a))
+RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt_kp, , a.K)
 
 /* Persistent per-cell adaptive solve with wave-aggregated work stealing: a
  * lane that finishes its cell takes the next index from the queue head; all

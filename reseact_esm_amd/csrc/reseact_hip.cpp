@@ -115,7 +115,7 @@ struct DevArray {
 
 struct ChemModule {
     hipModule_t mod = nullptr;
-    hipFunction_t attempt = nullptr, percell = nullptr, vjp = nullptr;
+    hipFunction_t attempt = nullptr, attempt_kp = nullptr, percell = nullptr, vjp = nullptr;
     rs::MechPlan plan;
     ~ChemModule() { if (mod) (void)hipModuleUnload(mod); }
 };
@@ -151,6 +151,7 @@ struct rs_ctx {
     int fplan[6] = {0, 0, 0, 0, 0, 0};  // fused plan: seg_w, seg_in, seg_halo, nseg, seg_lev, seg_chunks
     bool fplan_ok = false;
     unsigned chem_block = 64;    // chemistry threads per workgroup
+    bool chem_kp = false;        // attempt kernel reads the rate constants through a pointer
     DevArray<double> S[2], X, Y, cx, wz, wl, K, jfac, E, lam, LA, LB, LC, gE, partial, seqlog;
     DevArray<double> red;
     DevArray<unsigned long long> head;
@@ -429,9 +430,9 @@ bool chem_block_fits(const rs_ctx* c, unsigned block) { return chem_slab_bytes(c
 // RS_K_INLINE_MAX reactions, by the rate constants (the kernels' second
 // parameter; both are 8-byte aligned).
 void chem_launch(rs_ctx* c, hipFunction_t f, const void* argp, size_t argsz, unsigned grid,
-                 unsigned block, int kind) {
+                 unsigned block, int kind, bool k_arg = true) {
     const size_t lds = chem_slab_bytes(c, block);
-    const bool kin = c->nr > 0 && c->nr <= RS_K_INLINE_MAX;
+    const bool kin = k_arg && c->nr > 0 && c->nr <= RS_K_INLINE_MAX;
     const size_t off = (argsz + 7) & ~(size_t)7;
     size_t total = kin ? off + (size_t)c->nr * sizeof(double) : argsz;
     c->argbuf.assign(total, 0);
@@ -466,7 +467,8 @@ void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cu
     a.rtol = rtol;
     a.cur_static = cur_static;
     a.clamp = clamp;
-    chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), c->chem_block, P_CHEM);
+    if (c->chem_kp) chem_launch(c, c->chem->attempt_kp, &a, sizeof(a), chem_grid(c), c->chem_block, P_CHEM, false);
+    else chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), c->chem_block, P_CHEM);
 }
 
 // Sum of n device partials: folded on the device to at most 1024 values first.
@@ -1024,6 +1026,7 @@ int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const doub
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipModuleLoadData(&cm->mod, code.data()));
         HIPCHK(hipModuleGetFunction(&cm->attempt, cm->mod, "rs_chem_attempt"));
+        HIPCHK(hipModuleGetFunction(&cm->attempt_kp, cm->mod, "rs_chem_attempt_kp"));
         HIPCHK(hipModuleGetFunction(&cm->percell, cm->mod, "rs_chem_percell"));
         HIPCHK(hipModuleGetFunction(&cm->vjp, cm->mod, "rs_chem_vjp"));
         c->K.resize(std::max(nr, 1));
@@ -1439,6 +1442,15 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
             if (r.chem_ms[q] < best) { best = r.chem_ms[q]; r.chem_block = (int)bl[q]; }
         }
         c->chem_block = (unsigned)r.chem_block;
+        // the same attempt with the rate constants through a pointer
+        const bool kp0 = c->chem_kp;
+        c->chem_kp = !kp0;
+        const double alt = time_ms(c, [&]() {
+            enqueue_ros23(c, nullptr, c->S[0].p, c->S[1].p, c->cur, 1.0, 1e-6, 1e-3, 0);
+        }, 3);
+        if (alt >= best) c->chem_kp = kp0;
+        r.chem_kpointer = c->chem_kp ? 1 : 0;
+        r.chem_kp_ms = kp0 ? best : alt;
     }
     c->prof = prof;
     c->have_cand = false;

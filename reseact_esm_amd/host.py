@@ -82,7 +82,8 @@ class MechInfo(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("transport_tile_levels", C.c_int), ("chem_block", C.c_int),
                 ("transport_ms", C.c_double * 4), ("chem_ms", C.c_double * 3),
-                ("transport_fused", C.c_int), ("pad", C.c_int), ("transport_fused_ms", C.c_double)]
+                ("transport_fused", C.c_int), ("pad", C.c_int), ("transport_fused_ms", C.c_double),
+                ("chem_kpointer", C.c_int), ("pad2", C.c_int), ("chem_kp_ms", C.c_double)]
 
 
 class EnsembleStats(C.Structure):
