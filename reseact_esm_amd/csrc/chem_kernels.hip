@@ -18,7 +18,9 @@
  *   slot A: 2(k1 - f0)  ->  + c32 (k2 - f1)       (the two terms of k3's rhs)
  *   slot B: k1          ->  k1 - 2 k2             (the error's first term)
  *   slot C: u on entry  ->  atol + rtol max(|u|, |unew|) (unless KEEP_U)
- * The factors of W and three short-lived vectors stay in registers.  unew is
+ * W (the LU factors, stride WS) is a register array for small mechanisms and
+ * the lane's LDS slice for large ones; three short-lived vectors stay in
+ * registers.  unew is
  * handed to out(i, unew_i) as soon as it exists.  The exact build
  * (RS_EXACT_DIV) keeps the reference's evaluation order (f2 - a) - c for k3's
  * rhs; the fast build adds a + c first. */
@@ -34,18 +36,17 @@
 #define RS_SA(i) sv[(i) * ST]
 #define RS_SB(i) sv[(RS_NS + (i)) * ST]
 #define RS_SC(i) sv[(2 * RS_NS + (i)) * ST]
-template <int ST, bool KEEP_U, class Out>
+template <int ST, int WS, bool KEEP_U, class Out>
 RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
                               const double* __restrict__ E, long long es,
-                              double* __restrict__ sv, double dt, double atol, double rtol,
-                              Out out) {
-    double W[RS_NNZ];
+                              double* __restrict__ sv, double* __restrict__ W, double dt,
+                              double atol, double rtol, Out out) {
     double f[RS_NS], x[RS_NS], y[RS_NS];
     rs_rhs<ST>(K, jf, E, es, &RS_SC(0), f);                        /* f0, :282 */
     const double hg = dt * RS_ROS23_D;                             /* :291 */
-    rs_jac_W<ST>(K, jf, &RS_SC(0), hg, W);                         /* :287-296 */
-    rs_lu(W);
-    rs_solve(W, f, x);                                             /* k1, :298 */
+    rs_jac_W<ST, WS>(K, jf, &RS_SC(0), hg, W);                     /* :287-296 */
+    rs_lu<WS>(W);
+    rs_solve<WS>(W, f, x);                                         /* k1, :298 */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) {                              /* :323-324 */
         const double c = x[i] - f[i];
@@ -58,7 +59,7 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
     rs_rhs(K, jf, E, es, y, f);                                    /* f1, :303 */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) y[i] = f[i] - x[i];            /* :306-309 */
-    rs_solve(W, y, x);                                             /* :310 */
+    rs_solve<WS>(W, y, x);                                         /* :310 */
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) x[i] = x[i] + RS_SB(i);        /* k2, :311-313 */
 #pragma unroll
@@ -71,7 +72,10 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
         RS_SA(i) = ca + RS_SA(i);
 #endif
     }
+    /* error denominators: slot C (u is dead by then), or registers when u
+       must survive the attempt (KEEP_U) */
     double den[KEEP_U ? RS_NS : 1];
+#define RS_DEN(i) den[KEEP_U ? (i) : 0]
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) {
         const double u = RS_SC(i);
@@ -82,7 +86,7 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
         const double am = rs_max(RS_FABS(u), RS_FABS(un));        /* :335-346 */
         double d = am * rtol;
         d = d + atol;
-        if (KEEP_U) den[KEEP_U ? i : 0] = d;
+        if (KEEP_U) RS_DEN(i) = d;
         else RS_SC(i) = d;
         const double w2 = 2.0 * x[i];                              /* :333-334 */
         RS_SB(i) = RS_SB(i) - w2;
@@ -99,18 +103,19 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
         f[i] = f[i] - RS_SA(i);
 #endif
     }
-    rs_solve(W, f, x);                                             /* k3, :328 */
+    rs_solve<WS>(W, f, x);                                         /* k3, :328 */
     const double dto6 = dt / 6;
     double sse = 0.0;
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) {
         double ut = RS_SB(i) + x[i];
         ut = dto6 * ut;
-        const double at = ut / (KEEP_U ? den[KEEP_U ? i : 0] : RS_SC(i));
+        const double at = ut / (KEEP_U ? RS_DEN(i) : RS_SC(i));
         sse += at * at;
     }
     return sse;
 }
+#undef RS_DEN
 #undef RS_SA
 #undef RS_SB
 #undef RS_SC
@@ -133,18 +138,17 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
 #define RS_SA(i) sv[(i) * ST]
 #define RS_SB(i) sv[(RS_NS + (i)) * ST]
 #define RS_SC(i) sv[(2 * RS_NS + (i)) * ST]
-template <int ST, class LIn, class LOut>
+template <int ST, int WS, class LIn, class LOut>
 RS_FN void rs_ros23_vjp(const double* __restrict__ K, double jf, const double* __restrict__ E,
-                        long long es, double* __restrict__ sv, double dt,
+                        long long es, double* __restrict__ sv, double* __restrict__ W, double dt,
                         LIn lam_in, LOut lam_out, double* __restrict__ G) {
-    double W[RS_NNZ];
     double f[RS_NS], x[RS_NS], y[RS_NS];
     rs_rhs<ST>(K, jf, E, es, &RS_SA(0), f);
     const double hg = dt * RS_ROS23_D;
     RS_PHASE();
-    rs_jac_W<ST>(K, jf, &RS_SA(0), hg, W);
-    rs_lu(W);
-    rs_solve(W, f, x);                                             /* k1 */
+    rs_jac_W<ST, WS>(K, jf, &RS_SA(0), hg, W);
+    rs_lu<WS>(W);
+    rs_solve<WS>(W, f, x);                                         /* k1 */
     RS_PHASE();
     const double dto2 = dt / 2;
 #pragma unroll
@@ -158,14 +162,14 @@ RS_FN void rs_ros23_vjp(const double* __restrict__ K, double jf, const double* _
     RS_PHASE();
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) f[i] = f[i] - x[i];
-    rs_solve(W, f, y);
+    rs_solve<WS>(W, f, y);
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) y[i] = y[i] + x[i];            /* k2 */
     /* reverse */
     double ub[RS_NS];
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) { const double l = lam_in(i); f[i] = dt * l; ub[i] = l; }  /* k2bar */
-    rs_solveT(W, f, x);                                            /* z2 */
+    rs_solveT<WS>(W, f, x);                                        /* z2 */
     RS_PHASE();
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) { f[i] = f[i] - x[i]; y[i] = y[i] - RS_SC(i); }  /* k1bar; v = k2 - k1 */
@@ -176,7 +180,7 @@ RS_FN void rs_ros23_vjp(const double* __restrict__ K, double jf, const double* _
     RS_PHASE();
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) y[i] = x[i];                   /* keep z2 for Ebar */
-    rs_solveT(W, f, x);                                            /* z1 */
+    rs_solveT<WS>(W, f, x);                                        /* z1 */
     RS_PHASE();
     rs_jtv<ST>(K, jf, &RS_SA(0), x, ub);
     RS_PHASE();
@@ -224,8 +228,18 @@ RS_FN int rs_pi_update(double beta1, double beta2, double invqmin, double invqma
 
 #define RS_CHEM_BLOCK 256   /* largest block the launcher uses (64, 128 or 256) */
 
+/* Sum over the workgroup, deterministic: a partial wave (the large-mechanism
+ * layout) adds its lanes in order through LDS; whole waves use a fixed
+ * shuffle tree inside each wave, then wave order.  sh holds 64 doubles. */
 __device__ inline double rs_block_sum(double v, double* sh) {
-    /* deterministic: fixed shuffle tree inside the wave, then wave order */
+    if (blockDim.x < 64) {
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        double s = 0.0;
+        if (threadIdx.x == 0)
+            for (int i = 0; i < (int)blockDim.x; ++i) s += sh[i];
+        return s;
+    }
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) sh[wid] = v;
@@ -248,8 +262,12 @@ typedef struct { double k[RS_NR]; } RsKTab;
 #define RS_KPTR(a) ((a).K)
 #endif
 
-/* The lane's slab of three vectors: in LDS ([3][NS][64] per wave, sized by
- * the launcher) when it fits, otherwise a local array. */
+/* Storage of a lane's cell.  RS_WDECL(W): the LU factors, a register array
+ * (stride RS_W_ST = 1).  RS_SLAB(sv): three state vectors with stride
+ * RS_SLAB_ST, in LDS ([3][NS][64] per wave, sized by the launcher) when four
+ * waves per CU fit, otherwise a local array. */
+#define RS_W_ST 1
+#define RS_WDECL(name) double name##_arr[RS_NNZ]; double* name = name##_arr
 #if RS_SLOW_DOUBLES(RS_NS) > 0
 #define RS_SLAB_ST 64
 #define RS_SLAB(name) extern __shared__ double rs_slab_mem[]; \
@@ -267,8 +285,9 @@ typedef struct { double k[RS_NR]; } RsKTab;
  * B/lane more. */
 #define RS_CHEM_ATTEMPT_KERNEL(NAME, KPARAM, KEXPR)                                         \
     extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK) NAME(RsChemArgs a KPARAM) {  \
-        __shared__ double sh[RS_CHEM_BLOCK / 64];                                           \
+        __shared__ double sh[64];                                                           \
         RS_SLAB(sv);                                                                        \
+        RS_WDECL(W);                                                                        \
         const RsCtrl* c = a.ctrl;                                                           \
         if (c && c->done) return;                                                           \
         const int cur = c ? c->cur : a.cur_static;                                          \
@@ -283,8 +302,8 @@ typedef struct { double k[RS_NR]; } RsKTab;
             const double jf = a.jfac ? a.jfac[cell] : 1.0;                                  \
             const double* Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;               \
             const int clamp = a.clamp;                                                      \
-            sse = rs_ros23_attempt<RS_SLAB_ST, false>(                                      \
-                KEXPR, jf, Ec, a.ncol, sv, dt, a.atol, a.rtol, [&](int s, double un) {      \
+            sse = rs_ros23_attempt<RS_SLAB_ST, RS_W_ST, false>(                             \
+                KEXPR, jf, Ec, a.ncol, sv, W, dt, a.atol, a.rtol, [&](int s, double un) {   \
                     dst[(long long)s * a.nc + cell] = clamp ? rs_max(un, 0.0) : un;         \
                 });                                                                         \
         }                                                                                   \
@@ -298,11 +317,19 @@ RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt_kp, , a.K)
 /* Persistent per-cell adaptive solve with wave-aggregated work stealing: a
  * lane that finishes its cell takes the next index from the queue head; all
  * lanes run the same attempt code, each on its own cell and its own dt. */
-extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK)
+/* RS_PERCELL_WAVES=n caps the registers so n waves fit per SIMD (more spill,
+ * more latency hiding); default: the compiler's choice. */
+#ifdef RS_PERCELL_WAVES
+#define RS_PERCELL_ATTR __attribute__((amdgpu_waves_per_eu(RS_PERCELL_WAVES, RS_PERCELL_WAVES)))
+#else
+#define RS_PERCELL_ATTR
+#endif
+extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK) RS_PERCELL_ATTR
 rs_chem_percell(RsPercellArgs a RS_KPARAM) {
     const int lane = threadIdx.x & 63;
     long long cell = -1;     /* -1: needs work, -2: queue drained */
     RS_SLAB(sv);
+    RS_WDECL(W);
 #define RS_U(s) sv[(2 * RS_NS + (s)) * RS_SLAB_ST]
     const double* Ec = nullptr;
     double t = 0.0, dt = 0.0, qold = 0.0, jf = 1.0;
@@ -340,15 +367,17 @@ rs_chem_percell(RsPercellArgs a RS_KPARAM) {
             if ((t < tlim) && (iters < a.maxiters)) {
                 const double dtc = (dt < a.t1 - t) ? dt : a.t1 - t;
                 double un[RS_NS];
-                const double sse = rs_ros23_attempt<RS_SLAB_ST, true>(
-                    RS_KPTR(a), jf, Ec, a.ncol, sv, dtc, a.atol, a.rtol,
-                    [&](int s, double v) { un[s] = v; });
+#define RS_UN(s) un[s]
+                const double sse = rs_ros23_attempt<RS_SLAB_ST, RS_W_ST, true>(
+                    RS_KPTR(a), jf, Ec, a.ncol, sv, W, dtc, a.atol, a.rtol,
+                    [&](int s, double v) { RS_UN(s) = v; });
                 const double eest = RS_SQRT(sse / (double)RS_NS);
                 if (rs_pi_update(a.beta1, a.beta2, a.invqmin, a.invqmax, a.gamma, a.qsmin,
                                  a.qsmax, a.qoldinit, dtc, eest, &t, &dt, &qold, &nacc, &nrej)) {
 #pragma unroll
-                    for (int s = 0; s < RS_NS; ++s) RS_U(s) = a.clamp ? rs_max(un[s], 0.0) : un[s];
+                    for (int s = 0; s < RS_NS; ++s) RS_U(s) = a.clamp ? rs_max(RS_UN(s), 0.0) : RS_UN(s);
                 }
+#undef RS_UN
                 iters += 1;
             }
             if (!((t < tlim) && (iters < a.maxiters))) {
@@ -393,14 +422,15 @@ rs_chem_vjp(RsChemVjpArgs a RS_KPARAM) {
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= a.nc) return;
     RS_SLAB(sv);
+    RS_WDECL(W);
 #pragma unroll
     for (int s = 0; s < RS_NS; ++s) sv[s * RS_SLAB_ST] = a.u[(long long)s * a.nc + cell];
     const double jf = a.jfac ? a.jfac[cell] : 1.0;
     const bool surf = cell < a.ncol;
     const double* Ec = (surf && a.E) ? a.E + cell : nullptr;
     double* Gc = (surf && a.gE) ? a.gE + cell : nullptr;
-    rs_ros23_vjp<RS_SLAB_ST>(
-        RS_KPTR(a), jf, Ec, a.ncol, sv, a.dt,
+    rs_ros23_vjp<RS_SLAB_ST, RS_W_ST>(
+        RS_KPTR(a), jf, Ec, a.ncol, sv, W, a.dt,
         [&](int s) {
             double l = a.lam[(long long)s * a.nc + cell];
             if (a.uout && !(a.uout[(long long)s * a.nc + cell] > 0.0)) l = 0.0;

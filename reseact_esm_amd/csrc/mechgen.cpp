@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <sstream>
+#include <stdexcept>
 
 namespace rs {
 
@@ -194,14 +195,23 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         e << " }\n";
     }
     e << "}\n";
-    // ---- W = I - hg*J directly in LU slot storage
-    auto jname = [](int i, int a) { return "J" + std::to_string(i) + "_" + std::to_string(a); };
-    e << "template <int US = 1>\n";
+    // ---- W = I - hg*J directly in LU slot storage: J is accumulated in the
+    // slots of its pattern (each entry in reaction order), then turned into W
+    e << "template <int US = 1, int WS = 1>\n";
     e << "RS_FN void rs_jac_W(const double* __restrict__ K, double jf, const double* __restrict__ u,"
          " double hg, double* __restrict__ W) {\n";
+    std::vector<std::vector<int>> slot(n, std::vector<int>(n, -1));   // species (i, a) -> LU slot
+    for (int qi = 0; qi < n; ++qi)
+        for (int qj = 0; qj < n; ++qj)
+            if (p.idx[qi][qj] >= 0) slot[p.perm[qi]][p.perm[qj]] = p.idx[qi][qj];
     for (int i = 0; i < n; ++i)
         for (int a = 0; a < n; ++a)
-            if (p.jpat[i][a]) e << "  double " << jname(i, a) << " = 0.0;\n";
+            if (p.jpat[i][a] && slot[i][a] < 0)
+                throw std::logic_error("Jacobian entry outside the LU pattern");
+    auto wname = [&](int i, int a) { return "W[" + std::to_string(slot[i][a]) + " * WS]"; };
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < n; ++a)
+            if (p.jpat[i][a]) e << "  " << wname(i, a) << " = 0.0;\n";
     for (int r = 0; r < m.nr; ++r) {
         int a = m.ra[r], b = m.rb[r];
         e << "  { const double kc = " << kexpr(m, r) << ";";
@@ -209,35 +219,37 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         else if (a == b) e << " const double da = (2.0 * kc) * u[" << a << " * US];";
         else e << " const double da = kc * u[" << b << " * US]; const double db = kc * u[" << a << " * US];";
         for (auto& pr : p.net[r]) {
-            e << " " << accum(jname(pr.first, a), pr.second, "da");
-            if (b >= 0 && b != a) e << " " << accum(jname(pr.first, b), pr.second, "db");
+            e << " " << accum(wname(pr.first, a), pr.second, "da");
+            if (b >= 0 && b != a) e << " " << accum(wname(pr.first, b), pr.second, "db");
         }
         e << " }\n";
     }
     for (int qi = 0; qi < n; ++qi)
         for (int qj = 0; qj < n; ++qj) {
-            int s = p.idx[qi][qj];
-            if (s < 0) continue;
+            int sl = p.idx[qi][qj];
+            if (sl < 0) continue;
             int i = p.perm[qi], a = p.perm[qj];
+            const std::string w = "W[" + std::to_string(sl) + " * WS]";
             if (p.jpat[i][a])
-                e << "  W[" << s << "] = " << (i == a ? "1.0" : "0.0") << " - hg * " << jname(i, a) << ";\n";
+                e << "  " << w << " = " << (i == a ? "1.0" : "0.0") << " - hg * " << w << ";\n";
             else
-                e << "  W[" << s << "] = " << (i == a ? "1.0" : "0.0") << ";\n";
+                e << "  " << w << " = " << (i == a ? "1.0" : "0.0") << ";\n";
         }
     e << "}\n";
     // ---- LU without pivoting, in position order
+    e << "template <int WS = 1>\n";
     e << "RS_FN void rs_lu(double* __restrict__ W) {\n";
     for (int k = 0; k < n; ++k) {
-        e << "  { const double pv = RS_PIVOT(W[" << p.idx[k][k] << "]);\n";
+        e << "  { const double pv = RS_PIVOT(W[" << p.idx[k][k] << " * WS]);\n";
         for (int i = k + 1; i < n; ++i) {
             int sik = p.idx[i][k];
             if (sik < 0) continue;
-            e << "  { const double m = W[" << sik << "] * pv; W[" << sik << "] = m;";
+            e << "  { const double m = W[" << sik << " * WS] * pv; W[" << sik << " * WS] = m;";
             for (int j = k + 1; j < n; ++j) {
                 int skj = p.idx[k][j];
                 if (skj < 0) continue;
                 int sij = p.idx[i][j];
-                e << " W[" << sij << "] = W[" << sij << "] - m * W[" << skj << "];";
+                e << " W[" << sij << " * WS] = W[" << sij << " * WS] - m * W[" << skj << " * WS];";
             }
             e << " }\n";
         }
@@ -245,34 +257,36 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
     }
     e << "}\n";
     // ---- W x = b  (b, x in species coordinates)
+    e << "template <int WS = 1>\n";
     e << "RS_FN void rs_solve(const double* __restrict__ W,"
          " const double* __restrict__ b, double* __restrict__ x) {\n";
     for (int q = 0; q < n; ++q) {
         e << "  double y" << q << " = b[" << p.perm[q] << "];";
         for (int k = 0; k < q; ++k)
-            if (p.idx[q][k] >= 0) e << " y" << q << " = y" << q << " - W[" << p.idx[q][k] << "] * y" << k << ";";
+            if (p.idx[q][k] >= 0) e << " y" << q << " = y" << q << " - W[" << p.idx[q][k] << " * WS] * y" << k << ";";
         e << "\n";
     }
     for (int q = n - 1; q >= 0; --q) {
         e << "  { double acc = y" << q << ";";
         for (int j = q + 1; j < n; ++j)
-            if (p.idx[q][j] >= 0) e << " acc = acc - W[" << p.idx[q][j] << "] * x[" << p.perm[j] << "];";
-        e << " x[" << p.perm[q] << "] = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << "]); }\n";
+            if (p.idx[q][j] >= 0) e << " acc = acc - W[" << p.idx[q][j] << " * WS] * x[" << p.perm[j] << "];";
+        e << " x[" << p.perm[q] << "] = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << " * WS]); }\n";
     }
     e << "}\n";
     // ---- W^T x = b: U^T z = b (forward), then L^T x = z (backward)
+    e << "template <int WS = 1>\n";
     e << "RS_FN void rs_solveT(const double* __restrict__ W,"
          " const double* __restrict__ b, double* __restrict__ x) {\n";
     for (int q = 0; q < n; ++q) {
         e << "  double z" << q << "; { double acc = b[" << p.perm[q] << "];";
         for (int k = 0; k < q; ++k)
-            if (p.idx[k][q] >= 0) e << " acc = acc - W[" << p.idx[k][q] << "] * z" << k << ";";
-        e << " z" << q << " = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << "]); }\n";
+            if (p.idx[k][q] >= 0) e << " acc = acc - W[" << p.idx[k][q] << " * WS] * z" << k << ";";
+        e << " z" << q << " = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << " * WS]); }\n";
     }
     for (int q = n - 1; q >= 0; --q) {
         e << "  { double acc = z" << q << ";";
         for (int i = q + 1; i < n; ++i)
-            if (p.idx[i][q] >= 0) e << " acc = acc - W[" << p.idx[i][q] << "] * x[" << p.perm[i] << "];";
+            if (p.idx[i][q] >= 0) e << " acc = acc - W[" << p.idx[i][q] << " * WS] * x[" << p.perm[i] << "];";
         e << " x[" << p.perm[q] << "] = acc; }\n";
     }
     e << "}\n";

@@ -29,6 +29,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -258,8 +259,23 @@ std::string jit_cache_path(const std::string& key) {
     return dir + "/" + key + ".hsaco";
 }
 
+// Extra hipRTC options from RS_JIT_OPTIONS (space separated; developer knob
+// for kernel variants, e.g. -DRS_PERCELL_WAVES=2); part of the cache key.
+std::vector<std::string> jit_extra_options() {
+    std::vector<std::string> out;
+    if (const char* ev = std::getenv("RS_JIT_OPTIONS")) {
+        std::istringstream is(ev);
+        std::string w;
+        while (is >> w) out.push_back(w);
+    }
+    return out;
+}
+
 std::vector<char> jit_compile(const std::string& src, bool exact, std::string& log) {
-    const std::string key = std::to_string(fnv1a(src)) + "_" + std::to_string(src.size()) +
+    const std::vector<std::string> extra = jit_extra_options();
+    std::string xk;
+    for (auto& w : extra) xk += w + " ";
+    const std::string key = std::to_string(fnv1a(src + "|" + xk)) + "_" + std::to_string(src.size()) +
                             (exact ? "_x" : "_f");
     {
         std::lock_guard<std::mutex> g(g_jit_mu);
@@ -294,6 +310,7 @@ std::vector<char> jit_compile(const std::string& src, bool exact, std::string& l
         opts.push_back("-ffp-contract=off");
         opts.push_back("-DRS_EXACT_DIV");
     }
+    for (auto& w : extra) opts.push_back(w.c_str());
     hiprtcResult r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
     size_t ln = 0;
     hiprtcGetProgramLogSize(prog, &ln);
@@ -418,7 +435,9 @@ void enqueue_ssprk43(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* u
     { ProfScope ps(c, P_TLAST); HIPCHK(rs_launch_transport(KIND_LAST, c->tk, &a, c->stream)); }
 }
 
-// LDS slab of every runtime-compiled chemistry kernel for a workgroup of `block` lanes
+// Lanes per chemistry workgroup (one lane per cell).
+unsigned chem_threads(const rs_ctx* c) { return c->chem_block; }
+// LDS of every runtime-compiled chemistry kernel for a workgroup of `block` lanes
 size_t chem_slab_bytes(const rs_ctx* c, unsigned block) {
     return (size_t)block * RS_SLOW_DOUBLES(c->ns) * sizeof(double);
 }
@@ -448,7 +467,7 @@ void chem_launch(rs_ctx* c, hipFunction_t f, const void* argp, size_t argsz, uns
 
 
 
-unsigned chem_grid(rs_ctx* c) { return (unsigned)((c->nc + c->chem_block - 1) / c->chem_block); }
+unsigned chem_grid(rs_ctx* c) { return (unsigned)((c->nc + chem_threads(c) - 1) / chem_threads(c)); }
 
 void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cur_static,
                    double dt, double atol, double rtol, int clamp) {
@@ -467,8 +486,8 @@ void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cu
     a.rtol = rtol;
     a.cur_static = cur_static;
     a.clamp = clamp;
-    if (c->chem_kp) chem_launch(c, c->chem->attempt_kp, &a, sizeof(a), chem_grid(c), c->chem_block, P_CHEM, false);
-    else chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), c->chem_block, P_CHEM);
+    if (c->chem_kp) chem_launch(c, c->chem->attempt_kp, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEM, false);
+    else chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEM);
 }
 
 // Sum of n device partials: folded on the device to at most 1024 values first.
@@ -640,14 +659,15 @@ int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_ho
     HIPCHK(hipMemcpyAsync(c->stats.p, c->hstats, sizeof(init), hipMemcpyHostToDevice, c->stream));
     HIPCHK(rs_launch_fill_u64(c->head.p, 1, 0ULL, c->stream));
     int per_cu = 0;
-    const size_t slab = chem_slab_bytes(c, c->chem_block);
+    const unsigned thr = chem_threads(c);
+    const size_t slab = chem_slab_bytes(c, thr);
     HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c->chem->percell,
-                                                              (int)c->chem_block, slab));
+                                                              (int)thr, slab));
     per_cu = std::max(per_cu, 1);
     long long grid = (long long)per_cu * c->ncu;
-    grid = std::min<long long>(grid, (c->nc + c->chem_block - 1) / c->chem_block);
+    grid = std::min<long long>(grid, (c->nc + thr - 1) / thr);
     grid = std::max<long long>(grid, 1);
-    chem_launch(c, c->chem->percell, &a, sizeof(a), (unsigned)grid, c->chem_block, P_PERCELL);
+    chem_launch(c, c->chem->percell, &a, sizeof(a), (unsigned)grid, thr, P_PERCELL);
     HIPCHK(hipMemcpyAsync(c->hstats, c->stats.p, 8 * sizeof(long long), hipMemcpyDeviceToHost,
                           c->stream));
     if (dt_cells_host) d2h(dt_cells_host, c->dtcell.p, (size_t)c->nc, c->stream);
@@ -743,7 +763,7 @@ void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt
     a.nc = c->nc;
     a.ncol = c->ncol;
     a.dt = dt;
-    chem_launch(c, c->chem->vjp, &a, sizeof(a), chem_grid(c), c->chem_block, P_CHEMVJP);
+    chem_launch(c, c->chem->vjp, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEMVJP);
 }
 
 double loss_on_device(rs_ctx* c, const double* u, int species) {
@@ -1036,6 +1056,7 @@ int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const doub
         c->nr = nr;
         c->chem = std::move(cm);
         c->have_mech = true;
+        c->partial.ensure((size_t)chem_grid(c));   // one partial per chemistry workgroup
         return RS_OK;
     });
 }

@@ -20,8 +20,9 @@ extern "C" double hc_attempt(long long nc, long long ncol, const double* K, cons
         for (int s = 0; s < RS_NS; ++s) sv[2 * RS_NS + s] = u[s * nc + c];
         const double jf = jfac ? jfac[c] : 1.0;
         const double* Ec = (E && c < ncol) ? E + c : nullptr;
-        tot += rs_ros23_attempt<1, false>(K, jf, Ec, ncol, sv, dt, atol, rtol,
-                                          [&](int s, double un) { unew[s * nc + c] = un; });
+        double W[RS_NNZ];
+        tot += rs_ros23_attempt<1, 1, false>(K, jf, Ec, ncol, sv, W, dt, atol, rtol,
+                                             [&](int s, double un) { unew[s * nc + c] = un; });
     }
     return tot;
 }
@@ -37,7 +38,8 @@ extern "C" void hc_vjp(long long nc, long long ncol, const double* K, const doub
         const double jf = jfac ? jfac[c] : 1.0;
         const double* Ec = (E && c < ncol) ? E + c : nullptr;
         double* Gc = (c < ncol) ? gE + c : nullptr;
-        rs_ros23_vjp<1>(K, jf, Ec, ncol, sv, dt, [&](int s) { return lc[s]; },
+        double W[RS_NNZ];
+        rs_ros23_vjp<1, 1>(K, jf, Ec, ncol, sv, W, dt, [&](int s) { return lc[s]; },
                         [&](int s, double v) { lc[s] = v; }, Gc);
         for (int s = 0; s < RS_NS; ++s) lam[s * nc + c] = lc[s];
     }
