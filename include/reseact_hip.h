@@ -115,6 +115,17 @@ RS_API int rs_set_wind(rs_ctx* ctx, const double* cx, const double* wz, const do
  * (rx_traced_integrator.jl:151-253, prototypes/reseact_3d_chem/block_jac.jl:57-136). */
 RS_API int rs_set_mechanism(rs_ctx* ctx, int nr, const int* reactants, const int* products,
                      const double* k, const int* photo);
+/* The same with general products, for mechanisms ingested from reaction
+ * systems (fractional and multiple products, catalysts): reactants as above;
+ * the products of reaction r are prod_species[q] with coefficient
+ * prod_coef[q] (> 0) for q in [prod_off[r], prod_off[r+1]), prod_off[0] = 0.
+ * Rate laws are mass action, k[r] times the product of the reactants (no
+ * combinatoric factor for 2A).  Replaces the ReactionSystem the reference
+ * builds from an .esm `reaction_systems` entry (SuperFast,
+ * reseact.esm:12714-12717; prototypes/g2_depsink/g2_nodep.esm:450-1860). */
+RS_API int rs_set_mechanism_stoich(rs_ctx* ctx, int nr, const int* reactants, const int* prod_off,
+                                   const int* prod_species, const double* prod_coef,
+                                   const double* k, const int* photo);
 /* Per-cell photolysis factor (NC values, e.g. max(0, cos SZA)); NULL = 1. */
 RS_API int rs_set_photolysis(rs_ctx* ctx, const double* jfac);
 /* Surface emission source E[s*NLAT*NLON + j*NLON + i] (conc/s) added to the

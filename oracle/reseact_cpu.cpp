@@ -68,7 +68,7 @@ struct rs_ctx {
     int nr = 0;
     std::vector<int> ra, rb, pa, pb, photo;
     std::vector<double> kr;
-    std::vector<std::vector<std::pair<int, int>>> net;   // per reaction: (species, nu), nu != 0
+    std::vector<std::vector<std::pair<int, double>>> net;   // per reaction: (species, nu), nu != 0
     std::vector<double> jfac;   // nc
     std::vector<double> emis;   // ns*ncol, empty = none
     // state
@@ -312,7 +312,7 @@ struct CellChem {
         for (int i = 0; i < ns; ++i) f[i] = E ? E[(size_t)i * estride] : 0.0;
         for (int r = 0; r < c.nr; ++r) {
             double rt = rate(r, u);
-            for (auto& pr : c.net[r]) f[pr.first] += (double)pr.second * rt;
+            for (auto& pr : c.net[r]) f[pr.first] += pr.second * rt;
         }
     }
     // dense analytic Jacobian J[i*ns + a]
@@ -327,7 +327,7 @@ struct CellChem {
             else if (a == b) da = (2.0 * k) * u[a];
             else { da = k * u[b]; db = k * u[a]; }
             for (auto& pr : c.net[r]) {
-                double nu = (double)pr.second;
+                double nu = pr.second;
                 J[(size_t)pr.first * ns + a] += nu * da;
                 if (b >= 0 && b != a) J[(size_t)pr.first * ns + b] += nu * db;
             }
@@ -787,32 +787,73 @@ int rs_set_wind(rs_ctx* c, const double* cx, const double* wz, const double* wl)
     return RS_OK;
 }
 
+// Mass-action mechanism (rx_traced_integrator.jl:151-253 for the Jacobian
+// conventions): net stoichiometry per reaction, species ascending.
+static int set_mechanism_common(rs_ctx* c, int nr, const int* re, const double* k,
+                                const int* photo,
+                                const std::vector<std::vector<std::pair<int, double>>>& prods) {
+    c->nr = nr;
+    c->ra.resize(nr); c->rb.resize(nr); c->pa.assign(nr, -1); c->pb.assign(nr, -1);
+    c->photo.assign(nr, 0); c->kr.resize(nr); c->net.assign(nr, {});
+    for (int r = 0; r < nr; ++r) {
+        int a = re[2 * r], b = re[2 * r + 1];
+        if (a < 0 || a >= c->ns || b < -1 || b >= c->ns)
+            return fail(c, RS_E_ARG, "species index out of range in reaction " + std::to_string(r));
+        for (auto& q : prods[r]) {
+            if (q.first < 0 || q.first >= c->ns)
+                return fail(c, RS_E_ARG, "product species out of range in reaction " + std::to_string(r));
+            if (!(q.second > 0.0) || !(q.second < 1e6))
+                return fail(c, RS_E_ARG, "product coefficient must be positive and finite");
+        }
+        if (!std::isfinite(k[r])) return fail(c, RS_E_NONFINITE, "rate constant");
+        c->ra[r] = a; c->rb[r] = b;
+        c->kr[r] = k[r];
+        c->photo[r] = photo ? (photo[r] != 0) : 0;
+        if (c->photo[r] && b >= 0)
+            return fail(c, RS_E_ARG, "photolysis reaction " + std::to_string(r) + " must be first order");
+        std::vector<double> nu(c->ns, 0.0);
+        nu[a] -= 1.0;
+        if (b >= 0) nu[b] -= 1.0;
+        for (auto& q : prods[r]) nu[q.first] += q.second;
+        for (int s = 0; s < c->ns; ++s)
+            if (nu[s] != 0.0) c->net[r].emplace_back(s, nu[s]);
+    }
+    c->have_mech = true;
+    return RS_OK;
+}
+
 int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const double* k,
                      const int* photo) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
     if (nr < 0 || (nr > 0 && (!re || !pr || !k))) return fail(c, RS_E_ARG, "bad mechanism arrays");
-    c->nr = nr;
-    c->ra.resize(nr); c->rb.resize(nr); c->pa.resize(nr); c->pb.resize(nr);
-    c->photo.assign(nr, 0); c->kr.resize(nr); c->net.assign(nr, {});
+    std::vector<std::vector<std::pair<int, double>>> prods(nr);
     for (int r = 0; r < nr; ++r) {
-        int a = re[2 * r], b = re[2 * r + 1], p = pr[2 * r], q = pr[2 * r + 1];
-        if (a < 0 || a >= c->ns || b < -1 || b >= c->ns || p < -1 || p >= c->ns || q < -1 ||
-            q >= c->ns)
-            return fail(c, RS_E_ARG, "species index out of range in reaction " + std::to_string(r));
-        if (!std::isfinite(k[r])) return fail(c, RS_E_NONFINITE, "rate constant");
-        c->ra[r] = a; c->rb[r] = b; c->pa[r] = p; c->pb[r] = q;
-        c->kr[r] = k[r];
-        c->photo[r] = photo ? (photo[r] != 0) : 0;
-        std::vector<int> nu(c->ns, 0);
-        nu[a] -= 1;
-        if (b >= 0) nu[b] -= 1;
-        if (p >= 0) nu[p] += 1;
-        if (q >= 0) nu[q] += 1;
-        for (int s = 0; s < c->ns; ++s)
-            if (nu[s] != 0) c->net[r].emplace_back(s, nu[s]);
+        for (int j = 0; j < 2; ++j) {
+            const int p = pr[2 * r + j];
+            if (p < -1 || p >= c->ns)
+                return fail(c, RS_E_ARG, "species index out of range in reaction " + std::to_string(r));
+            if (p >= 0) prods[r].emplace_back(p, 1.0);
+        }
     }
-    c->have_mech = true;
+    int rc = set_mechanism_common(c, nr, re, k, photo, prods);
+    if (rc) return rc;
+    for (int r = 0; r < nr; ++r) { c->pa[r] = pr[2 * r]; c->pb[r] = pr[2 * r + 1]; }
     return RS_OK;
+}
+
+int rs_set_mechanism_stoich(rs_ctx* c, int nr, const int* re, const int* prod_off,
+                            const int* prod_species, const double* prod_coef, const double* k,
+                            const int* photo) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (nr < 0 || (nr > 0 && (!re || !prod_off || !k))) return fail(c, RS_E_ARG, "bad mechanism arrays");
+    if (nr > 0 && (prod_off[0] != 0 || (prod_off[nr] > 0 && (!prod_species || !prod_coef))))
+        return fail(c, RS_E_ARG, "bad product offsets");
+    std::vector<std::vector<std::pair<int, double>>> prods(nr);
+    for (int r = 0; r < nr; ++r) {
+        if (prod_off[r + 1] < prod_off[r]) return fail(c, RS_E_ARG, "bad product offsets");
+        for (int q = prod_off[r]; q < prod_off[r + 1]; ++q) prods[r].emplace_back(prod_species[q], prod_coef[q]);
+    }
+    return set_mechanism_common(c, nr, re, k, photo, prods);
 }
 
 int rs_set_photolysis(rs_ctx* c, const double* jfac) {

@@ -23,25 +23,44 @@ std::string validate_mech(const MechSpec& m) {
     if (m.ns < 1) return "mechanism needs at least one species";
     if (m.nr < 0) return "negative reaction count";
     auto bad = [&](int x, bool allow_none) { return x < (allow_none ? -1 : 0) || x >= m.ns; };
+    if (!m.prod.empty() && (int)m.prod.size() != m.nr) return "product lists: one per reaction";
     for (int r = 0; r < m.nr; ++r) {
-        if (bad(m.ra[r], false) || bad(m.rb[r], true) || bad(m.pa[r], true) || bad(m.pb[r], true))
+        if (bad(m.ra[r], false) || bad(m.rb[r], true))
             return "species index out of range in reaction " + std::to_string(r);
+        if (m.prod.empty()) {
+            if (bad(m.pa[r], true) || bad(m.pb[r], true))
+                return "species index out of range in reaction " + std::to_string(r);
+        } else {
+            for (auto& q : m.prod[r]) {
+                if (bad(q.first, false))
+                    return "product species out of range in reaction " + std::to_string(r);
+                if (!(q.second > 0.0) || !(q.second < 1e6))
+                    return "product coefficient must be positive and finite in reaction " + std::to_string(r);
+            }
+        }
         if (m.photo[r] && m.rb[r] >= 0)
             return "photolysis reaction " + std::to_string(r) + " must be first order";
     }
     return "";
 }
 
-static std::vector<std::vector<std::pair<int, int>>> net_stoich(const MechSpec& m) {
-    std::vector<std::vector<std::pair<int, int>>> net(m.nr);
+// Net stoichiometry per reaction, species ascending: substrates count -1 each
+// (a species twice for a second-order self-reaction), products their
+// coefficient; zero entries (a catalyst) are dropped.
+static std::vector<std::vector<std::pair<int, double>>> net_stoich(const MechSpec& m) {
+    std::vector<std::vector<std::pair<int, double>>> net(m.nr);
     for (int r = 0; r < m.nr; ++r) {
-        std::vector<int> nu(m.ns, 0);
-        nu[m.ra[r]] -= 1;
-        if (m.rb[r] >= 0) nu[m.rb[r]] -= 1;
-        if (m.pa[r] >= 0) nu[m.pa[r]] += 1;
-        if (m.pb[r] >= 0) nu[m.pb[r]] += 1;
+        std::vector<double> nu(m.ns, 0.0);
+        nu[m.ra[r]] -= 1.0;
+        if (m.rb[r] >= 0) nu[m.rb[r]] -= 1.0;
+        if (m.prod.empty()) {
+            if (m.pa[r] >= 0) nu[m.pa[r]] += 1.0;
+            if (m.pb[r] >= 0) nu[m.pb[r]] += 1.0;
+        } else {
+            for (auto& q : m.prod[r]) nu[q.first] += q.second;
+        }
         for (int s = 0; s < m.ns; ++s)
-            if (nu[s] != 0) net[r].emplace_back(s, nu[s]);
+            if (nu[s] != 0.0) net[r].emplace_back(s, nu[s]);
     }
     return net;
 }
@@ -163,12 +182,14 @@ std::string kexpr(const MechSpec& m, int r) {
 }
 
 // "x = x + nu*y" with the exact forms of the oracle's `x += (double)nu * y`.
-std::string accum(const std::string& x, int nu, const std::string& y) {
+// x = x + nu * y with the coefficient as an exact literal (x - y for -1).
+std::string accum(const std::string& x, double nu, const std::string& y) {
     std::ostringstream s;
-    if (nu == 1) s << x << " = " << x << " + " << y << ";";
-    else if (nu == -1) s << x << " = " << x << " - " << y << ";";
-    else if (nu > 0) s << x << " = " << x << " + " << nu << ".0 * " << y << ";";
-    else s << x << " = " << x << " - " << (-nu) << ".0 * " << y << ";";
+    s.precision(17);
+    if (nu == 1.0) s << x << " = " << x << " + " << y << ";";
+    else if (nu == -1.0) s << x << " = " << x << " - " << y << ";";
+    else if (nu > 0) s << x << " = " << x << " + " << std::showpoint << nu << " * " << y << ";";
+    else s << x << " = " << x << " - " << std::showpoint << (-nu) << " * " << y << ";";
     return s.str();
 }
 

@@ -20,6 +20,10 @@ namespace rs {
 struct MechSpec {
     int ns = 0, nr = 0;
     std::vector<int> ra, rb, pa, pb, photo;    // reactants a,b (-1), products a,b (-1)
+    // General products (empty: pa/pb with coefficient 1): per reaction a list
+    // of (species, coefficient), coefficients real (e.g. 0.87 CH2O), a
+    // species may appear on both sides (a catalyst).
+    std::vector<std::vector<std::pair<int, double>>> prod;
 };
 
 // Operation counts of the emitted code (FP64 flops; an FMA counts 2).
@@ -37,7 +41,7 @@ struct MechWork {
 struct MechPlan {
     int ns = 0, nr = 0;
     int ordering = 1;                           // 0 natural, 1 Markowitz
-    std::vector<std::vector<std::pair<int, int>>> net;   // per reaction: (species, nu != 0)
+    std::vector<std::vector<std::pair<int, double>>> net;   // per reaction: (species, nu != 0)
     std::vector<int> perm;                      // elimination position -> species
     std::vector<int> pos;                       // species -> position
     std::vector<std::vector<int>> idx;          // [row pos][col pos] -> W slot, -1 = zero

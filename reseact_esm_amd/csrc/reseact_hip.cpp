@@ -1017,19 +1017,15 @@ int rs_set_lu_ordering(rs_ctx* c, int ordering) {
     return RS_OK;
 }
 
-int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const double* k,
-                     const int* photo) {
-    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
-    if (nr < 0 || (nr > 0 && (!re || !pr || !k))) return fail(c, RS_E_ARG, "bad mechanism arrays");
-    rs::MechSpec m;
-    m.ns = c->ns;
-    m.nr = nr;
-    for (int r = 0; r < nr; ++r) {
-        m.ra.push_back(re[2 * r]); m.rb.push_back(re[2 * r + 1]);
-        m.pa.push_back(pr[2 * r]); m.pb.push_back(pr[2 * r + 1]);
-        m.photo.push_back(photo ? (photo[r] != 0) : 0);
+}  // extern "C"
+
+namespace {
+
+// Compiles and loads the generated chemistry kernels of a validated spec.
+int install_mechanism(rs_ctx* c, rs::MechSpec& m, const double* k) {
+    const int nr = m.nr;
+    for (int r = 0; r < nr; ++r)
         if (!std::isfinite(k[r])) return fail(c, RS_E_NONFINITE, "rate constant");
-    }
     std::string v = rs::validate_mech(m);
     if (!v.empty()) return fail(c, RS_E_ARG, v);
     return guarded(c, [&]() {
@@ -1059,6 +1055,47 @@ int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const doub
         c->partial.ensure((size_t)chem_grid(c));   // one partial per chemistry workgroup
         return RS_OK;
     });
+}
+
+}  // namespace
+
+extern "C" {
+
+int rs_set_mechanism(rs_ctx* c, int nr, const int* re, const int* pr, const double* k,
+                     const int* photo) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (nr < 0 || (nr > 0 && (!re || !pr || !k))) return fail(c, RS_E_ARG, "bad mechanism arrays");
+    rs::MechSpec m;
+    m.ns = c->ns;
+    m.nr = nr;
+    for (int r = 0; r < nr; ++r) {
+        m.ra.push_back(re[2 * r]); m.rb.push_back(re[2 * r + 1]);
+        m.pa.push_back(pr[2 * r]); m.pb.push_back(pr[2 * r + 1]);
+        m.photo.push_back(photo ? (photo[r] != 0) : 0);
+    }
+    return install_mechanism(c, m, k);
+}
+
+int rs_set_mechanism_stoich(rs_ctx* c, int nr, const int* re, const int* prod_off,
+                            const int* prod_species, const double* prod_coef, const double* k,
+                            const int* photo) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (nr < 0 || (nr > 0 && (!re || !prod_off || !k))) return fail(c, RS_E_ARG, "bad mechanism arrays");
+    if (nr > 0 && (prod_off[0] != 0 || (prod_off[nr] > 0 && (!prod_species || !prod_coef))))
+        return fail(c, RS_E_ARG, "bad product offsets");
+    rs::MechSpec m;
+    m.ns = c->ns;
+    m.nr = nr;
+    m.prod.resize(nr);
+    for (int r = 0; r < nr; ++r) {
+        if (prod_off[r + 1] < prod_off[r]) return fail(c, RS_E_ARG, "bad product offsets");
+        m.ra.push_back(re[2 * r]); m.rb.push_back(re[2 * r + 1]);
+        m.pa.push_back(-1); m.pb.push_back(-1);
+        m.photo.push_back(photo ? (photo[r] != 0) : 0);
+        for (int q = prod_off[r]; q < prod_off[r + 1]; ++q)
+            m.prod[r].emplace_back(prod_species[q], prod_coef[q]);
+    }
+    return install_mechanism(c, m, k);
 }
 
 int rs_set_photolysis(rs_ctx* c, const double* jfac) {

@@ -123,6 +123,7 @@ ABI = {
     "rs_set_grid": [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "rs_set_wind": [_P, _D, _D, _D],
     "rs_set_mechanism": [_P, C.c_int, _I, _I, _D, _I],
+    "rs_set_mechanism_stoich": [_P, C.c_int, _I, _I, _I, _D, _D, _I],
     "rs_set_photolysis": [_P, _D],
     "rs_set_emissions": [_P, _D],
     "rs_set_state": [_P, _D],
@@ -263,6 +264,18 @@ class Context:
         k = np.ascontiguousarray(m.k, np.float64)
         ph = np.ascontiguousarray(m.photo, np.int32)
         self._chk(self.lib.rs_set_mechanism(self.h, m.nr, _ip(re), _ip(pr), _dp(k), _ip(ph)))
+
+    def set_mechanism_stoich(self, m):
+        """General products (an esm.CompiledMechanism, or anything with the
+        same arrays): rs_set_mechanism_stoich."""
+        re = np.ascontiguousarray(m.reactants, np.int32).reshape(-1)
+        off = np.ascontiguousarray(m.prod_off, np.int32)
+        sp = np.ascontiguousarray(m.prod_species, np.int32)
+        co = np.ascontiguousarray(m.prod_coef, np.float64)
+        k = np.ascontiguousarray(m.k, np.float64)
+        ph = np.ascontiguousarray(m.photo, np.int32)
+        self._chk(self.lib.rs_set_mechanism_stoich(self.h, m.nr, _ip(re), _ip(off), _ip(sp), _dp(co),
+                                                   _dp(k), _ip(ph)))
 
     def set_lu_ordering(self, ordering: int):
         self._chk(self.lib.rs_set_lu_ordering(self.h, ordering))

@@ -28,11 +28,22 @@ def _cli() -> str:
 
 
 def generate(mech, ordering: int) -> tuple[str, str]:
-    lines = [f"{mech.ns} {mech.nr} {ordering}"]
-    for r in range(mech.nr):
-        a, b = mech.reactants[r]
-        p, q = mech.products[r]
-        lines.append(f"{a} {b} {p} {q} {int(mech.photo[r])}")
+    """Generated source for a problems.Mechanism (two product slots) or an
+    esm.CompiledMechanism (general products)."""
+    if hasattr(mech, "prod_off"):
+        lines = [f"{mech.ns} {mech.nr} {-1 - ordering}"]
+        for r in range(mech.nr):
+            a, b = mech.reactants[r]
+            prods = [(int(mech.prod_species[q]), float(mech.prod_coef[q]))
+                     for q in range(mech.prod_off[r], mech.prod_off[r + 1])]
+            lines.append(f"{a} {b} {int(mech.photo[r])} {len(prods)} " +
+                         " ".join(f"{s_} {c!r}" for s_, c in prods))
+    else:
+        lines = [f"{mech.ns} {mech.nr} {ordering}"]
+        for r in range(mech.nr):
+            a, b = mech.reactants[r]
+            p, q = mech.products[r]
+            lines.append(f"{a} {b} {p} {q} {int(mech.photo[r])}")
     res = subprocess.run([_cli()], input="\n".join(lines) + "\n", capture_output=True, text=True,
                          check=True)
     return res.stdout, res.stderr
