@@ -61,12 +61,28 @@ typedef struct rs_counts {
     double dt_end;    /* proposed dt carried into the next solve                */
 } rs_counts;
 
+/* Clamp flags of the adaptive solves (the `clamp_nonneg` argument and field):
+ * RS_CLAMP_NONNEG (1): max(u, 0) on every accepted inner step
+ *   (rx_traced_integrator.jl:466-470);
+ * RS_POSITIVE_DOMAIN (2): a trial state with a negative entry is rejected and
+ *   the step retried at half its size, values untouched (the out-of-domain
+ *   rejection the reference wants for the Rosenbrock23 trial sub-steps,
+ *   HELPERS.md:396-447, as an isoutofdomain rule with failfactor 2; per cell
+ *   in rs_chem_percell, any entry of the operator's species in the
+ *   global-norm solves).  In rs_split_opts it applies to the chemistry
+ *   operator only: the transport's linear flux-form scheme undershoots at a
+ *   sharp front by an amount proportional to the step, so halving never
+ *   clears it there.  The recorded step sequences and the adjoint see only
+ *   accepted steps. */
+#define RS_CLAMP_NONNEG 1
+#define RS_POSITIVE_DOMAIN 2
+
 /* Options of one Lie-Trotter macro step (adjoint_gradient.jl:609-619 and the
  * tolerances/controllers of run_reseact_reactant.jl:144,310-355). */
 typedef struct rs_split_opts {
     rs_pictrl ctrl_T, ctrl_C;
     double atol_T, rtol_T, atol_C, rtol_C;
-    int clamp_nonneg;        /* max(u,0) after every accepted inner step       */
+    int clamp_nonneg;        /* flags: RS_CLAMP_NONNEG | RS_POSITIVE_DOMAIN    */
     int chem_percell;        /* 0: one global error norm (reference), 1: per cell */
     long long maxiters;
 } rs_split_opts;
@@ -135,6 +151,9 @@ RS_API int rs_set_emissions(rs_ctx* ctx, const double* emis);
 /* ---- state ---------------------------------------------------------------- */
 RS_API int rs_set_state(rs_ctx* ctx, const double* u);   /* NS*NC, host -> device   */
 RS_API int rs_get_state(rs_ctx* ctx, double* u);         /* NS*NC, device -> host   */
+/* max(u, 0) on the whole state, in place: the macro-boundary clamp of the
+ * Lie-Trotter driver (op_split.jl:151). */
+RS_API int rs_clamp_state(rs_ctx* ctx);
 
 /* ---- single attempts ------------------------------------------------------
  * One SSPRK43 / ROS23 attempt from the current state with step dt; the raw

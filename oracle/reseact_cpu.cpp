@@ -616,6 +616,23 @@ void ros23_jvp_cell(const CellChem& cc, const double* u, double dt, const double
     for (int i = 0; i < ns; ++i) outd[i] = ud[i] + dt * (yd[i] + k1d[i]);
 }
 
+// Clamp flags: 1 = max(u, 0) on each accepted state (clamp_nonneg,
+// rx_traced_integrator.jl:466-470); 2 = positive domain: a trial state with a
+// negative entry is rejected and the step retried at half its size (the
+// failfactor-2 rule of an out-of-domain step in the ODE solvers the reference
+// configures with PositiveDomain, HELPERS.md:396-447), before the PI rule.
+bool out_of_domain(const std::vector<double>& x, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (x[i] < 0.0) return true;
+    return false;
+}
+
+void reject_out_of_domain(PIState& s, double dtc) {
+    s.dt = dtc / 2.0;
+    s.nrej += 1;
+    s.iters += 1;
+}
+
 // ----------------------------------------------------------------------------
 // Adaptive loops.
 // ----------------------------------------------------------------------------
@@ -632,8 +649,14 @@ int adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs_pic
         double ee = (op == Op::T) ? ssprk43_attempt(*c, c->u, raw, dtc, atol, rtol)
                                   : ros23_attempt(*c, c->u, raw, dtc, atol, rtol);
         double tbefore = s.t;
+        // the operator's own entries: transported species, or the whole state
+        const size_t nown = (op == Op::T) ? (size_t)c->nt * c->nc : raw.size();
+        if ((clamp & 2) && out_of_domain(raw, nown)) {
+            reject_out_of_domain(s, dtc);
+            continue;
+        }
         if (pi_update(ctrl, s, dtc, ee)) {
-            if (clamp) for (double& x : raw) x = std::max(x, 0.0);
+            if (clamp & 1) for (double& x : raw) x = std::max(x, 0.0);
             c->u.swap(raw);
             if (seq) seq->ts.emplace_back(tbefore, dtc);
         }
@@ -665,8 +688,12 @@ int percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells, const
                 double dtc = std::min(s.dt, t1 - s.t);
                 double sse = ros23_cell(cc, uc.data(), un.data(), dtc, atol, rtol, w);
                 double ee = std::sqrt(sse / (double)ns);
+                if ((clamp & 2) && out_of_domain(un, (size_t)ns)) {
+                    reject_out_of_domain(s, dtc);
+                    continue;
+                }
                 if (pi_update(ctrl, s, dtc, ee)) {
-                    for (int i = 0; i < ns; ++i) uc[i] = clamp ? std::max(un[i], 0.0) : un[i];
+                    for (int i = 0; i < ns; ++i) uc[i] = (clamp & 1) ? std::max(un[i], 0.0) : un[i];
                 }
             }
             if (s.iters >= maxiters && s.t < tlim) nfail += 1;
@@ -708,7 +735,8 @@ int macro(rs_ctx* c, double t0, double t1, double* dtT, double* dtC, const rs_sp
           rs_counts* cT, rs_counts* cC, Seq* sT, Seq* sC) {
     int rc = RS_OK;
     if (c->nt > 0) {
-        rc = adaptive(c, Op::T, t0, t1, dtT, o.ctrl_T, o.atol_T, o.rtol_T, o.clamp_nonneg,
+        rc = adaptive(c, Op::T, t0, t1, dtT, o.ctrl_T, o.atol_T, o.rtol_T,
+                      o.clamp_nonneg & RS_CLAMP_NONNEG,
                       o.maxiters, cT, sT);
         if (rc) return rc;
     }
@@ -886,6 +914,12 @@ int rs_get_state(rs_ctx* c, double* u) {
     return RS_OK;
 }
 
+int rs_clamp_state(rs_ctx* c) {
+    if (!c || !c->have_state) return fail(c, RS_E_STATE, "state not set");
+    for (double& x : c->u) x = std::max(x, 0.0);
+    return RS_OK;
+}
+
 int rs_ssprk43_step(rs_ctx* c, double t, double dt, double atol, double rtol, double* eest) {
     (void)t;
     int rc = check_ready(c, false, true);
@@ -914,7 +948,7 @@ int rs_get_candidate(rs_ctx* c, double* u) {
 
 int rs_accept_candidate(rs_ctx* c, int clamp) {
     if (!c || !c->have_cand) return fail(c, RS_E_STATE, "no candidate");
-    if (clamp) for (double& x : c->cand) x = std::max(x, 0.0);
+    if (clamp & 1) for (double& x : c->cand) x = std::max(x, 0.0);
     c->u.swap(c->cand);
     c->have_cand = false;
     return RS_OK;
@@ -1030,7 +1064,7 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         double* L = c->lam.data() + (size_t)loss_species * c->nc;
         for (size_t x = 0; x < c->ncol; ++x) L[x] = 2.0 * q[x];
     }
-    const int clampf = o->clamp_nonneg;
+    const int clampf = o->clamp_nonneg & 1;
     for (int m = nmacro - 1; m >= 0; --m) {
         auto tr0 = clk::now();
         // replay_fixed: tape of step inputs, each followed by its clamped output
@@ -1092,12 +1126,12 @@ int rs_replay_loss(rs_ctx* c, int loss_kind, int loss_species, double* J_out) {
     for (size_t m = 0; m < c->seqT.size(); ++m) {
         for (auto& p : c->seqT[m].ts) {
             ssprk43_attempt(*c, c->u, raw, p.second, o.atol_T, o.rtol_T);
-            if (o.clamp_nonneg) for (double& x : raw) x = std::max(x, 0.0);
+            if (o.clamp_nonneg & 1) for (double& x : raw) x = std::max(x, 0.0);
             c->u.swap(raw);
         }
         for (auto& p : c->seqC[m].ts) {
             ros23_attempt(*c, c->u, raw, p.second, o.atol_C, o.rtol_C);
-            if (o.clamp_nonneg) for (double& x : raw) x = std::max(x, 0.0);
+            if (o.clamp_nonneg & 1) for (double& x : raw) x = std::max(x, 0.0);
             c->u.swap(raw);
         }
     }

@@ -228,18 +228,9 @@ RS_FN int rs_pi_update(double beta1, double beta2, double invqmin, double invqma
 
 #define RS_CHEM_BLOCK 256   /* largest block the launcher uses (64, 128 or 256) */
 
-/* Sum over the workgroup, deterministic: a partial wave (the large-mechanism
- * layout) adds its lanes in order through LDS; whole waves use a fixed
- * shuffle tree inside each wave, then wave order.  sh holds 64 doubles. */
+/* Sum over the workgroup (whole waves), deterministic: a fixed shuffle tree
+ * inside each wave, then wave order; valid in lane 0. */
 __device__ inline double rs_block_sum(double v, double* sh) {
-    if (blockDim.x < 64) {
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        double s = 0.0;
-        if (threadIdx.x == 0)
-            for (int i = 0; i < (int)blockDim.x; ++i) s += sh[i];
-        return s;
-    }
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) sh[wid] = v;
@@ -285,7 +276,7 @@ typedef struct { double k[RS_NR]; } RsKTab;
  * B/lane more. */
 #define RS_CHEM_ATTEMPT_KERNEL(NAME, KPARAM, KEXPR)                                         \
     extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK) NAME(RsChemArgs a KPARAM) {  \
-        __shared__ double sh[64];                                                           \
+        __shared__ double sh[RS_CHEM_BLOCK / 64];                                           \
         RS_SLAB(sv);                                                                        \
         RS_WDECL(W);                                                                        \
         const RsCtrl* c = a.ctrl;                                                           \
@@ -295,20 +286,26 @@ typedef struct { double k[RS_NR]; } RsKTab;
         const double* __restrict__ src = cur ? a.S1 : a.S0;                                 \
         double* __restrict__ dst = cur ? a.S0 : a.S1;                                       \
         const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;            \
-        double sse = 0.0;                                                                   \
+        double sse = 0.0, neg = 0.0;                                                        \
         if (cell < a.nc) {                                                                  \
             _Pragma("unroll") for (int s = 0; s < RS_NS; ++s)                               \
                 sv[(2 * RS_NS + s) * RS_SLAB_ST] = src[(long long)s * a.nc + cell];         \
             const double jf = a.jfac ? a.jfac[cell] : 1.0;                                  \
             const double* Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;               \
-            const int clamp = a.clamp;                                                      \
+            const int clamp = a.clamp, posdom = a.posdom;                                  \
             sse = rs_ros23_attempt<RS_SLAB_ST, RS_W_ST, false>(                             \
                 KEXPR, jf, Ec, a.ncol, sv, W, dt, a.atol, a.rtol, [&](int s, double un) {   \
+                    if (posdom && un < 0.0) neg += 1.0;                                     \
                     dst[(long long)s * a.nc + cell] = clamp ? rs_max(un, 0.0) : un;         \
                 });                                                                         \
         }                                                                                   \
         const double tot = rs_block_sum(sse, sh);                                           \
         if (threadIdx.x == 0) a.partial[blockIdx.x] = tot;                                  \
+        if (a.negpart) {                                                                    \
+            __syncthreads();                                                                \
+            const double nt = rs_block_sum(neg, sh);                                        \
+            if (threadIdx.x == 0) a.negpart[blockIdx.x] = nt;                               \
+        }                                                                                   \
     }
 
 RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt, RS_KPARAM, RS_KPTR(a))
@@ -372,8 +369,17 @@ rs_chem_percell(RsPercellArgs a RS_KPARAM) {
                     RS_KPTR(a), jf, Ec, a.ncol, sv, W, dtc, a.atol, a.rtol,
                     [&](int s, double v) { RS_UN(s) = v; });
                 const double eest = RS_SQRT(sse / (double)RS_NS);
-                if (rs_pi_update(a.beta1, a.beta2, a.invqmin, a.invqmax, a.gamma, a.qsmin,
-                                 a.qsmax, a.qoldinit, dtc, eest, &t, &dt, &qold, &nacc, &nrej)) {
+                bool isout = false;
+                if (a.posdom) {
+#pragma unroll
+                    for (int s = 0; s < RS_NS; ++s) isout = isout || RS_UN(s) < 0.0;
+                }
+                if (isout) {
+                    /* out-of-domain trial state: retry at half the step */
+                    dt = dtc / 2.0;
+                    nrej += 1;
+                } else if (rs_pi_update(a.beta1, a.beta2, a.invqmin, a.invqmax, a.gamma, a.qsmin,
+                                        a.qsmax, a.qoldinit, dtc, eest, &t, &dt, &qold, &nacc, &nrej)) {
 #pragma unroll
                     for (int s = 0; s < RS_NS; ++s) RS_U(s) = a.clamp ? rs_max(RS_UN(s), 0.0) : RS_UN(s);
                 }

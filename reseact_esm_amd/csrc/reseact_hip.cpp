@@ -47,7 +47,8 @@ long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt, int tk);
 int rs_transport_fused_plan(int nlon, int nlev, int* plan);
 long long rs_transport_fused_nblocks(int nlat, int nt, int nseg);
 hipError_t rs_launch_transport_fused(int adj, const RsTransArgs* a, hipStream_t st);
-hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog, hipStream_t st);
+hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog,
+                          const double* negpart, hipStream_t st);
 hipError_t rs_launch_fill(double* x, long long n, double v, hipStream_t st);
 hipError_t rs_launch_fill_u64(unsigned long long* x, int n, unsigned long long v, hipStream_t st);
 hipError_t rs_launch_mask(double* lam, const double* uout, long long n, hipStream_t st);
@@ -153,7 +154,7 @@ struct rs_ctx {
     bool fplan_ok = false;
     unsigned chem_block = 64;    // chemistry threads per workgroup
     bool chem_kp = false;        // attempt kernel reads the rate constants through a pointer
-    DevArray<double> S[2], X, Y, cx, wz, wl, K, jfac, E, lam, LA, LB, LC, gE, partial, seqlog;
+    DevArray<double> S[2], X, Y, cx, wz, wl, K, jfac, E, lam, LA, LB, LC, gE, partial, negpart, seqlog;
     DevArray<double> red;
     DevArray<unsigned long long> head;
     DevArray<long long> stats;
@@ -387,11 +388,20 @@ void set_fused_plan(RsTransArgs& a, const rs_ctx* c) {
     a.seg_chunks = c->fplan[5];
 }
 
+// Clamp flags of every step entry point: 1 = max(u, 0) on the accepted state
+// (clamp_nonneg), 2 = positive domain (a trial state with a negative entry is
+// rejected and retried at half the step).
+inline int clamp_nonneg_of(int flags) { return flags & 1; }
+inline int posdom_of(int flags) { return (flags >> 1) & 1; }
+
 // One SSPRK43 attempt: either from the state buffers under a controller (ctrl
 // != null: dt/cur/done from the device), or between explicit buffers.
 void enqueue_ssprk43(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* uout, double dt,
-                     double atol, double rtol, int clamp) {
+                     double atol, double rtol, int flags) {
+    const int clamp = clamp_nonneg_of(flags);
     RsTransArgs a = trans_args(c);
+    a.posdom = posdom_of(flags);
+    a.negpart = a.posdom ? c->negpart.p : nullptr;
     a.ctrl = ctrl;
     a.dt_static = dt;
     a.cur_static = c->cur;
@@ -470,8 +480,11 @@ void chem_launch(rs_ctx* c, hipFunction_t f, const void* argp, size_t argsz, uns
 unsigned chem_grid(rs_ctx* c) { return (unsigned)((c->nc + chem_threads(c) - 1) / chem_threads(c)); }
 
 void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cur_static,
-                   double dt, double atol, double rtol, int clamp) {
+                   double dt, double atol, double rtol, int flags) {
+    const int clamp = clamp_nonneg_of(flags);
     RsChemArgs a{};
+    a.posdom = posdom_of(flags);
+    a.negpart = a.posdom ? c->negpart.p : nullptr;
     a.S0 = S0;
     a.S1 = S1;
     a.ctrl = ctrl;
@@ -568,7 +581,8 @@ void adaptive_chunk(rs_ctx* c, const AdaptiveRun& ar, int n) {
         if (ar.op == Op::T) enqueue_ssprk43(c, c->dctrl, nullptr, nullptr, 0.0, ar.atol, ar.rtol, ar.clamp);
         else enqueue_ros23(c, c->dctrl, c->S[0].p, c->S[1].p, 0, 0.0, ar.atol, ar.rtol, ar.clamp);
         ProfScope ps(c, P_CTRL);
-        HIPCHK(rs_launch_ctrl(c->dctrl, c->partial.p, npart, logp, c->stream));
+        HIPCHK(rs_launch_ctrl(c->dctrl, c->partial.p, npart, logp,
+                              posdom_of(ar.clamp) ? c->negpart.p : nullptr, c->stream));
     }
 }
 
@@ -652,7 +666,8 @@ int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_ho
     a.qsmin = pc.qsteady_min;
     a.qsmax = pc.qsteady_max;
     a.qoldinit = pc.qoldinit;
-    a.clamp = clamp;
+    a.clamp = clamp_nonneg_of(clamp);
+    a.posdom = posdom_of(clamp);
     // queue head and statistics (min starts at the largest value)
     long long init[8] = {0, 0, 0, 0, LLONG_MAX, 0, 0, 0};
     std::memcpy(c->hstats, init, sizeof(init));
@@ -699,7 +714,8 @@ int macro(rs_ctx* c, double t0, double t1, double* dtT, double* dtC, const rs_sp
           std::vector<std::pair<double, double>>* sC) {
     int rc = RS_OK;
     if (c->nt > 0) {
-        rc = run_adaptive(c, Op::T, t0, t1, dtT, o.ctrl_T, o.atol_T, o.rtol_T, o.clamp_nonneg,
+        rc = run_adaptive(c, Op::T, t0, t1, dtT, o.ctrl_T, o.atol_T, o.rtol_T,
+                          o.clamp_nonneg & RS_CLAMP_NONNEG,
                           o.maxiters, cT, sT);
         if (rc) return rc;
     }
@@ -798,7 +814,8 @@ size_t replay_into_tape(rs_ctx* c, const double* u0, const std::vector<std::pair
     for (auto& p : sT) {
         double* in = tp + j * N;
         double* outp = tp + (j + 1) * N;
-        enqueue_ssprk43(c, nullptr, in, outp, p.second, o.atol_T, o.rtol_T, o.clamp_nonneg);
+        enqueue_ssprk43(c, nullptr, in, outp, p.second, o.atol_T, o.rtol_T,
+                        clamp_nonneg_of(o.clamp_nonneg));
         if (c->nt < c->ns) {
             const size_t off = (size_t)c->nt * c->nc;
             d2d(outp + off, in + off, (size_t)(c->ns - c->nt) * c->nc, c->stream);
@@ -807,7 +824,7 @@ size_t replay_into_tape(rs_ctx* c, const double* u0, const std::vector<std::pair
     }
     for (auto& p : sC) {
         enqueue_ros23(c, nullptr, tp + j * N, tp + (j + 1) * N, 0, p.second, o.atol_C, o.rtol_C,
-                      o.clamp_nonneg);
+                      clamp_nonneg_of(o.clamp_nonneg));
         ++j;
     }
     return nslots;
@@ -860,11 +877,11 @@ void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* re
     const size_t nT = sT.size();
     for (size_t q = sC.size(); q-- > 0;) {
         const size_t slot = nT + q;
-        enqueue_ros23_vjp(c, tp + slot * N, o.clamp_nonneg ? tp + (slot + 1) * N : nullptr,
+        enqueue_ros23_vjp(c, tp + slot * N, clamp_nonneg_of(o.clamp_nonneg) ? tp + (slot + 1) * N : nullptr,
                           sC[q].second);
     }
     for (size_t q = nT; q-- > 0;) {
-        if (o.clamp_nonneg) {
+        if (clamp_nonneg_of(o.clamp_nonneg)) {
             ProfScope ps(c, P_UTIL);
             HIPCHK(rs_launch_mask(c->lam.p, tp + (q + 1) * N, (long long)N, c->stream));
         }
@@ -974,6 +991,7 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
         if (c->fplan_ok)
             np = std::max<long long>(np, rs_transport_fused_nblocks(nlat, std::max(nt, 1), c->fplan[3]));
         c->partial.resize((size_t)np);
+        c->negpart.resize((size_t)np);
         c->red.resize(1024);
         HIPCHK(rs_launch_fill(c->lam.p, (long long)N, 0.0, c->stream));
         HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
@@ -1053,6 +1071,7 @@ int install_mechanism(rs_ctx* c, rs::MechSpec& m, const double* k) {
         c->chem = std::move(cm);
         c->have_mech = true;
         c->partial.ensure((size_t)chem_grid(c));   // one partial per chemistry workgroup
+        c->negpart.ensure((size_t)chem_grid(c));
         return RS_OK;
     });
 }
@@ -1140,6 +1159,15 @@ int rs_get_state(rs_ctx* c, double* u) {
     return guarded(c, [&]() { d2h(u, c->S[c->cur].p, c->N(), c->stream); return RS_OK; });
 }
 
+int rs_clamp_state(rs_ctx* c) {
+    if (!c || !c->have_state) return fail(c, RS_E_STATE, "state not set");
+    return guarded(c, [&]() {
+        HIPCHK(rs_launch_clamp(c->S[c->cur].p, (long long)c->N(), c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return RS_OK;
+    });
+}
+
 int rs_ssprk43_step(rs_ctx* c, double t, double dt, double atol, double rtol, double* eest) {
     (void)t;
     int rc = check_ready(c, false, true);
@@ -1183,7 +1211,7 @@ int rs_accept_candidate(rs_ctx* c, int clamp) {
     if (!c || !c->have_cand) return fail(c, RS_E_STATE, "no candidate");
     return guarded(c, [&]() {
         c->cur ^= 1;
-        if (clamp) HIPCHK(rs_launch_clamp(c->S[c->cur].p, (long long)c->N(), c->stream));
+        if (clamp_nonneg_of(clamp)) HIPCHK(rs_launch_clamp(c->S[c->cur].p, (long long)c->N(), c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         c->have_cand = false;
         return RS_OK;
@@ -1658,7 +1686,8 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
             HIPCHK(hipStreamSynchronize(c->stream));
         }
         const int CH = 8;   // attempts per member per graph launch
-        AdaptiveRun proto_T{Op::T, o->atol_T, o->rtol_T, o->clamp_nonneg, o->maxiters, 0, true};
+        AdaptiveRun proto_T{Op::T, o->atol_T, o->rtol_T, o->clamp_nonneg & RS_CLAMP_NONNEG,
+                            o->maxiters, 0, true};
         AdaptiveRun proto_C{Op::C, o->atol_C, o->rtol_C, o->clamp_nonneg, o->maxiters, 0, true};
         hipGraphExec_t gT = withT.empty() ? nullptr
                                           : ens_capture(e, withT, [&](int d) { adaptive_chunk(e->m[d], proto_T, CH); }, nullptr);
@@ -1673,7 +1702,9 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
                 ar[d] = adaptive_begin(c, op, tstops[mstep], tstops[mstep + 1],
                                        op == Op::T ? dtT[d] : dtC[d], pc,
                                        op == Op::T ? o->atol_T : o->atol_C,
-                                       op == Op::T ? o->rtol_T : o->rtol_C, o->clamp_nonneg,
+                                       op == Op::T ? o->rtol_T : o->rtol_C,
+                                       op == Op::T ? o->clamp_nonneg & RS_CLAMP_NONNEG
+                                                   : o->clamp_nonneg,
                                        o->maxiters, true, e->origin);
             }
             for (;;) {

@@ -37,6 +37,8 @@ typedef struct RsChemArgs {
     long long nc, ncol;
     double dt_static, atol, rtol;
     int cur_static, clamp;
+    double* negpart;        /* positive domain: negative entries per block, or 0 */
+    int posdom, pad;
 } RsChemArgs;
 
 /* Per-cell adaptive chemistry, persistent work-stealing launch. */
@@ -51,7 +53,7 @@ typedef struct RsPercellArgs {
     long long nc, ncol, maxiters;
     double t0, t1, dt0, atol, rtol;
     double beta1, beta2, invqmin, invqmax, gamma, qsmin, qsmax, qoldinit;
-    int clamp, pad;
+    int clamp, posdom;      /* posdom: reject a trial state with a negative entry */
 } RsPercellArgs;
 
 /* One ROS23 step VJP over all cells (stages recomputed from u). */
@@ -88,6 +90,8 @@ typedef struct RsTransArgs {
      * lane owns one column and seg_lev consecutive levels, seg_chunks lanes
      * per column */
     int seg_w, seg_in, seg_halo, nseg, seg_lev, seg_chunks;
+    double* negpart;        /* positive domain: negative entries per block, or 0 */
+    int posdom, pad;
 } RsTransArgs;
 
 #endif

@@ -55,6 +55,21 @@ __device__ __forceinline__ double face3(double qmm, double qm, double qp) {
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 
+// Sum over a workgroup of NT lanes (whole waves), deterministic: shuffle tree
+// inside each wave, then wave order; the result is valid in lane 0.  `red`
+// holds NT/64 doubles.
+template <int NT>
+__device__ __forceinline__ double block_sum_fixed(double v, double* red, int tid) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (tid == 0)
+        for (int q = 0; q < NT / 64; ++q) t += red[q];
+    return t;
+}
+
 __device__ __forceinline__ int wrapi(int i, int n) {
     i %= n;
     return i < 0 ? i + n : i;
@@ -177,7 +192,7 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
     const int ii = tid & 63, ky = tid >> 6;
     const int i = i0 + ii;
     const double cxj = A.cx[j];
-    double sq = 0.0;
+    double sq = 0.0, ng = 0.0;
 #pragma unroll
     for (int rr = 0; rr < TKV / 4; ++rr) {
         const int kk = ky + 4 * rr;
@@ -201,6 +216,7 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
                 den = den + A.atol;
                 const double at = ut / den;
                 sq += at * at;
+                if (A.posdom && un < 0.0) ng += 1.0;
                 po[g] = A.clamp ? dmax(un, 0.0) : un;
             } else {
                 po[g] = un;
@@ -213,14 +229,12 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
         }
     }
     if (KIND == KIND_LAST) {
-        for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
-        if ((tid & 63) == 0) red[tid >> 6] = sq;
-        __syncthreads();
-        if (tid == 0) {
-            double t = 0.0;
-            for (int w = 0; w < TB / 64; ++w) t += red[w];
-            const long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-            A.partial[bid] = t;
+        const long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        const double t = block_sum_fixed<TB>(sq, red, tid);
+        if (tid == 0) A.partial[bid] = t;
+        if (A.negpart) {
+            const double nt = block_sum_fixed<TB>(ng, red, tid);
+            if (tid == 0) A.negpart[bid] = nt;
         }
     }
 }
@@ -468,7 +482,7 @@ __global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
     }
     const bool keep = periodic || (il >= FHALO && il < FHALO + P.seg_in &&
                                    seg * P.seg_in + il - FHALO < nlon);
-    double sq = 0.0;
+    double sq = 0.0, ng = 0.0;
     if (ADJ) {
         // out = 2/3 lambda4 + (x + dt/2 L^T x)
         fstage_adj(C, nk, h2, [&](int jj, double z) {
@@ -488,17 +502,17 @@ __global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
             den = den + P.atol;
             const double at = ut / den;
             sq += at * at;
+            if (P.posdom && un < 0.0) ng += 1.0;
             po[g0 + (k0 + jj) * plane] = P.clamp ? dmax(un, 0.0) : un;
         });
     }
     if (!ADJ) {
-        for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
-        if ((tid & 63) == 0) red[tid >> 6] = sq;
-        __syncthreads();
-        if (tid == 0) {
-            double t = 0.0;
-            for (int q = 0; q < FT / 64; ++q) t += red[q];
-            P.partial[blockIdx.y * gridDim.x + blockIdx.x] = t;
+        const long long bid = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+        const double t = block_sum_fixed<FT>(sq, red, tid);
+        if (tid == 0) P.partial[bid] = t;
+        if (P.negpart) {
+            const double nt = block_sum_fixed<FT>(ng, red, tid);
+            if (tid == 0) P.negpart[bid] = nt;
         }
     }
 }
@@ -509,27 +523,39 @@ __global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
 // forming one dependent chain), then a fixed tree sums the lanes.  The order
 // depends only on npart, so EEst is reproducible run to run.
 #define CTRL_THREADS 1024
-__global__ void __launch_bounds__(CTRL_THREADS) rs_ctrl_update(RsCtrl* c, const double* partial,
-                                                               int npart, double* seqlog) {
-    __shared__ double sh[CTRL_THREADS];
-    if (c->done) return;
+// Fixed-order sum of n values by the controller's CTRL_THREADS lanes (four
+// independent accumulators per lane, then a tree); valid in lane 0.
+__device__ __forceinline__ double ctrl_sum(const double* v, int n, double* sh) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int x = threadIdx.x;
-    for (; x + 3 * CTRL_THREADS < npart; x += 4 * CTRL_THREADS) {
-        s0 += partial[x];
-        s1 += partial[x + CTRL_THREADS];
-        s2 += partial[x + 2 * CTRL_THREADS];
-        s3 += partial[x + 3 * CTRL_THREADS];
+    for (; x + 3 * CTRL_THREADS < n; x += 4 * CTRL_THREADS) {
+        s0 += v[x];
+        s1 += v[x + CTRL_THREADS];
+        s2 += v[x + 2 * CTRL_THREADS];
+        s3 += v[x + 3 * CTRL_THREADS];
     }
-    for (; x < npart; x += CTRL_THREADS) s0 += partial[x];
+    for (; x < n; x += CTRL_THREADS) s0 += v[x];
+    __syncthreads();
     sh[threadIdx.x] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     for (int w = CTRL_THREADS / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
+    return sh[0];
+}
+
+// negpart (may be 0): per-block counts of negative entries of the trial state;
+// any makes the attempt a rejection with dt <- dtc / 2 (the failfactor rule of
+// an out-of-domain trial step, positive-domain mode).
+__global__ void __launch_bounds__(CTRL_THREADS) rs_ctrl_update(RsCtrl* c, const double* partial,
+                                                               int npart, double* seqlog,
+                                                               const double* negpart) {
+    __shared__ double sh[CTRL_THREADS];
+    if (c->done) return;
+    const double sse = ctrl_sum(partial, npart, sh);
+    const double nneg = negpart ? ctrl_sum(negpart, npart, sh) : 0.0;
     if (threadIdx.x != 0) return;
-    const double sse = sh[0];
     const double raw = sqrt(sse / c->nnorm);
     const double EEst = (raw != raw) ? 1.0e10 : raw;
     const double q11 = pow(dmax(EEst, 1.0e-35), c->beta1);
@@ -537,7 +563,8 @@ __global__ void __launch_bounds__(CTRL_THREADS) rs_ctrl_update(RsCtrl* c, const 
     double q = q11 / qden;
     const double qg = q / c->gamma;
     q = dmax(c->invqmax, (c->invqmin < qg) ? c->invqmin : qg);
-    const int accept = EEst <= 1.0;
+    const int isout = nneg > 0.0;
+    const int accept = EEst <= 1.0 && !isout;
     const int insteady = (c->qsmin <= q) && (q <= c->qsmax);
     const double qa = insteady ? 1.0 : q;
     const double dtc = c->dtc;
@@ -554,6 +581,9 @@ __global__ void __launch_bounds__(CTRL_THREADS) rs_ctrl_update(RsCtrl* c, const 
         c->qold = dmax(EEst, c->qoldinit);
         c->nacc += 1;
         c->cur ^= 1;
+    } else if (isout) {
+        c->dt = dtc / 2.0;
+        c->nrej += 1;
     } else {
         const double q11g = q11 / c->gamma;
         const double qr = (c->invqmin < q11g) ? c->invqmin : q11g;
@@ -691,8 +721,9 @@ long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt, int tk) {
 }
 
 hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog,
-                          hipStream_t st) {
-    hipLaunchKernelGGL(rs_ctrl_update, dim3(1), dim3(CTRL_THREADS), 0, st, c, partial, npart, seqlog);
+                          const double* negpart, hipStream_t st) {
+    hipLaunchKernelGGL(rs_ctrl_update, dim3(1), dim3(CTRL_THREADS), 0, st, c, partial, npart, seqlog,
+                       negpart);
     return hipGetLastError();
 }
 

@@ -98,10 +98,28 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_longlong), ("total_ms", C.c_double)]
 
 
+# clamp flags of the adaptive solves (include/reseact_hip.h): RS_CLAMP_NONNEG
+# zeroes negative entries of every accepted state (the op_split.jl:108-109
+# callback); RS_POSITIVE_DOMAIN rejects a trial state with a negative entry of
+# the operator's own block and retries at half the step (PositiveDomain
+# isoutofdomain, HELPERS.md:396-447)
+RS_CLAMP_NONNEG = 1
+RS_POSITIVE_DOMAIN = 2
+
+
+def _flags(clamp) -> int:
+    """True/False -> RS_CLAMP_NONNEG or 0; an int passes as the flag set."""
+    v = int(clamp)
+    if v & ~(RS_CLAMP_NONNEG | RS_POSITIVE_DOMAIN):
+        raise ValueError(f"unknown clamp flags {v:#x}")
+    return v
+
+
 def split_opts(p: Problem, clamp: bool = True, percell: bool = False,
-               maxiters: int = 20000) -> SplitOpts:
+               maxiters: int = 20000, positive_domain: bool = False) -> SplitOpts:
+    flags = _flags(clamp) | (RS_POSITIVE_DOMAIN if positive_domain else 0)
     return SplitOpts(pictrl_ssprk43(), pictrl_ros23(), p.atol_T, p.rtol_T, p.atol_C,
-                     p.rtol_C, 1 if clamp else 0, 1 if percell else 0, maxiters)
+                     p.rtol_C, flags, 1 if percell else 0, maxiters)
 
 
 class ReseactError(RuntimeError):
@@ -128,6 +146,7 @@ ABI = {
     "rs_set_emissions": [_P, _D],
     "rs_set_state": [_P, _D],
     "rs_get_state": [_P, _D],
+    "rs_clamp_state": [_P],
     "rs_ssprk43_step": [_P, C.c_double, C.c_double, C.c_double, C.c_double, _D],
     "rs_ros23_step": [_P, C.c_double, C.c_double, C.c_double, C.c_double, _D],
     "rs_get_candidate": [_P, _D],
@@ -321,6 +340,9 @@ class Context:
         self._chk(self.lib.rs_get_state(self.h, _dp(out)))
         return out
 
+    def clamp_state(self):
+        self._chk(self.lib.rs_clamp_state(self.h))
+
     def load(self, p: Problem, state: bool = True):
         self.set_grid(p.grid, p.ns, p.nt)
         self.set_wind(p.wind)
@@ -349,7 +371,7 @@ class Context:
         return out
 
     def accept_candidate(self, clamp: bool):
-        self._chk(self.lib.rs_accept_candidate(self.h, 1 if clamp else 0))
+        self._chk(self.lib.rs_accept_candidate(self.h, _flags(clamp)))
 
     # ---- adaptive -------------------------------------------------------------
     def adaptive_transport(self, t0, t1, dt, ctrl=None, atol=1e-6, rtol=1e-3, clamp=True,
@@ -358,7 +380,7 @@ class Context:
         c = Counts()
         ctrl = pictrl_ssprk43() if ctrl is None else ctrl
         self._chk(self.lib.rs_adaptive_transport(self.h, t0, t1, C.byref(d), C.byref(ctrl), atol,
-                                                 rtol, 1 if clamp else 0, maxiters, C.byref(c)))
+                                                 rtol, _flags(clamp), maxiters, C.byref(c)))
         return d.value, c
 
     def adaptive_chem(self, t0, t1, dt, ctrl=None, atol=1e-6, rtol=1e-3, clamp=True,
@@ -367,7 +389,7 @@ class Context:
         c = Counts()
         ctrl = pictrl_ros23() if ctrl is None else ctrl
         self._chk(self.lib.rs_adaptive_chem(self.h, t0, t1, C.byref(d), C.byref(ctrl), atol,
-                                            rtol, 1 if clamp else 0, maxiters, C.byref(c)))
+                                            rtol, _flags(clamp), maxiters, C.byref(c)))
         return d.value, c
 
     def chem_percell(self, t0, t1, dt0, dt_cells=None, ctrl=None, atol=1e-6, rtol=1e-3,
@@ -376,7 +398,7 @@ class Context:
         ctrl = pictrl_ros23() if ctrl is None else ctrl
         dc = None if dt_cells is None else np.ascontiguousarray(dt_cells, np.float64)
         self._chk(self.lib.rs_chem_percell(self.h, t0, t1, dt0, _dp(dc), C.byref(ctrl), atol,
-                                           rtol, 1 if clamp else 0, maxiters, C.byref(st)))
+                                           rtol, _flags(clamp), maxiters, C.byref(st)))
         return st
 
     def macro_step(self, t0, t1, dtT, dtC, opts: SplitOpts):

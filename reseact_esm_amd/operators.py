@@ -8,8 +8,11 @@ the reference finds the calls it already makes:
   ros23_step(ctx, u, t, dt, abstol, reltol) -> (unew, EEst)        (:279-349, jac=:sym)
   adaptive_solve(step, ctx, u0, t0, tend, dt0, ctrl; maxiters, clamp_nonneg)
       -> (u, t, dt, nacc, nrej)                                    (:455-507)
-  lie_trotter_solve(ctx, u0, tspan; macro_dt, reltols, abstols, clamp_nonneg)
+  lie_trotter_solve(ctx, u0, tspan; macro_dt, reltols, abstols, refresh,
+                    forcing_tstops, clamp_nonneg)
       -> dict(u, retcode, naT, nrT, naC, nrC, nmacro)              (op_split.jl:100-164)
+  lie_trotter_solve_bisect(ctx, u0, tspan; macro_dt, min_macro_dt, on_bisect, ...)
+      -> the same dict                                             (op_split.jl:219-242)
   ssprk43_step_vjp(ctx, u, t, dt, lam) -> (lam_in, None)           (:618-623)
   ros23_step_vjp(ctx, u, t, dt, lam) -> (lam_in, grad_emis)        (:610-617)
 
@@ -23,7 +26,8 @@ from __future__ import annotations
 
 import numpy as np
 
-from .host import Context, PICtrl, SplitOpts, pictrl_ros23, pictrl_ssprk43  # noqa: F401
+from .host import (RS_E_MAXITERS, RS_E_NONFINITE, RS_POSITIVE_DOMAIN, Context, PICtrl,  # noqa: F401
+                   ReseactError, SplitOpts, pictrl_ros23, pictrl_ssprk43)
 
 
 def ssprk43_step(ctx: Context, u, t, dt, abstol, reltol):
@@ -51,28 +55,114 @@ def adaptive_solve(step, ctx: Context, u0, t0, tend, dt0, ctrl: PICtrl, abstol=1
     return ctx.get_state(), c.t_end, dt, c.naccept, c.nreject
 
 
+def forcing_stops(t0: float, tf: float, macro_dt: float, forcing_tstops=()):
+    """Stop grid of one Lie-Trotter window (op_split.jl:122-144): the
+    macro-step lattice t0 + k*macro_dt short of tf, the forcing cadence
+    boundaries in the HALF-OPEN interval t0 < x <= tf (rounded to 6 digits),
+    and tf itself, sorted and unique.  Returns (stops, fstops).
+
+    The right end is included on purpose: a caller that drives one macro step
+    at a time would otherwise drop every boundary that lands on a step edge
+    (excluded as the end of one step and again as the start of the next), and
+    the forcing would never refresh.  The strict left end keeps a boundary at
+    t0 from firing twice."""
+    t0, tf, macro_dt = float(t0), float(tf), float(macro_dt)
+    if not macro_dt > 0.0:
+        raise ValueError("macro_dt must be positive")
+    fstops = {round(float(t), 6) for t in forcing_tstops if t0 + 1e-6 < float(t) <= tf + 1e-6}
+    grid, k = [], 1
+    while t0 + k * macro_dt <= tf - 1e-9:
+        grid.append(t0 + k * macro_dt)
+        k += 1
+    return sorted(set(grid) | fstops | {tf}), fstops
+
+
+# Inner-solve errors that end a window with a failed retcode instead of
+# raising (the inner integrators' own failure retcodes in the reference).
+_FAILURES = {RS_E_MAXITERS: "MaxIters", RS_E_NONFINITE: "Unstable"}
+
+
 def lie_trotter_solve(ctx: Context, u0, tspan, macro_dt, reltols=(1e-3, 1e-3),
-                      abstols=(1e-6, 1e-6), clamp_nonneg=True, dt0=(15.0, 0.5),
-                      chem_percell=False, maxiters=20000):
-    """Macro-step lattice t0 + k*macro_dt then tf (op_split.jl:142-156)."""
+                      abstols=(1e-6, 1e-6), refresh=None, forcing_tstops=(),
+                      clamp_nonneg=True, dt0=(15.0, 0.5), chem_percell=False, maxiters=20000,
+                      positive_domain=False):
+    """One Lie-Trotter window over `tspan` (op_split.jl:100-164).
+
+    Steps through `forcing_stops`: transport then chemistry over each stop
+    interval (rs_macro_step), then max(u, 0) on the state when
+    `clamp_nonneg` (:151), then `refresh(t)` when the interval ended on a
+    forcing boundary (:154), so the next interval sees fresh forcing.  The
+    inner solvers start from `dt0` and carry their step sizes across the
+    window.  `positive_domain` rejects negative chemistry trial states
+    (RS_POSITIVE_DOMAIN), which the reference wants and cannot yet pass
+    through its splitting package (HELPERS.md:396-447).
+
+    An inner solve that runs out of iterations or goes non-finite ends the
+    window: the result carries retcode "MaxIters" / "Unstable" and the state
+    reached.  Returns dict(u, retcode, naT, nrT, naC, nrC, nmacro)."""
     t0, tf = float(tspan[0]), float(tspan[1])
-    grid = list(np.arange(t0 + macro_dt, tf - 1e-9, macro_dt))
-    stops = sorted(set(grid + [tf]))
+    stops, fstops = forcing_stops(t0, tf, macro_dt, forcing_tstops)
+    flags = RS_POSITIVE_DOMAIN if positive_domain else 0
     opts = SplitOpts(pictrl_ssprk43(), pictrl_ros23(), abstols[0], reltols[0], abstols[1],
-                     reltols[1], 1 if clamp_nonneg else 0, 1 if chem_percell else 0, maxiters)
+                     reltols[1], flags, 1 if chem_percell else 0, maxiters)
     ctx.set_state(u0)
     dtT, dtC = dt0
     tcur = t0
+    retcode = "Success"
     naT = nrT = naC = nrC = nmacro = 0
     for tnext in stops:
         if tnext <= tcur + 1e-9:
             continue
-        dtT, dtC, cT, cC = ctx.macro_step(tcur, tnext, dtT, dtC, opts)
+        try:
+            dtT, dtC, cT, cC = ctx.macro_step(tcur, tnext, dtT, dtC, opts)
+        except ReseactError as e:
+            if e.code not in _FAILURES:
+                raise
+            retcode = _FAILURES[e.code]
+            break
         naT += cT.naccept; nrT += cT.nreject; naC += cC.naccept; nrC += cC.nreject
+        if clamp_nonneg:
+            ctx.clamp_state()
         tcur = tnext
         nmacro += 1
-    return dict(u=ctx.get_state(), retcode="Success", naT=naT, nrT=nrT, naC=naC, nrC=nrC,
+        if refresh is not None and round(tcur, 6) in fstops:
+            refresh(tcur)
+    return dict(u=ctx.get_state(), retcode=retcode, naT=naT, nrT=nrT, naC=naC, nrC=nrC,
                 nmacro=nmacro)
+
+
+def _ok(r) -> bool:
+    return r["retcode"] in ("Success", "Default")
+
+
+def lie_trotter_solve_bisect(ctx: Context, u0, tspan, macro_dt, min_macro_dt=9.0,
+                             on_bisect=None, **kw):
+    """`lie_trotter_solve`, with a failed window retried as two half windows,
+    recursively, until a window would be no longer than `min_macro_dt`; then
+    the failure propagates (op_split.jl:219-242).  Each half starts its inner
+    solvers afresh; `on_bisect(t0, tf)` is called before each split.  Counts
+    are summed over the halves that succeeded (the failed attempt's work is
+    not in them).  Call it one macro step at a time: a failed multi-step
+    window is redone whole."""
+    t0, tf = float(tspan[0]), float(tspan[1])
+    r = lie_trotter_solve(ctx, u0, (t0, tf), macro_dt, **kw)
+    if _ok(r) or (tf - t0) <= min_macro_dt + 1e-9:
+        return r
+    tm = 0.5 * (t0 + tf)
+    if on_bisect is not None:
+        on_bisect(t0, tf)
+    r1 = lie_trotter_solve_bisect(ctx, u0, (t0, tm), min(float(macro_dt), tm - t0),
+                                  min_macro_dt, on_bisect, **kw)
+    if not _ok(r1):
+        return r1
+    r2 = lie_trotter_solve_bisect(ctx, r1["u"], (tm, tf), min(float(macro_dt), tf - tm),
+                                  min_macro_dt, on_bisect, **kw)
+    if not _ok(r2):
+        return r2
+    out = dict(u=r2["u"], retcode=r2["retcode"])
+    for k in ("naT", "nrT", "naC", "nrC", "nmacro"):
+        out[k] = r1[k] + r2[k]
+    return out
 
 
 def ssprk43_step_vjp(ctx: Context, u, t, dt, lam):
