@@ -2,7 +2,8 @@
 #
 # Mirrors the operator API of tools/reactant_handoff/rx_traced_integrator.jl
 # (PICtrl, ssprk43_step, ros23_step, adaptive_solve, *_step_vjp) and
-# tools/reactant_handoff/op_split.jl (lie_trotter_solve) on top of the C ABI of
+# tools/reactant_handoff/op_split.jl (lie_trotter_solve, lie_trotter_solve_bisect)
+# on top of the C ABI of
 # include/reseact_hip.h, called with `ccall`.  The RHS closures of the
 # reference are replaced by a `Context` that holds the grid, wind, mechanism
 # and emissions on the GPU.  `rrule(split_loss, ...)` gives ChainRulesCore the
@@ -154,28 +155,72 @@ function adaptive_solve(stepfn, c::Context, u0, t0, tend, dt0, ctrl::PICtrl;
     return get_state(c), cnt[].t_end, dt[], Float64(cnt[].naccept), Float64(cnt[].nreject)
 end
 
-# ---- Lie-Trotter driver (op_split.jl:100-164) --------------------------------
+# ---- Lie-Trotter driver (op_split.jl:100-164, :219-242) ---------------------
+const RS_CLAMP_NONNEG = Cint(1)
+const RS_POSITIVE_DOMAIN = Cint(2)
+const RS_E_NONFINITE = Cint(3)
+const RS_E_MAXITERS = Cint(4)
+
+clamp_state!(c::Context) = check(c, ccall((:rs_clamp_state, LIB), Cint, (Ptr{Cvoid},), c.ptr))
+
+# Stop grid (op_split.jl:122-144): macro lattice short of tf, forcing boundaries
+# in the half-open t0 < x <= tf, and tf.
+function forcing_stops(t0::Float64, tf::Float64, macro_dt::Real, forcing_tstops)
+    fstops = Set(round(t; digits = 6) for t in forcing_tstops if t0 + 1e-6 < t <= tf + 1e-6)
+    grid = collect((t0 + macro_dt):macro_dt:(tf - 1e-9))
+    return sort!(unique!(vcat(grid, collect(fstops), Float64[tf]))), fstops
+end
+
 function lie_trotter_solve(c::Context, u0, tspan; macro_dt::Real,
                            reltols = (1e-3, 1e-3), abstols = (1e-6, 1e-6),
-                           clamp_nonneg::Bool = true, dt0 = (15.0, 0.5), maxiters = 20000)
+                           refresh = (_t -> nothing), forcing_tstops = Float64[],
+                           clamp_nonneg::Bool = true, positive_domain::Bool = false,
+                           dt0 = (15.0, 0.5), maxiters = 20000)
     t0, tf = Float64(tspan[1]), Float64(tspan[2])
-    stops = sort!(unique!(vcat(collect((t0 + macro_dt):macro_dt:(tf - 1e-9)), [tf])))
+    stops, fstops = forcing_stops(t0, tf, macro_dt, forcing_tstops)
+    flags = positive_domain ? RS_POSITIVE_DOMAIN : Cint(0)
     opts = SplitOpts(pictrl_ssprk43(), pictrl_ros23(), abstols[1], reltols[1], abstols[2],
-                     reltols[2], clamp_nonneg, 0, maxiters)
+                     reltols[2], flags, 0, maxiters)
     set_state!(c, u0)
     dtT = Ref(Float64(dt0[1])); dtC = Ref(Float64(dt0[2]))
-    naT = nrT = naC = nrC = 0; nmacro = 0; tcur = t0
+    naT = nrT = naC = nrC = 0; nmacro = 0; tcur = t0; retcode = :Success
     for tnext in stops
         tnext <= tcur + 1e-9 && continue
         cT = Ref(Counts()); cC = Ref(Counts())
-        check(c, ccall((:rs_macro_step, LIB), Cint,
-                       (Ptr{Cvoid}, Cdouble, Cdouble, Ref{Cdouble}, Ref{Cdouble}, Ref{SplitOpts},
-                        Ref{Counts}, Ref{Counts}),
-                       c.ptr, tcur, tnext, dtT, dtC, Ref(opts), cT, cC))
+        rc = ccall((:rs_macro_step, LIB), Cint,
+                   (Ptr{Cvoid}, Cdouble, Cdouble, Ref{Cdouble}, Ref{Cdouble}, Ref{SplitOpts},
+                    Ref{Counts}, Ref{Counts}),
+                   c.ptr, tcur, tnext, dtT, dtC, Ref(opts), cT, cC)
+        if rc == RS_E_MAXITERS || rc == RS_E_NONFINITE
+            retcode = rc == RS_E_MAXITERS ? :MaxIters : :Unstable
+            break
+        end
+        check(c, rc)
         naT += cT[].naccept; nrT += cT[].nreject; naC += cC[].naccept; nrC += cC[].nreject
+        clamp_nonneg && clamp_state!(c)
         tcur = tnext; nmacro += 1
+        round(tcur; digits = 6) in fstops && refresh(tcur)
     end
-    return (; u = get_state(c), retcode = :Success, naT, nrT, naC, nrC, nmacro)
+    return (; u = get_state(c), retcode, naT, nrT, naC, nrC, nmacro)
+end
+
+_ltg_ok(r) = r.retcode === :Success || r.retcode === :Default
+
+function lie_trotter_solve_bisect(c::Context, u0, tspan; macro_dt, min_macro_dt::Real = 9.0,
+                                  on_bisect = (_t0, _tf) -> nothing, kwargs...)
+    t0 = Float64(tspan[1]); tf = Float64(tspan[2])
+    r = lie_trotter_solve(c, u0, (t0, tf); macro_dt = macro_dt, kwargs...)
+    (_ltg_ok(r) || (tf - t0) <= min_macro_dt + 1e-9) && return r
+    tm = 0.5 * (t0 + tf)
+    on_bisect(t0, tf)
+    r1 = lie_trotter_solve_bisect(c, u0, (t0, tm); macro_dt = min(Float64(macro_dt), tm - t0),
+                                  min_macro_dt, on_bisect, kwargs...)
+    _ltg_ok(r1) || return r1
+    r2 = lie_trotter_solve_bisect(c, r1.u, (tm, tf); macro_dt = min(Float64(macro_dt), tf - tm),
+                                  min_macro_dt, on_bisect, kwargs...)
+    _ltg_ok(r2) || return r2
+    return (; u = r2.u, retcode = r2.retcode, naT = r1.naT + r2.naT, nrT = r1.nrT + r2.nrT,
+            naC = r1.naC + r2.naC, nrC = r1.nrC + r2.nrC, nmacro = r1.nmacro + r2.nmacro)
 end
 
 # ---- one-step adjoints (:610-623) ---------------------------------------------
