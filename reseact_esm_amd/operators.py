@@ -13,6 +13,8 @@ the reference finds the calls it already makes:
       -> dict(u, retcode, naT, nrT, naC, nrC, nmacro)              (op_split.jl:100-164)
   lie_trotter_solve_bisect(ctx, u0, tspan; macro_dt, min_macro_dt, on_bisect, ...)
       -> the same dict                                             (op_split.jl:219-242)
+  simulate(ctx, u0, tspan, macro_dt; sink, out_every, flush_every, statedump, ...)
+      -> dict(u, t, ok, retcode, nstep, nT, nC, nbisect)           (run_reseact.jl:339-406)
   ssprk43_step_vjp(ctx, u, t, dt, lam) -> (lam_in, None)           (:618-623)
   ros23_step_vjp(ctx, u, t, dt, lam) -> (lam_in, grad_emis)        (:610-617)
 
@@ -163,6 +165,60 @@ def lie_trotter_solve_bisect(ctx: Context, u0, tspan, macro_dt, min_macro_dt=9.0
     for k in ("naT", "nrT", "naC", "nrC", "nmacro"):
         out[k] = r1[k] + r2[k]
     return out
+
+
+def simulate(ctx: Context, u0, tspan, macro_dt, min_macro_dt=9.0, sink=None, out_every=1,
+             flush_every=1, statedump: str | None = None, names=None, on_bisect=None,
+             report=None, **kw):
+    """The run drivers' macro-step loop (run_reseact.jl:339-406): one
+    `lie_trotter_solve_bisect` call per macro step (so a bisection halves the
+    failed step itself), the state written to `sink` at t0 and after every
+    `out_every` steps, the sink flushed every `out_every * flush_every` steps
+    and at the end.  A step that still fails stops the run without advancing
+    t; with `statedump` the state it started from (restartable) and the state
+    where it gave up are written there (run_reseact.jl:378-391).  `report(t,
+    u)` runs after each step.  `kw` goes to lie_trotter_solve (tolerances,
+    refresh, forcing_tstops, clamp_nonneg, ...).
+
+    Returns dict(u, t, ok, retcode, nstep, nT, nC, nbisect)."""
+    from .output import write_statedump
+
+    t0, tend = float(tspan[0]), float(tspan[1])
+    u = np.array(u0, np.float64, copy=True)
+    nbisect = 0
+
+    def bis(a, b):
+        nonlocal nbisect
+        nbisect += 1
+        if on_bisect is not None:
+            on_bisect(a, b)
+
+    if sink is not None:
+        sink.write(t0, u)
+    t, nstep, nT, nC, ok, retcode = t0, 0, 0, 0, True, "Success"
+    while t < tend - 1e-9:
+        nstep += 1
+        tnext = min(t + macro_dt, tend)
+        res = lie_trotter_solve_bisect(ctx, u, (t, tnext), tnext - t, min_macro_dt=min_macro_dt,
+                                       on_bisect=bis, **kw)
+        if not _ok(res):
+            ok, retcode = False, res["retcode"]
+            if statedump:
+                write_statedump(statedump, ctx.ns, ctx.nc, t, u, res["u"], names)
+            break
+        u = res["u"]
+        nT += res["naT"]
+        nC += res["naC"]
+        t = tnext
+        if report is not None:
+            report(t, u)
+        if sink is not None and nstep % out_every == 0:
+            sink.write(t, u)
+            if nstep % (out_every * flush_every) == 0:
+                sink.flush()
+    if sink is not None:
+        sink.flush()
+    return dict(u=u, t=t, ok=ok, retcode=retcode, nstep=nstep, nT=nT, nC=nC, nbisect=nbisect)
 
 
 def ssprk43_step_vjp(ctx: Context, u, t, dt, lam):
