@@ -123,6 +123,57 @@ RS_API int rs_set_grid(rs_ctx* ctx, int nlon, int nlat, int nlev, int ns, int nt
  * 3rd-order upwind operator of the workloads). */
 RS_API int rs_set_wind(rs_ctx* ctx, const double* cx, const double* wz, const double* wl);
 
+/* ---- monotone PPM transport in CWC mixing-ratio form ------------------------
+ * The reference's own transport (reseact.esm:1536-3100; the regional
+ * prototype prototypes/transport_3d/transport_3d.esm): flux-form PPM with the
+ * CW84 monotone slopes and parabola limiter, open lateral walls with
+ * prescribed inflow mixing ratios, and the hybrid sigma-pressure mass-form
+ * vertical with no flux through the surface and the lid.  The discretisation
+ * rules are imported by the reference from EarthSciDiscretizations
+ * (reseact.esm:61-110), which is not in its tree; the operator here restates
+ * Colella & Woodward (1984) eqs. 1.6-1.10 in method-of-lines form (the face
+ * value is the limited edge value of the upwind cell).
+ *
+ * With scheme RS_TRANSPORT_PPM the transported block is
+ *   species 0        air mass m (Pa):  dm/dt = -div(M)
+ *   species 1..NT-1  mixing ratios q:  dq/dt = (-div(M q_f) + q div(M)) / m
+ * where div(F) = (Fx[i+1]-Fx[i]) dphi_lat[j] / (dlon dS_lat[j])
+ *              + (coslat_e[j+1] Fy[j+1] - coslat_e[j] Fy[j]) / dS_lat[j]
+ *              + (Fz[k+1] - Fz[k])   (k = 0 at the surface, +ve upward).
+ * A uniform q stays uniform to the last bit under any air-mass flux (the
+ * reference's CWC gate, transport_3d.esm:269).  rs_set_wind is not needed. */
+#define RS_TRANSPORT_UPWIND3 0   /* the synthetic workloads' operator (default) */
+#define RS_TRANSPORT_PPM     1
+RS_API int rs_set_transport_scheme(rs_ctx* ctx, int scheme);
+
+/* Grid metrics on the unit sphere: lon_periodic (1) or open west/east walls
+ * (0); dlon (rad); coslat_e[NLAT+1] meridional face lengths (0 at a pole);
+ * dS_lat[NLAT] cell area weights sin(phie[j+1]) - sin(phie[j]); dphi_lat[NLAT]
+ * zonal face lengths (rad).  reseact.esm:146-177 (phie, coslat_e, dS_lat,
+ * dphi_lat). */
+RS_API int rs_set_ppm_geometry(rs_ctx* ctx, int lon_periodic, double dlon,
+                               const double* coslat_e, const double* dS_lat,
+                               const double* dphi_lat);
+
+/* Face air-mass fluxes (Pa/s) and layer thicknesses (Pa), lon fastest:
+ *   Mx[(k*NLAT + j)*(NLON+1) + i], i = 0..NLON (west face of cell i; with a
+ *      periodic longitude Mx[..NLON] must equal Mx[..0]);
+ *   My[(k*(NLAT+1) + j)*NLON + i], j = 0..NLAT (south face of row j);
+ *   Mz[(k*NLAT + j)*NLON + i],     k = 0..NLEV (lower face of level k, +ve
+ *      upward; zero at k = 0 and k = NLEV: no flux through surface and lid);
+ *   dp[(k*NLAT + j)*NLON + i] > 0, the widths of the vertical parabolae
+ *      (reseact.esm:178-197 Mx, My; :225-234 dp; :282-291 Mz). */
+RS_API int rs_set_mass_fluxes(rs_ctx* ctx, const double* Mx, const double* My,
+                              const double* Mz, const double* dp);
+
+/* Inflow mixing ratios of the NT-1 mixing-ratio species in the 3-cell halos
+ * of the open walls (qbc_w_X/qbc_e_X/qbc_s_X/qbc_n_X of reseact.esm:726-1121,
+ * one set per species X): qbc_w[(s*NLEV + k)*NLAT + j], the same for qbc_e;
+ * qbc_s[(s*NLEV + k)*NLON + i], the same for qbc_n; s = 0 is species 1.
+ * NULL leaves that wall's halo at 0.  A periodic longitude ignores w/e. */
+RS_API int rs_set_inflow(rs_ctx* ctx, const double* qbc_w, const double* qbc_e,
+                         const double* qbc_s, const double* qbc_n);
+
 /* Mass-action mechanism, NR reactions: reactants[2r], reactants[2r+1]
  * (second = -1 for unimolecular; equal entries = 2A), products[2r],
  * products[2r+1] (-1 = none), rate constants k[r] (1/s or 1/(conc s)),

@@ -64,6 +64,14 @@ struct rs_ctx {
     size_t nc = 0, ncol = 0;
     std::vector<double> cx, wz, wl;
     bool have_wind = false, have_mech = false, have_state = false;
+    // PPM transport (scheme RS_TRANSPORT_PPM)
+    int scheme = RS_TRANSPORT_UPWIND3;
+    int lon_periodic = 1;
+    double dlon = 0.0;
+    std::vector<double> coslat_e, dS_lat, dphi_lat;   // nlat+1, nlat, nlat
+    std::vector<double> Mx, My, Mz, dp;               // faces / cells, lon fastest
+    std::vector<double> qw, qe, qs, qn;               // inflow halos, (nt-1) species
+    bool have_ppm_geom = false, have_ppm_flux = false;
     // mechanism
     int nr = 0;
     std::vector<int> ra, rb, pa, pb, photo;
@@ -238,48 +246,255 @@ double fixed_sum(const std::vector<double>& v) {
     return s;
 }
 
+// ----------------------------------------------------------------------------
+// Monotone PPM in CWC mixing-ratio form (reseact.esm:1536-3100 and
+// prototypes/transport_3d/transport_3d.esm; the discretisation rules of
+// reseact.esm:61-110 are not in the reference tree).  Restatement of
+// Colella & Woodward (1984), method-of-lines form:
+//   eq. 1.7-1.8  monotone slope  dm_i = sgn(da) min(|da|, 2|a_i-a_{i-1}|,
+//                2|a_{i+1}-a_i|) if (a_{i+1}-a_i)(a_i-a_{i-1}) > 0, else 0,
+//                da = (a_{i+1}-a_{i-1})/2 (uniform) or the weighted form;
+//   eq. 1.6      edge value a_{i+1/2} = a_i + (a_{i+1}-a_i)/2 - (dm_{i+1}-dm_i)/6
+//                (uniform) or the width-weighted form (vertical, widths dp);
+//   eq. 1.10     parabola limiter on (aL, aR) of each cell;
+// the face value is the limited edge of the upwind cell.  Lateral open walls
+// fill 3 halo cells with the prescribed inflow mixing ratio; the vertical
+// clamps its stencil at the surface and the lid (no flux there).
+// ----------------------------------------------------------------------------
+inline double ppm_slope(double am, double a0, double ap) {
+    const double dl = a0 - am, dr = ap - a0;
+    if (dr * dl > 0.0) {
+        const double da = 0.5 * (ap - am);
+        const double m = std::min(std::fabs(da), std::min(2.0 * std::fabs(dl), 2.0 * std::fabs(dr)));
+        return std::copysign(m, da);
+    }
+    return 0.0;
+}
+
+inline double ppm_edge(double a0, double a1, double d0, double d1) {
+    const double e = a0 + 0.5 * (a1 - a0);
+    return e - (d1 - d0) * C16;
+}
+
+// Width-weighted slope (eq. 1.7) of cell 0 with neighbour widths hm, hp.
+inline double ppm_slope_nu(double hm, double h0, double hp, double am, double a0, double ap) {
+    const double dl = a0 - am, dr = ap - a0;
+    if (dr * dl > 0.0) {
+        const double w = h0 / (hm + h0 + hp);
+        const double cr = (2.0 * hm + h0) / (hp + h0);
+        const double cl = (h0 + 2.0 * hp) / (hm + h0);
+        const double da = w * (cr * dr + cl * dl);
+        const double m = std::min(std::fabs(da), std::min(2.0 * std::fabs(dl), 2.0 * std::fabs(dr)));
+        return std::copysign(m, da);
+    }
+    return 0.0;
+}
+
+// Width-weighted edge between cells 0 and 1 (eq. 1.6); hm, h2 the outer widths.
+inline double ppm_edge_nu(double hm, double h0, double h1, double h2, double a0, double a1,
+                          double d0, double d1) {
+    // the width-only coefficients first (a kernel may tabulate them per edge)
+    const double cA = h0 / (h0 + h1);
+    const double g = (2.0 * h1 * h0) / (h0 + h1);
+    const double t1 = (hm + h0) / (2.0 * h0 + h1);
+    const double t2 = (h2 + h1) / (2.0 * h1 + h0);
+    const double cB = g * (t1 - t2);
+    const double cC = h0 * t1;
+    const double cD = h1 * ((h1 + h2) / (h0 + 2.0 * h1));
+    const double cS = 1.0 / (((hm + h0) + h1) + h2);
+    const double da = a1 - a0;
+    const double e0 = a0 + cA * da;
+    const double br = ((cB * da) - (cC * d1)) + (cD * d0);
+    return e0 + br * cS;
+}
+
+// Parabola limiter (eq. 1.10) of a cell with mean a0 and edges aL, aR.
+inline void ppm_limit(double a0, double& aL, double& aR) {
+    if ((aR - a0) * (a0 - aL) <= 0.0) {
+        aL = a0;
+        aR = a0;
+        return;
+    }
+    const double d = aR - aL;
+    const double a6 = 6.0 * (a0 - 0.5 * (aL + aR));
+    const double dd = d * d;
+    const double da6 = d * a6;
+    if (da6 > dd) aL = 3.0 * a0 - 2.0 * aR;
+    else if (-dd > da6) aR = 3.0 * a0 - 2.0 * aL;
+}
+
+// Face value from the six cells v[0..5] = a_{f-3} .. a_{f+2} around face f,
+// uniform spacing: the limited right edge of cell f-1 for M >= 0, the limited
+// left edge of cell f otherwise.
+inline double ppm_face(const double* v, double M) {
+    if (M >= 0.0) {
+        const double dm2 = ppm_slope(v[0], v[1], v[2]);
+        const double dm1 = ppm_slope(v[1], v[2], v[3]);
+        const double d0 = ppm_slope(v[2], v[3], v[4]);
+        double aL = ppm_edge(v[1], v[2], dm2, dm1);
+        double aR = ppm_edge(v[2], v[3], dm1, d0);
+        ppm_limit(v[2], aL, aR);
+        return aR;
+    }
+    const double dm1 = ppm_slope(v[1], v[2], v[3]);
+    const double d0 = ppm_slope(v[2], v[3], v[4]);
+    const double dp1 = ppm_slope(v[3], v[4], v[5]);
+    double aL = ppm_edge(v[2], v[3], dm1, d0);
+    double aR = ppm_edge(v[3], v[4], d0, dp1);
+    ppm_limit(v[3], aL, aR);
+    return aL;
+}
+
+// The same with widths h[0..5] of the six cells (vertical).
+inline double ppm_face_nu(const double* v, const double* h, double M) {
+    if (M >= 0.0) {
+        const double dm2 = ppm_slope_nu(h[0], h[1], h[2], v[0], v[1], v[2]);
+        const double dm1 = ppm_slope_nu(h[1], h[2], h[3], v[1], v[2], v[3]);
+        const double d0 = ppm_slope_nu(h[2], h[3], h[4], v[2], v[3], v[4]);
+        double aL = ppm_edge_nu(h[0], h[1], h[2], h[3], v[1], v[2], dm2, dm1);
+        double aR = ppm_edge_nu(h[1], h[2], h[3], h[4], v[2], v[3], dm1, d0);
+        ppm_limit(v[2], aL, aR);
+        return aR;
+    }
+    const double dm1 = ppm_slope_nu(h[1], h[2], h[3], v[1], v[2], v[3]);
+    const double d0 = ppm_slope_nu(h[2], h[3], h[4], v[2], v[3], v[4]);
+    const double dp1 = ppm_slope_nu(h[3], h[4], h[5], v[3], v[4], v[5]);
+    double aL = ppm_edge_nu(h[1], h[2], h[3], h[4], v[2], v[3], dm1, d0);
+    double aR = ppm_edge_nu(h[2], h[3], h[4], h[5], v[3], v[4], d0, dp1);
+    ppm_limit(v[3], aL, aR);
+    return aL;
+}
+
+// Flux divergence of one cell from its six face fluxes, in the operator's
+// fixed order ((x + y) + z); gx = dphi_lat / (dlon dS_lat), rdS = 1 / dS_lat.
+inline double ppm_div(double fw, double fe, double fs, double fn, double fb, double ft,
+                      double gx, double ces, double cen, double rdS) {
+    const double dx = (fe - fw) * gx;
+    const double dy = (cen * fn - ces * fs) * rdS;
+    const double dz = ft - fb;
+    return (dx + dy) + dz;
+}
+
+// f = L_ppm u over the transported block (species 0 = air mass).
+void ppm_rhs_all(const rs_ctx& c, const double* u, double* f) {
+    const int nlon = c.nlon, nlat = c.nlat, nlev = c.nlev, nt = c.nt;
+    const size_t nc = c.nc, plane = (size_t)nlon * nlat;
+    const double* m = u;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < nlev; ++k) {
+        for (int j = 0; j < nlat; ++j) {
+            const double gx = c.dphi_lat[j] / (c.dlon * c.dS_lat[j]);
+            const double ces = c.coslat_e[j], cen = c.coslat_e[j + 1], rdS = 1.0 / c.dS_lat[j];
+            for (int i = 0; i < nlon; ++i) {
+                const size_t cell = (size_t)k * plane + (size_t)j * nlon + i;
+                const double* mxr = c.Mx.data() + ((size_t)k * nlat + j) * (nlon + 1);
+                const double mw = mxr[i], me = mxr[i + 1];
+                const double ms = c.My[((size_t)k * (nlat + 1) + j) * nlon + i];
+                const double mn = c.My[((size_t)k * (nlat + 1) + j + 1) * nlon + i];
+                const double mb = c.Mz[(size_t)k * plane + (size_t)j * nlon + i];
+                const double mt = c.Mz[(size_t)(k + 1) * plane + (size_t)j * nlon + i];
+                const double divM = ppm_div(mw, me, ms, mn, mb, mt, gx, ces, cen, rdS);
+                f[cell] = -divM;
+                for (int s = 1; s < nt; ++s) {
+                    const double* q = u + (size_t)s * nc;
+                    const int sq = s - 1;
+                    // lon line (periodic wrap or the w/e inflow halo)
+                    auto X = [&](int ii) -> double {
+                        if (c.lon_periodic) {
+                            ii %= nlon;
+                            if (ii < 0) ii += nlon;
+                            return q[(size_t)k * plane + (size_t)j * nlon + ii];
+                        }
+                        if (ii < 0) return c.qw[((size_t)sq * nlev + k) * nlat + j];
+                        if (ii >= nlon) return c.qe[((size_t)sq * nlev + k) * nlat + j];
+                        return q[(size_t)k * plane + (size_t)j * nlon + ii];
+                    };
+                    auto Y = [&](int jj) -> double {
+                        if (jj < 0) return c.qs[((size_t)sq * nlev + k) * nlon + i];
+                        if (jj >= nlat) return c.qn[((size_t)sq * nlev + k) * nlon + i];
+                        return q[(size_t)k * plane + (size_t)jj * nlon + i];
+                    };
+                    auto Zc = [&](int kk) { return std::min(std::max(kk, 0), nlev - 1); };
+                    double v[6], h[6];
+                    for (int d = 0; d < 6; ++d) v[d] = X(i - 3 + d);
+                    const double qw_ = ppm_face(v, mw);
+                    for (int d = 0; d < 6; ++d) v[d] = X(i - 2 + d);
+                    const double qe_ = ppm_face(v, me);
+                    for (int d = 0; d < 6; ++d) v[d] = Y(j - 3 + d);
+                    const double qs_ = ppm_face(v, ms);
+                    for (int d = 0; d < 6; ++d) v[d] = Y(j - 2 + d);
+                    const double qn_ = ppm_face(v, mn);
+                    for (int d = 0; d < 6; ++d) {
+                        const size_t x = (size_t)Zc(k - 3 + d) * plane + (size_t)j * nlon + i;
+                        v[d] = q[x];
+                        h[d] = c.dp[x];
+                    }
+                    const double qb_ = ppm_face_nu(v, h, mb);
+                    for (int d = 0; d < 6; ++d) {
+                        const size_t x = (size_t)Zc(k - 2 + d) * plane + (size_t)j * nlon + i;
+                        v[d] = q[x];
+                        h[d] = c.dp[x];
+                    }
+                    const double qt_ = ppm_face_nu(v, h, mt);
+                    const double divF = ppm_div(mw * qw_, me * qe_, ms * qs_, mn * qn_, mb * qb_,
+                                                mt * qt_, gx, ces, cen, rdS);
+                    const double qc = q[cell];
+                    const double num = qc * divM - divF;
+                    f[(size_t)s * nc + cell] = num / m[cell];
+                }
+            }
+        }
+    }
+}
+
+// f = L u over the transported block, for either scheme.
+void transport_rhs_all(const rs_ctx& c, const double* u, double* f) {
+    if (c.scheme == RS_TRANSPORT_PPM) {
+        ppm_rhs_all(c, u, f);
+        return;
+    }
+    for (int s = 0; s < c.nt; ++s)
+        transport_rhs_block(c, u + (size_t)s * c.nc, f + (size_t)s * c.nc);
+}
+
 // One SSPRK43 attempt (ssprk43_step_unrolled, rx_traced_integrator.jl:404-437).
 // Writes the raw result into unew (all species; non-transported copied), returns EEst.
+// Stages run over the whole transported block (the PPM tendency couples the
+// mixing ratios to the air mass); every element's arithmetic is per element.
 double ssprk43_attempt(const rs_ctx& c, const std::vector<double>& u, std::vector<double>& unew,
                        double dt, double atol, double rtol) {
     const size_t nc = c.nc, nT = (size_t)c.nt * nc;
-    std::vector<double> k(nc), u1(nT), u2(nT), u3(nT), u4(nT);
+    std::vector<double> k(nT), u1(nT), u2(nT), u3(nT), u4(nT);
     unew.assign(u.begin(), u.end());
     const double dt_2 = dt / 2;
-    std::vector<double> sq(nc * (size_t)c.nt);
-    for (int s = 0; s < c.nt; ++s) {
-        const double* us = u.data() + s * nc;
-        double* a1 = u1.data() + s * nc;
-        double* a2 = u2.data() + s * nc;
-        double* a3 = u3.data() + s * nc;
-        double* a4 = u4.data() + s * nc;
-        transport_rhs_block(c, us, k.data());
-        for (size_t x = 0; x < nc; ++x) { double du = dt_2 * k[x]; a1[x] = us[x] + du; }
-        transport_rhs_block(c, a1, k.data());
-        for (size_t x = 0; x < nc; ++x) { double du = dt_2 * k[x]; a2[x] = a1[x] + du; }
-        transport_rhs_block(c, a2, k.data());
-        for (size_t x = 0; x < nc; ++x) { double du = dt_2 * k[x]; a3[x] = a2[x] + du; }
-        for (size_t x = 0; x < nc; ++x) {
-            double a = (2.0 / 3.0) * us[x];
-            double b = (1.0 / 3.0) * a3[x];
-            a4[x] = a + b;
-        }
-        transport_rhs_block(c, a4, k.data());
-        double* un = unew.data() + s * nc;
-        for (size_t x = 0; x < nc; ++x) {
-            double du = dt_2 * k[x];
-            un[x] = a4[x] + du;
-            double a = (1.0 / 3.0) * us[x];
-            double b = (2.0 / 3.0) * a3[x];
-            double ut = a + b;
-            double ud = ut - un[x];
-            ut = 0.5 * ud;
-            double am = std::max(std::fabs(us[x]), std::fabs(un[x]));
-            double den = am * rtol;
-            den = den + atol;
-            double at = ut / den;
-            sq[s * nc + x] = at * at;
-        }
+    std::vector<double> sq(nT);
+    const double* us = u.data();
+    transport_rhs_all(c, us, k.data());
+    for (size_t x = 0; x < nT; ++x) { double du = dt_2 * k[x]; u1[x] = us[x] + du; }
+    transport_rhs_all(c, u1.data(), k.data());
+    for (size_t x = 0; x < nT; ++x) { double du = dt_2 * k[x]; u2[x] = u1[x] + du; }
+    transport_rhs_all(c, u2.data(), k.data());
+    for (size_t x = 0; x < nT; ++x) { double du = dt_2 * k[x]; u3[x] = u2[x] + du; }
+    for (size_t x = 0; x < nT; ++x) {
+        double a = (2.0 / 3.0) * us[x];
+        double b = (1.0 / 3.0) * u3[x];
+        u4[x] = a + b;
+    }
+    transport_rhs_all(c, u4.data(), k.data());
+    double* un = unew.data();
+    for (size_t x = 0; x < nT; ++x) {
+        double du = dt_2 * k[x];
+        un[x] = u4[x] + du;
+        double a = (1.0 / 3.0) * us[x];
+        double b = (2.0 / 3.0) * u3[x];
+        double ut = a + b;
+        double ud = ut - un[x];
+        ut = 0.5 * ud;
+        double am = std::max(std::fabs(us[x]), std::fabs(un[x]));
+        double den = am * rtol;
+        den = den + atol;
+        double at = ut / den;
+        sq[x] = at * at;
     }
     double sse = fixed_sum(sq);
     double N = (double)c.ns * (double)nc;
@@ -719,7 +934,14 @@ int check_ready(rs_ctx* c, bool need_mech, bool need_wind) {
     if (c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
     if (!c->have_state) return fail(c, RS_E_STATE, "state not set");
     if (need_mech && !c->have_mech) return fail(c, RS_E_STATE, "mechanism not set");
-    if (need_wind && !c->have_wind && c->nt > 0) return fail(c, RS_E_STATE, "wind not set");
+    if (need_wind && c->nt > 0) {
+        if (c->scheme == RS_TRANSPORT_PPM) {
+            if (!c->have_ppm_geom || !c->have_ppm_flux)
+                return fail(c, RS_E_STATE, "PPM geometry or mass fluxes not set");
+        } else if (!c->have_wind) {
+            return fail(c, RS_E_STATE, "wind not set");
+        }
+    }
     return RS_OK;
 }
 
@@ -799,6 +1021,11 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
     c->gemis.assign((size_t)ns * c->ncol, 0.0);
     c->jfac.clear(); c->emis.clear();
     c->have_state = c->have_mech = c->have_wind = false;
+    c->scheme = RS_TRANSPORT_UPWIND3;
+    c->have_ppm_geom = c->have_ppm_flux = false;
+    const size_t nq = nt > 1 ? (size_t)(nt - 1) : 0;
+    c->qw.assign(nq * nlev * nlat, 0.0); c->qe = c->qw;
+    c->qs.assign(nq * nlev * nlon, 0.0); c->qn = c->qs;
     return RS_OK;
 }
 
@@ -812,6 +1039,70 @@ int rs_set_wind(rs_ctx* c, const double* cx, const double* wz, const double* wl)
     for (double x : c->wz) if (!std::isfinite(x)) return fail(c, RS_E_NONFINITE, "wz");
     for (double x : c->wl) if (!std::isfinite(x)) return fail(c, RS_E_NONFINITE, "wl");
     c->have_wind = true;
+    return RS_OK;
+}
+
+int rs_set_transport_scheme(rs_ctx* c, int scheme) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (scheme != RS_TRANSPORT_UPWIND3 && scheme != RS_TRANSPORT_PPM)
+        return fail(c, RS_E_ARG, "unknown transport scheme");
+    if (scheme == RS_TRANSPORT_PPM && c->nt < 1)
+        return fail(c, RS_E_ARG, "PPM needs the air mass as transported species 0");
+    c->scheme = scheme;
+    return RS_OK;
+}
+
+static bool all_finite(const std::vector<double>& v) {
+    for (double x : v) if (!std::isfinite(x)) return false;
+    return true;
+}
+
+int rs_set_ppm_geometry(rs_ctx* c, int lon_periodic, double dlon, const double* coslat_e,
+                        const double* dS_lat, const double* dphi_lat) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!coslat_e || !dS_lat || !dphi_lat) return fail(c, RS_E_ARG, "null geometry array");
+    if (!(dlon > 0.0) || !std::isfinite(dlon)) return fail(c, RS_E_ARG, "dlon must be positive");
+    c->lon_periodic = lon_periodic ? 1 : 0;
+    c->dlon = dlon;
+    c->coslat_e.assign(coslat_e, coslat_e + c->nlat + 1);
+    c->dS_lat.assign(dS_lat, dS_lat + c->nlat);
+    c->dphi_lat.assign(dphi_lat, dphi_lat + c->nlat);
+    if (!all_finite(c->coslat_e) || !all_finite(c->dphi_lat)) return fail(c, RS_E_NONFINITE, "geometry");
+    for (double x : c->dS_lat) if (!(x > 0.0) || !std::isfinite(x)) return fail(c, RS_E_ARG, "dS_lat must be positive");
+    c->have_ppm_geom = true;
+    return RS_OK;
+}
+
+int rs_set_mass_fluxes(rs_ctx* c, const double* Mx, const double* My, const double* Mz,
+                       const double* dp) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!Mx || !My || !Mz || !dp) return fail(c, RS_E_ARG, "null flux array");
+    const size_t nlon = c->nlon, nlat = c->nlat, nlev = c->nlev;
+    c->Mx.assign(Mx, Mx + nlev * nlat * (nlon + 1));
+    c->My.assign(My, My + nlev * (nlat + 1) * nlon);
+    c->Mz.assign(Mz, Mz + (nlev + 1) * nlat * nlon);
+    c->dp.assign(dp, dp + c->nc);
+    if (!all_finite(c->Mx) || !all_finite(c->My) || !all_finite(c->Mz))
+        return fail(c, RS_E_NONFINITE, "mass flux");
+    for (double x : c->dp) if (!(x > 0.0) || !std::isfinite(x)) return fail(c, RS_E_ARG, "dp must be positive");
+    for (size_t x = 0; x < c->ncol; ++x)
+        if (c->Mz[x] != 0.0 || c->Mz[nlev * c->ncol + x] != 0.0)
+            return fail(c, RS_E_ARG, "Mz must vanish at the surface and the lid");
+    c->have_ppm_flux = true;
+    return RS_OK;
+}
+
+int rs_set_inflow(rs_ctx* c, const double* qbc_w, const double* qbc_e, const double* qbc_s,
+                  const double* qbc_n) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    const size_t nq = c->nt > 1 ? (size_t)(c->nt - 1) : 0;
+    const size_t nwe = nq * c->nlev * c->nlat, nsn = nq * c->nlev * c->nlon;
+    auto put = [&](std::vector<double>& d, const double* src, size_t n) {
+        if (src) d.assign(src, src + n); else d.assign(n, 0.0);
+    };
+    put(c->qw, qbc_w, nwe); put(c->qe, qbc_e, nwe); put(c->qs, qbc_s, nsn); put(c->qn, qbc_n, nsn);
+    if (!all_finite(c->qw) || !all_finite(c->qe) || !all_finite(c->qs) || !all_finite(c->qn))
+        return fail(c, RS_E_NONFINITE, "inflow mixing ratio");
     return RS_OK;
 }
 
@@ -1014,6 +1305,7 @@ int rs_ssprk43_vjp(rs_ctx* c, double t, double dt) {
     (void)t;
     int rc = check_ready(c, false, true);
     if (rc) return rc;
+    if (c->scheme == RS_TRANSPORT_PPM) return fail(c, RS_E_ARG, "no adjoint of the PPM transport");
     ssprk43_vjp(*c, c->lam, dt);
     return RS_OK;
 }
@@ -1032,6 +1324,8 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
     int rc = check_ready(c, true, true);
     if (rc) return rc;
     if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
+    if (c->scheme == RS_TRANSPORT_PPM && c->nt > 0)
+        return fail(c, RS_E_ARG, "no adjoint of the PPM transport");
     if (o->chem_percell) return fail(c, RS_E_ARG, "adjoint needs the global-norm chemistry");
     if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
         return fail(c, RS_E_ARG, "bad loss");
@@ -1203,6 +1497,7 @@ int rs_oracle_ros23_jvp(rs_ctx* c, double dt, const double* v, const double* Ev,
 int rs_oracle_ssprk43_jvp(rs_ctx* c, double dt, const double* v, double* out) {
     int rc = check_ready(c, false, true);
     if (rc) return rc;
+    if (c->scheme == RS_TRANSPORT_PPM) return fail(c, RS_E_ARG, "no tangent of the PPM transport");
     std::vector<double> vv(v, v + (size_t)c->ns * c->nc), o;
     ssprk43_jvp(*c, vv, o, dt);
     std::copy(o.begin(), o.end(), out);
@@ -1212,15 +1507,16 @@ int rs_oracle_ssprk43_jvp(rs_ctx* c, double dt, const double* v, double* out) {
 // Transport tendency f = L q (species < NT) and the dense per-cell Jacobian,
 // exposed for the operator-level checks.
 int rs_oracle_transport_rhs(rs_ctx* c, const double* q, double* f) {
-    if (!c || !c->have_wind) return fail(c, RS_E_STATE, "wind not set");
-    for (int s = 0; s < c->ns; ++s) {
-        if (s < c->nt) transport_rhs_block(*c, q + (size_t)s * c->nc, f + (size_t)s * c->nc);
-        else std::fill(f + (size_t)s * c->nc, f + (size_t)(s + 1) * c->nc, 0.0);
-    }
+    if (!c) return RS_E_ARG;
+    if (c->scheme == RS_TRANSPORT_PPM ? !(c->have_ppm_geom && c->have_ppm_flux) : !c->have_wind)
+        return fail(c, RS_E_STATE, "transport not set up");
+    transport_rhs_all(*c, q, f);
+    std::fill(f + (size_t)c->nt * c->nc, f + (size_t)c->ns * c->nc, 0.0);
     return RS_OK;
 }
 int rs_oracle_transport_rhs_adj(rs_ctx* c, const double* lam, double* g) {
     if (!c || !c->have_wind) return fail(c, RS_E_STATE, "wind not set");
+    if (c->scheme == RS_TRANSPORT_PPM) return fail(c, RS_E_ARG, "no adjoint of the PPM transport");
     std::fill(g, g + (size_t)c->ns * c->nc, 0.0);
     for (int s = 0; s < c->nt; ++s)
         transport_rhs_adj_block(*c, lam + (size_t)s * c->nc, g + (size_t)s * c->nc);

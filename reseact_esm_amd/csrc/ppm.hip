@@ -1,0 +1,353 @@
+// ppm.hip -- monotone PPM transport stages in CWC mixing-ratio form, gfx950.
+//
+// The reference's own transport (reseact.esm:1536-3100; regional prototype
+// prototypes/transport_3d/transport_3d.esm): flux-form PPM with the
+// Colella-Woodward (1984) monotone slopes (eq. 1.7-1.8), edge interpolation
+// (eq. 1.6: uniform in longitude and latitude, width-weighted in the
+// vertical) and parabola limiter (eq. 1.10), in method-of-lines form: the
+// face value is the limited edge of the upwind cell.
+//   species 0        air mass m:      dm/dt = -div(M)
+//   species 1..NT-1  mixing ratio q:  dq/dt = (q div(M) - div(M q_f)) / m
+// Lateral open walls read 3 halo cells of prescribed inflow mixing ratio;
+// the vertical clamps its stencil at the surface and the lid (Mz = 0 there).
+//
+// One lane per cell, all species of that cell: the air-mass fluxes, metrics,
+// the divergence of M and the vertical width coefficients are loaded once
+// and serve every species; the stencils come through L1/L2 (neighbouring
+// lanes share them).  The width-only vertical coefficients are tabulated per
+// column when the fluxes are set (rs_ppm_coefs), so a stage does no divisions
+// besides the final one by m.  Built with -ffp-contract=off: every statement
+// rounds as the oracle's, so stages agree with it bit for bit.
+
+#include <hip/hip_runtime.h>
+
+#include "rs_launch.h"
+
+namespace {
+
+constexpr int PB = 256;                 // lanes per workgroup
+constexpr double C13 = 1.0 / 3.0;
+constexpr double C23 = 2.0 / 3.0;
+constexpr double C16 = 1.0 / 6.0;
+
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+
+// eq. 1.7-1.8, uniform spacing
+__device__ __forceinline__ double slope_u(double am, double a0, double ap) {
+    const double dl = a0 - am, dr = ap - a0;
+    if (dr * dl > 0.0) {
+        const double da = 0.5 * (ap - am);
+        const double m = dmin(fabs(da), dmin(2.0 * fabs(dl), 2.0 * fabs(dr)));
+        return copysign(m, da);
+    }
+    return 0.0;
+}
+
+// eq. 1.6, uniform spacing: edge between cells with means a0, a1 and slopes d0, d1
+__device__ __forceinline__ double edge_u(double a0, double a1, double d0, double d1) {
+    const double e = a0 + 0.5 * (a1 - a0);
+    return e - (d1 - d0) * C16;
+}
+
+// eq. 1.7-1.8 with tabulated width weights (w, cr, cl) of the cell
+__device__ __forceinline__ double slope_w(double w, double cr, double cl, double am, double a0,
+                                          double ap) {
+    const double dl = a0 - am, dr = ap - a0;
+    if (dr * dl > 0.0) {
+        const double da = w * (cr * dr + cl * dl);
+        const double m = dmin(fabs(da), dmin(2.0 * fabs(dl), 2.0 * fabs(dr)));
+        return copysign(m, da);
+    }
+    return 0.0;
+}
+
+// eq. 1.6 with tabulated width coefficients (cA, cB, cC, cD, cS) of the edge
+__device__ __forceinline__ double edge_w(const double* c, double a0, double a1, double d0,
+                                         double d1) {
+    const double da = a1 - a0;
+    const double e0 = a0 + c[0] * da;
+    const double br = ((c[1] * da) - (c[2] * d1)) + (c[3] * d0);
+    return e0 + br * c[4];
+}
+
+// eq. 1.10 on a cell with mean a0 and edges aL, aR
+__device__ __forceinline__ void limit(double a0, double& aL, double& aR) {
+    if ((aR - a0) * (a0 - aL) <= 0.0) {
+        aL = a0;
+        aR = a0;
+        return;
+    }
+    const double d = aR - aL;
+    const double a6 = 6.0 * (a0 - 0.5 * (aL + aR));
+    const double dd = d * d;
+    const double da6 = d * a6;
+    if (da6 > dd) aL = 3.0 * a0 - 2.0 * aR;
+    else if (-dd > da6) aR = 3.0 * a0 - 2.0 * aL;
+}
+
+// Uniform face value from v[0..5] = cells f-3 .. f+2 around face f.
+__device__ __forceinline__ double face_u(const double* v, double M) {
+    if (M >= 0.0) {
+        const double dm2 = slope_u(v[0], v[1], v[2]);
+        const double dm1 = slope_u(v[1], v[2], v[3]);
+        const double d0 = slope_u(v[2], v[3], v[4]);
+        double aL = edge_u(v[1], v[2], dm2, dm1);
+        double aR = edge_u(v[2], v[3], dm1, d0);
+        limit(v[2], aL, aR);
+        return aR;
+    }
+    const double dm1 = slope_u(v[1], v[2], v[3]);
+    const double d0 = slope_u(v[2], v[3], v[4]);
+    const double dp1 = slope_u(v[3], v[4], v[5]);
+    double aL = edge_u(v[2], v[3], dm1, d0);
+    double aR = edge_u(v[3], v[4], d0, dp1);
+    limit(v[3], aL, aR);
+    return aL;
+}
+
+__device__ __forceinline__ double div6(double fw, double fe, double fs, double fn, double fb,
+                                       double ft, double gx, double ces, double cen, double rdS) {
+    const double dx = (fe - fw) * gx;
+    const double dy = (cen * fn - ces * fs) * rdS;
+    const double dz = ft - fb;
+    return (dx + dy) + dz;
+}
+
+// Sum over a workgroup of PB lanes in a fixed order; valid in lane 0.
+__device__ __forceinline__ double block_sum(double v, double* red, int tid) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (tid == 0)
+        for (int q = 0; q < PB / 64; ++q) t += red[q];
+    return t;
+}
+
+}  // namespace
+
+// Width coefficients of the vertical reconstruction per column, from the
+// layer widths dp with the stencil clamped to the column: slopes of the
+// cells x = -2..NLEV+1 (w, cr, cl) and edges x = -2..NLEV between cells x and
+// x+1 (cA, cB, cC, cD, cS), each with the oracle's expressions.
+__global__ void __launch_bounds__(PB) rs_ppm_coefs(const double* __restrict__ dp,
+                                                   double* __restrict__ vs,
+                                                   double* __restrict__ ve, int nlon, int nlat,
+                                                   int nlev) {
+    const long long ncol = (long long)nlon * nlat;
+    const long long t = (long long)blockIdx.x * PB + threadIdx.x;
+    const long long nrow = (long long)(nlev + 4);
+    if (t >= nrow * ncol) return;
+    const long long col = t % ncol;
+    const int x = (int)(t / ncol) - 2;
+    auto H = [&](int kk) { return dp[(long long)clampi(kk, 0, nlev - 1) * ncol + col]; };
+    {
+        const double hm = H(x - 1), h0 = H(x), hp = H(x + 1);
+        const double w = h0 / (hm + h0 + hp);
+        const double cr = (2.0 * hm + h0) / (hp + h0);
+        const double cl = (h0 + 2.0 * hp) / (hm + h0);
+        const long long r = (long long)(x + 2) * ncol + col;
+        vs[r] = w;
+        vs[nrow * ncol + r] = cr;
+        vs[2 * nrow * ncol + r] = cl;
+    }
+    if (x <= nlev) {
+        const long long nrowe = (long long)(nlev + 3);
+        const double hm = H(x - 1), h0 = H(x), h1 = H(x + 1), h2 = H(x + 2);
+        const double cA = h0 / (h0 + h1);
+        const double g = (2.0 * h1 * h0) / (h0 + h1);
+        const double t1 = (hm + h0) / (2.0 * h0 + h1);
+        const double t2 = (h2 + h1) / (2.0 * h1 + h0);
+        const double cB = g * (t1 - t2);
+        const double cC = h0 * t1;
+        const double cD = h1 * ((h1 + h2) / (h0 + 2.0 * h1));
+        const double cS = 1.0 / (((hm + h0) + h1) + h2);
+        const long long r = (long long)(x + 2) * ncol + col;
+        ve[r] = cA;
+        ve[nrowe * ncol + r] = cB;
+        ve[2 * nrowe * ncol + r] = cC;
+        ve[3 * nrowe * ncol + r] = cD;
+        ve[4 * nrowe * ncol + r] = cS;
+    }
+}
+
+// One SSPRK43 stage over the transported block.  KIND 0: out = a + h/2 L a;
+// KIND 1 (last): a4 = 2/3 u + 1/3 u3 (u = a, u3 = b), unew = a4 + h/2 L a4,
+// error partial and (positive domain) the count of negative entries.
+template <int KIND>
+__global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
+    __shared__ double red[PB / 64];
+    const RsCtrl* ctl = A.ctrl;
+    if (ctl && ctl->done) return;
+    const int cur = ctl ? ctl->cur : A.cur_static;
+    const double dt = ctl ? ctl->dtc : A.dt_static;
+    const double h2 = dt / 2;
+    const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev, nt = A.nt;
+    const long long nc = A.nc, plane = (long long)nlon * nlat;
+    const double* __restrict__ pa = A.a_is_state ? (cur ? A.S1 : A.S0) : A.a;
+    const double* __restrict__ pb = A.b;
+    double* __restrict__ po = A.out_is_state ? (cur ? A.S0 : A.S1) : A.out;
+    const int tid = threadIdx.x;
+    const long long cell = (long long)blockIdx.x * PB + tid;
+    double sq = 0.0, ng = 0.0;
+    if (cell < nc) {
+        const int i = (int)(cell % nlon);
+        const int j = (int)((cell / nlon) % nlat);
+        const int k = (int)(cell / plane);
+        const long long col = (long long)j * nlon + i;
+        // stage value at a cell of species s (the last stage combines u, u3)
+        auto V = [&](long long g) -> double {
+            if (KIND == 1) {
+                const double a = C23 * pa[g];
+                const double b = C13 * pb[g];
+                return a + b;
+            }
+            return pa[g];
+        };
+        const double* mxr = A.Mx + ((long long)k * nlat + j) * (nlon + 1);
+        const double mw = mxr[i], me = mxr[i + 1];
+        const double ms = A.My[((long long)k * (nlat + 1) + j) * nlon + i];
+        const double mn = A.My[((long long)k * (nlat + 1) + j + 1) * nlon + i];
+        const double mb = A.Mz[(long long)k * plane + col];
+        const double mt = A.Mz[(long long)(k + 1) * plane + col];
+        const double gx = A.gx[j], ces = A.ce[j], cen = A.ce[j + 1], rdS = A.rdS[j];
+        const double divM = div6(mw, me, ms, mn, mb, mt, gx, ces, cen, rdS);
+        const double mval = V(cell);
+        // vertical coefficients of the faces k (bottom) and k+1 (top): slopes
+        // of cells k-2..k+2, edges k-2..k+1
+        const long long nrs = (long long)(nlev + 4) * plane, nre = (long long)(nlev + 3) * plane;
+        double sw[5], sr[5], sl[5], ec[4][5];
+#pragma unroll
+        for (int d = 0; d < 5; ++d) {
+            const long long r = (long long)(k - 2 + d + 2) * plane + col;
+            sw[d] = A.vs[r];
+            sr[d] = A.vs[nrs + r];
+            sl[d] = A.vs[2 * nrs + r];
+        }
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const long long r = (long long)(k - 2 + d + 2) * plane + col;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) ec[d][q] = A.ve[q * nre + r];
+        }
+        for (int s = 0; s < nt; ++s) {
+            const long long sb = (long long)s * nc;
+            const long long g = sb + cell;
+            double f;
+            if (s == 0) {
+                f = -divM;
+            } else {
+                const int sq1 = s - 1;
+                const long long rowb = sb + (long long)k * plane + (long long)j * nlon;
+                double vx[7], vy[7], vz[7];
+#pragma unroll
+                for (int d = 0; d < 7; ++d) {
+                    const int ii = i - 3 + d;
+                    if (A.lon_periodic) vx[d] = V(rowb + wrapi(ii, nlon));
+                    else if (ii < 0) vx[d] = A.qw[((long long)sq1 * nlev + k) * nlat + j];
+                    else if (ii >= nlon) vx[d] = A.qe[((long long)sq1 * nlev + k) * nlat + j];
+                    else vx[d] = V(rowb + ii);
+                }
+#pragma unroll
+                for (int d = 0; d < 7; ++d) {
+                    const int jj = j - 3 + d;
+                    if (jj < 0) vy[d] = A.qs[((long long)sq1 * nlev + k) * nlon + i];
+                    else if (jj >= nlat) vy[d] = A.qn[((long long)sq1 * nlev + k) * nlon + i];
+                    else vy[d] = V(sb + (long long)k * plane + (long long)jj * nlon + i);
+                }
+#pragma unroll
+                for (int d = 0; d < 7; ++d)
+                    vz[d] = V(sb + (long long)clampi(k - 3 + d, 0, nlev - 1) * plane + col);
+                const double qw_ = face_u(vx, mw);
+                const double qe_ = face_u(vx + 1, me);
+                const double qs_ = face_u(vy, ms);
+                const double qn_ = face_u(vy + 1, mn);
+                // vertical faces: physical cell x has d = x - k + 3 in vz, slope
+                // index x - k + 2 in sw/sr/sl, edge index x - k + 2 in ec
+                auto sl_ = [&](int x) {
+                    const int d = x - k + 3, c = x - k + 2;
+                    return slope_w(sw[c], sr[c], sl[c], vz[d - 1], vz[d], vz[d + 1]);
+                };
+                auto fz = [&](int fface, double M) -> double {
+                    if (M >= 0.0) {
+                        const double dm2 = sl_(fface - 2), dm1 = sl_(fface - 1), d0 = sl_(fface);
+                        const int e1 = fface - 2 - k + 2, e2 = fface - 1 - k + 2;
+                        const int c1 = fface - 2 - k + 3;   // vz index of cell fface-2
+                        double aL = edge_w(ec[e1], vz[c1], vz[c1 + 1], dm2, dm1);
+                        double aR = edge_w(ec[e2], vz[c1 + 1], vz[c1 + 2], dm1, d0);
+                        limit(vz[c1 + 1], aL, aR);
+                        return aR;
+                    }
+                    const double dm1 = sl_(fface - 1), d0 = sl_(fface), dp1 = sl_(fface + 1);
+                    const int e1 = fface - 1 - k + 2, e2 = fface - k + 2;
+                    const int c1 = fface - 1 - k + 3;       // vz index of cell fface-1
+                    double aL = edge_w(ec[e1], vz[c1], vz[c1 + 1], dm1, d0);
+                    double aR = edge_w(ec[e2], vz[c1 + 1], vz[c1 + 2], d0, dp1);
+                    limit(vz[c1 + 1], aL, aR);
+                    return aL;
+                };
+                const double qb_ = fz(k, mb);
+                const double qt_ = fz(k + 1, mt);
+                const double divF = div6(mw * qw_, me * qe_, ms * qs_, mn * qn_, mb * qb_, mt * qt_,
+                                         gx, ces, cen, rdS);
+                const double qc = V(g);
+                const double num = qc * divM - divF;
+                f = num / mval;
+            }
+            const double ac = (s == 0) ? mval : V(g);
+            const double du = h2 * f;
+            const double un = ac + du;
+            if (KIND == 1) {
+                const double u = pa[g], u3 = pb[g];
+                const double a = C13 * u;
+                const double b = C23 * u3;
+                double ut = a + b;
+                const double ud = ut - un;
+                ut = 0.5 * ud;
+                const double am = dmax(fabs(u), fabs(un));
+                double den = am * A.rtol;
+                den = den + A.atol;
+                const double at = ut / den;
+                sq += at * at;
+                if (A.posdom && un < 0.0) ng += 1.0;
+                po[g] = A.clamp ? dmax(un, 0.0) : un;
+            } else {
+                po[g] = un;
+            }
+        }
+    }
+    if (KIND == 1) {
+        const double t = block_sum(sq, red, tid);
+        if (tid == 0) A.partial[blockIdx.x] = t;
+        if (A.negpart) {
+            const double n = block_sum(ng, red, tid);
+            if (tid == 0) A.negpart[blockIdx.x] = n;
+        }
+    }
+}
+
+long long rs_ppm_nblocks(long long nc) { return (nc + PB - 1) / PB; }
+
+hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlon, int nlat,
+                               int nlev, hipStream_t st) {
+    const long long n = (long long)(nlev + 4) * nlon * nlat;
+    hipLaunchKernelGGL(rs_ppm_coefs, dim3((unsigned)((n + PB - 1) / PB)), dim3(PB), 0, st, dp, vs,
+                       ve, nlon, nlat, nlev);
+    return hipGetLastError();
+}
+
+hipError_t rs_launch_ppm_stage(int last, const RsPPMArgs* a, hipStream_t st) {
+    const dim3 grid((unsigned)rs_ppm_nblocks(a->nc));
+    if (last) hipLaunchKernelGGL(rs_ppm_stage<1>, grid, dim3(PB), 0, st, *a);
+    else hipLaunchKernelGGL(rs_ppm_stage<0>, grid, dim3(PB), 0, st, *a);
+    return hipGetLastError();
+}

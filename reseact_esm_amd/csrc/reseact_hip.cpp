@@ -60,6 +60,11 @@ hipError_t rs_launch_sum(const double* x, long long n, double* part, hipStream_t
 hipError_t rs_launch_loss_seed(double* lam, const double* u, long long ntot, long long sbase,
                                long long ncol, hipStream_t st);
 }
+// from ppm.hip
+long long rs_ppm_nblocks(long long nc);
+hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlon, int nlat,
+                               int nlev, hipStream_t st);
+hipError_t rs_launch_ppm_stage(int last, const RsPPMArgs* a, hipStream_t st);
 
 // embedded sources of the runtime-compiled chemistry kernels
 #include "jit_sources.inc"
@@ -155,6 +160,11 @@ struct rs_ctx {
     unsigned chem_block = 64;    // chemistry threads per workgroup
     bool chem_kp = false;        // attempt kernel reads the rate constants through a pointer
     DevArray<double> S[2], X, Y, cx, wz, wl, K, jfac, E, lam, LA, LB, LC, gE, partial, negpart, seqlog;
+    // monotone PPM transport (scheme RS_TRANSPORT_PPM)
+    int scheme = RS_TRANSPORT_UPWIND3;
+    int lon_periodic = 1;
+    bool have_ppm_geom = false, have_ppm_flux = false;
+    DevArray<double> pMx, pMy, pMz, pdp, pce, pgx, prdS, pvs, pve, pqw, pqe, pqs, pqn;
     DevArray<double> red;
     DevArray<unsigned long long> head;
     DevArray<long long> stats;
@@ -372,9 +382,11 @@ RsTransArgs trans_args(rs_ctx* c) {
     return a;
 }
 
-bool trans_fused(const rs_ctx* c) { return c->tmode == 1 && c->fplan_ok; }
+bool trans_ppm(const rs_ctx* c) { return c->scheme == RS_TRANSPORT_PPM; }
+bool trans_fused(const rs_ctx* c) { return c->tmode == 1 && c->fplan_ok && !trans_ppm(c); }
 
 long long trans_nblocks(rs_ctx* c) {
+    if (trans_ppm(c)) return rs_ppm_nblocks(c->nc);
     if (trans_fused(c)) return rs_transport_fused_nblocks(c->nlat, c->nt, c->fplan[3]);
     return rs_transport_nblocks(c->nlon, c->nlat, c->nlev, c->nt, c->tk);
 }
@@ -396,8 +408,58 @@ inline int posdom_of(int flags) { return (flags >> 1) & 1; }
 
 // One SSPRK43 attempt: either from the state buffers under a controller (ctrl
 // != null: dt/cur/done from the device), or between explicit buffers.
+// One SSPRK43 attempt of the PPM transport: three stages X, Y, X (u3) and the
+// last stage from u and u3; input and output must differ (neighbours read
+// the input while the output is written).
+void enqueue_ssprk43_ppm(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* uout,
+                         double dt, double atol, double rtol, int flags) {
+    RsPPMArgs a{};
+    a.S0 = c->S[0].p;
+    a.S1 = c->S[1].p;
+    a.ctrl = ctrl;
+    a.partial = c->partial.p;
+    a.posdom = posdom_of(flags);
+    a.negpart = a.posdom ? c->negpart.p : nullptr;
+    a.Mx = c->pMx.p; a.My = c->pMy.p; a.Mz = c->pMz.p;
+    a.ce = c->pce.p; a.gx = c->pgx.p; a.rdS = c->prdS.p;
+    a.vs = c->pvs.p; a.ve = c->pve.p;
+    a.qw = c->pqw.p; a.qe = c->pqe.p; a.qs = c->pqs.p; a.qn = c->pqn.p;
+    a.nc = c->nc;
+    a.dt_static = dt;
+    a.atol = atol;
+    a.rtol = rtol;
+    a.nlon = c->nlon; a.nlat = c->nlat; a.nlev = c->nlev; a.nt = c->nt;
+    a.cur_static = c->cur;
+    a.lon_periodic = c->lon_periodic;
+    const bool st = (uin == nullptr);
+    a.a_is_state = st ? 1 : 0;
+    a.a = uin;
+    a.out_is_state = 0;
+    a.out = c->X.p;
+    a.clamp = 0;
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
+    a.a_is_state = 0;
+    a.a = c->X.p;
+    a.out = c->Y.p;
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
+    a.a = c->Y.p;
+    a.out = c->X.p;
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
+    a.a_is_state = st ? 1 : 0;
+    a.a = uin;
+    a.b = c->X.p;
+    a.out_is_state = st ? 1 : 0;
+    a.out = uout;
+    a.clamp = clamp_nonneg_of(flags);
+    { ProfScope ps(c, P_TLAST); HIPCHK(rs_launch_ppm_stage(1, &a, c->stream)); }
+}
+
 void enqueue_ssprk43(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* uout, double dt,
                      double atol, double rtol, int flags) {
+    if (trans_ppm(c)) {
+        enqueue_ssprk43_ppm(c, ctrl, uin, uout, dt, atol, rtol, flags);
+        return;
+    }
     const int clamp = clamp_nonneg_of(flags);
     RsTransArgs a = trans_args(c);
     a.posdom = posdom_of(flags);
@@ -705,7 +767,14 @@ int check_ready(rs_ctx* c, bool need_mech, bool need_wind) {
     if (c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
     if (!c->have_state) return fail(c, RS_E_STATE, "state not set");
     if (need_mech && !c->have_mech) return fail(c, RS_E_STATE, "mechanism not set");
-    if (need_wind && c->nt > 0 && !c->have_wind) return fail(c, RS_E_STATE, "wind not set");
+    if (need_wind && c->nt > 0) {
+        if (trans_ppm(c)) {
+            if (!c->have_ppm_geom || !c->have_ppm_flux)
+                return fail(c, RS_E_STATE, "PPM geometry or mass fluxes not set");
+        } else if (!c->have_wind) {
+            return fail(c, RS_E_STATE, "wind not set");
+        }
+    }
     return RS_OK;
 }
 
@@ -996,6 +1065,22 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
         HIPCHK(rs_launch_fill(c->lam.p, (long long)N, 0.0, c->stream));
         HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
         c->cur = 0;
+        c->scheme = RS_TRANSPORT_UPWIND3;
+        c->have_ppm_geom = c->have_ppm_flux = false;
+        {
+            const size_t nq = nt > 1 ? (size_t)(nt - 1) : 0;
+            const size_t nwe = nq * nlev * nlat, nsn = nq * nlev * nlon;
+            c->pqw.resize(nwe); c->pqe.resize(nwe); c->pqs.resize(nsn); c->pqn.resize(nsn);
+            if (nwe) {
+                HIPCHK(rs_launch_fill(c->pqw.p, (long long)nwe, 0.0, c->stream));
+                HIPCHK(rs_launch_fill(c->pqe.p, (long long)nwe, 0.0, c->stream));
+            }
+            if (nsn) {
+                HIPCHK(rs_launch_fill(c->pqs.p, (long long)nsn, 0.0, c->stream));
+                HIPCHK(rs_launch_fill(c->pqn.p, (long long)nsn, 0.0, c->stream));
+            }
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
         c->have_state = c->have_wind = c->have_mech = c->have_cand = false;
         c->has_jfac = c->has_E = false;
         c->chem.reset();
@@ -1026,6 +1111,98 @@ int rs_set_transport_kernel(rs_ctx* c, int mode) {
     if (mode != 0 && mode != 1) return fail(c, RS_E_ARG, "mode must be 0 or 1");
     c->tmode = mode;
     return RS_OK;
+}
+
+int rs_set_transport_scheme(rs_ctx* c, int scheme) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (scheme != RS_TRANSPORT_UPWIND3 && scheme != RS_TRANSPORT_PPM)
+        return fail(c, RS_E_ARG, "unknown transport scheme");
+    if (scheme == RS_TRANSPORT_PPM && c->nt < 1)
+        return fail(c, RS_E_ARG, "PPM needs the air mass as transported species 0");
+    c->scheme = scheme;
+    return RS_OK;
+}
+
+int rs_set_ppm_geometry(rs_ctx* c, int lon_periodic, double dlon, const double* coslat_e,
+                        const double* dS_lat, const double* dphi_lat) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!coslat_e || !dS_lat || !dphi_lat) return fail(c, RS_E_ARG, "null geometry array");
+    if (!(dlon > 0.0) || !std::isfinite(dlon)) return fail(c, RS_E_ARG, "dlon must be positive");
+    const int nlat = c->nlat;
+    for (int j = 0; j <= nlat; ++j)
+        if (!std::isfinite(coslat_e[j])) return fail(c, RS_E_NONFINITE, "geometry");
+    for (int j = 0; j < nlat; ++j) {
+        if (!std::isfinite(dphi_lat[j])) return fail(c, RS_E_NONFINITE, "geometry");
+        if (!(dS_lat[j] > 0.0) || !std::isfinite(dS_lat[j])) return fail(c, RS_E_ARG, "dS_lat must be positive");
+    }
+    std::vector<double> gx(nlat), rdS(nlat);
+    for (int j = 0; j < nlat; ++j) {
+        gx[j] = dphi_lat[j] / (dlon * dS_lat[j]);
+        rdS[j] = 1.0 / dS_lat[j];
+    }
+    return guarded(c, [&]() {
+        c->pce.resize(nlat + 1); c->pgx.resize(nlat); c->prdS.resize(nlat);
+        h2d(c->pce.p, coslat_e, nlat + 1, c->stream);
+        h2d(c->pgx.p, gx.data(), nlat, c->stream);
+        h2d(c->prdS.p, rdS.data(), nlat, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->lon_periodic = lon_periodic ? 1 : 0;
+        c->have_ppm_geom = true;
+        return RS_OK;
+    });
+}
+
+int rs_set_mass_fluxes(rs_ctx* c, const double* Mx, const double* My, const double* Mz,
+                       const double* dp) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!Mx || !My || !Mz || !dp) return fail(c, RS_E_ARG, "null flux array");
+    const size_t nlon = c->nlon, nlat = c->nlat, nlev = c->nlev, ncol = c->ncol;
+    const size_t nx = nlev * nlat * (nlon + 1), ny = nlev * (nlat + 1) * nlon, nz = (nlev + 1) * ncol;
+    for (size_t x = 0; x < nx; ++x) if (!std::isfinite(Mx[x])) return fail(c, RS_E_NONFINITE, "mass flux");
+    for (size_t x = 0; x < ny; ++x) if (!std::isfinite(My[x])) return fail(c, RS_E_NONFINITE, "mass flux");
+    for (size_t x = 0; x < nz; ++x) if (!std::isfinite(Mz[x])) return fail(c, RS_E_NONFINITE, "mass flux");
+    for (size_t x = 0; x < (size_t)c->nc; ++x)
+        if (!(dp[x] > 0.0) || !std::isfinite(dp[x])) return fail(c, RS_E_ARG, "dp must be positive");
+    for (size_t x = 0; x < ncol; ++x)
+        if (Mz[x] != 0.0 || Mz[nlev * ncol + x] != 0.0)
+            return fail(c, RS_E_ARG, "Mz must vanish at the surface and the lid");
+    return guarded(c, [&]() {
+        c->pMx.resize(nx); c->pMy.resize(ny); c->pMz.resize(nz); c->pdp.resize(c->nc);
+        c->pvs.resize(3 * (nlev + 4) * ncol); c->pve.resize(5 * (nlev + 3) * ncol);
+        h2d(c->pMx.p, Mx, nx, c->stream);
+        h2d(c->pMy.p, My, ny, c->stream);
+        h2d(c->pMz.p, Mz, nz, c->stream);
+        h2d(c->pdp.p, dp, c->nc, c->stream);
+        HIPCHK(rs_launch_ppm_coefs(c->pdp.p, c->pvs.p, c->pve.p, c->nlon, c->nlat, c->nlev,
+                                   c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->have_ppm_flux = true;
+        return RS_OK;
+    });
+}
+
+int rs_set_inflow(rs_ctx* c, const double* qbc_w, const double* qbc_e, const double* qbc_s,
+                  const double* qbc_n) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    const size_t nq = c->nt > 1 ? (size_t)(c->nt - 1) : 0;
+    const size_t nwe = nq * c->nlev * c->nlat, nsn = nq * c->nlev * c->nlon;
+    const double* src[4] = {qbc_w, qbc_e, qbc_s, qbc_n};
+    const size_t n[4] = {nwe, nwe, nsn, nsn};
+    for (int w = 0; w < 4; ++w)
+        if (src[w])
+            for (size_t x = 0; x < n[w]; ++x)
+                if (!std::isfinite(src[w][x])) return fail(c, RS_E_NONFINITE, "inflow mixing ratio");
+    return guarded(c, [&]() {
+        DevArray<double>* dst[4] = {&c->pqw, &c->pqe, &c->pqs, &c->pqn};
+        for (int w = 0; w < 4; ++w) {
+            dst[w]->resize(n[w]);
+            if (!n[w]) continue;
+            if (src[w]) h2d(dst[w]->p, src[w], n[w], c->stream);
+            else HIPCHK(rs_launch_fill(dst[w]->p, (long long)n[w], 0.0, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return RS_OK;
+    });
 }
 
 int rs_set_lu_ordering(rs_ctx* c, int ordering) {
@@ -1296,6 +1473,7 @@ int rs_ssprk43_vjp(rs_ctx* c, double t, double dt) {
     (void)t;
     int rc = check_ready(c, false, true);
     if (rc) return rc;
+    if (trans_ppm(c)) return fail(c, RS_E_ARG, "no adjoint of the PPM transport");
     return guarded(c, [&]() {
         if (c->nt > 0) enqueue_ssprk43_vjp(c, dt);
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1320,6 +1498,7 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
     int rc = check_ready(c, true, true);
     if (rc) return rc;
     if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
+    if (trans_ppm(c) && c->nt > 0) return fail(c, RS_E_ARG, "no adjoint of the PPM transport");
     if (o->chem_percell) return fail(c, RS_E_ARG, "adjoint needs the global-norm chemistry");
     if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
         return fail(c, RS_E_ARG, "bad loss");
@@ -1661,6 +1840,8 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
         int rc = check_ready(e->m[d], true, true);
         if (rc) return ens_fail(e, rc, "member " + std::to_string(d) + ": " + e->m[d]->err);
         if (loss_species < 0 || loss_species >= e->m[d]->ns) return ens_fail(e, RS_E_ARG, "bad loss species");
+        if (trans_ppm(e->m[d]) && e->m[d]->nt > 0)
+            return ens_fail(e, RS_E_ARG, "member " + std::to_string(d) + ": no adjoint of the PPM transport");
     }
     using clk = std::chrono::steady_clock;
     rs_ensemble_stats ES{};

@@ -94,4 +94,34 @@ typedef struct RsTransArgs {
     int posdom, pad;
 } RsTransArgs;
 
+/* Monotone PPM transport stage (CWC mixing-ratio form; species 0 = air
+ * mass).  Stage input a (or the state buffer), LAST also u (a) and u3 (b). */
+typedef struct RsPPMArgs {
+    double* S0;
+    double* S1;
+    const double* a;
+    const double* b;
+    double* out;
+    const RsCtrl* ctrl;
+    double* partial;
+    double* negpart;
+    const double* Mx;       /* nlev x nlat x (nlon+1) */
+    const double* My;       /* nlev x (nlat+1) x nlon */
+    const double* Mz;       /* (nlev+1) x nlat x nlon */
+    const double* ce;       /* nlat+1: meridional face lengths */
+    const double* gx;       /* nlat: dphi_lat / (dlon dS_lat) */
+    const double* rdS;      /* nlat: 1 / dS_lat */
+    const double* vs;       /* vertical slope coefficients, 3 x (nlev+4) x ncol */
+    const double* ve;       /* vertical edge coefficients, 5 x (nlev+3) x ncol */
+    const double* qw;       /* inflow halos of species 1..nt-1 */
+    const double* qe;
+    const double* qs;
+    const double* qn;
+    long long nc;
+    double dt_static, atol, rtol;
+    int nlon, nlat, nlev, nt;
+    int cur_static, a_is_state, out_is_state, clamp;
+    int lon_periodic, posdom;
+} RsPPMArgs;
+
 #endif

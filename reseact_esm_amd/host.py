@@ -106,6 +106,10 @@ class KernelTime(C.Structure):
 RS_CLAMP_NONNEG = 1
 RS_POSITIVE_DOMAIN = 2
 
+# transport schemes (include/reseact_hip.h)
+RS_TRANSPORT_UPWIND3 = 0
+RS_TRANSPORT_PPM = 1
+
 
 def _flags(clamp) -> int:
     """True/False -> RS_CLAMP_NONNEG or 0; an int passes as the flag set."""
@@ -140,6 +144,10 @@ ABI = {
     "rs_backend": [],
     "rs_set_grid": [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "rs_set_wind": [_P, _D, _D, _D],
+    "rs_set_transport_scheme": [_P, C.c_int],
+    "rs_set_ppm_geometry": [_P, C.c_int, C.c_double, _D, _D, _D],
+    "rs_set_mass_fluxes": [_P, _D, _D, _D, _D],
+    "rs_set_inflow": [_P, _D, _D, _D, _D],
     "rs_set_mechanism": [_P, C.c_int, _I, _I, _D, _I],
     "rs_set_mechanism_stoich": [_P, C.c_int, _I, _I, _I, _D, _D, _I],
     "rs_set_photolysis": [_P, _D],
@@ -330,6 +338,22 @@ class Context:
         a = None if e is None else np.ascontiguousarray(e, np.float64)
         self._chk(self.lib.rs_set_emissions(self.h, _dp(a)))
 
+    def set_transport_scheme(self, scheme: int):
+        self._chk(self.lib.rs_set_transport_scheme(self.h, scheme))
+
+    def set_ppm(self, ppm):
+        """Monotone PPM transport: scheme, geometry, air-mass fluxes, inflow."""
+        f = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
+        self.set_transport_scheme(RS_TRANSPORT_PPM)
+        self._ppm_keep = [f(x) for x in (ppm.coslat_e, ppm.dS_lat, ppm.dphi_lat, ppm.Mx, ppm.My,
+                                         ppm.Mz, ppm.dp, ppm.qbc_w, ppm.qbc_e, ppm.qbc_s,
+                                         ppm.qbc_n)]
+        ce, dS, dphi, Mx, My, Mz, dp, qw, qe, qs, qn = self._ppm_keep
+        self._chk(self.lib.rs_set_ppm_geometry(self.h, 1 if ppm.lon_periodic else 0,
+                                               float(ppm.dlon), _dp(ce), _dp(dS), _dp(dphi)))
+        self._chk(self.lib.rs_set_mass_fluxes(self.h, _dp(Mx), _dp(My), _dp(Mz), _dp(dp)))
+        self._chk(self.lib.rs_set_inflow(self.h, _dp(qw), _dp(qe), _dp(qs), _dp(qn)))
+
     def set_state(self, u):
         u = np.ascontiguousarray(u, np.float64)
         assert u.size == self.ns * self.nc
@@ -345,7 +369,10 @@ class Context:
 
     def load(self, p: Problem, state: bool = True):
         self.set_grid(p.grid, p.ns, p.nt)
-        self.set_wind(p.wind)
+        if p.ppm is not None:
+            self.set_ppm(p.ppm)
+        else:
+            self.set_wind(p.wind)
         if p.mech is not None:
             self.set_mechanism(p.mech)
         self.set_photolysis(p.jfac)

@@ -246,6 +246,7 @@ class Problem:
     dt0_T: float = 15.0      # run_reseact_reactant.jl:134
     dt0_C: float = 0.5       # run_reseact_reactant.jl:135
     extra: dict = dataclasses.field(default_factory=dict)
+    ppm: "PPMSetup | None" = None   # set: monotone PPM transport (species 0 = air mass)
 
     @property
     def nc(self) -> int:
@@ -339,3 +340,149 @@ def toy_problem() -> Problem:
     u0 = 0.5 + rng.random(15)
     return Problem(name="toy-3sp-5cells", grid=g, ns=3, nt=0,
                    wind=solid_body_wind(g, 1.0, 0.0), mech=mech, u0=u0)
+
+
+# ---------------------------------------------------------------------------
+# Monotone PPM transport in CWC mixing-ratio form (the reference's transport,
+# reseact.esm:1536-3100): geometry, air-mass fluxes, inflow halos.
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class PPMSetup:
+    lon_periodic: bool
+    dlon: float                 # rad
+    coslat_e: np.ndarray        # (nlat+1,)
+    dS_lat: np.ndarray          # (nlat,)
+    dphi_lat: np.ndarray        # (nlat,)
+    Mx: np.ndarray              # (nlev, nlat, nlon+1)  Pa/s
+    My: np.ndarray              # (nlev, nlat+1, nlon)
+    Mz: np.ndarray              # (nlev+1, nlat, nlon)
+    dp: np.ndarray              # (nlev, nlat, nlon)    Pa
+    qbc_w: np.ndarray | None = None   # (nq, nlev, nlat)
+    qbc_e: np.ndarray | None = None
+    qbc_s: np.ndarray | None = None   # (nq, nlev, nlon)
+    qbc_n: np.ndarray | None = None
+
+
+# GEOS-FP hybrid coefficients at the lowest 8 level edges (surface first),
+# prototypes/transport_3d/gen_t3d.py:8-9.
+GEOSFP_AP8 = [0.0, 4.804826, 659.3752, 1313.48, 1961.311, 2609.201, 3257.081, 3898.201]
+GEOSFP_BP8 = [1.0, 0.984952, 0.963406, 0.941865, 0.920387, 0.898908, 0.877429, 0.856018]
+
+
+def lat_edges_deg(lat_points_deg: np.ndarray) -> np.ndarray:
+    """Cell edges of POINT-based latitude rows: midpoints between points,
+    half a spacing beyond the end points, clamped to +-90 (the half-polar
+    caps of a global grid), reseact.esm:146-153."""
+    d = float(lat_points_deg[1] - lat_points_deg[0]) if lat_points_deg.size > 1 else 1.0
+    e = np.concatenate([[lat_points_deg[0] - d / 2], 0.5 * (lat_points_deg[1:] + lat_points_deg[:-1]),
+                        [lat_points_deg[-1] + d / 2]])
+    return np.clip(e, -90.0, 90.0)
+
+
+def sphere_metrics(phie_deg: np.ndarray):
+    """coslat_e, dS_lat = sin(phie[j+1]) - sin(phie[j]), dphi_lat (rad);
+    a face at a pole has length exactly 0."""
+    phie = phie_deg * D2R
+    ce = np.cos(phie)
+    ce[np.abs(phie_deg) == 90.0] = 0.0
+    dS = np.sin(phie[1:]) - np.sin(phie[:-1])
+    dphi = phie[1:] - phie[:-1]
+    return ce, dS, dphi
+
+
+def transport3d_regional(nlon: int = 7, nlat: int = 7, nlev: int = 7, nq: int = 1,
+                         qbc_scale: float = 1.0) -> Problem:
+    """The reference's CWC gate case (prototypes/transport_3d/transport_3d.esm,
+    test freestream_q_stays_one_regional_inflow): a CONUS patch of GEOS-FP 4x5
+    cells (west edge -112.5 deg, 5 deg; latitude points from 26 deg, 4 deg),
+    open walls with inflow W and S, outflow E and N; analytic air-mass fluxes
+    Mx = 0.7 cos(lon_face)(1 + 0.3 sin(lat_c))(1 + 0.05 k),
+    My = 0.5 sin(2 lon_c) cos(lat_face)(1 + 0.05 k),
+    Mz = 0.05*4 s(1-s)(1-2s), s = k/NLEV (zero at surface and lid);
+    dp from the GEOS-FP hybrid table at PS = 101325 Pa; m(0) = 2 + 0.5 sin(20 +
+    5 j deg) + 0.1 cos(5 (i + 1/2) deg) and every mixing ratio 1, with the
+    inflow halos at `qbc_scale` (1: the gate; 1.000001: its falsifier)."""
+    g = Grid(nlon, nlat, nlev)
+    lon0, dlon_deg, lat0, dlat_deg = -112.5, 5.0, 26.0, 4.0
+    phie = lat_edges_deg(lat0 + dlat_deg * np.arange(nlat))
+    ce, dS, dphi = sphere_metrics(phie)
+    k = np.arange(nlev)[:, None, None]
+    lon_face = (lon0 + dlon_deg * np.arange(nlon + 1))[None, None, :]
+    lat_c = (0.5 * (phie[:-1] + phie[1:]))[None, :, None]
+    Mx = 0.7 * np.cos(D2R * lon_face) * (1.0 + 0.3 * np.sin(D2R * lat_c)) * (1.0 + 0.05 * k)
+    lon_c = (lon0 + dlon_deg * (np.arange(nlon) + 0.5))[None, None, :]
+    My = 0.5 * np.sin(2.0 * D2R * lon_c) * np.cos(D2R * phie[None, :, None]) * (1.0 + 0.05 * k)
+    sf = np.arange(nlev + 1) / float(nlev)
+    mz = 0.05 * (4.0 * sf * (1.0 - sf) * (1.0 - 2.0 * sf))
+    mz[0] = mz[-1] = 0.0
+    Mz = np.broadcast_to(mz[:, None, None], (nlev + 1, nlat, nlon))
+    ap = np.interp(np.linspace(0, 7, nlev + 1), np.arange(8), GEOSFP_AP8)
+    bp = np.interp(np.linspace(0, 7, nlev + 1), np.arange(8), GEOSFP_BP8)
+    dpk = (ap[:-1] - ap[1:]) + (bp[:-1] - bp[1:]) * 101325.0
+    dp = np.broadcast_to(dpk[:, None, None], (nlev, nlat, nlon))
+    jj = np.arange(nlat)[None, :, None]
+    ii = np.arange(nlon)[None, None, :]
+    m0 = np.broadcast_to(2.0 + 0.5 * np.sin(D2R * (20.0 + jj * 5.0)) + 0.1 * np.cos(D2R * ((ii + 0.5) * 5.0)),
+                         (nlev, nlat, nlon))
+    ns = 1 + nq
+    u0 = np.concatenate([np.ascontiguousarray(m0).reshape(-1), np.ones(nq * g.nc)])
+    halo_we = np.full((nq, nlev, nlat), qbc_scale)
+    halo_sn = np.full((nq, nlev, nlon), qbc_scale)
+    ppm = PPMSetup(False, dlon_deg * D2R, ce, dS, dphi, np.ascontiguousarray(Mx),
+                   np.ascontiguousarray(My), np.ascontiguousarray(Mz), np.ascontiguousarray(dp),
+                   halo_we, halo_we.copy(), halo_sn, halo_sn.copy())
+    return Problem(name=f"transport3d-regional-{nlon}x{nlat}x{nlev}", grid=g, ns=ns, nt=ns,
+                   wind=solid_body_wind(g, 1.0, 0.0), mech=None, u0=u0, ppm=ppm,
+                   dt_macro=0.1, nmacro=1)
+
+
+def continuity_mz(g: Grid, ppm_geom, Mx: np.ndarray, My: np.ndarray, dp: np.ndarray) -> np.ndarray:
+    """Vertical air-mass flux diagnosed from continuity with the column
+    pressure fixer at dPS/dt = 0 (reseact.esm:262-291): divh_fix(k) = divh(k)
+    - divh_col dp(k)/dp_col, Mz[k] = -sum_{k'<k} divh_fix(k'), zero at the
+    surface and (set exactly) at the lid."""
+    dlon, ce, dS, dphi = ppm_geom
+    gx = (dphi / (dlon * dS))[None, :, None]
+    divh = (Mx[:, :, 1:] - Mx[:, :, :-1]) * gx \
+        + (ce[None, 1:, None] * My[:, 1:, :] - ce[None, :-1, None] * My[:, :-1, :]) / dS[None, :, None]
+    col = divh.sum(0)
+    fix = divh - col[None] * dp / dp.sum(0)[None]
+    Mz = np.zeros((g.nlev + 1, g.nlat, g.nlon))
+    Mz[1:] = -np.cumsum(fix, axis=0)
+    Mz[-1] = 0.0
+    return Mz
+
+
+def ppm_global(nlon: int = 360, nlat: int = 180, nlev: int = 72, nq: int = 20,
+               seed: int = 1006) -> Problem:
+    """Global PPM transport: periodic longitude, poles (zero-length faces),
+    hybrid levels from a smooth surface-to-top pressure profile, air-mass
+    fluxes of u = 20 cos(phi) + 5 sin(2 lambda) cos(phi) m/s and
+    v = 5 sin(lambda) cos(phi)^2 m/s, Mz from continuity with the pressure
+    fixer, mixing ratios lognormal.
+    Species 0 is the air mass; NT = NS = 1 + nq."""
+    g = Grid(nlon, nlat, nlev)
+    phie = np.linspace(-90.0, 90.0, nlat + 1)
+    ce, dS, dphi = sphere_metrics(phie)
+    dlon = 2.0 * math.pi / nlon
+    pe = 101325.0 * np.exp(-np.linspace(0.0, 3.0, nlev + 1))       # level edges, Pa
+    dpk = pe[:-1] - pe[1:]
+    dp = np.ascontiguousarray(np.broadcast_to(dpk[:, None, None], (nlev, nlat, nlon)))
+    lam_f = (dlon * np.arange(nlon + 1))[None, None, :]
+    lam_c = (dlon * (np.arange(nlon) + 0.5))[None, None, :]
+    phc = (0.5 * (phie[:-1] + phie[1:]) * D2R)[None, :, None]
+    phf = (phie * D2R)[None, :, None]
+    dpx = dpk[:, None, None]
+    U = (20.0 + 5.0 * np.sin(2.0 * lam_f)) * np.cos(phc)
+    V = 5.0 * np.sin(lam_c) * np.cos(phf) ** 2
+    Mx = np.ascontiguousarray(dpx * U / A_EARTH * np.ones((nlev, nlat, 1)))
+    Mx[:, :, -1] = Mx[:, :, 0]
+    My = np.ascontiguousarray(dpx * V / A_EARTH * np.ones((nlev, 1, nlon)))
+    Mz = continuity_mz(g, (dlon, ce, dS, dphi), Mx, My, dp)
+    rng = np.random.default_rng([seed, 2])
+    q = rng.lognormal(0.0, 0.5, size=nq * g.nc)
+    u0 = np.concatenate([dp.reshape(-1), q])
+    ppm = PPMSetup(True, dlon, ce, dS, dphi, Mx, My, np.ascontiguousarray(Mz), dp)
+    return Problem(name=f"transport-ppm-{nlon}x{nlat}x{nlev}-{nq}tr", grid=g, ns=1 + nq, nt=1 + nq,
+                   wind=solid_body_wind(g, 1.0, 0.0), mech=None, u0=u0, ppm=ppm,
+                   dt_macro=600.0, nmacro=1)
