@@ -41,6 +41,11 @@ from reseact_esm_amd import hip_library, problems as P, split_opts  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# PPM transport: flops per mixing-ratio species-cell and stage, counted from the
+# operator's statements (ppm_roofline); the counter pass in the PPM kernel
+# profile reads 381 FP64 VALU lane operations for the same work (min/max and
+# the final division's Newton steps included there)
+PPM_FLOPS_SPECIES_CELL_STAGE = 330.0
 
 
 def log(msg: str) -> None:
@@ -163,6 +168,58 @@ def transport_roofline(lib, nsteps: int = 60, nt: int = 20):
                           "achieved_GBs": round(72.0 * cells / (s["ms_step"] * 1e-3) / 1e9, 1),
                           "launches_per_step": round(s["launches"] / nsteps, 2)},
             "fused_speedup": round(s["ms_step"] / f["ms_step"], 3)}
+
+
+def ppm_roofline(lib, nsteps: int = 10, nq: int = 20):
+    """The reference's transport: monotone PPM in CWC mixing-ratio form on the
+    global 360x180x72 grid, air mass + 20 mixing ratios, one SSPRK43 attempt
+    per step (four stage launches) from the same state."""
+    p = P.ppm_global(360, 180, 72, nq=nq)
+    nt = p.nt
+    ctx = lib.context().load(p)
+    dt = 60.0
+    ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)          # warm-up
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+    t1 = time.perf_counter()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ctx.close()
+    ns_, ms_s = kernel_ms(prof, "transport_stage")
+    nl_, ms_l = kernel_ms(prof, "transport_last")
+    ms = (ms_s + ms_l) / nsteps
+    g = p.grid
+    # algorithmic bytes per attempt: per species-cell 16 B in stages 1-2
+    # (read, write), 32 B in stage 3 (reads a and u, writes u3 and a4) and 32 B
+    # in the last (reads a4, u, u3, writes unew); per cell and stage the face
+    # fluxes Mx, My, Mz and the vertical width factors (2 per slope row, 3 per
+    # edge row), each read once
+    per_cell_stage = 8.0 * ((g.nlon + 1) / g.nlon + (g.nlat + 1) / g.nlat + (g.nlev + 1) / g.nlev
+                            + 2.0 * (g.nlev + 4) / g.nlev + 3.0 * (g.nlev + 3) / g.nlev)
+    bytes_attempt = p.nc * (96.0 * nt + 4.0 * per_cell_stage)
+    gbs = bytes_attempt / (ms * 1e-3) / 1e9
+    # algorithmic flops per mixing-ratio species-cell and stage, counted from
+    # the operator's statements (min/max and compares not counted): four
+    # uniform faces x 52 (3 slopes, 2 edges, limiter), two width-weighted
+    # vertical faces x 51, the six face fluxes and the divergence 15, the CWC
+    # tendency 3, the stage update 2; the last stage adds 9 for the error
+    # norm; the air mass takes 11 per stage (divergence and update)
+    flops_attempt = p.nc * ((nt - 1) * (4.0 * PPM_FLOPS_SPECIES_CELL_STAGE + 9.0) + 4.0 * 11.0 + 9.0)
+    tf = flops_attempt / (ms * 1e-3) / 1e12
+    log(f"ppm {p.name}: {ms:.3f} ms/attempt (stage {ms_s / max(ns_, 1):.3f} ms, last "
+        f"{ms_l / max(nl_, 1):.3f} ms), {tf:.2f} TFLOP/s, {gbs:.0f} GB/s algorithmic, "
+        f"wall {t1 - t0:.2f} s")
+    return {"workload": p.name, "kernels": ["transport_stage", "transport_last"], "bound": "mfma",
+            "compute": "fp64 valu", "achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 4),
+            "flops_per_attempt": flops_attempt,
+            "hbm": {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_attempt": bytes_attempt},
+            "ms_per_attempt": round(ms, 4),
+            "avg_stage_ms": round(ms_s / max(ns_, 1), 4), "avg_last_ms": round(ms_l / max(nl_, 1), 4),
+            "tracer_cell_attempts_per_s": round(nt * p.nc / (ms * 1e-3), 1)}
 
 
 def chemistry_roofline(lib):
@@ -356,6 +413,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-domain ensemble")
     ap.add_argument("--probe", action="store_true",
                     help="only the short hardware-counter run (copy + one transport and chemistry)")
+    ap.add_argument("--ppm-only", action="store_true",
+                    help="only the PPM transport leg (for profiling it)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure HBM traffic")
     a = ap.parse_args()
@@ -367,11 +426,15 @@ def main() -> int:
     if a.probe:
         print(json.dumps({"probe": probe(lib)}), flush=True)
         return 0
+    if a.ppm_only:
+        print(json.dumps({"ppm_transport": ppm_roofline(lib, nsteps=a.steps)}), flush=True)
+        return 0
     c = coupled(lib, a.steps, a.warmup)
     extra = {}
     if not a.quick:
         extra["transport_hbm"] = transport_roofline(lib)
         extra["chemistry_fp64"] = chemistry_roofline(lib)
+        extra["ppm_transport"] = ppm_roofline(lib)
     if not a.quick and not a.no_ensemble:
         extra["ensemble"] = ensemble_run(lib)
     cpu = None if a.no_cpu else cpu_baseline()

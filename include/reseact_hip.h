@@ -174,6 +174,18 @@ RS_API int rs_set_mass_fluxes(rs_ctx* ctx, const double* Mx, const double* My,
 RS_API int rs_set_inflow(rs_ctx* ctx, const double* qbc_w, const double* qbc_e,
                          const double* qbc_s, const double* qbc_n);
 
+/* Column pressure fixer of the air-mass equation (reseact.esm:5 caveat 4 and
+ * the divh_fix variable, :262-281): with dPSdt[NLAT*NLON] (Pa/s, surface
+ * pressure tendency) the air mass follows
+ *   dm/dt = -div(M) + (dPSdt + divh_col) dp(k) / dp_col,
+ * divh_col = sum_k of the horizontal divergence, dp_col = sum_k dp (sums in
+ * level order), i.e. the continuity equation reads the corrected horizontal
+ * divergence while the mixing ratios keep the raw div(M).  With Mz diagnosed
+ * from the same corrected divergence, dm/dt = dB dPSdt up to round-off.
+ * NULL switches it off.  Recomputed whenever the fluxes or the geometry are
+ * set again; call after both. */
+RS_API int rs_set_pressure_fixer(rs_ctx* ctx, const double* dPSdt);
+
 /* Mass-action mechanism, NR reactions: reactants[2r], reactants[2r+1]
  * (second = -1 for unimolecular; equal entries = 2A), products[2r],
  * products[2r+1] (-1 = none), rate constants k[r] (1/s or 1/(conc s)),

@@ -71,6 +71,7 @@ struct rs_ctx {
     std::vector<double> coslat_e, dS_lat, dphi_lat;   // nlat+1, nlat, nlat
     std::vector<double> Mx, My, Mz, dp;               // faces / cells, lon fastest
     std::vector<double> qw, qe, qs, qn;               // inflow halos, (nt-1) species
+    std::vector<double> dPSdt, msrc;                  // pressure fixer (ncol), air-mass source (nc)
     bool have_ppm_geom = false, have_ppm_flux = false;
     // mechanism
     int nr = 0;
@@ -276,36 +277,40 @@ inline double ppm_edge(double a0, double a1, double d0, double d1) {
     return e - (d1 - d0) * C16;
 }
 
-// Width-weighted slope (eq. 1.7) of cell 0 with neighbour widths hm, hp.
+// Width-weighted slope (eq. 1.7) of cell 0 with neighbour widths hm, hp:
+// da = h0/(hm+h0+hp) [(2hm+h0)/(hp+h0) dr + (h0+2hp)/(hm+h0) dl], written with
+// the two width-only factors cR, cL (a kernel may tabulate them per cell).
 inline double ppm_slope_nu(double hm, double h0, double hp, double am, double a0, double ap) {
     const double dl = a0 - am, dr = ap - a0;
     if (dr * dl > 0.0) {
         const double w = h0 / (hm + h0 + hp);
-        const double cr = (2.0 * hm + h0) / (hp + h0);
-        const double cl = (h0 + 2.0 * hp) / (hm + h0);
-        const double da = w * (cr * dr + cl * dl);
+        const double cR = w * ((2.0 * hm + h0) / (hp + h0));
+        const double cL = w * ((h0 + 2.0 * hp) / (hm + h0));
+        const double da = cR * dr + cL * dl;
         const double m = std::min(std::fabs(da), std::min(2.0 * std::fabs(dl), 2.0 * std::fabs(dr)));
         return std::copysign(m, da);
     }
     return 0.0;
 }
 
-// Width-weighted edge between cells 0 and 1 (eq. 1.6); hm, h2 the outer widths.
+// Width-weighted edge between cells 0 and 1 (eq. 1.6); hm, h2 the outer widths:
+// a_{1/2} = a0 + h0/(h0+h1) da + S^-1 { 2 h1 h0/(h0+h1) [t1 - t2] da
+//           - h0 t1 d1 + h1 (h1+h2)/(h0+2h1) d0 },  S = hm+h0+h1+h2,
+// t1 = (hm+h0)/(2h0+h1), t2 = (h2+h1)/(2h1+h0); written with the three
+// width-only factors cE, cF, cG (a kernel may tabulate them per edge).
 inline double ppm_edge_nu(double hm, double h0, double h1, double h2, double a0, double a1,
                           double d0, double d1) {
-    // the width-only coefficients first (a kernel may tabulate them per edge)
     const double cA = h0 / (h0 + h1);
     const double g = (2.0 * h1 * h0) / (h0 + h1);
     const double t1 = (hm + h0) / (2.0 * h0 + h1);
     const double t2 = (h2 + h1) / (2.0 * h1 + h0);
-    const double cB = g * (t1 - t2);
-    const double cC = h0 * t1;
-    const double cD = h1 * ((h1 + h2) / (h0 + 2.0 * h1));
     const double cS = 1.0 / (((hm + h0) + h1) + h2);
+    const double cE = cA + (g * (t1 - t2)) * cS;
+    const double cF = (h0 * t1) * cS;
+    const double cG = (h1 * ((h1 + h2) / (h0 + 2.0 * h1))) * cS;
     const double da = a1 - a0;
-    const double e0 = a0 + cA * da;
-    const double br = ((cB * da) - (cC * d1)) + (cD * d0);
-    return e0 + br * cS;
+    const double e0 = a0 + cE * da;
+    return (e0 - cF * d1) + cG * d0;
 }
 
 // Parabola limiter (eq. 1.10) of a cell with mean a0 and edges aL, aR.
@@ -394,7 +399,7 @@ void ppm_rhs_all(const rs_ctx& c, const double* u, double* f) {
                 const double mb = c.Mz[(size_t)k * plane + (size_t)j * nlon + i];
                 const double mt = c.Mz[(size_t)(k + 1) * plane + (size_t)j * nlon + i];
                 const double divM = ppm_div(mw, me, ms, mn, mb, mt, gx, ces, cen, rdS);
-                f[cell] = -divM;
+                f[cell] = c.msrc.empty() ? -divM : -divM + c.msrc[cell];
                 for (int s = 1; s < nt; ++s) {
                     const double* q = u + (size_t)s * nc;
                     const int sq = s - 1;
@@ -1023,6 +1028,8 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
     c->have_state = c->have_mech = c->have_wind = false;
     c->scheme = RS_TRANSPORT_UPWIND3;
     c->have_ppm_geom = c->have_ppm_flux = false;
+    c->dPSdt.clear();
+    c->msrc.clear();
     const size_t nq = nt > 1 ? (size_t)(nt - 1) : 0;
     c->qw.assign(nq * nlev * nlat, 0.0); c->qe = c->qw;
     c->qs.assign(nq * nlev * nlon, 0.0); c->qn = c->qs;
@@ -1041,6 +1048,8 @@ int rs_set_wind(rs_ctx* c, const double* cx, const double* wz, const double* wl)
     c->have_wind = true;
     return RS_OK;
 }
+
+static void ppm_update_source(rs_ctx* c);
 
 int rs_set_transport_scheme(rs_ctx* c, int scheme) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
@@ -1070,6 +1079,7 @@ int rs_set_ppm_geometry(rs_ctx* c, int lon_periodic, double dlon, const double* 
     if (!all_finite(c->coslat_e) || !all_finite(c->dphi_lat)) return fail(c, RS_E_NONFINITE, "geometry");
     for (double x : c->dS_lat) if (!(x > 0.0) || !std::isfinite(x)) return fail(c, RS_E_ARG, "dS_lat must be positive");
     c->have_ppm_geom = true;
+    ppm_update_source(c);
     return RS_OK;
 }
 
@@ -1089,6 +1099,52 @@ int rs_set_mass_fluxes(rs_ctx* c, const double* Mx, const double* My, const doub
         if (c->Mz[x] != 0.0 || c->Mz[nlev * c->ncol + x] != 0.0)
             return fail(c, RS_E_ARG, "Mz must vanish at the surface and the lid");
     c->have_ppm_flux = true;
+    ppm_update_source(c);
+    return RS_OK;
+}
+
+// Air-mass source of the column pressure fixer: (dPSdt + divh_col) dp/dp_col,
+// divh_col and dp_col summed over levels in order.
+static void ppm_update_source(rs_ctx* c) {
+    c->msrc.clear();
+    if (c->dPSdt.empty() || !c->have_ppm_geom || !c->have_ppm_flux) return;
+    const int nlon = c->nlon, nlat = c->nlat, nlev = c->nlev;
+    const size_t plane = (size_t)nlon * nlat;
+    c->msrc.assign(c->nc, 0.0);
+    for (int j = 0; j < nlat; ++j) {
+        const double gx = c->dphi_lat[j] / (c->dlon * c->dS_lat[j]);
+        const double rdS = 1.0 / c->dS_lat[j];
+        const double ces = c->coslat_e[j], cen = c->coslat_e[j + 1];
+        for (int i = 0; i < nlon; ++i) {
+            const size_t col = (size_t)j * nlon + i;
+            double dcol = 0.0, pcol = 0.0;
+            for (int k = 0; k < nlev; ++k) {
+                const double* mxr = c->Mx.data() + ((size_t)k * nlat + j) * (nlon + 1);
+                const double ms = c->My[((size_t)k * (nlat + 1) + j) * nlon + i];
+                const double mn = c->My[((size_t)k * (nlat + 1) + j + 1) * nlon + i];
+                const double dx = (mxr[i + 1] - mxr[i]) * gx;
+                const double dy = (cen * mn - ces * ms) * rdS;
+                dcol += dx + dy;
+                pcol += c->dp[(size_t)k * plane + col];
+            }
+            const double a = (c->dPSdt[col] + dcol) / pcol;
+            for (int k = 0; k < nlev; ++k) c->msrc[(size_t)k * plane + col] = a * c->dp[(size_t)k * plane + col];
+        }
+    }
+}
+
+int rs_set_pressure_fixer(rs_ctx* c, const double* dPSdt) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!dPSdt) {
+        c->dPSdt.clear();
+        c->msrc.clear();
+        return RS_OK;
+    }
+    if (!c->have_ppm_geom || !c->have_ppm_flux)
+        return fail(c, RS_E_STATE, "set the PPM geometry and mass fluxes first");
+    c->dPSdt.assign(dPSdt, dPSdt + c->ncol);
+    if (!all_finite(c->dPSdt)) return fail(c, RS_E_NONFINITE, "dPSdt");
+    ppm_update_source(c);
     return RS_OK;
 }
 

@@ -57,25 +57,23 @@ __device__ __forceinline__ double edge_u(double a0, double a1, double d0, double
     return e - (d1 - d0) * C16;
 }
 
-// eq. 1.7-1.8 with tabulated width weights (w, cr, cl) of the cell
-__device__ __forceinline__ double slope_w(double w, double cr, double cl, double am, double a0,
-                                          double ap) {
+// eq. 1.7-1.8 with the tabulated width factors (cR, cL) of the cell
+__device__ __forceinline__ double slope_w(double cR, double cL, double am, double a0, double ap) {
     const double dl = a0 - am, dr = ap - a0;
     if (dr * dl > 0.0) {
-        const double da = w * (cr * dr + cl * dl);
+        const double da = cR * dr + cL * dl;
         const double m = dmin(fabs(da), dmin(2.0 * fabs(dl), 2.0 * fabs(dr)));
         return copysign(m, da);
     }
     return 0.0;
 }
 
-// eq. 1.6 with tabulated width coefficients (cA, cB, cC, cD, cS) of the edge
+// eq. 1.6 with the tabulated width factors (cE, cF, cG) of the edge
 __device__ __forceinline__ double edge_w(const double* c, double a0, double a1, double d0,
                                          double d1) {
     const double da = a1 - a0;
     const double e0 = a0 + c[0] * da;
-    const double br = ((c[1] * da) - (c[2] * d1)) + (c[3] * d0);
-    return e0 + br * c[4];
+    return (e0 - c[1] * d1) + c[2] * d0;
 }
 
 // eq. 1.10 on a cell with mean a0 and edges aL, aR
@@ -135,10 +133,10 @@ __device__ __forceinline__ double block_sum(double v, double* red, int tid) {
 
 }  // namespace
 
-// Width coefficients of the vertical reconstruction per column, from the
-// layer widths dp with the stencil clamped to the column: slopes of the
-// cells x = -2..NLEV+1 (w, cr, cl) and edges x = -2..NLEV between cells x and
-// x+1 (cA, cB, cC, cD, cS), each with the oracle's expressions.
+// Width factors of the vertical reconstruction per column, from the layer
+// widths dp with the stencil clamped to the column: slopes of the cells
+// x = -2..NLEV+1 (cR, cL) and edges x = -2..NLEV between cells x and x+1
+// (cE, cF, cG), each with the oracle's expressions.
 __global__ void __launch_bounds__(PB) rs_ppm_coefs(const double* __restrict__ dp,
                                                    double* __restrict__ vs,
                                                    double* __restrict__ ve, int nlon, int nlat,
@@ -153,12 +151,11 @@ __global__ void __launch_bounds__(PB) rs_ppm_coefs(const double* __restrict__ dp
     {
         const double hm = H(x - 1), h0 = H(x), hp = H(x + 1);
         const double w = h0 / (hm + h0 + hp);
-        const double cr = (2.0 * hm + h0) / (hp + h0);
-        const double cl = (h0 + 2.0 * hp) / (hm + h0);
+        const double cR = w * ((2.0 * hm + h0) / (hp + h0));
+        const double cL = w * ((h0 + 2.0 * hp) / (hm + h0));
         const long long r = (long long)(x + 2) * ncol + col;
-        vs[r] = w;
-        vs[nrow * ncol + r] = cr;
-        vs[2 * nrow * ncol + r] = cl;
+        vs[r] = cR;
+        vs[nrow * ncol + r] = cL;
     }
     if (x <= nlev) {
         const long long nrowe = (long long)(nlev + 3);
@@ -167,22 +164,21 @@ __global__ void __launch_bounds__(PB) rs_ppm_coefs(const double* __restrict__ dp
         const double g = (2.0 * h1 * h0) / (h0 + h1);
         const double t1 = (hm + h0) / (2.0 * h0 + h1);
         const double t2 = (h2 + h1) / (2.0 * h1 + h0);
-        const double cB = g * (t1 - t2);
-        const double cC = h0 * t1;
-        const double cD = h1 * ((h1 + h2) / (h0 + 2.0 * h1));
         const double cS = 1.0 / (((hm + h0) + h1) + h2);
+        const double cE = cA + (g * (t1 - t2)) * cS;
+        const double cF = (h0 * t1) * cS;
+        const double cG = (h1 * ((h1 + h2) / (h0 + 2.0 * h1))) * cS;
         const long long r = (long long)(x + 2) * ncol + col;
-        ve[r] = cA;
-        ve[nrowe * ncol + r] = cB;
-        ve[2 * nrowe * ncol + r] = cC;
-        ve[3 * nrowe * ncol + r] = cD;
-        ve[4 * nrowe * ncol + r] = cS;
+        ve[r] = cE;
+        ve[nrowe * ncol + r] = cF;
+        ve[2 * nrowe * ncol + r] = cG;
     }
 }
 
 // One SSPRK43 stage over the transported block.  KIND 0: out = a + h/2 L a;
-// KIND 1 (last): a4 = 2/3 u + 1/3 u3 (u = a, u3 = b), unew = a4 + h/2 L a4,
-// error partial and (positive domain) the count of negative entries.
+// KIND 2 (third stage): the same, and out2 = a4 = 2/3 u + 1/3 out (u = b);
+// KIND 1 (last): unew = a4 + h/2 L a4 (a4 = c4), the error partial from
+// u = a and u3 = b, and (positive domain) the count of negative entries.
 template <int KIND>
 __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
     __shared__ double red[PB / 64];
@@ -194,7 +190,8 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
     const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev, nt = A.nt;
     const long long nc = A.nc, plane = (long long)nlon * nlat;
     const double* __restrict__ pa = A.a_is_state ? (cur ? A.S1 : A.S0) : A.a;
-    const double* __restrict__ pb = A.b;
+    const double* __restrict__ pb = A.b_is_state ? (cur ? A.S1 : A.S0) : A.b;
+    const double* __restrict__ ps = (KIND == 1) ? A.c4 : pa;   // stencil source
     double* __restrict__ po = A.out_is_state ? (cur ? A.S0 : A.S1) : A.out;
     const int tid = threadIdx.x;
     const long long cell = (long long)blockIdx.x * PB + tid;
@@ -204,15 +201,8 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
         const int j = (int)((cell / nlon) % nlat);
         const int k = (int)(cell / plane);
         const long long col = (long long)j * nlon + i;
-        // stage value at a cell of species s (the last stage combines u, u3)
-        auto V = [&](long long g) -> double {
-            if (KIND == 1) {
-                const double a = C23 * pa[g];
-                const double b = C13 * pb[g];
-                return a + b;
-            }
-            return pa[g];
-        };
+        // stage value at a cell (the last stage: a4 from the third stage)
+        auto V = [&](long long g) -> double { return ps[g]; };
         const double* mxr = A.Mx + ((long long)k * nlat + j) * (nlon + 1);
         const double mw = mxr[i], me = mxr[i + 1];
         const double ms = A.My[((long long)k * (nlat + 1) + j) * nlon + i];
@@ -222,77 +212,78 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
         const double gx = A.gx[j], ces = A.ce[j], cen = A.ce[j + 1], rdS = A.rdS[j];
         const double divM = div6(mw, me, ms, mn, mb, mt, gx, ces, cen, rdS);
         const double mval = V(cell);
-        // vertical coefficients of the faces k (bottom) and k+1 (top): slopes
+        // vertical width factors of the faces k (bottom) and k+1 (top): slopes
         // of cells k-2..k+2, edges k-2..k+1
         const long long nrs = (long long)(nlev + 4) * plane, nre = (long long)(nlev + 3) * plane;
-        double sw[5], sr[5], sl[5], ec[4][5];
+        double sR[5], sL[5], ec[4][3];
 #pragma unroll
         for (int d = 0; d < 5; ++d) {
-            const long long r = (long long)(k - 2 + d + 2) * plane + col;
-            sw[d] = A.vs[r];
-            sr[d] = A.vs[nrs + r];
-            sl[d] = A.vs[2 * nrs + r];
+            const long long r = (long long)(k + d) * plane + col;      // row x + 2, x = k - 2 + d
+            sR[d] = A.vs[r];
+            sL[d] = A.vs[nrs + r];
         }
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-            const long long r = (long long)(k - 2 + d + 2) * plane + col;
+            const long long r = (long long)(k + d) * plane + col;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) ec[d][q] = A.ve[q * nre + r];
+            for (int q = 0; q < 3; ++q) ec[d][q] = A.ve[q * nre + r];
         }
         for (int s = 0; s < nt; ++s) {
             const long long sb = (long long)s * nc;
             const long long g = sb + cell;
             double f;
             if (s == 0) {
-                f = -divM;
+                f = A.msrc ? -divM + A.msrc[cell] : -divM;
             } else {
                 const int sq1 = s - 1;
                 const long long rowb = sb + (long long)k * plane + (long long)j * nlon;
-                double vx[7], vy[7], vz[7];
+                double v[7];
+                // longitude: periodic wrap or the w/e inflow halo
 #pragma unroll
                 for (int d = 0; d < 7; ++d) {
                     const int ii = i - 3 + d;
-                    if (A.lon_periodic) vx[d] = V(rowb + wrapi(ii, nlon));
-                    else if (ii < 0) vx[d] = A.qw[((long long)sq1 * nlev + k) * nlat + j];
-                    else if (ii >= nlon) vx[d] = A.qe[((long long)sq1 * nlev + k) * nlat + j];
-                    else vx[d] = V(rowb + ii);
+                    if (A.lon_periodic) v[d] = V(rowb + wrapi(ii, nlon));
+                    else if (ii < 0) v[d] = A.qw[((long long)sq1 * nlev + k) * nlat + j];
+                    else if (ii >= nlon) v[d] = A.qe[((long long)sq1 * nlev + k) * nlat + j];
+                    else v[d] = V(rowb + ii);
                 }
+                const double qw_ = face_u(v, mw);
+                const double qe_ = face_u(v + 1, me);
+                // latitude: the s/n inflow halo
 #pragma unroll
                 for (int d = 0; d < 7; ++d) {
                     const int jj = j - 3 + d;
-                    if (jj < 0) vy[d] = A.qs[((long long)sq1 * nlev + k) * nlon + i];
-                    else if (jj >= nlat) vy[d] = A.qn[((long long)sq1 * nlev + k) * nlon + i];
-                    else vy[d] = V(sb + (long long)k * plane + (long long)jj * nlon + i);
+                    if (jj < 0) v[d] = A.qs[((long long)sq1 * nlev + k) * nlon + i];
+                    else if (jj >= nlat) v[d] = A.qn[((long long)sq1 * nlev + k) * nlon + i];
+                    else v[d] = V(sb + (long long)k * plane + (long long)jj * nlon + i);
                 }
+                const double qs_ = face_u(v, ms);
+                const double qn_ = face_u(v + 1, mn);
+                // vertical: stencil clamped to the column; physical cell x is
+                // v[x - k + 3], its slope factors index x - k + 2, edge x index x - k + 2
 #pragma unroll
                 for (int d = 0; d < 7; ++d)
-                    vz[d] = V(sb + (long long)clampi(k - 3 + d, 0, nlev - 1) * plane + col);
-                const double qw_ = face_u(vx, mw);
-                const double qe_ = face_u(vx + 1, me);
-                const double qs_ = face_u(vy, ms);
-                const double qn_ = face_u(vy + 1, mn);
-                // vertical faces: physical cell x has d = x - k + 3 in vz, slope
-                // index x - k + 2 in sw/sr/sl, edge index x - k + 2 in ec
+                    v[d] = V(sb + (long long)clampi(k - 3 + d, 0, nlev - 1) * plane + col);
                 auto sl_ = [&](int x) {
                     const int d = x - k + 3, c = x - k + 2;
-                    return slope_w(sw[c], sr[c], sl[c], vz[d - 1], vz[d], vz[d + 1]);
+                    return slope_w(sR[c], sL[c], v[d - 1], v[d], v[d + 1]);
                 };
                 auto fz = [&](int fface, double M) -> double {
                     if (M >= 0.0) {
                         const double dm2 = sl_(fface - 2), dm1 = sl_(fface - 1), d0 = sl_(fface);
                         const int e1 = fface - 2 - k + 2, e2 = fface - 1 - k + 2;
-                        const int c1 = fface - 2 - k + 3;   // vz index of cell fface-2
-                        double aL = edge_w(ec[e1], vz[c1], vz[c1 + 1], dm2, dm1);
-                        double aR = edge_w(ec[e2], vz[c1 + 1], vz[c1 + 2], dm1, d0);
-                        limit(vz[c1 + 1], aL, aR);
+                        const int c1 = fface - 2 - k + 3;   // v index of cell fface-2
+                        double aL = edge_w(ec[e1], v[c1], v[c1 + 1], dm2, dm1);
+                        double aR = edge_w(ec[e2], v[c1 + 1], v[c1 + 2], dm1, d0);
+                        limit(v[c1 + 1], aL, aR);
                         return aR;
                     }
                     const double dm1 = sl_(fface - 1), d0 = sl_(fface), dp1 = sl_(fface + 1);
                     const int e1 = fface - 1 - k + 2, e2 = fface - k + 2;
-                    const int c1 = fface - 1 - k + 3;       // vz index of cell fface-1
-                    double aL = edge_w(ec[e1], vz[c1], vz[c1 + 1], dm1, d0);
-                    double aR = edge_w(ec[e2], vz[c1 + 1], vz[c1 + 2], d0, dp1);
-                    limit(vz[c1 + 1], aL, aR);
+                    const int c1 = fface - 1 - k + 3;       // v index of cell fface-1
+                    double aL = edge_w(ec[e1], v[c1], v[c1 + 1], dm1, d0);
+                    double aR = edge_w(ec[e2], v[c1 + 1], v[c1 + 2], d0, dp1);
+                    limit(v[c1 + 1], aL, aR);
                     return aL;
                 };
                 const double qb_ = fz(k, mb);
@@ -322,6 +313,11 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
                 po[g] = A.clamp ? dmax(un, 0.0) : un;
             } else {
                 po[g] = un;
+                if (KIND == 2) {
+                    const double a = C23 * pb[g];
+                    const double b = C13 * un;
+                    A.out2[g] = a + b;
+                }
             }
         }
     }
@@ -335,7 +331,47 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
     }
 }
 
+// Air-mass source of the column pressure fixer, one lane per column:
+// (dPSdt + divh_col) dp / dp_col with divh_col and dp_col summed in level order.
+__global__ void __launch_bounds__(PB) rs_ppm_source(const double* __restrict__ Mx,
+                                                    const double* __restrict__ My,
+                                                    const double* __restrict__ dp,
+                                                    const double* __restrict__ gxa,
+                                                    const double* __restrict__ ce,
+                                                    const double* __restrict__ rdSa,
+                                                    const double* __restrict__ dPSdt,
+                                                    double* __restrict__ msrc, int nlon, int nlat,
+                                                    int nlev) {
+    const long long plane = (long long)nlon * nlat;
+    const long long col = (long long)blockIdx.x * PB + threadIdx.x;
+    if (col >= plane) return;
+    const int i = (int)(col % nlon), j = (int)(col / nlon);
+    const double gx = gxa[j], rdS = rdSa[j], ces = ce[j], cen = ce[j + 1];
+    double dcol = 0.0, pcol = 0.0;
+    for (int k = 0; k < nlev; ++k) {
+        const double* mxr = Mx + ((long long)k * nlat + j) * (nlon + 1);
+        const double ms = My[((long long)k * (nlat + 1) + j) * nlon + i];
+        const double mn = My[((long long)k * (nlat + 1) + j + 1) * nlon + i];
+        const double dx = (mxr[i + 1] - mxr[i]) * gx;
+        const double dy = (cen * mn - ces * ms) * rdS;
+        dcol += dx + dy;
+        pcol += dp[(long long)k * plane + col];
+    }
+    const double a = (dPSdt[col] + dcol) / pcol;
+    for (int k = 0; k < nlev; ++k) msrc[(long long)k * plane + col] = a * dp[(long long)k * plane + col];
+}
+
 long long rs_ppm_nblocks(long long nc) { return (nc + PB - 1) / PB; }
+
+hipError_t rs_launch_ppm_source(const double* Mx, const double* My, const double* dp,
+                                const double* gx, const double* ce, const double* rdS,
+                                const double* dPSdt, double* msrc, int nlon, int nlat, int nlev,
+                                hipStream_t st) {
+    const long long n = (long long)nlon * nlat;
+    hipLaunchKernelGGL(rs_ppm_source, dim3((unsigned)((n + PB - 1) / PB)), dim3(PB), 0, st, Mx, My,
+                       dp, gx, ce, rdS, dPSdt, msrc, nlon, nlat, nlev);
+    return hipGetLastError();
+}
 
 hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlon, int nlat,
                                int nlev, hipStream_t st) {
@@ -345,9 +381,11 @@ hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlo
     return hipGetLastError();
 }
 
-hipError_t rs_launch_ppm_stage(int last, const RsPPMArgs* a, hipStream_t st) {
+// kind 0: first and second stages; 2: third stage (also a4); 1: last stage
+hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st) {
     const dim3 grid((unsigned)rs_ppm_nblocks(a->nc));
-    if (last) hipLaunchKernelGGL(rs_ppm_stage<1>, grid, dim3(PB), 0, st, *a);
+    if (kind == 1) hipLaunchKernelGGL(rs_ppm_stage<1>, grid, dim3(PB), 0, st, *a);
+    else if (kind == 2) hipLaunchKernelGGL(rs_ppm_stage<2>, grid, dim3(PB), 0, st, *a);
     else hipLaunchKernelGGL(rs_ppm_stage<0>, grid, dim3(PB), 0, st, *a);
     return hipGetLastError();
 }

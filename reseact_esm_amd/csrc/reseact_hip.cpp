@@ -64,7 +64,11 @@ hipError_t rs_launch_loss_seed(double* lam, const double* u, long long ntot, lon
 long long rs_ppm_nblocks(long long nc);
 hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlon, int nlat,
                                int nlev, hipStream_t st);
-hipError_t rs_launch_ppm_stage(int last, const RsPPMArgs* a, hipStream_t st);
+hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st);
+hipError_t rs_launch_ppm_source(const double* Mx, const double* My, const double* dp,
+                                const double* gx, const double* ce, const double* rdS,
+                                const double* dPSdt, double* msrc, int nlon, int nlat, int nlev,
+                                hipStream_t st);
 
 // embedded sources of the runtime-compiled chemistry kernels
 #include "jit_sources.inc"
@@ -164,7 +168,9 @@ struct rs_ctx {
     int scheme = RS_TRANSPORT_UPWIND3;
     int lon_periodic = 1;
     bool have_ppm_geom = false, have_ppm_flux = false;
-    DevArray<double> pMx, pMy, pMz, pdp, pce, pgx, prdS, pvs, pve, pqw, pqe, pqs, pqn;
+    DevArray<double> pMx, pMy, pMz, pdp, pce, pgx, prdS, pvs, pve, pqw, pqe, pqs, pqn, pZ;
+    DevArray<double> pdPS, pmsrc;   // pressure fixer: dPS/dt per column, air-mass source per cell
+    bool have_fixer = false;
     DevArray<double> red;
     DevArray<unsigned long long> head;
     DevArray<long long> stats;
@@ -424,6 +430,7 @@ void enqueue_ssprk43_ppm(rs_ctx* c, const RsCtrl* ctrl, const double* uin, doubl
     a.ce = c->pce.p; a.gx = c->pgx.p; a.rdS = c->prdS.p;
     a.vs = c->pvs.p; a.ve = c->pve.p;
     a.qw = c->pqw.p; a.qe = c->pqe.p; a.qs = c->pqs.p; a.qn = c->pqn.p;
+    a.msrc = c->pmsrc.n ? c->pmsrc.p : nullptr;
     a.nc = c->nc;
     a.dt_static = dt;
     a.atol = atol;
@@ -442,12 +449,20 @@ void enqueue_ssprk43_ppm(rs_ctx* c, const RsCtrl* ctrl, const double* uin, doubl
     a.a = c->X.p;
     a.out = c->Y.p;
     { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
+    // third stage: u3 into X and a4 = 2/3 u + 1/3 u3 into Z
     a.a = c->Y.p;
     a.out = c->X.p;
-    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
+    a.b_is_state = st ? 1 : 0;
+    a.b = uin;
+    a.out2 = c->pZ.p;
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(2, &a, c->stream)); }
+    // last stage: stencils on a4, error from u and u3
     a.a_is_state = st ? 1 : 0;
     a.a = uin;
+    a.b_is_state = 0;
     a.b = c->X.p;
+    a.c4 = c->pZ.p;
+    a.out2 = nullptr;
     a.out_is_state = st ? 1 : 0;
     a.out = uout;
     a.clamp = clamp_nonneg_of(flags);
@@ -1066,7 +1081,10 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
         HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
         c->cur = 0;
         c->scheme = RS_TRANSPORT_UPWIND3;
+        c->pZ.resize(0);
         c->have_ppm_geom = c->have_ppm_flux = false;
+        c->have_fixer = false;
+        c->pmsrc.resize(0);
         {
             const size_t nq = nt > 1 ? (size_t)(nt - 1) : 0;
             const size_t nwe = nq * nlev * nlat, nsn = nq * nlev * nlon;
@@ -1119,8 +1137,45 @@ int rs_set_transport_scheme(rs_ctx* c, int scheme) {
         return fail(c, RS_E_ARG, "unknown transport scheme");
     if (scheme == RS_TRANSPORT_PPM && c->nt < 1)
         return fail(c, RS_E_ARG, "PPM needs the air mass as transported species 0");
-    c->scheme = scheme;
-    return RS_OK;
+    return guarded(c, [&]() {
+        // the PPM attempt keeps a4 of the third stage in a buffer of its own
+        if (scheme == RS_TRANSPORT_PPM) c->pZ.resize((size_t)c->nt * (size_t)c->nc);
+        else c->pZ.resize(0);
+        c->scheme = scheme;
+        return RS_OK;
+    });
+}
+
+// Air-mass source of the column pressure fixer (recomputed with the fluxes
+// and the geometry).
+void ppm_update_source(rs_ctx* c) {
+    if (!c->have_fixer || !c->have_ppm_geom || !c->have_ppm_flux) {
+        c->pmsrc.resize(0);
+        return;
+    }
+    c->pmsrc.resize(c->nc);
+    HIPCHK(rs_launch_ppm_source(c->pMx.p, c->pMy.p, c->pdp.p, c->pgx.p, c->pce.p, c->prdS.p,
+                                c->pdPS.p, c->pmsrc.p, c->nlon, c->nlat, c->nlev, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+int rs_set_pressure_fixer(rs_ctx* c, const double* dPSdt) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!dPSdt) {
+        c->have_fixer = false;
+        return guarded(c, [&]() { ppm_update_source(c); return RS_OK; });
+    }
+    if (!c->have_ppm_geom || !c->have_ppm_flux)
+        return fail(c, RS_E_STATE, "set the PPM geometry and mass fluxes first");
+    for (long long x = 0; x < c->ncol; ++x)
+        if (!std::isfinite(dPSdt[x])) return fail(c, RS_E_NONFINITE, "dPSdt");
+    return guarded(c, [&]() {
+        c->pdPS.resize(c->ncol);
+        h2d(c->pdPS.p, dPSdt, c->ncol, c->stream);
+        c->have_fixer = true;
+        ppm_update_source(c);
+        return RS_OK;
+    });
 }
 
 int rs_set_ppm_geometry(rs_ctx* c, int lon_periodic, double dlon, const double* coslat_e,
@@ -1148,6 +1203,7 @@ int rs_set_ppm_geometry(rs_ctx* c, int lon_periodic, double dlon, const double* 
         HIPCHK(hipStreamSynchronize(c->stream));
         c->lon_periodic = lon_periodic ? 1 : 0;
         c->have_ppm_geom = true;
+        ppm_update_source(c);
         return RS_OK;
     });
 }
@@ -1168,7 +1224,7 @@ int rs_set_mass_fluxes(rs_ctx* c, const double* Mx, const double* My, const doub
             return fail(c, RS_E_ARG, "Mz must vanish at the surface and the lid");
     return guarded(c, [&]() {
         c->pMx.resize(nx); c->pMy.resize(ny); c->pMz.resize(nz); c->pdp.resize(c->nc);
-        c->pvs.resize(3 * (nlev + 4) * ncol); c->pve.resize(5 * (nlev + 3) * ncol);
+        c->pvs.resize(2 * (nlev + 4) * ncol); c->pve.resize(3 * (nlev + 3) * ncol);
         h2d(c->pMx.p, Mx, nx, c->stream);
         h2d(c->pMy.p, My, ny, c->stream);
         h2d(c->pMz.p, Mz, nz, c->stream);
@@ -1177,6 +1233,7 @@ int rs_set_mass_fluxes(rs_ctx* c, const double* Mx, const double* My, const doub
                                    c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         c->have_ppm_flux = true;
+        ppm_update_source(c);
         return RS_OK;
     });
 }

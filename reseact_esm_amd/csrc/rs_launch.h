@@ -95,13 +95,16 @@ typedef struct RsTransArgs {
 } RsTransArgs;
 
 /* Monotone PPM transport stage (CWC mixing-ratio form; species 0 = air
- * mass).  Stage input a (or the state buffer), LAST also u (a) and u3 (b). */
+ * mass).  Stage input a (or the state buffer); the third stage also writes
+ * a4 from u (b); the last stage reads a4 (c4), u (a) and u3 (b). */
 typedef struct RsPPMArgs {
     double* S0;
     double* S1;
-    const double* a;
-    const double* b;
+    const double* a;        /* stage input                               */
+    const double* b;        /* u (third stage) or u3 (last stage)        */
+    const double* c4;       /* last stage: a4 = 2/3 u + 1/3 u3           */
     double* out;
+    double* out2;           /* third stage: a4                            */
     const RsCtrl* ctrl;
     double* partial;
     double* negpart;
@@ -111,17 +114,18 @@ typedef struct RsPPMArgs {
     const double* ce;       /* nlat+1: meridional face lengths */
     const double* gx;       /* nlat: dphi_lat / (dlon dS_lat) */
     const double* rdS;      /* nlat: 1 / dS_lat */
-    const double* vs;       /* vertical slope coefficients, 3 x (nlev+4) x ncol */
-    const double* ve;       /* vertical edge coefficients, 5 x (nlev+3) x ncol */
+    const double* vs;       /* vertical slope factors, 2 x (nlev+4) x ncol */
+    const double* ve;       /* vertical edge factors, 3 x (nlev+3) x ncol */
     const double* qw;       /* inflow halos of species 1..nt-1 */
     const double* qe;
     const double* qs;
     const double* qn;
+    const double* msrc;     /* air-mass source of the pressure fixer, or 0 */
     long long nc;
     double dt_static, atol, rtol;
     int nlon, nlat, nlev, nt;
     int cur_static, a_is_state, out_is_state, clamp;
-    int lon_periodic, posdom;
+    int lon_periodic, posdom, b_is_state, pad;
 } RsPPMArgs;
 
 #endif

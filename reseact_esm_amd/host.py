@@ -148,6 +148,7 @@ ABI = {
     "rs_set_ppm_geometry": [_P, C.c_int, C.c_double, _D, _D, _D],
     "rs_set_mass_fluxes": [_P, _D, _D, _D, _D],
     "rs_set_inflow": [_P, _D, _D, _D, _D],
+    "rs_set_pressure_fixer": [_P, _D],
     "rs_set_mechanism": [_P, C.c_int, _I, _I, _D, _I],
     "rs_set_mechanism_stoich": [_P, C.c_int, _I, _I, _I, _D, _D, _I],
     "rs_set_photolysis": [_P, _D],
@@ -353,6 +354,9 @@ class Context:
                                                float(ppm.dlon), _dp(ce), _dp(dS), _dp(dphi)))
         self._chk(self.lib.rs_set_mass_fluxes(self.h, _dp(Mx), _dp(My), _dp(Mz), _dp(dp)))
         self._chk(self.lib.rs_set_inflow(self.h, _dp(qw), _dp(qe), _dp(qs), _dp(qn)))
+        fix = None if ppm.dPSdt is None else np.ascontiguousarray(ppm.dPSdt, np.float64)
+        self._ppm_keep.append(fix)
+        self._chk(self.lib.rs_set_pressure_fixer(self.h, _dp(fix)))
 
     def set_state(self, u):
         u = np.ascontiguousarray(u, np.float64)

@@ -361,6 +361,7 @@ class PPMSetup:
     qbc_e: np.ndarray | None = None
     qbc_s: np.ndarray | None = None   # (nq, nlev, nlon)
     qbc_n: np.ndarray | None = None
+    dPSdt: np.ndarray | None = None   # (nlat, nlon): column pressure fixer of the air mass
 
 
 # GEOS-FP hybrid coefficients at the lowest 8 level edges (surface first),
@@ -459,7 +460,8 @@ def ppm_global(nlon: int = 360, nlat: int = 180, nlev: int = 72, nq: int = 20,
     hybrid levels from a smooth surface-to-top pressure profile, air-mass
     fluxes of u = 20 cos(phi) + 5 sin(2 lambda) cos(phi) m/s and
     v = 5 sin(lambda) cos(phi)^2 m/s, Mz from continuity with the pressure
-    fixer, mixing ratios lognormal.
+    fixer at dPS/dt = 0 and the same fixer on the air-mass equation (so m is
+    stationary to round-off), mixing ratios lognormal.
     Species 0 is the air mass; NT = NS = 1 + nq."""
     g = Grid(nlon, nlat, nlev)
     phie = np.linspace(-90.0, 90.0, nlat + 1)
@@ -482,7 +484,8 @@ def ppm_global(nlon: int = 360, nlat: int = 180, nlev: int = 72, nq: int = 20,
     rng = np.random.default_rng([seed, 2])
     q = rng.lognormal(0.0, 0.5, size=nq * g.nc)
     u0 = np.concatenate([dp.reshape(-1), q])
-    ppm = PPMSetup(True, dlon, ce, dS, dphi, Mx, My, np.ascontiguousarray(Mz), dp)
+    ppm = PPMSetup(True, dlon, ce, dS, dphi, Mx, My, np.ascontiguousarray(Mz), dp,
+                   dPSdt=np.zeros((nlat, nlon)))
     return Problem(name=f"transport-ppm-{nlon}x{nlat}x{nlev}-{nq}tr", grid=g, ns=1 + nq, nt=1 + nq,
                    wind=solid_body_wind(g, 1.0, 0.0), mech=None, u0=u0, ppm=ppm,
                    dt_macro=600.0, nmacro=1)

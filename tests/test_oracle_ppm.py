@@ -15,6 +15,8 @@ against the reference's own test and what the mathematics fixes.  CPU only.
 * Conservation on the global grid: sum of A*dm/dt and of A*d(m q)/dt vanish
   to round-off (periodic longitude, zero-length polar faces, no flux through
   surface and lid); the air mass is conserved through a solve.
+* The column pressure fixer: the air-mass column tendency is dPS/dt for any
+  fluxes; with Mz from the corrected continuity, m is stationary.
 * Monotonicity: a square wave carried by a uniform zonal flow makes no new
   extrema beyond 1e-10 of the jump (the 3rd-order upwind scheme of the
   synthetic workloads overshoots by percents on the same profile).
@@ -153,6 +155,7 @@ def test_weighted_form_reduces_to_uniform(oracle):
 # ------------------------------------------------------------ conservation
 def test_conservation_global(oracle):
     p = P.ppm_global(24, 12, 8, nq=3, seed=7)
+    p.ppm.dPSdt = None          # the raw continuity: dm/dt = -div(M) alone
     ctx = oracle.context().load(p)
     nc, g, pp = p.nc, p.grid, p.ppm
     A = np.broadcast_to((pp.dlon * pp.dS_lat)[None, :, None], (g.nlev, g.nlat, g.nlon)).reshape(-1)
@@ -171,6 +174,35 @@ def test_conservation_global(oracle):
     ctx.adaptive_transport(0.0, 3600.0, 60.0, pictrl_ssprk43(), 1e-6, 1e-3, 0)
     mt = ctx.get_state()[:nc]
     assert abs(np.sum(A * mt) - np.sum(A * m)) <= 1e-13 * np.sum(A * m)
+    ctx.close()
+
+
+def test_pressure_fixer(oracle):
+    """The column pressure fixer (reseact.esm:5 caveat 4): the air-mass column
+    tendency equals dPS/dt per column whatever the fluxes; with Mz diagnosed
+    from the same corrected divergence and dPS/dt = 0 the air mass is
+    stationary to round-off; the mixing-ratio tendencies do not see it."""
+    p = P.ppm_global(24, 12, 8, nq=2, seed=5)
+    g, pp, nc = p.grid, p.ppm, p.nc
+    ctx = oracle.context().load(p)
+    f = rhs(ctx, p.u0)
+    fm = f[:nc].reshape(g.nlev, g.nlat, g.nlon)
+    divscale = np.max(np.abs(np.diff(pp.Mx, axis=2)))
+    assert np.max(np.abs(fm)) <= 1e-12 * divscale            # stationary (the fixer on)
+    ctx.close()
+    rng = np.random.default_rng(2)
+    Mz = np.array(pp.Mz) * rng.uniform(0.5, 1.5, pp.Mz.shape)  # any Mz, zero at the ends
+    dps = rng.normal(0.0, 1e-3, (g.nlat, g.nlon))
+    pp2 = P.PPMSetup(**{**pp.__dict__, "Mz": Mz, "dPSdt": dps})
+    p2 = P.Problem(**{**p.__dict__, "ppm": pp2})
+    ctx = oracle.context().load(p2)
+    f2 = rhs(ctx, p.u0)
+    col = f2[:nc].reshape(g.nlev, g.nlat, g.nlon).sum(0)
+    assert np.max(np.abs(col - dps)) <= 1e-12 * (np.max(np.abs(dps)) + divscale)
+    ctx._chk(ctx.lib.rs_set_pressure_fixer(ctx.h, None))
+    f3 = rhs(ctx, p.u0)
+    assert np.array_equal(f3[nc:], f2[nc:])                   # mixing ratios: raw div(M)
+    assert not np.array_equal(f3[:nc], f2[:nc])
     ctx.close()
 
 
