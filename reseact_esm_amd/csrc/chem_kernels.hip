@@ -268,13 +268,15 @@ typedef struct { double k[RS_NR]; } RsKTab;
 #define RS_SLAB(name) double name##_arr[3 * RS_NS]; double* name = name##_arr
 #endif
 
-/* One attempt over all cells, in two entry points: the rate constants as a
+/* One attempt over all cells, in three entry points: the rate constants as a
  * kernel argument (scalar loads; RS_KPARAM) or through the pointer in the
- * argument block (vector loads, fewer SGPRs held); the autotuner keeps the
- * faster on the context's grid.  The body is spelled out in each kernel (a
- * macro): shared through a device function, the compiler spilled 100-140
- * B/lane more. */
-#define RS_CHEM_ATTEMPT_KERNEL(NAME, KPARAM, KEXPR)                                         \
+ * argument block (vector loads, fewer SGPRs held), the autotuner keeping the
+ * faster on the context's grid; and, for positive-domain solves only, the
+ * pointer form that also counts negative entries of the trial state per
+ * workgroup (a compile-time switch: the count costs the other two 6%).  The
+ * body is spelled out in each kernel (a macro): shared through a device
+ * function, the compiler spilled 100-140 B/lane more. */
+#define RS_CHEM_ATTEMPT_KERNEL(NAME, KPARAM, KEXPR, POSDOM)                                 \
     extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK) NAME(RsChemArgs a KPARAM) {  \
         __shared__ double sh[RS_CHEM_BLOCK / 64];                                           \
         RS_SLAB(sv);                                                                        \
@@ -292,24 +294,25 @@ typedef struct { double k[RS_NR]; } RsKTab;
                 sv[(2 * RS_NS + s) * RS_SLAB_ST] = src[(long long)s * a.nc + cell];         \
             const double jf = a.jfac ? a.jfac[cell] : 1.0;                                  \
             const double* Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;               \
-            const int clamp = a.clamp, posdom = a.posdom;                                  \
+            const int clamp = a.clamp;                                                      \
             sse = rs_ros23_attempt<RS_SLAB_ST, RS_W_ST, false>(                             \
                 KEXPR, jf, Ec, a.ncol, sv, W, dt, a.atol, a.rtol, [&](int s, double un) {   \
-                    if (posdom && un < 0.0) neg += 1.0;                                     \
+                    if (POSDOM && un < 0.0) neg += 1.0;                                     \
                     dst[(long long)s * a.nc + cell] = clamp ? rs_max(un, 0.0) : un;         \
                 });                                                                         \
         }                                                                                   \
         const double tot = rs_block_sum(sse, sh);                                           \
         if (threadIdx.x == 0) a.partial[blockIdx.x] = tot;                                  \
-        if (a.negpart) {                                                                    \
+        if (POSDOM) {                                                                       \
             __syncthreads();                                                                \
             const double nt = rs_block_sum(neg, sh);                                        \
             if (threadIdx.x == 0) a.negpart[blockIdx.x] = nt;                               \
         }                                                                                   \
     }
 
-RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt, RS_KPARAM, RS_KPTR(a))
-RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt_kp, , a.K)
+RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt, RS_KPARAM, RS_KPTR(a), 0)
+RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt_kp, , a.K, 0)
+RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt_pd, , a.K, 1)
 
 /* Persistent per-cell adaptive solve with wave-aggregated work stealing: a
  * lane that finishes its cell takes the next index from the queue head; all

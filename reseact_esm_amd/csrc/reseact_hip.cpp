@@ -126,7 +126,8 @@ struct DevArray {
 
 struct ChemModule {
     hipModule_t mod = nullptr;
-    hipFunction_t attempt = nullptr, attempt_kp = nullptr, percell = nullptr, vjp = nullptr;
+    hipFunction_t attempt = nullptr, attempt_kp = nullptr, attempt_pd = nullptr, percell = nullptr,
+                  vjp = nullptr;
     rs::MechPlan plan;
     ~ChemModule() { if (mod) (void)hipModuleUnload(mod); }
 };
@@ -576,7 +577,8 @@ void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cu
     a.rtol = rtol;
     a.cur_static = cur_static;
     a.clamp = clamp;
-    if (c->chem_kp) chem_launch(c, c->chem->attempt_kp, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEM, false);
+    if (a.posdom) chem_launch(c, c->chem->attempt_pd, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEM, false);
+    else if (c->chem_kp) chem_launch(c, c->chem->attempt_kp, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEM, false);
     else chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEM);
 }
 
@@ -1295,6 +1297,7 @@ int install_mechanism(rs_ctx* c, rs::MechSpec& m, const double* k) {
         HIPCHK(hipModuleLoadData(&cm->mod, code.data()));
         HIPCHK(hipModuleGetFunction(&cm->attempt, cm->mod, "rs_chem_attempt"));
         HIPCHK(hipModuleGetFunction(&cm->attempt_kp, cm->mod, "rs_chem_attempt_kp"));
+        HIPCHK(hipModuleGetFunction(&cm->attempt_pd, cm->mod, "rs_chem_attempt_pd"));
         HIPCHK(hipModuleGetFunction(&cm->percell, cm->mod, "rs_chem_percell"));
         HIPCHK(hipModuleGetFunction(&cm->vjp, cm->mod, "rs_chem_vjp"));
         c->K.resize(std::max(nr, 1));

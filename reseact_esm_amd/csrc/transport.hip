@@ -384,7 +384,9 @@ __device__ __forceinline__ void fstage_adj(const FCol& C, int nk, double h2, Out
 
 }  // namespace
 
-template <int ADJ>
+// PD: positive-domain solves also count the negative entries of the trial
+// state per workgroup (compile-time, so other solves do not carry it).
+template <int ADJ, int PD>
 __global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
     extern __shared__ double lds[];
     __shared__ double red[FT / 64];
@@ -502,7 +504,7 @@ __global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
             den = den + P.atol;
             const double at = ut / den;
             sq += at * at;
-            if (P.posdom && un < 0.0) ng += 1.0;
+            if (PD && un < 0.0) ng += 1.0;
             po[g0 + (k0 + jj) * plane] = P.clamp ? dmax(un, 0.0) : un;
         });
     }
@@ -510,7 +512,7 @@ __global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
         const long long bid = (long long)blockIdx.y * gridDim.x + blockIdx.x;
         const double t = block_sum_fixed<FT>(sq, red, tid);
         if (tid == 0) P.partial[bid] = t;
-        if (P.negpart) {
+        if (PD) {
             const double nt = block_sum_fixed<FT>(ng, red, tid);
             if (tid == 0) P.negpart[bid] = nt;
         }
@@ -822,8 +824,9 @@ hipError_t rs_launch_transport_fused(int adj, const RsTransArgs* a, hipStream_t 
     const size_t lds = ((size_t)a->seg_w * fpitch(a->nlev) + a->seg_w + a->nlev + 4) * sizeof(double);
     if (lds > 65536 - 256) return hipErrorInvalidValue;
     dim3 grid((unsigned)a->nseg, (unsigned)(a->nlat * a->nt));
-    if (adj) hipLaunchKernelGGL(rs_transport_fused<1>, grid, dim3(FT), lds, st, *a);
-    else hipLaunchKernelGGL(rs_transport_fused<0>, grid, dim3(FT), lds, st, *a);
+    if (adj) hipLaunchKernelGGL((rs_transport_fused<1, 0>), grid, dim3(FT), lds, st, *a);
+    else if (a->negpart) hipLaunchKernelGGL((rs_transport_fused<0, 1>), grid, dim3(FT), lds, st, *a);
+    else hipLaunchKernelGGL((rs_transport_fused<0, 0>), grid, dim3(FT), lds, st, *a);
     return hipGetLastError();
 }
 
