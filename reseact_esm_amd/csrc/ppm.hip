@@ -12,9 +12,10 @@
 // the vertical clamps its stencil at the surface and the lid (Mz = 0 there).
 //
 // One lane per cell, all species of that cell: the air-mass fluxes, metrics,
-// the divergence of M and the vertical width coefficients are loaded once
-// and serve every species; the stencils come through L1/L2 (neighbouring
-// lanes share them).  The width-only vertical coefficients are tabulated per
+// the divergence of M and the vertical width factors are loaded once and
+// serve every species.  The horizontal faces of a workgroup's tile are
+// reconstructed once each from an LDS copy of the tile; the vertical ones per
+// cell from the column.  The width-only vertical factors are tabulated per
 // column when the fluxes are set (rs_ppm_coefs), so a stage does no divisions
 // besides the final one by m.  Built with -ffp-contract=off: every statement
 // rounds as the oracle's, so stages agree with it bit for bit.
@@ -179,9 +180,23 @@ __global__ void __launch_bounds__(PB) rs_ppm_coefs(const double* __restrict__ dp
 // KIND 2 (third stage): the same, and out2 = a4 = 2/3 u + 1/3 out (u = b);
 // KIND 1 (last): unew = a4 + h/2 L a4 (a4 = c4), the error partial from
 // u = a and u3 = b, and (positive domain) the count of negative entries.
+//
+// A workgroup is a tile of PTX longitudes x PTY latitudes at one level.  Per
+// mixing-ratio species the tile and its 3-cell halo are staged in LDS, every
+// longitude and latitude face of the tile is reconstructed once (by one lane)
+// into a face-flux array, and each lane then takes its cell's divergence
+// from the four horizontal face fluxes and its own two vertical faces (the
+// vertical stencil comes from global memory, clamped to the column).
+constexpr int PTX = 64, PTY = 4;
+constexpr int PWX = PTX + 6, PWY = PTY + 6;                 // tile + halo
+constexpr int NFX = (PTX + 1) * PTY, NFY = PTX * (PTY + 1);  // faces per tile
+
 template <int KIND>
 __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
     __shared__ double red[PB / 64];
+    __shared__ double Q[PWY][PWX];
+    __shared__ double FX[PTY][PTX + 1];
+    __shared__ double FY[PTY + 1][PTX];
     const RsCtrl* ctl = A.ctrl;
     if (ctl && ctl->done) return;
     const int cur = ctl ? ctl->cur : A.cur_static;
@@ -194,31 +209,36 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
     const double* __restrict__ ps = (KIND == 1) ? A.c4 : pa;   // stencil source
     double* __restrict__ po = A.out_is_state ? (cur ? A.S0 : A.S1) : A.out;
     const int tid = threadIdx.x;
-    const long long cell = (long long)blockIdx.x * PB + tid;
-    double sq = 0.0, ng = 0.0;
-    if (cell < nc) {
-        const int i = (int)(cell % nlon);
-        const int j = (int)((cell / nlon) % nlat);
-        const int k = (int)(cell / plane);
-        const long long col = (long long)j * nlon + i;
-        // stage value at a cell (the last stage: a4 from the third stage)
-        auto V = [&](long long g) -> double { return ps[g]; };
+    const int tx = tid % PTX, ty = tid / PTX;
+    const int i0 = blockIdx.x * PTX, j0 = blockIdx.y * PTY, k = blockIdx.z;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool valid = (i < nlon) && (j < nlat);
+    const long long col = (long long)j * nlon + i;
+    const long long cell = (long long)k * plane + col;
+    // per-cell quantities shared by every species
+    double mw = 0.0, me = 0.0, ms = 0.0, mn = 0.0, mb = 0.0, mt = 0.0;
+    double gx = 0.0, ces = 0.0, cen = 0.0, rdS = 0.0, divM = 0.0, mval = 1.0;
+    double sR[5], sL[5], ec[4][3];
+    if (valid) {
         const double* mxr = A.Mx + ((long long)k * nlat + j) * (nlon + 1);
-        const double mw = mxr[i], me = mxr[i + 1];
-        const double ms = A.My[((long long)k * (nlat + 1) + j) * nlon + i];
-        const double mn = A.My[((long long)k * (nlat + 1) + j + 1) * nlon + i];
-        const double mb = A.Mz[(long long)k * plane + col];
-        const double mt = A.Mz[(long long)(k + 1) * plane + col];
-        const double gx = A.gx[j], ces = A.ce[j], cen = A.ce[j + 1], rdS = A.rdS[j];
-        const double divM = div6(mw, me, ms, mn, mb, mt, gx, ces, cen, rdS);
-        const double mval = V(cell);
-        // vertical width factors of the faces k (bottom) and k+1 (top): slopes
-        // of cells k-2..k+2, edges k-2..k+1
+        mw = mxr[i];
+        me = mxr[i + 1];
+        ms = A.My[((long long)k * (nlat + 1) + j) * nlon + i];
+        mn = A.My[((long long)k * (nlat + 1) + j + 1) * nlon + i];
+        mb = A.Mz[(long long)k * plane + col];
+        mt = A.Mz[(long long)(k + 1) * plane + col];
+        gx = A.gx[j];
+        ces = A.ce[j];
+        cen = A.ce[j + 1];
+        rdS = A.rdS[j];
+        divM = div6(mw, me, ms, mn, mb, mt, gx, ces, cen, rdS);
+        mval = ps[cell];
+        // vertical width factors of the faces k (bottom) and k+1 (top):
+        // slopes of cells k-2..k+2, edges k-2..k+1
         const long long nrs = (long long)(nlev + 4) * plane, nre = (long long)(nlev + 3) * plane;
-        double sR[5], sL[5], ec[4][3];
 #pragma unroll
         for (int d = 0; d < 5; ++d) {
-            const long long r = (long long)(k + d) * plane + col;      // row x + 2, x = k - 2 + d
+            const long long r = (long long)(k + d) * plane + col;   // row x + 2, x = k - 2 + d
             sR[d] = A.vs[r];
             sL[d] = A.vs[nrs + r];
         }
@@ -228,42 +248,70 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) ec[d][q] = A.ve[q * nre + r];
         }
-        for (int s = 0; s < nt; ++s) {
-            const long long sb = (long long)s * nc;
-            const long long g = sb + cell;
-            double f;
-            if (s == 0) {
-                f = A.msrc ? -divM + A.msrc[cell] : -divM;
-            } else {
-                const int sq1 = s - 1;
-                const long long rowb = sb + (long long)k * plane + (long long)j * nlon;
+    }
+    double sq = 0.0, ng = 0.0;
+    for (int s = 0; s < nt; ++s) {
+        const long long sb = (long long)s * nc;
+        const long long g = sb + cell;
+        double f = 0.0;
+        if (s == 0) {
+            if (valid) f = A.msrc ? -divM + A.msrc[cell] : -divM;
+        } else {
+            const int sq1 = s - 1;
+            // ---- stage the tile and its halo (corners are never read)
+            for (int x = tid; x < PWX * PWY; x += PB) {
+                const int r = x / PWX, q = x - r * PWX;
+                const int gi = i0 - 3 + q, gj = j0 - 3 + r;
+                double v = 0.0;
+                const bool inlat = gj >= 0 && gj < nlat;
+                const bool inlon = gi >= 0 && gi < nlon;
+                if (inlat) {
+                    const long long rowb = sb + (long long)k * plane + (long long)gj * nlon;
+                    if (inlon) v = ps[rowb + gi];
+                    else if (A.lon_periodic) v = ps[rowb + wrapi(gi, nlon)];
+                    else if (gi < 0) v = A.qw[((long long)sq1 * nlev + k) * nlat + gj];
+                    else v = A.qe[((long long)sq1 * nlev + k) * nlat + gj];
+                } else if (inlon) {
+                    if (gj < 0) v = A.qs[((long long)sq1 * nlev + k) * nlon + gi];
+                    else v = A.qn[((long long)sq1 * nlev + k) * nlon + gi];
+                }
+                Q[r][q] = v;
+            }
+            __syncthreads();
+            // ---- longitude faces i0 + fx (fx = 0..PTX) of rows j0 + fr
+            for (int x = tid; x < NFX; x += PB) {
+                const int fr = x / (PTX + 1), fx = x - fr * (PTX + 1);
+                const int gj = j0 + fr, gi = i0 + fx;
+                double F = 0.0;
+                if (gj < nlat && gi <= nlon) {
+                    const double M = A.Mx[((long long)k * nlat + gj) * (nlon + 1) + gi];
+                    F = M * face_u(&Q[fr + 3][fx], M);
+                }
+                FX[fr][fx] = F;
+            }
+            // ---- latitude faces j0 + fy (fy = 0..PTY) of columns i0 + fc
+            for (int x = tid; x < NFY; x += PB) {
+                const int fy = x / PTX, fc = x - fy * PTX;
+                const int gj = j0 + fy, gi = i0 + fc;
+                double F = 0.0;
+                if (gj <= nlat && gi < nlon) {
+                    const double M = A.My[((long long)k * (nlat + 1) + gj) * nlon + gi];
+                    double v[6];
+#pragma unroll
+                    for (int d = 0; d < 6; ++d) v[d] = Q[fy + d][fc + 3];
+                    F = M * face_u(v, M);
+                }
+                FY[fy][fc] = F;
+            }
+            // ---- this cell's vertical faces
+            double fb = 0.0, ft = 0.0;
+            if (valid) {
                 double v[7];
-                // longitude: periodic wrap or the w/e inflow halo
-#pragma unroll
-                for (int d = 0; d < 7; ++d) {
-                    const int ii = i - 3 + d;
-                    if (A.lon_periodic) v[d] = V(rowb + wrapi(ii, nlon));
-                    else if (ii < 0) v[d] = A.qw[((long long)sq1 * nlev + k) * nlat + j];
-                    else if (ii >= nlon) v[d] = A.qe[((long long)sq1 * nlev + k) * nlat + j];
-                    else v[d] = V(rowb + ii);
-                }
-                const double qw_ = face_u(v, mw);
-                const double qe_ = face_u(v + 1, me);
-                // latitude: the s/n inflow halo
-#pragma unroll
-                for (int d = 0; d < 7; ++d) {
-                    const int jj = j - 3 + d;
-                    if (jj < 0) v[d] = A.qs[((long long)sq1 * nlev + k) * nlon + i];
-                    else if (jj >= nlat) v[d] = A.qn[((long long)sq1 * nlev + k) * nlon + i];
-                    else v[d] = V(sb + (long long)k * plane + (long long)jj * nlon + i);
-                }
-                const double qs_ = face_u(v, ms);
-                const double qn_ = face_u(v + 1, mn);
-                // vertical: stencil clamped to the column; physical cell x is
-                // v[x - k + 3], its slope factors index x - k + 2, edge x index x - k + 2
 #pragma unroll
                 for (int d = 0; d < 7; ++d)
-                    v[d] = V(sb + (long long)clampi(k - 3 + d, 0, nlev - 1) * plane + col);
+                    v[d] = ps[sb + (long long)clampi(k - 3 + d, 0, nlev - 1) * plane + col];
+                // physical cell x is v[x - k + 3]; its slope factors and the
+                // edge between x and x+1 sit at index x - k + 2
                 auto sl_ = [&](int x) {
                     const int d = x - k + 3, c = x - k + 2;
                     return slope_w(sR[c], sL[c], v[d - 1], v[d], v[d + 1]);
@@ -286,15 +334,20 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
                     limit(v[c1 + 1], aL, aR);
                     return aL;
                 };
-                const double qb_ = fz(k, mb);
-                const double qt_ = fz(k + 1, mt);
-                const double divF = div6(mw * qw_, me * qe_, ms * qs_, mn * qn_, mb * qb_, mt * qt_,
-                                         gx, ces, cen, rdS);
-                const double qc = V(g);
+                fb = mb * fz(k, mb);
+                ft = mt * fz(k + 1, mt);
+            }
+            __syncthreads();
+            if (valid) {
+                const double divF = div6(FX[ty][tx], FX[ty][tx + 1], FY[ty][tx], FY[ty + 1][tx],
+                                         fb, ft, gx, ces, cen, rdS);
+                const double qc = Q[ty + 3][tx + 3];
                 const double num = qc * divM - divF;
                 f = num / mval;
             }
-            const double ac = (s == 0) ? mval : V(g);
+        }
+        if (valid) {
+            const double ac = (s == 0) ? mval : Q[ty + 3][tx + 3];
             const double du = h2 * f;
             const double un = ac + du;
             if (KIND == 1) {
@@ -322,11 +375,12 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
         }
     }
     if (KIND == 1) {
+        const long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         const double t = block_sum(sq, red, tid);
-        if (tid == 0) A.partial[blockIdx.x] = t;
+        if (tid == 0) A.partial[bid] = t;
         if (A.negpart) {
             const double n = block_sum(ng, red, tid);
-            if (tid == 0) A.negpart[blockIdx.x] = n;
+            if (tid == 0) A.negpart[bid] = n;
         }
     }
 }
@@ -361,7 +415,9 @@ __global__ void __launch_bounds__(PB) rs_ppm_source(const double* __restrict__ M
     for (int k = 0; k < nlev; ++k) msrc[(long long)k * plane + col] = a * dp[(long long)k * plane + col];
 }
 
-long long rs_ppm_nblocks(long long nc) { return (nc + PB - 1) / PB; }
+long long rs_ppm_nblocks(int nlon, int nlat, int nlev) {
+    return (long long)((nlon + PTX - 1) / PTX) * ((nlat + PTY - 1) / PTY) * nlev;
+}
 
 hipError_t rs_launch_ppm_source(const double* Mx, const double* My, const double* dp,
                                 const double* gx, const double* ce, const double* rdS,
@@ -383,7 +439,8 @@ hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlo
 
 // kind 0: first and second stages; 2: third stage (also a4); 1: last stage
 hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st) {
-    const dim3 grid((unsigned)rs_ppm_nblocks(a->nc));
+    const dim3 grid((unsigned)((a->nlon + PTX - 1) / PTX), (unsigned)((a->nlat + PTY - 1) / PTY),
+                    (unsigned)a->nlev);
     if (kind == 1) hipLaunchKernelGGL(rs_ppm_stage<1>, grid, dim3(PB), 0, st, *a);
     else if (kind == 2) hipLaunchKernelGGL(rs_ppm_stage<2>, grid, dim3(PB), 0, st, *a);
     else hipLaunchKernelGGL(rs_ppm_stage<0>, grid, dim3(PB), 0, st, *a);
