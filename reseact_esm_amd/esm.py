@@ -167,11 +167,19 @@ class CompiledMechanism:
 
 
 def compile_system(sys: EsmSystem, parameter_overrides: dict | None = None,
-                   species_values: dict | None = None) -> CompiledMechanism:
+                   species_values: dict | None = None,
+                   sinks: dict | None = None) -> CompiledMechanism:
     """Evaluates every rate with the parameters (defaults, then overrides)
     and folds constant substrates (their default or `species_values`) into k.
     Substrates beyond two slots, or a stoichiometry that is not 1 or 2, are
-    outside the mass-action kernels' reach and raise."""
+    outside the mass-action kernels' reach and raise.
+
+    `sinks` {species: k (1/s)} appends first-order losses D(X) = -k X as
+    reactions X -> (nothing) named "sink_X": the form of the deposition sink
+    the reference merges onto the chemistry by operator_compose
+    (prototypes/g2_depsink/g2_dep.esm, D(X) ~ -(k_dry + k_wet) X; the sink
+    model itself, superfast_deposition_sink.esm, is not in the reference
+    tree, so which species take which k is the caller's)."""
     values = {s: d["default"] for s, d in sys.species.items()}
     if species_values:
         values.update(species_values)
@@ -207,6 +215,15 @@ def compile_system(sys: EsmSystem, parameter_overrides: dict | None = None,
         off.append(len(psp))
         kk.append(k)
         ids.append(r.id)
+    for sp, kd in (sinks or {}).items():
+        if sp not in index:
+            raise KeyError(f"sink on {sp!r}, which is not an active species")
+        if not (kd >= 0.0 and math.isfinite(kd)):
+            raise ValueError(f"sink rate of {sp} must be finite and >= 0")
+        re_.append([index[sp], -1])
+        off.append(len(psp))
+        kk.append(float(kd))
+        ids.append(f"sink_{sp}")
     return CompiledMechanism(active, np.array(re_, np.int32).reshape(-1, 2),
                              np.array(off, np.int32), np.array(psp, np.int32),
                              np.array(pco, np.float64), np.array(kk, np.float64),

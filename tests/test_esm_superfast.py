@@ -149,3 +149,37 @@ def test_unsupported_reactions_raise(sf):
     bad.reactions[0].rate = {"op": "*", "args": ["OH", 2.0]}   # an active species inside a rate
     with pytest.raises(KeyError):
         esm.compile_system(bad)
+
+
+# the exaggerated deposition rates of prototypes/g2_depsink/g2_run.jl:32-39
+# (k_wet on the soluble species is the caller's choice: the sink model that
+# assigns it is not in the reference tree)
+DEP = {"O3": 0.05, "NO2": 0.03, "HNO3": 0.08 + 0.02, "H2O2": 0.05 + 0.02, "CH2O": 0.04}
+
+
+def test_deposition_sinks(sf, oracle):
+    """First-order sinks from compile_system(sinks=...): alone they decay as
+    exp(-k t); on SuperFast over 1 s (g2_run.jl) every species with a sink
+    ends lower than without, and the NOy total NO + NO2 + HNO3 loses what
+    the NO2 and HNO3 sinks remove."""
+    m0 = esm.compile_system(sf)
+    md = esm.compile_system(sf, sinks=DEP)
+    assert md.nr == m0.nr + len(DEP) and md.ids[-1] == "sink_CH2O"
+    with pytest.raises(KeyError):
+        esm.compile_system(sf, sinks={"N2O5": 1.0})
+    # sinks alone: zero every chemical rate constant
+    import dataclasses
+    only = dataclasses.replace(md, k=np.concatenate([np.zeros(m0.nr), md.k[m0.nr:]]))
+    u0 = esm.initial_state(sf)
+    run = _library_run(oracle, rtol=1e-10, atol=1e-30)
+    got = run(only, u0, [1.0], 1.0)[1.0]
+    for sp, kd in DEP.items():
+        i = md.species.index(sp)
+        assert abs(got[i] - u0[i] * np.exp(-kd)) <= 1e-8 * u0[i], sp
+    a = run(m0, u0, [1.0], 1.0)[1.0]
+    b = run(md, u0, [1.0], 1.0)[1.0]
+    for sp in DEP:
+        i = md.species.index(sp)
+        assert b[i] < a[i], sp
+    noy = lambda u: sum(u[md.species.index(x)] for x in ("NO", "NO2", "HNO3"))
+    assert noy(b) < noy(a) <= noy(u0) * (1 + 1e-9)

@@ -94,3 +94,18 @@ def test_superfast_global_norm_same_sequence(sf, hip, oracle):
     assert abs(dg - do) <= 1e-8 * do
     floor = np.maximum(np.abs(uo), 1e-6)
     assert float(np.max(np.abs(ug - uo) / floor)) <= 1e-8
+
+
+def test_superfast_with_deposition_sinks(sf, hip, oracle):
+    """SuperFast with first-order deposition sinks (compile_system(sinks=...),
+    the rates of prototypes/g2_depsink/g2_run.jl:32-39) on 128 cells over one
+    hour: per-cell adaptive chemistry agrees with the oracle."""
+    dep = {"O3": 0.05, "NO2": 0.03, "HNO3": 0.1, "H2O2": 0.07, "CH2O": 0.04}
+    m = esm.compile_system(sf, sinks=dep)
+    u = _cells(m, esm.initial_state(sf), 21)
+    g, ag = _run(hip, m, u, [3600.0])
+    o, ao = _run(oracle, m, u, [3600.0])
+    print(f"with sinks: attempts gpu {ag} oracle {ao}", flush=True)
+    assert abs(ag - ao) <= max(2, 0.002 * ao)
+    rel = np.abs(g[3600.0] - o[3600.0]) / np.maximum(np.abs(o[3600.0]), 1e-12)
+    assert float(rel.max()) <= 1e-6
