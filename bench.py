@@ -394,7 +394,7 @@ def traffic(outdir: str) -> dict:
     f_write = copy / (max(wr[cname]) * 1024.0)
     res = {"calibration": {"kernel": cname, "bytes": copy, "fetch_factor": round(f_fetch, 4),
                            "write_factor": round(f_write, 4)}}
-    for key, pat in (("transport_fused", "rs_transport_fused<0>"), ("chem_attempt", "rs_chem_attempt")):
+    for key, pat in (("transport_fused", "rs_transport_fused<0,"), ("chem_attempt", "rs_chem_attempt")):
         kf = next(k for k in fe if pat in k)
         kw = next(k for k in wr if pat in k)
         rd = sum(fe[kf]) / len(fe[kf]) * 1024.0 * f_fetch
@@ -444,7 +444,7 @@ def main() -> int:
             tr = traffic(os.path.join(REPO, "gpurun_out", "bench_pmc"))
             log(f"traffic: {tr}")
         except Exception as e:  # the counters are a report, not the measurement
-            log(f"traffic counters unavailable: {e}")
+            log(f"traffic counters unavailable: {type(e).__name__} {e}")
     p = c["p"]
     if tr:
         c["roof"]["traffic"] = tr["chem_attempt"]["bytes"]

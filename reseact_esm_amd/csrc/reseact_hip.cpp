@@ -759,6 +759,12 @@ int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_ho
                                                               (int)thr, slab));
     per_cu = std::max(per_cu, 1);
     long long grid = (long long)per_cu * c->ncu;
+    // RS_PERCELL_WAVES: developer knob capping the resident waves of the
+    // persistent kernel (a smaller spill footprint can stay in the caches)
+    if (const char* ev = std::getenv("RS_PERCELL_WAVES")) {
+        const long long w = std::atoll(ev);
+        if (w > 0) grid = std::min<long long>(grid, std::max<long long>(1, w / (thr / 64)));
+    }
     grid = std::min<long long>(grid, (c->nc + thr - 1) / thr);
     grid = std::max<long long>(grid, 1);
     chem_launch(c, c->chem->percell, &a, sizeof(a), (unsigned)grid, thr, P_PERCELL);
