@@ -12,6 +12,8 @@ int rs_oracle_ros23_jvp(rs_ctx* c, double dt, const double* v, const double* Ev,
 int rs_oracle_ssprk43_jvp(rs_ctx* c, double dt, const double* v, double* out);
 int rs_oracle_transport_rhs(rs_ctx* c, const double* q, double* f);
 int rs_oracle_transport_rhs_adj(rs_ctx* c, const double* lam, double* g);
+/* PPM only: J(u) v at the context's state u */
+int rs_oracle_transport_rhs_jvp(rs_ctx* c, const double* v, double* out);
 int rs_oracle_chem_rhs(rs_ctx* c, const double* u, double* f);
 int rs_oracle_chem_jac(rs_ctx* c, const double* u, size_t cell, double* J);
 int rs_oracle_threads(void);

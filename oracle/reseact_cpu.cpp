@@ -452,6 +452,371 @@ void ppm_rhs_all(const rs_ctx& c, const double* u, double* f) {
     }
 }
 
+// ----------------------------------------------------------------------------
+// Derivatives of the PPM tendency, on the branch the forward takes at u.  The
+// slope and parabola limiters switch on products of neighbour differences, so
+// L_ppm is only piecewise smooth; these are the derivatives of the active
+// branch, the derivative almost everywhere (the reference's caveat,
+// DIFFERENTIABILITY_PLAN.md:709-736: on an exactly uniform field every switch
+// sits on its surface and finite differences see other branches).
+// Two independent derivations: a forward-mode tangent (value-and-tangent per
+// operation, for the JVP) and the local gradient of each face value with
+// respect to its six stencil cells (reverse rules, for the VJP).
+// ----------------------------------------------------------------------------
+struct Tan { double v, d; };
+
+// slope of a cell from (am, a0, ap): value and tangent; nu selects the
+// width-weighted form with widths (hm, h0, hp)
+inline Tan ppm_slope_tan(Tan am, Tan a0, Tan ap, bool nu, double hm, double h0, double hp) {
+    const double val = nu ? ppm_slope_nu(hm, h0, hp, am.v, a0.v, ap.v) : ppm_slope(am.v, a0.v, ap.v);
+    const double dl = a0.v - am.v, dr = ap.v - a0.v;
+    if (!(dr * dl > 0.0)) return {val, 0.0};
+    double da, dda;
+    if (nu) {
+        const double w = h0 / (hm + h0 + hp);
+        const double cR = w * ((2.0 * hm + h0) / (hp + h0));
+        const double cL = w * ((h0 + 2.0 * hp) / (hm + h0));
+        da = cR * dr + cL * dl;
+        dda = cR * (ap.d - a0.d) + cL * (a0.d - am.d);
+    } else {
+        da = 0.5 * (ap.v - am.v);
+        dda = 0.5 * (ap.d - am.d);
+    }
+    // value = sgn(da) min(|da|, min(2|dl|, 2|dr|)); dl, dr and da share one sign
+    const double x = std::fabs(da), y = 2.0 * std::fabs(dl), z = 2.0 * std::fabs(dr);
+    const double in = (z < y) ? z : y;
+    if (in < x) return {val, (z < y) ? 2.0 * (ap.d - a0.d) : 2.0 * (a0.d - am.d)};
+    return {val, dda};
+}
+
+inline Tan ppm_edge_tan(Tan a0, Tan a1, Tan d0, Tan d1, bool nu, double hm, double h0, double h1,
+                        double h2) {
+    if (!nu) {
+        const double val = ppm_edge(a0.v, a1.v, d0.v, d1.v);
+        const double t = a0.d + 0.5 * (a1.d - a0.d);
+        return {val, t - (d1.d - d0.d) * C16};
+    }
+    const double val = ppm_edge_nu(hm, h0, h1, h2, a0.v, a1.v, d0.v, d1.v);
+    // the edge is linear in (a0, a1, d0, d1): its tangent is the edge of the tangents
+    return {val, ppm_edge_nu(hm, h0, h1, h2, a0.d, a1.d, d0.d, d1.d)};
+}
+
+inline void ppm_limit_tan(Tan a0, Tan& aL, Tan& aR) {
+    double l = aL.v, r = aR.v;
+    ppm_limit(a0.v, l, r);
+    if ((aR.v - a0.v) * (a0.v - aL.v) <= 0.0) {
+        aL = {l, a0.d};
+        aR = {r, a0.d};
+        return;
+    }
+    const double d = aR.v - aL.v;
+    const double a6 = 6.0 * (a0.v - 0.5 * (aL.v + aR.v));
+    const double dd = d * d;
+    const double da6 = d * a6;
+    if (da6 > dd) aL = {l, 3.0 * a0.d - 2.0 * aR.d};
+    else if (-dd > da6) aR = {r, 3.0 * a0.d - 2.0 * aL.d};
+}
+
+// tangent of the face value of face f from v[0..5] (cells f-3..f+2), widths h
+// (null: uniform)
+inline double ppm_face_tan(const double* v, const double* vd, const double* h, double M) {
+    const bool nu = h != nullptr;
+    auto H = [&](int x) { return nu ? h[x] : 1.0; };
+    Tan t[6];
+    for (int d = 0; d < 6; ++d) t[d] = {v[d], vd[d]};
+    const int c = M >= 0.0 ? 2 : 3;             // upwind cell
+    auto sl = [&](int x) { return ppm_slope_tan(t[x - 1], t[x], t[x + 1], nu, H(x - 1), H(x), H(x + 1)); };
+    const Tan sm = sl(c - 1), s0 = sl(c), sp = sl(c + 1);
+    Tan aL = ppm_edge_tan(t[c - 1], t[c], sm, s0, nu, H(c - 2), H(c - 1), H(c), H(c + 1));
+    Tan aR = ppm_edge_tan(t[c], t[c + 1], s0, sp, nu, H(c - 1), H(c), H(c + 1), H(c + 2));
+    ppm_limit_tan(t[c], aL, aR);
+    return M >= 0.0 ? aR.d : aL.d;
+}
+
+// Local gradient g[d] = d(face value)/d v[d] of the active branch (reverse rules).
+inline void ppm_slope_grad(const double* v, const double* h, int x, double g[3]) {
+    g[0] = g[1] = g[2] = 0.0;
+    const double am = v[x - 1], a0 = v[x], ap = v[x + 1];
+    const double dl = a0 - am, dr = ap - a0;
+    if (!(dr * dl > 0.0)) return;
+    double da, cR = 0.5, cL = 0.5;
+    if (h) {
+        const double w = h[x] / (h[x - 1] + h[x] + h[x + 1]);
+        cR = w * ((2.0 * h[x - 1] + h[x]) / (h[x + 1] + h[x]));
+        cL = w * ((h[x] + 2.0 * h[x + 1]) / (h[x - 1] + h[x]));
+        da = cR * dr + cL * dl;
+    } else {
+        da = 0.5 * (ap - am);
+    }
+    const double ax = std::fabs(da), y = 2.0 * std::fabs(dl), z = 2.0 * std::fabs(dr);
+    const double in = (z < y) ? z : y;
+    if (in < ax) {
+        if (z < y) { g[2] = 2.0; g[1] = -2.0; }
+        else { g[1] = 2.0; g[0] = -2.0; }
+        return;
+    }
+    // da = cR (ap - a0) + cL (a0 - am)
+    g[2] = cR;
+    g[1] = cL - cR;
+    g[0] = -cL;
+}
+
+// Edge between cells x and x+1: gradient with respect to (a_x, a_x+1, s_x, s_x+1).
+inline void ppm_edge_grad(const double* h, int x, double e[4]) {
+    if (!h) {
+        e[0] = 0.5; e[1] = 0.5; e[2] = C16; e[3] = -C16;
+        return;
+    }
+    const double hm = h[x - 1], h0 = h[x], h1 = h[x + 1], h2 = h[x + 2];
+    const double cA = h0 / (h0 + h1);
+    const double gg = (2.0 * h1 * h0) / (h0 + h1);
+    const double t1 = (hm + h0) / (2.0 * h0 + h1);
+    const double t2 = (h2 + h1) / (2.0 * h1 + h0);
+    const double cS = 1.0 / (((hm + h0) + h1) + h2);
+    const double cE = cA + (gg * (t1 - t2)) * cS;
+    const double cF = (h0 * t1) * cS;
+    const double cG = (h1 * ((h1 + h2) / (h0 + 2.0 * h1))) * cS;
+    e[0] = 1.0 - cE; e[1] = cE; e[2] = cG; e[3] = -cF;
+}
+
+inline void ppm_face_grad(const double* v, const double* h, double M, double g[6]) {
+    for (int d = 0; d < 6; ++d) g[d] = 0.0;
+    const int c = M >= 0.0 ? 2 : 3;
+    double sm[3], s0[3], sp[3], eL[4], eR[4];
+    ppm_slope_grad(v, h, c - 1, sm);
+    ppm_slope_grad(v, h, c, s0);
+    ppm_slope_grad(v, h, c + 1, sp);
+    auto slope_val = [&](int x) {
+        return h ? ppm_slope_nu(h[x - 1], h[x], h[x + 1], v[x - 1], v[x], v[x + 1])
+                 : ppm_slope(v[x - 1], v[x], v[x + 1]);
+    };
+    auto edge_val = [&](int x, double d0, double d1) {
+        return h ? ppm_edge_nu(h[x - 1], h[x], h[x + 1], h[x + 2], v[x], v[x + 1], d0, d1)
+                 : ppm_edge(v[x], v[x + 1], d0, d1);
+    };
+    const double dsm = slope_val(c - 1), ds0 = slope_val(c), dsp = slope_val(c + 1);
+    const double aL = edge_val(c - 1, dsm, ds0), aR = edge_val(c, ds0, dsp);
+    ppm_edge_grad(h, c - 1, eL);
+    ppm_edge_grad(h, c, eR);
+    // adjoints of (a_c, aL, aR) from the limiter branch
+    double bc = 0.0, bL = 0.0, bR = 0.0;
+    const bool right = M >= 0.0;
+    if ((aR - v[c]) * (v[c] - aL) <= 0.0) {
+        bc = 1.0;
+    } else {
+        const double d = aR - aL;
+        const double a6 = 6.0 * (v[c] - 0.5 * (aL + aR));
+        const double dd = d * d;
+        const double da6 = d * a6;
+        if (da6 > dd) {             // aL <- 3 a_c - 2 aR
+            if (right) bR = 1.0;
+            else { bc = 3.0; bR = -2.0; }
+        } else if (-dd > da6) {     // aR <- 3 a_c - 2 aL
+            if (right) { bc = 3.0; bL = -2.0; }
+            else bL = 1.0;
+        } else {
+            if (right) bR = 1.0;
+            else bL = 1.0;
+        }
+    }
+    g[c] += bc;
+    // aL = edge(c-1): (a_c-1, a_c, s_c-1, s_c); aR = edge(c): (a_c, a_c+1, s_c, s_c+1)
+    g[c - 1] += bL * eL[0];
+    g[c] += bL * eL[1];
+    const double bsm = bL * eL[2];
+    double bs0 = bL * eL[3];
+    g[c] += bR * eR[0];
+    g[c + 1] += bR * eR[1];
+    bs0 += bR * eR[2];
+    const double bsp = bR * eR[3];
+    for (int q = 0; q < 3; ++q) {
+        g[c - 2 + q] += bsm * sm[q];
+        g[c - 1 + q] += bs0 * s0[q];
+        g[c + q] += bsp * sp[q];
+    }
+}
+
+
+// out = J(u) v over the transported block (tangent of ppm_rhs_all).
+void ppm_rhs_jvp_all(const rs_ctx& c, const double* u, const double* vt, double* out) {
+    const int nlon = c.nlon, nlat = c.nlat, nlev = c.nlev, nt = c.nt;
+    const size_t nc = c.nc, plane = (size_t)nlon * nlat;
+    std::vector<double> f((size_t)nt * nc);
+    ppm_rhs_all(c, u, f.data());
+    const double* m = u;
+    const double* vm = vt;
+    for (size_t x = 0; x < nc; ++x) out[x] = 0.0;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < nlev; ++k) {
+        for (int j = 0; j < nlat; ++j) {
+            const double gx = c.dphi_lat[j] / (c.dlon * c.dS_lat[j]);
+            const double ces = c.coslat_e[j], cen = c.coslat_e[j + 1], rdS = 1.0 / c.dS_lat[j];
+            for (int i = 0; i < nlon; ++i) {
+                const size_t cell = (size_t)k * plane + (size_t)j * nlon + i;
+                const double* mxr = c.Mx.data() + ((size_t)k * nlat + j) * (nlon + 1);
+                const double mw = mxr[i], me = mxr[i + 1];
+                const double ms = c.My[((size_t)k * (nlat + 1) + j) * nlon + i];
+                const double mn = c.My[((size_t)k * (nlat + 1) + j + 1) * nlon + i];
+                const double mb = c.Mz[(size_t)k * plane + (size_t)j * nlon + i];
+                const double mt = c.Mz[(size_t)(k + 1) * plane + (size_t)j * nlon + i];
+                const double divM = ppm_div(mw, me, ms, mn, mb, mt, gx, ces, cen, rdS);
+                for (int s = 1; s < nt; ++s) {
+                    const double* q = u + (size_t)s * nc;
+                    const double* qd = vt + (size_t)s * nc;
+                    const int sq = s - 1;
+                    // value and tangent of the lon / lat lines (halo cells: tangent 0)
+                    auto X = [&](int ii, double* val, double* tan) {
+                        if (c.lon_periodic) {
+                            ii %= nlon;
+                            if (ii < 0) ii += nlon;
+                        } else if (ii < 0 || ii >= nlon) {
+                            *val = ii < 0 ? c.qw[((size_t)sq * nlev + k) * nlat + j]
+                                          : c.qe[((size_t)sq * nlev + k) * nlat + j];
+                            *tan = 0.0;
+                            return;
+                        }
+                        const size_t x = (size_t)k * plane + (size_t)j * nlon + ii;
+                        *val = q[x];
+                        *tan = qd[x];
+                    };
+                    auto Y = [&](int jj, double* val, double* tan) {
+                        if (jj < 0 || jj >= nlat) {
+                            *val = jj < 0 ? c.qs[((size_t)sq * nlev + k) * nlon + i]
+                                          : c.qn[((size_t)sq * nlev + k) * nlon + i];
+                            *tan = 0.0;
+                            return;
+                        }
+                        const size_t x = (size_t)k * plane + (size_t)jj * nlon + i;
+                        *val = q[x];
+                        *tan = qd[x];
+                    };
+                    auto Zc = [&](int kk) { return std::min(std::max(kk, 0), nlev - 1); };
+                    double v[6], vd[6], h[6];
+                    for (int d = 0; d < 6; ++d) X(i - 3 + d, &v[d], &vd[d]);
+                    const double tw = ppm_face_tan(v, vd, nullptr, mw);
+                    for (int d = 0; d < 6; ++d) X(i - 2 + d, &v[d], &vd[d]);
+                    const double te = ppm_face_tan(v, vd, nullptr, me);
+                    for (int d = 0; d < 6; ++d) Y(j - 3 + d, &v[d], &vd[d]);
+                    const double ts = ppm_face_tan(v, vd, nullptr, ms);
+                    for (int d = 0; d < 6; ++d) Y(j - 2 + d, &v[d], &vd[d]);
+                    const double tn = ppm_face_tan(v, vd, nullptr, mn);
+                    for (int d = 0; d < 6; ++d) {
+                        const size_t x = (size_t)Zc(k - 3 + d) * plane + (size_t)j * nlon + i;
+                        v[d] = q[x]; vd[d] = qd[x]; h[d] = c.dp[x];
+                    }
+                    const double tb = ppm_face_tan(v, vd, h, mb);
+                    for (int d = 0; d < 6; ++d) {
+                        const size_t x = (size_t)Zc(k - 2 + d) * plane + (size_t)j * nlon + i;
+                        v[d] = q[x]; vd[d] = qd[x]; h[d] = c.dp[x];
+                    }
+                    const double tt = ppm_face_tan(v, vd, h, mt);
+                    const double ddivF = ppm_div(mw * tw, me * te, ms * ts, mn * tn, mb * tb,
+                                                 mt * tt, gx, ces, cen, rdS);
+                    const double dnum = qd[cell] * divM - ddivF;
+                    out[(size_t)s * nc + cell] = (dnum - f[(size_t)s * nc + cell] * vm[cell]) / m[cell];
+                }
+            }
+        }
+    }
+}
+
+// out = J(u)^T lam over the transported block: the air mass gathers
+// -lam_s f_s / m over the mixing ratios; each mixing ratio gets
+// mu divM (mu = lam / m) and, from every face of every cell, the face's local
+// gradient times that cell's coefficient of the face flux, scattered to the
+// face's stencil cells (halo cells drop out; clamped vertical indices
+// accumulate on the clamped cell).  Tracers run in parallel, each serially.
+void ppm_rhs_vjp_all(const rs_ctx& c, const double* u, const double* lam, double* out) {
+    const int nlon = c.nlon, nlat = c.nlat, nlev = c.nlev, nt = c.nt;
+    const size_t nc = c.nc, plane = (size_t)nlon * nlat;
+    std::vector<double> f((size_t)nt * nc);
+    ppm_rhs_all(c, u, f.data());
+    const double* m = u;
+    for (size_t x = 0; x < nc; ++x) {
+        double a = 0.0;
+        for (int s = 1; s < nt; ++s) {
+            const double t = lam[(size_t)s * nc + x] * f[(size_t)s * nc + x];
+            a -= t / m[x];
+        }
+        out[x] = a;
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int s = 1; s < nt; ++s) {
+        const double* q = u + (size_t)s * nc;
+        const double* L = lam + (size_t)s * nc;
+        double* o = out + (size_t)s * nc;
+        for (size_t x = 0; x < nc; ++x) o[x] = 0.0;
+        for (int k = 0; k < nlev; ++k) {
+            for (int j = 0; j < nlat; ++j) {
+                const double gx = c.dphi_lat[j] / (c.dlon * c.dS_lat[j]);
+                const double ces = c.coslat_e[j], cen = c.coslat_e[j + 1], rdS = 1.0 / c.dS_lat[j];
+                for (int i = 0; i < nlon; ++i) {
+                    const size_t cell = (size_t)k * plane + (size_t)j * nlon + i;
+                    const double* mxr = c.Mx.data() + ((size_t)k * nlat + j) * (nlon + 1);
+                    const double mw = mxr[i], me = mxr[i + 1];
+                    const double ms = c.My[((size_t)k * (nlat + 1) + j) * nlon + i];
+                    const double mn = c.My[((size_t)k * (nlat + 1) + j + 1) * nlon + i];
+                    const double mb = c.Mz[(size_t)k * plane + (size_t)j * nlon + i];
+                    const double mt = c.Mz[(size_t)(k + 1) * plane + (size_t)j * nlon + i];
+                    const double divM = ppm_div(mw, me, ms, mn, mb, mt, gx, ces, cen, rdS);
+                    const double mu = L[cell] / m[cell];
+                    o[cell] += mu * divM;
+                    double v[6], h[6], g[6];
+                    long long id[6];
+                    // f_cell = (q divM - divF) / m: coefficient of each face's value
+                    auto scatter = [&](double coef) {
+                        for (int d = 0; d < 6; ++d)
+                            if (id[d] >= 0) o[id[d]] += coef * g[d];
+                    };
+                    auto line_x = [&](int i0) {
+                        for (int d = 0; d < 6; ++d) {
+                            int ii = i0 + d;
+                            if (c.lon_periodic) {
+                                ii %= nlon;
+                                if (ii < 0) ii += nlon;
+                            } else if (ii < 0 || ii >= nlon) {
+                                id[d] = -1;
+                                v[d] = ii < 0 ? c.qw[((size_t)(s - 1) * nlev + k) * nlat + j]
+                                              : c.qe[((size_t)(s - 1) * nlev + k) * nlat + j];
+                                continue;
+                            }
+                            id[d] = (long long)((size_t)k * plane + (size_t)j * nlon + ii);
+                            v[d] = q[id[d]];
+                        }
+                    };
+                    auto line_y = [&](int j0) {
+                        for (int d = 0; d < 6; ++d) {
+                            const int jj = j0 + d;
+                            if (jj < 0 || jj >= nlat) {
+                                id[d] = -1;
+                                v[d] = jj < 0 ? c.qs[((size_t)(s - 1) * nlev + k) * nlon + i]
+                                              : c.qn[((size_t)(s - 1) * nlev + k) * nlon + i];
+                                continue;
+                            }
+                            id[d] = (long long)((size_t)k * plane + (size_t)jj * nlon + i);
+                            v[d] = q[id[d]];
+                        }
+                    };
+                    auto line_z = [&](int k0) {
+                        for (int d = 0; d < 6; ++d) {
+                            const int kk = std::min(std::max(k0 + d, 0), nlev - 1);
+                            id[d] = (long long)((size_t)kk * plane + (size_t)j * nlon + i);
+                            v[d] = q[id[d]];
+                            h[d] = c.dp[id[d]];
+                        }
+                    };
+                    line_x(i - 3); ppm_face_grad(v, nullptr, mw, g); scatter(mu * (mw * gx));
+                    line_x(i - 2); ppm_face_grad(v, nullptr, me, g); scatter(-mu * (me * gx));
+                    line_y(j - 3); ppm_face_grad(v, nullptr, ms, g); scatter(mu * (ces * ms * rdS));
+                    line_y(j - 2); ppm_face_grad(v, nullptr, mn, g); scatter(-mu * (cen * mn * rdS));
+                    line_z(k - 3); ppm_face_grad(v, h, mb, g); scatter(mu * mb);
+                    line_z(k - 2); ppm_face_grad(v, h, mt, g); scatter(-mu * mt);
+                }
+            }
+        }
+    }
+}
+
 // f = L u over the transported block, for either scheme.
 void transport_rhs_all(const rs_ctx& c, const double* u, double* f) {
     if (c.scheme == RS_TRANSPORT_PPM) {
@@ -700,6 +1065,71 @@ void ssprk43_vjp(const rs_ctx& c, std::vector<double>& lam, double dt) {
         apply(l1.data(), l0.data());                                // u1 = u + h/2 L u
         for (size_t x = 0; x < nc; ++x) L[x] = ub[x] + l0[x];
     }
+}
+
+// SSPRK43 at fixed dt with the PPM tendency, which is nonlinear (limiters,
+// division by the air mass): the stages u1, u2, u4 are recomputed from the
+// step input u as in ssprk43_attempt, then reversed with J at each stage state:
+//   l4 = lam + h/2 J(u4)^T lam;  ub = 2/3 l4;  l3 = 1/3 l4
+//   l2 = l3 + h/2 J(u2)^T l3;  l1 = l2 + h/2 J(u1)^T l2;  l0 = l1 + h/2 J(u)^T l1
+//   lam_u = ub + l0
+// Species beyond the transported block pass through.
+void ssprk43_ppm_stage_states(const rs_ctx& c, const double* u, double dt, std::vector<double>& u1,
+                              std::vector<double>& u2, std::vector<double>& u4) {
+    const size_t nT = (size_t)c.nt * c.nc;
+    const double dt_2 = dt / 2;
+    std::vector<double> k(nT), u3(nT);
+    u1.resize(nT); u2.resize(nT); u4.resize(nT);
+    transport_rhs_all(c, u, k.data());
+    for (size_t x = 0; x < nT; ++x) { double du = dt_2 * k[x]; u1[x] = u[x] + du; }
+    transport_rhs_all(c, u1.data(), k.data());
+    for (size_t x = 0; x < nT; ++x) { double du = dt_2 * k[x]; u2[x] = u1[x] + du; }
+    transport_rhs_all(c, u2.data(), k.data());
+    for (size_t x = 0; x < nT; ++x) { double du = dt_2 * k[x]; u3[x] = u2[x] + du; }
+    for (size_t x = 0; x < nT; ++x) {
+        double a = (2.0 / 3.0) * u[x];
+        double b = (1.0 / 3.0) * u3[x];
+        u4[x] = a + b;
+    }
+}
+
+void ssprk43_vjp_ppm(const rs_ctx& c, const double* u, std::vector<double>& lam, double dt) {
+    const size_t nT = (size_t)c.nt * c.nc;
+    const double h2 = dt / 2;
+    std::vector<double> u1, u2, u4, g(nT), l4(nT), l3(nT), ub(nT);
+    ssprk43_ppm_stage_states(c, u, dt, u1, u2, u4);
+    double* L = lam.data();
+    auto apply = [&](const double* at, const double* in, double* out) {   // out = in + h/2 J(at)^T in
+        ppm_rhs_vjp_all(c, at, in, g.data());
+        for (size_t x = 0; x < nT; ++x) out[x] = in[x] + h2 * g[x];
+    };
+    apply(u4.data(), L, l4.data());
+    for (size_t x = 0; x < nT; ++x) { ub[x] = (2.0 / 3.0) * l4[x]; l3[x] = (1.0 / 3.0) * l4[x]; }
+    apply(u2.data(), l3.data(), l4.data());     // l2
+    apply(u1.data(), l4.data(), l3.data());     // l1
+    apply(u, l3.data(), l4.data());             // l0
+    for (size_t x = 0; x < nT; ++x) L[x] = ub[x] + l4[x];
+}
+
+// Tangent of the same step: v1 = v + h/2 J(u) v, v2 = v1 + h/2 J(u1) v1,
+// v3 = v2 + h/2 J(u2) v2, v4 = 2/3 v + 1/3 v3, out = v4 + h/2 J(u4) v4.
+void ssprk43_jvp_ppm(const rs_ctx& c, const double* u, const double* v, double* out, double dt) {
+    const size_t nT = (size_t)c.nt * c.nc;
+    const double h2 = dt / 2;
+    std::vector<double> u1, u2, u4, g(nT), v1(nT), v2(nT), v4(nT);
+    ssprk43_ppm_stage_states(c, u, dt, u1, u2, u4);
+    ppm_rhs_jvp_all(c, u, v, g.data());
+    for (size_t x = 0; x < nT; ++x) v1[x] = v[x] + h2 * g[x];
+    ppm_rhs_jvp_all(c, u1.data(), v1.data(), g.data());
+    for (size_t x = 0; x < nT; ++x) v2[x] = v1[x] + h2 * g[x];
+    ppm_rhs_jvp_all(c, u2.data(), v2.data(), g.data());
+    for (size_t x = 0; x < nT; ++x) {
+        const double v3 = v2[x] + h2 * g[x];
+        v4[x] = (2.0 / 3.0) * v[x] + (1.0 / 3.0) * v3;
+    }
+    ppm_rhs_jvp_all(c, u4.data(), v4.data(), g.data());
+    for (size_t x = 0; x < nT; ++x) out[x] = v4[x] + h2 * g[x];
+    for (size_t x = nT; x < (size_t)c.ns * c.nc; ++x) out[x] = v[x];
 }
 
 void ssprk43_jvp(const rs_ctx& c, const std::vector<double>& v, std::vector<double>& out, double dt) {
@@ -1361,8 +1791,9 @@ int rs_ssprk43_vjp(rs_ctx* c, double t, double dt) {
     (void)t;
     int rc = check_ready(c, false, true);
     if (rc) return rc;
-    if (c->scheme == RS_TRANSPORT_PPM) return fail(c, RS_E_ARG, "no adjoint of the PPM transport");
-    ssprk43_vjp(*c, c->lam, dt);
+    // PPM: J depends on the state; the step input is the context's state
+    if (c->scheme == RS_TRANSPORT_PPM) ssprk43_vjp_ppm(*c, c->u.data(), c->lam, dt);
+    else ssprk43_vjp(*c, c->lam, dt);
     return RS_OK;
 }
 
@@ -1380,8 +1811,6 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
     int rc = check_ready(c, true, true);
     if (rc) return rc;
     if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
-    if (c->scheme == RS_TRANSPORT_PPM && c->nt > 0)
-        return fail(c, RS_E_ARG, "no adjoint of the PPM transport");
     if (o->chem_percell) return fail(c, RS_E_ARG, "adjoint needs the global-norm chemistry");
     if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
         return fail(c, RS_E_ARG, "bad loss");
@@ -1453,7 +1882,10 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         for (int jx = (int)tapeT.size() - 1; jx >= 0; --jx) {
             const std::vector<double>& outj = (jx + 1 < (int)tapeT.size()) ? tapeT[jx + 1] : uT;
             mask(outj);
-            ssprk43_vjp(*c, c->lam, c->seqT[m].ts[jx].second);
+            if (c->scheme == RS_TRANSPORT_PPM)
+                ssprk43_vjp_ppm(*c, tapeT[jx].data(), c->lam, c->seqT[m].ts[jx].second);
+            else
+                ssprk43_vjp(*c, c->lam, c->seqT[m].ts[jx].second);
             S.nvjp_T += 1;
         }
     }
@@ -1553,7 +1985,10 @@ int rs_oracle_ros23_jvp(rs_ctx* c, double dt, const double* v, const double* Ev,
 int rs_oracle_ssprk43_jvp(rs_ctx* c, double dt, const double* v, double* out) {
     int rc = check_ready(c, false, true);
     if (rc) return rc;
-    if (c->scheme == RS_TRANSPORT_PPM) return fail(c, RS_E_ARG, "no tangent of the PPM transport");
+    if (c->scheme == RS_TRANSPORT_PPM) {       // tangent at the context's state
+        ssprk43_jvp_ppm(*c, c->u.data(), v, out, dt);
+        return RS_OK;
+    }
     std::vector<double> vv(v, v + (size_t)c->ns * c->nc), o;
     ssprk43_jvp(*c, vv, o, dt);
     std::copy(o.begin(), o.end(), out);
@@ -1570,12 +2005,25 @@ int rs_oracle_transport_rhs(rs_ctx* c, const double* q, double* f) {
     std::fill(f + (size_t)c->nt * c->nc, f + (size_t)c->ns * c->nc, 0.0);
     return RS_OK;
 }
+// PPM: J(u)^T lam and J(u) v at the context's state u (the tendency is nonlinear).
 int rs_oracle_transport_rhs_adj(rs_ctx* c, const double* lam, double* g) {
+    if (c && c->scheme == RS_TRANSPORT_PPM) {
+        if (!(c->have_ppm_geom && c->have_ppm_flux)) return fail(c, RS_E_STATE, "transport not set up");
+        std::fill(g, g + (size_t)c->ns * c->nc, 0.0);
+        ppm_rhs_vjp_all(*c, c->u.data(), lam, g);
+        return RS_OK;
+    }
     if (!c || !c->have_wind) return fail(c, RS_E_STATE, "wind not set");
-    if (c->scheme == RS_TRANSPORT_PPM) return fail(c, RS_E_ARG, "no adjoint of the PPM transport");
     std::fill(g, g + (size_t)c->ns * c->nc, 0.0);
     for (int s = 0; s < c->nt; ++s)
         transport_rhs_adj_block(*c, lam + (size_t)s * c->nc, g + (size_t)s * c->nc);
+    return RS_OK;
+}
+int rs_oracle_transport_rhs_jvp(rs_ctx* c, const double* v, double* out) {
+    if (!c || c->scheme != RS_TRANSPORT_PPM || !(c->have_ppm_geom && c->have_ppm_flux))
+        return fail(c, RS_E_STATE, "PPM transport not set up");
+    std::fill(out, out + (size_t)c->ns * c->nc, 0.0);
+    ppm_rhs_jvp_all(*c, c->u.data(), v, out);
     return RS_OK;
 }
 int rs_oracle_chem_rhs(rs_ctx* c, const double* u, double* f) {

@@ -489,3 +489,31 @@ def ppm_global(nlon: int = 360, nlat: int = 180, nlev: int = 72, nq: int = 20,
     return Problem(name=f"transport-ppm-{nlon}x{nlat}x{nlev}-{nq}tr", grid=g, ns=1 + nq, nt=1 + nq,
                    wind=solid_body_wind(g, 1.0, 0.0), mech=None, u0=u0, ppm=ppm,
                    dt_macro=600.0, nmacro=1)
+
+
+def ppm_coupled_regional(nlon: int = 10, nlat: int = 8, nlev: int = 6, nq: int = 3,
+                         seed: int = 5, nmacro: int = 2) -> Problem:
+    """The regional PPM patch (open walls) with lognormal mixing ratios and
+    inflow halos, coupled to a small mass-action mechanism among the mixing
+    ratios (q1 -> q2, q2 + q3 -> q1, q3 -> q2; the air mass, species 0, takes
+    no part): the workload of the discrete adjoint through the PPM transport."""
+    if nq < 3:
+        raise ValueError("the mechanism needs three mixing ratios")
+    p = transport3d_regional(nlon, nlat, nlev, nq=nq)
+    rng = np.random.default_rng([seed, 3])
+    nc = p.nc
+    p.u0 = p.u0.copy()
+    p.u0[nc:] = rng.lognormal(0.0, 0.5, nq * nc)
+    pp = p.ppm
+    for name in ("qbc_w", "qbc_e", "qbc_s", "qbc_n"):
+        setattr(pp, name, rng.lognormal(0.0, 0.5, getattr(pp, name).shape))
+    re = np.array([[1, -1], [2, 3], [3, -1]], dtype=np.int32)
+    pr = np.array([[2, -1], [1, -1], [2, -1]], dtype=np.int32)
+    p.mech = Mechanism(ns=1 + nq, reactants=re, products=pr, k=np.array([2.0, 0.5, 1.0]),
+                       photo=np.zeros(3, dtype=np.int32))
+    p.name = f"ppm-coupled-regional-{nlon}x{nlat}x{nlev}-{nq}q"
+    p.dt_macro = 0.05
+    p.nmacro = nmacro
+    p.dt0_T = 0.01
+    p.dt0_C = 0.01
+    return p

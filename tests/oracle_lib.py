@@ -17,6 +17,7 @@ EXTRA = {
     "rs_oracle_ssprk43_jvp": [_P, C.c_double, _D, _D],
     "rs_oracle_transport_rhs": [_P, _D, _D],
     "rs_oracle_transport_rhs_adj": [_P, _D, _D],
+    "rs_oracle_transport_rhs_jvp": [_P, _D, _D],
     "rs_oracle_chem_rhs": [_P, _D, _D],
     "rs_oracle_chem_jac": [_P, _D, C.c_size_t, _D],
     "rs_oracle_threads": [],
