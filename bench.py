@@ -185,8 +185,22 @@ def ppm_roofline(lib, nsteps: int = 10, nq: int = 20):
         ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)
     t1 = time.perf_counter()
     prof = ctx.profile_read()
+    # the step VJP at the same state: three forward stages recompute the stage
+    # states, four adjoint applications (ppm_vjp) reverse them
+    ctx.set_lambda(np.ones(p.ns * p.nc))
+    ctx.ssprk43_vjp(0.0, dt)                       # warm-up
+    ctx.profile(True)
+    nvjp = max(2, nsteps // 2)
+    t2 = time.perf_counter()
+    for _ in range(nvjp):
+        ctx.ssprk43_vjp(0.0, dt)
+    t3 = time.perf_counter()
+    profv = ctx.profile_read()
     ctx.profile(False)
     ctx.close()
+    nv_, ms_v = kernel_ms(profv, "ppm_vjp")
+    nvs_, ms_vs = kernel_ms(profv, "transport_stage")
+    vjp_ms = (ms_v + ms_vs) / nvjp
     ns_, ms_s = kernel_ms(prof, "transport_stage")
     nl_, ms_l = kernel_ms(prof, "transport_last")
     ms = (ms_s + ms_l) / nsteps
@@ -210,7 +224,8 @@ def ppm_roofline(lib, nsteps: int = 10, nq: int = 20):
     tf = flops_attempt / (ms * 1e-3) / 1e12
     log(f"ppm {p.name}: {ms:.3f} ms/attempt (stage {ms_s / max(ns_, 1):.3f} ms, last "
         f"{ms_l / max(nl_, 1):.3f} ms), {tf:.2f} TFLOP/s, {gbs:.0f} GB/s algorithmic, "
-        f"wall {t1 - t0:.2f} s")
+        f"wall {t1 - t0:.2f} s; step VJP {vjp_ms:.3f} ms ({ms_v / max(nv_, 1):.3f} ms per "
+        f"ppm_vjp launch), VJP/attempt {vjp_ms / ms:.2f}")
     return {"workload": p.name, "kernels": ["transport_stage", "transport_last"], "bound": "mfma",
             "compute": "fp64 valu", "achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 4),
@@ -219,7 +234,10 @@ def ppm_roofline(lib, nsteps: int = 10, nq: int = 20):
                     "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_attempt": bytes_attempt},
             "ms_per_attempt": round(ms, 4),
             "avg_stage_ms": round(ms_s / max(ns_, 1), 4), "avg_last_ms": round(ms_l / max(nl_, 1), 4),
-            "tracer_cell_attempts_per_s": round(nt * p.nc / (ms * 1e-3), 1)}
+            "tracer_cell_attempts_per_s": round(nt * p.nc / (ms * 1e-3), 1),
+            "vjp": {"ms_per_step_vjp": round(vjp_ms, 4), "avg_ppm_vjp_ms": round(ms_v / max(nv_, 1), 4),
+                    "ppm_vjp_launches_per_vjp": round(nv_ / nvjp, 2),
+                    "vjp_over_attempt": round(vjp_ms / ms, 3), "wall_s": round(t3 - t2, 3)}}
 
 
 def chemistry_roofline(lib):

@@ -258,7 +258,11 @@ RS_API int rs_macro_step(rs_ctx* ctx, double t0, double t1, double* dtT_io, doub
 /* ---- discrete adjoint of one step (fixed dt) ------------------------------
  * lambda <- (d unew / d u)^T lambda at the current state; for ROS23 also
  * grad_emis += (d unew / d E)^T lambda.  The Rosenbrock stages are recomputed
- * from the state, not stored.
+ * from the state, not stored.  With the PPM transport (nonlinear: limiters,
+ * division by the air mass) the SSPRK43 stage states are recomputed from the
+ * state too, and the derivative is that of the limiter branches the forward
+ * takes there (the derivative almost everywhere; DIFFERENTIABILITY_PLAN.md:
+ * 709-736 for the caveat at exactly uniform fields).
  * Reference: ssprk43_step_vjp / ros23_step_vjp rx_traced_integrator.jl:610-623. */
 RS_API int rs_set_lambda(rs_ctx* ctx, const double* lam);      /* NS*NC               */
 RS_API int rs_get_lambda(rs_ctx* ctx, double* lam);

@@ -415,6 +415,274 @@ __global__ void __launch_bounds__(PB) rs_ppm_source(const double* __restrict__ M
     for (int k = 0; k < nlev; ++k) msrc[(long long)k * plane + col] = a * dp[(long long)k * plane + col];
 }
 
+// ---- adjoint of the tendency ------------------------------------------------
+// Derivatives on the branch the forward takes at the linearisation state (the
+// limiters switch on products of neighbour differences, so the tendency is
+// piecewise smooth; these are its derivatives almost everywhere).
+//
+// Local gradient g[d] = d(face value)/d v[d] of one face from its six stencil
+// values v[0..5] (cells f-3 .. f+2): the upwind cell c (2 for M >= 0, 3
+// otherwise), the slopes of cells c-1..c+1 and the two edges of c, then the
+// limiter's branch.  NU: width-weighted (vertical), with the slope factors
+// (cR, cL) of stencil cell x at sR[x], sL[x] and the edge factors of the edge
+// between stencil cells x and x+1 at ec[x][0..2]; values from the forward's
+// own expressions, so branch decisions are the forward's.
+template <bool NU>
+__device__ __forceinline__ void face_grad(const double* v, double M, const double* sR,
+                                          const double* sL, const double (*ec)[3], double* g) {
+#pragma unroll
+    for (int d = 0; d < 6; ++d) g[d] = 0.0;
+    const int c = M >= 0.0 ? 2 : 3;
+    double sv[3], sg[3][3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int x = c - 1 + q;
+        const double am = v[x - 1], a0 = v[x], ap = v[x + 1];
+        const double cR = NU ? sR[x] : 0.5, cL = NU ? sL[x] : 0.5;
+        sv[q] = NU ? slope_w(cR, cL, am, a0, ap) : slope_u(am, a0, ap);
+        sg[q][0] = sg[q][1] = sg[q][2] = 0.0;
+        const double dl = a0 - am, dr = ap - a0;
+        if (dr * dl > 0.0) {
+            const double da = NU ? cR * dr + cL * dl : 0.5 * (ap - am);
+            const double ax = fabs(da), y = 2.0 * fabs(dl), z = 2.0 * fabs(dr);
+            const double in = (z < y) ? z : y;
+            if (in < ax) {
+                if (z < y) { sg[q][2] = 2.0; sg[q][1] = -2.0; }
+                else { sg[q][1] = 2.0; sg[q][0] = -2.0; }
+            } else {
+                sg[q][2] = cR; sg[q][1] = cL - cR; sg[q][0] = -cL;
+            }
+        }
+    }
+    double aL, aR, eL[4], eR[4];
+    if (NU) {
+        aL = edge_w(ec[c - 1], v[c - 1], v[c], sv[0], sv[1]);
+        aR = edge_w(ec[c], v[c], v[c + 1], sv[1], sv[2]);
+        eL[0] = 1.0 - ec[c - 1][0]; eL[1] = ec[c - 1][0]; eL[2] = ec[c - 1][2]; eL[3] = -ec[c - 1][1];
+        eR[0] = 1.0 - ec[c][0]; eR[1] = ec[c][0]; eR[2] = ec[c][2]; eR[3] = -ec[c][1];
+    } else {
+        aL = edge_u(v[c - 1], v[c], sv[0], sv[1]);
+        aR = edge_u(v[c], v[c + 1], sv[1], sv[2]);
+        eL[0] = eR[0] = 0.5; eL[1] = eR[1] = 0.5; eL[2] = eR[2] = C16; eL[3] = eR[3] = -C16;
+    }
+    const bool right = M >= 0.0;
+    double bc = 0.0, bL = 0.0, bR = 0.0;
+    if ((aR - v[c]) * (v[c] - aL) <= 0.0) {
+        bc = 1.0;
+    } else {
+        const double d = aR - aL;
+        const double a6 = 6.0 * (v[c] - 0.5 * (aL + aR));
+        const double dd = d * d;
+        const double da6 = d * a6;
+        if (da6 > dd) {                 // aL <- 3 a_c - 2 aR
+            if (right) bR = 1.0;
+            else { bc = 3.0; bR = -2.0; }
+        } else if (-dd > da6) {         // aR <- 3 a_c - 2 aL
+            if (right) { bc = 3.0; bL = -2.0; }
+            else bL = 1.0;
+        } else {
+            if (right) bR = 1.0;
+            else bL = 1.0;
+        }
+    }
+    g[c] += bc;
+    g[c - 1] += bL * eL[0];
+    g[c] += bL * eL[1];
+    const double bsm = bL * eL[2];
+    const double bs0 = bL * eL[3] + bR * eR[2];
+    g[c] += bR * eR[0];
+    g[c + 1] += bR * eR[1];
+    const double bsp = bR * eR[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        g[c - 2 + q] += bsm * sg[0][q];
+        g[c - 1 + q] += bs0 * sg[1][q];
+        g[c + q] += bsp * sg[2][q];
+    }
+}
+
+// out = [beta b +] x + h2 J(at)^T x, x = alpha a, J the Jacobian of the PPM
+// tendency at `at`.  One lane per cell, all transported species of the cell:
+//   air mass:  x_m - h2 sum_s x_s f_s / m   (f_s the cell's mixing-ratio
+//              tendency at `at`, s in species order)
+//   ratio s:   x_s + h2 (mu divM + sum over the faces whose stencil holds the
+//              cell of W_f g_f(cell)),  mu = x_s / m,
+// with W_f the face's flux times its divergence coefficients in the two cells
+// it separates, g_f the face's local gradient (summed over every stencil
+// position that maps to the cell: periodic wrap, clamped vertical stencil);
+// inflow halo cells are not state and drop out.  A gather: deterministic, no
+// atomics.
+__global__ void __launch_bounds__(PB) rs_ppm_vjp(RsPPMVjpArgs A) {
+    const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev, nt = A.nt;
+    const long long nc = A.nc, plane = (long long)nlon * nlat;
+    const long long cell = (long long)blockIdx.x * PB + threadIdx.x;
+    if (cell >= nc) return;
+    const int i = (int)(cell % nlon), j = (int)((cell / nlon) % nlat), k = (int)(cell / plane);
+    const long long col = (long long)j * nlon + i;
+    const double* __restrict__ at = A.at;
+    const double gx = A.gx[j], ces = A.ce[j], cen = A.ce[j + 1], rdS = A.rdS[j];
+    const double* mxr = A.Mx + ((long long)k * nlat + j) * (nlon + 1);
+    const double mw = mxr[i], me = mxr[i + 1];
+    const double ms = A.My[((long long)k * (nlat + 1) + j) * nlon + i];
+    const double mn = A.My[((long long)k * (nlat + 1) + j + 1) * nlon + i];
+    const double mb = A.Mz[(long long)k * plane + col];
+    const double mt = A.Mz[(long long)(k + 1) * plane + col];
+    const double divM = div6(mw, me, ms, mn, mb, mt, gx, ces, cen, rdS);
+    const double mval = at[cell];
+    const double alpha = A.alpha;
+    auto X = [&](const double* p, long long x) { return alpha == 1.0 ? p[x] : alpha * p[x]; };
+    double msum = 0.0;
+    const long long nrs = (long long)(nlev + 4) * plane, nre = (long long)(nlev + 3) * plane;
+    for (int s = 1; s < nt; ++s) {
+        const long long sb = (long long)s * nc;
+        const int sq1 = s - 1;
+        const double* q = at + sb;
+        const double* la = A.a + sb;
+        // mu of a state cell (x / m)
+        auto mu = [&](long long x) { return X(la, x) / at[x]; };
+        // ---- longitude: values with wrap or inflow halo; in-domain flag
+        auto qx = [&](int ii, bool* st) {
+            if (A.lon_periodic) { *st = true; return q[(long long)k * plane + (long long)j * nlon + wrapi(ii, nlon)]; }
+            if (ii < 0) { *st = false; return A.qw[((long long)sq1 * nlev + k) * nlat + j]; }
+            if (ii >= nlon) { *st = false; return A.qe[((long long)sq1 * nlev + k) * nlat + j]; }
+            *st = true;
+            return q[(long long)k * plane + (long long)j * nlon + ii];
+        };
+        auto qy = [&](int jj, bool* st) {
+            if (jj < 0) { *st = false; return A.qs[((long long)sq1 * nlev + k) * nlon + i]; }
+            if (jj >= nlat) { *st = false; return A.qn[((long long)sq1 * nlev + k) * nlon + i]; }
+            *st = true;
+            return q[(long long)k * plane + (long long)jj * nlon + i];
+        };
+        double gsum = mu(cell) * divM;
+        double v[6], g[6];
+        bool in[6];
+        // own faces' values for the cell's tendency (forward expressions)
+        double fw_ = 0.0, fe_ = 0.0, fs_ = 0.0, fn_ = 0.0, fb_ = 0.0, ft_ = 0.0;
+        // ---- longitude faces: residues i-2..i+3 (all residues if nlon < 6),
+        // face index F in [0, nlon] (face nlon also for residue 0)
+        {
+            const bool allres = A.lon_periodic && nlon < 6;
+            const int nloop = allres ? nlon : 6;
+            for (int o = 0; o < nloop; ++o) {
+                int r;
+                if (allres) r = o;
+                else if (A.lon_periodic) r = wrapi(i - 2 + o, nlon);
+                else r = i - 2 + o;
+                for (int rep = 0; rep < 2; ++rep) {
+                    int F;
+                    if (rep == 0) F = r;
+                    else if (A.lon_periodic && r == 0) F = nlon;
+                    else break;
+                    if (F < 0 || F > nlon) continue;
+                    const double M = mxr[F];
+                    bool hit = false;
+#pragma unroll
+                    for (int d = 0; d < 6; ++d) {
+                        v[d] = qx(F - 3 + d, &in[d]);
+                        const int cc = A.lon_periodic ? wrapi(F - 3 + d, nlon) : F - 3 + d;
+                        in[d] = in[d] && cc == i;
+                        hit = hit || in[d];
+                    }
+                    if (F == i) fw_ = M * face_u(v, M);
+                    if (F == i + 1) fe_ = M * face_u(v, M);
+                    if (!hit || M == 0.0) continue;
+                    face_grad<false>(v, M, nullptr, nullptr, nullptr, g);
+                    double gs = 0.0;
+#pragma unroll
+                    for (int d = 0; d < 6; ++d) if (in[d]) gs += g[d];
+                    const long long rowb = (long long)k * plane + (long long)j * nlon;
+                    const double mr = F < nlon ? mu(rowb + F) : 0.0;
+                    const double ml = F >= 1 ? mu(rowb + F - 1) : 0.0;
+                    gsum += (gx * M) * (mr - ml) * gs;
+                }
+            }
+        }
+        // ---- latitude faces F in [j-2, j+3] n [0, nlat]
+        for (int o = 0; o < 6; ++o) {
+            const int F = j - 2 + o;
+            if (F < 0 || F > nlat) continue;
+            const double M = A.My[((long long)k * (nlat + 1) + F) * nlon + i];
+            bool hit = false;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                v[d] = qy(F - 3 + d, &in[d]);
+                in[d] = in[d] && (F - 3 + d == j);
+                hit = hit || in[d];
+            }
+            if (F == j) fs_ = M * face_u(v, M);
+            if (F == j + 1) fn_ = M * face_u(v, M);
+            if (!hit || M == 0.0) continue;
+            face_grad<false>(v, M, nullptr, nullptr, nullptr, g);
+            double gs = 0.0;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) if (in[d]) gs += g[d];
+            const double cf = A.ce[F];
+            const double mr = F < nlat ? mu((long long)k * plane + (long long)F * nlon + i) * A.rdS[F] : 0.0;
+            const double ml = F >= 1 ? mu((long long)k * plane + (long long)(F - 1) * nlon + i) * A.rdS[F - 1] : 0.0;
+            gsum += (cf * M) * (mr - ml) * gs;
+        }
+        // ---- vertical faces F in [k-2, k+3] n [0, nlev], stencil clamped
+        {
+            double sR[6], sL[6], ec[6][3];
+            for (int o = 0; o < 6; ++o) {
+                const int F = k - 2 + o;
+                if (F < 0 || F > nlev) continue;
+                const double M = A.Mz[(long long)F * plane + col];
+                bool hit = false;
+#pragma unroll
+                for (int d = 0; d < 6; ++d) {
+                    const int L = F - 3 + d;                  // logical level
+                    const int kk = clampi(L, 0, nlev - 1);
+                    v[d] = q[(long long)kk * plane + col];
+                    in[d] = kk == k;
+                    hit = hit || in[d];
+                    // slope factors of logical cell L (row L + 2), edge (L, L+1) (row L + 2)
+                    const int rs = clampi(L + 2, 0, nlev + 3), re = clampi(L + 2, 0, nlev + 2);
+                    sR[d] = A.vs[(long long)rs * plane + col];
+                    sL[d] = A.vs[nrs + (long long)rs * plane + col];
+                    ec[d][0] = A.ve[(long long)re * plane + col];
+                    ec[d][1] = A.ve[nre + (long long)re * plane + col];
+                    ec[d][2] = A.ve[2 * nre + (long long)re * plane + col];
+                }
+                if (F == k || F == k + 1) {
+                    // the forward's vertical face value (rs_ppm_stage fz)
+                    const int c = M >= 0.0 ? 2 : 3;
+                    const double dm = slope_w(sR[c - 1], sL[c - 1], v[c - 2], v[c - 1], v[c]);
+                    const double d0 = slope_w(sR[c], sL[c], v[c - 1], v[c], v[c + 1]);
+                    const double dp = slope_w(sR[c + 1], sL[c + 1], v[c], v[c + 1], v[c + 2]);
+                    double aL = edge_w(ec[c - 1], v[c - 1], v[c], dm, d0);
+                    double aR = edge_w(ec[c], v[c], v[c + 1], d0, dp);
+                    limit(v[c], aL, aR);
+                    const double fv = M * (M >= 0.0 ? aR : aL);
+                    if (F == k) fb_ = fv;
+                    else ft_ = fv;
+                }
+                if (!hit || M == 0.0) continue;
+                face_grad<true>(v, M, sR, sL, ec, g);
+                double gs = 0.0;
+#pragma unroll
+                for (int d = 0; d < 6; ++d) if (in[d]) gs += g[d];
+                const double mr = F < nlev ? mu((long long)F * plane + col) : 0.0;
+                const double ml = F >= 1 ? mu((long long)(F - 1) * plane + col) : 0.0;
+                gsum += M * (mr - ml) * gs;
+            }
+        }
+        const double xs = X(la, cell);
+        const double divF = div6(fw_, fe_, fs_, fn_, fb_, ft_, gx, ces, cen, rdS);
+        const double num = q[cell] * divM - divF;
+        const double fsv = num / mval;
+        const double t = xs * fsv;
+        msum -= t / mval;
+        double o = xs + A.h2 * gsum;
+        if (A.b) o = A.beta * A.b[sb + cell] + o;
+        A.out[sb + cell] = o;
+    }
+    double o = X(A.a, cell) + A.h2 * msum;
+    if (A.b) o = A.beta * A.b[cell] + o;
+    A.out[cell] = o;
+}
+
 long long rs_ppm_nblocks(int nlon, int nlat, int nlev) {
     return (long long)((nlon + PTX - 1) / PTX) * ((nlat + PTY - 1) / PTY) * nlev;
 }
@@ -444,5 +712,10 @@ hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st) {
     if (kind == 1) hipLaunchKernelGGL(rs_ppm_stage<1>, grid, dim3(PB), 0, st, *a);
     else if (kind == 2) hipLaunchKernelGGL(rs_ppm_stage<2>, grid, dim3(PB), 0, st, *a);
     else hipLaunchKernelGGL(rs_ppm_stage<0>, grid, dim3(PB), 0, st, *a);
+    return hipGetLastError();
+}
+
+hipError_t rs_launch_ppm_vjp(const RsPPMVjpArgs* a, hipStream_t st) {
+    hipLaunchKernelGGL(rs_ppm_vjp, dim3((unsigned)((a->nc + PB - 1) / PB)), dim3(PB), 0, st, *a);
     return hipGetLastError();
 }

@@ -44,7 +44,7 @@
 extern "C" {
 hipError_t rs_launch_transport(int kind, int tk, const RsTransArgs* a, hipStream_t st);
 long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt, int tk);
-int rs_transport_fused_plan(int nlon, int nlev, int* plan);
+int rs_transport_fused_plan(int nlon, int nlev, long long lds_bytes, int* plan);
 long long rs_transport_fused_nblocks(int nlat, int nt, int nseg);
 hipError_t rs_launch_transport_fused(int adj, const RsTransArgs* a, hipStream_t st);
 hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog,
@@ -65,6 +65,7 @@ long long rs_ppm_nblocks(int nlon, int nlat, int nlev);
 hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlon, int nlat,
                                int nlev, hipStream_t st);
 hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st);
+hipError_t rs_launch_ppm_vjp(const RsPPMVjpArgs* a, hipStream_t st);
 hipError_t rs_launch_ppm_source(const double* Mx, const double* My, const double* dp,
                                 const double* gx, const double* ce, const double* rdS,
                                 const double* dPSdt, double* msrc, int nlon, int nlat, int nlev,
@@ -83,11 +84,11 @@ hipError_t rs_launch_ppm_source(const double* Mx, const double* My, const double
 namespace {
 
 enum ProfKind { P_TSTAGE, P_TLAST, P_CTRL, P_CHEM, P_PERCELL, P_CHEMVJP, P_TADJ, P_UTIL,
-                P_TFUSED, P_TFUSEDADJ, P_COUNT };
+                P_TFUSED, P_TFUSEDADJ, P_TPPMADJ, P_COUNT };
 const char* kProfNames[P_COUNT] = {"transport_stage", "transport_last", "ctrl_update",
                                    "chem_attempt",    "chem_percell",   "chem_vjp",
                                    "transport_adj",   "util",           "transport_fused",
-                                   "transport_fused_adj"};
+                                   "transport_fused_adj", "ppm_vjp"};
 
 struct HipError {
     hipError_t e;
@@ -418,15 +419,11 @@ inline int posdom_of(int flags) { return (flags >> 1) & 1; }
 // One SSPRK43 attempt of the PPM transport: three stages X, Y, X (u3) and the
 // last stage from u and u3; input and output must differ (neighbours read
 // the input while the output is written).
-void enqueue_ssprk43_ppm(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* uout,
-                         double dt, double atol, double rtol, int flags) {
+RsPPMArgs ppm_args(rs_ctx* c, double dt) {
     RsPPMArgs a{};
     a.S0 = c->S[0].p;
     a.S1 = c->S[1].p;
-    a.ctrl = ctrl;
     a.partial = c->partial.p;
-    a.posdom = posdom_of(flags);
-    a.negpart = a.posdom ? c->negpart.p : nullptr;
     a.Mx = c->pMx.p; a.My = c->pMy.p; a.Mz = c->pMz.p;
     a.ce = c->pce.p; a.gx = c->pgx.p; a.rdS = c->prdS.p;
     a.vs = c->pvs.p; a.ve = c->pve.p;
@@ -434,11 +431,20 @@ void enqueue_ssprk43_ppm(rs_ctx* c, const RsCtrl* ctrl, const double* uin, doubl
     a.msrc = c->pmsrc.n ? c->pmsrc.p : nullptr;
     a.nc = c->nc;
     a.dt_static = dt;
-    a.atol = atol;
-    a.rtol = rtol;
     a.nlon = c->nlon; a.nlat = c->nlat; a.nlev = c->nlev; a.nt = c->nt;
     a.cur_static = c->cur;
     a.lon_periodic = c->lon_periodic;
+    return a;
+}
+
+void enqueue_ssprk43_ppm(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* uout,
+                         double dt, double atol, double rtol, int flags) {
+    RsPPMArgs a = ppm_args(c, dt);
+    a.ctrl = ctrl;
+    a.posdom = posdom_of(flags);
+    a.negpart = a.posdom ? c->negpart.p : nullptr;
+    a.atol = atol;
+    a.rtol = rtol;
     const bool st = (uin == nullptr);
     a.a_is_state = st ? 1 : 0;
     a.a = uin;
@@ -759,12 +765,6 @@ int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_ho
                                                               (int)thr, slab));
     per_cu = std::max(per_cu, 1);
     long long grid = (long long)per_cu * c->ncu;
-    // RS_PERCELL_WAVES: developer knob capping the resident waves of the
-    // persistent kernel (a smaller spill footprint can stay in the caches)
-    if (const char* ev = std::getenv("RS_PERCELL_WAVES")) {
-        const long long w = std::atoll(ev);
-        if (w > 0) grid = std::min<long long>(grid, std::max<long long>(1, w / (thr / 64)));
-    }
     grid = std::min<long long>(grid, (c->nc + thr - 1) / thr);
     grid = std::max<long long>(grid, 1);
     chem_launch(c, c->chem->percell, &a, sizeof(a), (unsigned)grid, thr, P_PERCELL);
@@ -857,6 +857,48 @@ void enqueue_ssprk43_vjp(rs_ctx* c, double dt) {
     a.b = c->LA.p;
     a.out = c->lam.p;
     { ProfScope ps(c, P_TADJ); HIPCHK(rs_launch_transport(KIND_ADJ_D, c->tk, &a, c->stream)); }
+}
+
+// lam <- SSPRK43 VJP of the PPM transport at the step input u (species >= nt
+// untouched).  The tendency is nonlinear, so the stage states are recomputed
+// with the forward stage kernels (X = u1, Y = u2, LA = u3, Z = u4 = 2/3 u +
+// 1/3 u3, the forward's bits), then reversed with J at each stage state:
+//   LB = lam + h/2 J(u4)^T lam                  (l4)
+//   LC = x + h/2 J(u2)^T x,   x = 1/3 LB        (l2)
+//   LA = LC + h/2 J(u1)^T LC                    (l1)
+//   lam = 2/3 LB + (LA + h/2 J(u)^T LA)
+void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
+    RsPPMArgs a = ppm_args(c, dt);
+    a.a = u;
+    a.out = c->X.p;
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
+    a.a = c->X.p;
+    a.out = c->Y.p;
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
+    a.a = c->Y.p;
+    a.out = c->LA.p;
+    a.b = u;
+    a.out2 = c->pZ.p;
+    { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(2, &a, c->stream)); }
+    RsPPMVjpArgs v{};
+    v.Mx = c->pMx.p; v.My = c->pMy.p; v.Mz = c->pMz.p;
+    v.ce = c->pce.p; v.gx = c->pgx.p; v.rdS = c->prdS.p;
+    v.vs = c->pvs.p; v.ve = c->pve.p;
+    v.qw = c->pqw.p; v.qe = c->pqe.p; v.qs = c->pqs.p; v.qn = c->pqn.p;
+    v.nc = c->nc;
+    v.h2 = dt / 2;
+    v.nlon = c->nlon; v.nlat = c->nlat; v.nlev = c->nlev; v.nt = c->nt;
+    v.lon_periodic = c->lon_periodic;
+    auto apply = [&](const double* at, const double* in, double alpha, const double* b, double beta,
+                     double* out) {
+        v.at = at; v.a = in; v.alpha = alpha; v.b = b; v.beta = beta; v.out = out;
+        ProfScope ps(c, P_TPPMADJ);
+        HIPCHK(rs_launch_ppm_vjp(&v, c->stream));
+    };
+    apply(c->pZ.p, c->lam.p, 1.0, nullptr, 0.0, c->LB.p);
+    apply(c->Y.p, c->LB.p, 1.0 / 3.0, nullptr, 0.0, c->LC.p);
+    apply(c->X.p, c->LC.p, 1.0, nullptr, 0.0, c->LA.p);
+    apply(u, c->LA.p, 1.0, c->LB.p, 2.0 / 3.0, c->lam.p);
 }
 
 void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt) {
@@ -977,7 +1019,8 @@ void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* re
             ProfScope ps(c, P_UTIL);
             HIPCHK(rs_launch_mask(c->lam.p, tp + (q + 1) * N, (long long)N, c->stream));
         }
-        enqueue_ssprk43_vjp(c, sT[q].second);
+        if (trans_ppm(c)) enqueue_ssprk43_vjp_ppm(c, tp + q * N, sT[q].second);
+        else enqueue_ssprk43_vjp(c, sT[q].second);
     }
 }
 
@@ -1077,7 +1120,7 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
         c->lam.resize(N);
         c->LA.resize(NT); c->LB.resize(NT); c->LC.resize(NT);
         c->gE.resize((size_t)ns * c->ncol);
-        c->fplan_ok = rs_transport_fused_plan(nlon, nlev, c->fplan) != 0;
+        c->fplan_ok = rs_transport_fused_plan(nlon, nlev, (long long)c->lds_max, c->fplan) != 0;
         long long np = std::max<long long>(rs_transport_nblocks(nlon, nlat, nlev, std::max(nt, 1), 4),
                                            (c->nc + 63) / 64);
         if (c->fplan_ok)
@@ -1541,9 +1584,10 @@ int rs_ssprk43_vjp(rs_ctx* c, double t, double dt) {
     (void)t;
     int rc = check_ready(c, false, true);
     if (rc) return rc;
-    if (trans_ppm(c)) return fail(c, RS_E_ARG, "no adjoint of the PPM transport");
     return guarded(c, [&]() {
-        if (c->nt > 0) enqueue_ssprk43_vjp(c, dt);
+        // PPM: the tendency is nonlinear; the step input is the context's state
+        if (c->nt > 0 && trans_ppm(c)) enqueue_ssprk43_vjp_ppm(c, c->S[c->cur].p, dt);
+        else if (c->nt > 0) enqueue_ssprk43_vjp(c, dt);
         HIPCHK(hipStreamSynchronize(c->stream));
         return RS_OK;
     });
@@ -1566,7 +1610,6 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
     int rc = check_ready(c, true, true);
     if (rc) return rc;
     if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
-    if (trans_ppm(c) && c->nt > 0) return fail(c, RS_E_ARG, "no adjoint of the PPM transport");
     if (o->chem_percell) return fail(c, RS_E_ARG, "adjoint needs the global-norm chemistry");
     if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
         return fail(c, RS_E_ARG, "bad loss");

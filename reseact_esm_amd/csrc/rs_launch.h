@@ -128,4 +128,29 @@ typedef struct RsPPMArgs {
     int lon_periodic, posdom, b_is_state, pad;
 } RsPPMArgs;
 
+/* Adjoint of the PPM tendency at the state `at` (species 0 = air mass):
+ * out = [beta b +] x + h2 J(at)^T x with x = alpha a, over the transported
+ * block; a and out must differ (each cell gathers from its neighbours). */
+typedef struct RsPPMVjpArgs {
+    const double* at;
+    const double* a;
+    const double* b;        /* or 0 */
+    double* out;
+    const double* Mx;
+    const double* My;
+    const double* Mz;
+    const double* ce;
+    const double* gx;
+    const double* rdS;
+    const double* vs;
+    const double* ve;
+    const double* qw;
+    const double* qe;
+    const double* qs;
+    const double* qn;
+    long long nc;
+    double h2, alpha, beta;
+    int nlon, nlat, nlev, nt, lon_periodic, pad;
+} RsPPMVjpArgs;
+
 #endif
