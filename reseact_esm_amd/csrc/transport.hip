@@ -277,6 +277,9 @@ __device__ __forceinline__ int fcol(int i, int w, int periodic) {
 
 __host__ __device__ constexpr int fpitch(int nlev) { return (nlev + 4) | 1; }
 
+// Largest dynamic LDS a fused launch asks for (gfx950: 160 KB per workgroup).
+#define FUSED_LDS_MAX (160 * 1024)
+
 // One lane's view: pointers at level k0 of its own column and of the four
 // zonal neighbours, and at wz[k0]; x[j] is level k0 + j.
 struct FCol {
@@ -781,9 +784,10 @@ extern "C" {
 // Segment plan for the fused kernels on an nlon x nlev slab: plan[0..5] =
 // seg_w, seg_in, seg_halo, nseg, seg_lev, seg_chunks.  Returns 0 if the fused
 // kernels cannot take this grid (nlon < 4, or too many levels for a useful
-// segment: at most FL levels per lane and FT lanes per workgroup).
-int rs_transport_fused_plan(int nlon, int nlev, int* plan) {
-    const long long lds_doubles = (65536 - 256) / 8;   // dynamic LDS per workgroup
+// segment: at most FL levels per lane and FT lanes per workgroup, and the
+// tile within lds_bytes of LDS per workgroup).
+int rs_transport_fused_plan(int nlon, int nlev, long long lds_bytes, int* plan) {
+    const long long lds_doubles = (std::min<long long>(lds_bytes, FUSED_LDS_MAX) - 256) / 8;
     auto fits = [&](long long w, int* lev, int* chunks) {
         if (w < 4 || w > FT) return false;
         if (w * fpitch(nlev) + w + nlev + 4 > lds_doubles) return false;
@@ -822,7 +826,7 @@ hipError_t rs_launch_transport_fused(int adj, const RsTransArgs* a, hipStream_t 
         (long long)a->seg_w * a->seg_chunks > FT || (long long)a->seg_lev * a->seg_chunks < a->nlev)
         return hipErrorInvalidValue;
     const size_t lds = ((size_t)a->seg_w * fpitch(a->nlev) + a->seg_w + a->nlev + 4) * sizeof(double);
-    if (lds > 65536 - 256) return hipErrorInvalidValue;
+    if (lds > FUSED_LDS_MAX - 256) return hipErrorInvalidValue;
     dim3 grid((unsigned)a->nseg, (unsigned)(a->nlat * a->nt));
     if (adj) hipLaunchKernelGGL((rs_transport_fused<1, 0>), grid, dim3(FT), lds, st, *a);
     else if (a->negpart) hipLaunchKernelGGL((rs_transport_fused<0, 1>), grid, dim3(FT), lds, st, *a);

@@ -96,3 +96,29 @@ def test_fused_adaptive_same_sequence(hip, oracle, dims):
     for a in (cf.get_state(), cs.get_state()):
         assert float(np.max(np.abs(a - b)) / np.max(np.abs(b))) <= 1e-12
         assert float(np.linalg.norm(a - b) / np.linalg.norm(b)) <= 1e-12
+
+
+# more slabs than workgroups: each persistent workgroup runs several slabs,
+# streaming the next one into LDS while it computes (whole rows: 64 x 5 = 320
+# slabs; segmented: 4 segments x 40 x 2 = 320 slabs)
+@pytest.mark.parametrize("dims,nt", [((48, 64, 10), 5), ((360, 40, 72), 2)])
+def test_fused_many_slabs_bitwise(hip, oracle, dims, nt):
+    print(f"fused many slabs dims={dims} nt={nt}", flush=True)
+    p = problem(dims, flip=False, nt=nt, ns=nt + 1)
+    cf, cs = contexts(hip, p)
+    co = oracle.context().load(p)
+    ef = cf.ssprk43_step(0.0, 600.0, 1e-6, 1e-3)
+    es = cs.ssprk43_step(0.0, 600.0, 1e-6, 1e-3)
+    eo = co.ssprk43_step(0.0, 600.0, 1e-6, 1e-3)
+    uf, uo = cf.get_candidate(), co.get_candidate()
+    assert np.array_equal(uf, cs.get_candidate())
+    assert np.array_equal(uf, uo)
+    assert abs(ef - es) <= 1e-13 * max(es, 1e-300)
+    assert abs(ef - eo) <= 1e-12 * max(eo, 1.0)
+    lam = np.random.default_rng(12).standard_normal(p.ns * p.nc)
+    for c in (cf, cs, co):
+        c.set_lambda(lam)
+        c.ssprk43_vjp(0.0, 300.0)
+    lf, lo = cf.get_lambda(), co.get_lambda()
+    assert np.array_equal(lf, cs.get_lambda())
+    assert float(np.linalg.norm(lf - lo) / np.linalg.norm(lo)) <= 1e-12
