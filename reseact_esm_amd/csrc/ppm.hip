@@ -176,7 +176,8 @@ __global__ void __launch_bounds__(PB) rs_ppm_coefs(const double* __restrict__ dp
     }
 }
 
-// One SSPRK43 stage over the transported block.  KIND 0: out = a + h/2 L a;
+// One SSPRK43 stage over the transported block (KIND 3: out = L a, the
+// tendency alone, for the adjoint).  KIND 0: out = a + h/2 L a;
 // KIND 2 (third stage): the same, and out2 = a4 = 2/3 u + 1/3 out (u = b);
 // KIND 1 (last): unew = a4 + h/2 L a4 (a4 = c4), the error partial from
 // u = a and u3 = b, and (positive domain) the count of negative entries.
@@ -350,7 +351,9 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
             const double ac = (s == 0) ? mval : Q[ty + 3][tx + 3];
             const double du = h2 * f;
             const double un = ac + du;
-            if (KIND == 1) {
+            if (KIND == 3) {
+                po[g] = f;                 // the tendency itself (adjoint's m row)
+            } else if (KIND == 1) {
                 const double u = pa[g], u3 = pb[g];
                 const double a = C13 * u;
                 const double b = C23 * u3;
@@ -683,6 +686,292 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp(RsPPMVjpArgs A) {
     A.out[cell] = o;
 }
 
+// ---- adjoint of the tendency as three line sweeps --------------------------
+// The same J^T x as rs_ppm_vjp, with each face gradient computed once: a lane
+// walks one grid line (a longitude row, a latitude column or a vertical
+// column of one mixing ratio) face by face, and adds the face's contribution
+// W_f g_f[d] to the six cells of its stencil, held in a register window of
+// "logical" cells f-3 .. f+2 that slides by one per face.  A cell leaves the
+// window complete (no later face's stencil reaches back to it) and is
+// written once: deterministic, no atomics.  Logical cells outside the line
+// are inflow halos (dropped), periodic images (longitude: folded onto their
+// cell at the ends of the walk) or clamped vertical indices (added to the
+// end cell).  The sweeps run z, then y (adding), then x (finishing):
+//   G   = mu divM + acc_z;   G += acc_y;   out = [beta b +] x + h2 (G + acc_x)
+// and the air-mass row from the tendency F = L(at):
+//   out_m = [beta b_m +] x_m - h2 sum_s x_s F_s / m.
+
+// Local gradient of a face value over the five cells w[0..4] = the upwind
+// cell's neighbourhood c-2 .. c+2 (the six-cell stencil shifted by one for
+// M < 0); `right`: the value is the right edge of w[2] (M >= 0), else its left
+// edge.  Width factors (NU): slopes of w[1..3] (sR, sL), edges (w1, w2) and
+// (w2, w3) (e1, e2).  Branch-free in the flux sign.
+template <bool NU>
+__device__ __forceinline__ double face_grad5(const double* w, bool right, const double* sR,
+                                             const double* sL, const double* e1, const double* e2,
+                                             double* g) {
+    double sv[3], sg[3][3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double am = w[q], a0 = w[q + 1], ap = w[q + 2];
+        const double cR = NU ? sR[q] : 0.5, cL = NU ? sL[q] : 0.5;
+        sv[q] = NU ? slope_w(cR, cL, am, a0, ap) : slope_u(am, a0, ap);
+        const double dl = a0 - am, dr = ap - a0;
+        const bool on = dr * dl > 0.0;
+        const double da = NU ? cR * dr + cL * dl : 0.5 * (ap - am);
+        const double ax = fabs(da), y = 2.0 * fabs(dl), z = 2.0 * fabs(dr);
+        const bool zl = z < y;
+        const double in = zl ? z : y;
+        const bool lim = in < ax;
+        // gradient of the chosen term: da (cR, cL), 2 dr or 2 dl
+        double g0 = -cL, g1 = cL - cR, g2 = cR;
+        if (lim) { g0 = zl ? 0.0 : -2.0; g1 = 2.0 * (zl ? -1.0 : 1.0); g2 = zl ? 2.0 : 0.0; }
+        sg[q][0] = on ? g0 : 0.0;
+        sg[q][1] = on ? g1 : 0.0;
+        sg[q][2] = on ? g2 : 0.0;
+    }
+    double aL, aR, eL0, eL1, eL2, eL3, eR0, eR1, eR2, eR3;
+    if (NU) {
+        aL = edge_w(e1, w[1], w[2], sv[0], sv[1]);
+        aR = edge_w(e2, w[2], w[3], sv[1], sv[2]);
+        eL0 = 1.0 - e1[0]; eL1 = e1[0]; eL2 = e1[2]; eL3 = -e1[1];
+        eR0 = 1.0 - e2[0]; eR1 = e2[0]; eR2 = e2[2]; eR3 = -e2[1];
+    } else {
+        aL = edge_u(w[1], w[2], sv[0], sv[1]);
+        aR = edge_u(w[2], w[3], sv[1], sv[2]);
+        eL0 = eR0 = 0.5; eL1 = eR1 = 0.5; eL2 = eR2 = C16; eL3 = eR3 = -C16;
+    }
+    const double a0 = w[2];
+    // limiter branch -> adjoints (bc, bL, bR) of (a0, aL, aR) for the output
+    const bool flat = (aR - a0) * (a0 - aL) <= 0.0;
+    const double d = aR - aL;
+    const double a6 = 6.0 * (a0 - 0.5 * (aL + aR));
+    const double dd = d * d;
+    const double da6 = d * a6;
+    const bool resetL = da6 > dd, resetR = !resetL && (-dd > da6);
+    double bc, bL, bR;
+    if (flat) { bc = 1.0; bL = 0.0; bR = 0.0; }
+    else if (right) { bc = resetR ? 3.0 : 0.0; bL = resetR ? -2.0 : 0.0; bR = resetR ? 0.0 : 1.0; }
+    else { bc = resetL ? 3.0 : 0.0; bR = resetL ? -2.0 : 0.0; bL = resetL ? 0.0 : 1.0; }
+    // the face value (the forward's limiter)
+    double lL = aL, lR = aR;
+    limit(a0, lL, lR);
+    g[0] = 0.0; g[1] = 0.0; g[2] = bc; g[3] = 0.0; g[4] = 0.0;
+    g[1] += bL * eL0;
+    g[2] += bL * eL1 + bR * eR0;
+    g[3] += bR * eR1;
+    const double bs0 = bL * eL2, bs1 = bL * eL3 + bR * eR2, bs2 = bR * eR3;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        g[q] += bs0 * sg[0][q];
+        g[q + 1] += bs1 * sg[1][q];
+        g[q + 2] += bs2 * sg[2][q];
+    }
+    return right ? lR : lL;
+}
+
+// Face divergence per cell, div(M) in the operator's order (species-independent).
+__global__ void __launch_bounds__(PB) rs_ppm_divm(RsPPMVjpArgs A, double* __restrict__ divm) {
+    const int nlon = A.nlon, nlat = A.nlat;
+    const long long plane = (long long)nlon * nlat;
+    const long long cell = (long long)blockIdx.x * PB + threadIdx.x;
+    if (cell >= A.nc) return;
+    const int i = (int)(cell % nlon), j = (int)((cell / nlon) % nlat), k = (int)(cell / plane);
+    const long long col = (long long)j * nlon + i;
+    const double* mxr = A.Mx + ((long long)k * nlat + j) * (nlon + 1);
+    divm[cell] = div6(mxr[i], mxr[i + 1], A.My[((long long)k * (nlat + 1) + j) * nlon + i],
+                      A.My[((long long)k * (nlat + 1) + j + 1) * nlon + i], A.Mz[(long long)k * plane + col],
+                      A.Mz[(long long)(k + 1) * plane + col], A.gx[j], A.ce[j], A.ce[j + 1], A.rdS[j]);
+}
+
+// One line sweep.  DIR 2: vertical columns (lane = species-1, column), writes
+// G = mu divM + acc; DIR 1: latitude columns (lane = species-1, level, lon),
+// adds to G; DIR 0: longitude rows (lane = species-1, level, lat), writes out.
+template <int DIR>
+__global__ void __launch_bounds__(PB) rs_ppm_vjp_sweep(RsPPMVjpArgs A, const double* __restrict__ divm,
+                                                       double* __restrict__ G) {
+    const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev, nq = A.nt - 1;
+    const long long nc = A.nc, plane = (long long)nlon * nlat;
+    const long long lane = (long long)blockIdx.x * PB + threadIdx.x;
+    const int n = DIR == 0 ? nlon : (DIR == 1 ? nlat : nlev);          // cells on the line
+    const long long nlines = (long long)nq * (DIR == 0 ? (long long)nlev * nlat
+                                              : (DIR == 1 ? (long long)nlev * nlon : plane));
+    if (lane >= nlines) return;
+    int s, k = 0, j = 0, i = 0;
+    long long col = 0;
+    if (DIR == 2) { s = 1 + (int)(lane / plane); col = lane % plane; }
+    else if (DIR == 1) { s = 1 + (int)(lane / ((long long)nlev * nlon)); const long long r = lane % ((long long)nlev * nlon); k = (int)(r / nlon); i = (int)(r % nlon); }
+    else { s = 1 + (int)(lane / ((long long)nlev * nlat)); const long long r = lane % ((long long)nlev * nlat); k = (int)(r / nlat); j = (int)(r % nlat); }
+    const long long sb = (long long)s * nc;
+    const double* __restrict__ q = A.at + sb;
+    const double* __restrict__ la = A.a + sb;
+    const double* __restrict__ m = A.at;
+    const int sq1 = s - 1;
+    const bool periodic = DIR == 0 && A.lon_periodic;
+    // cell index of position x (0 <= x < n) on the line
+    auto cidx = [&](int x) -> long long {
+        if (DIR == 2) return (long long)x * plane + col;
+        if (DIR == 1) return (long long)k * plane + (long long)x * nlon + i;
+        return (long long)k * plane + (long long)j * nlon + x;
+    };
+    // value of logical position x (halo, wrap or clamp)
+    auto val = [&](int x) -> double {
+        if (DIR == 2) return q[cidx(clampi(x, 0, nlev - 1))];
+        if (DIR == 1) {
+            if (x < 0) return A.qs[((long long)sq1 * nlev + k) * nlon + i];
+            if (x >= nlat) return A.qn[((long long)sq1 * nlev + k) * nlon + i];
+            return q[cidx(x)];
+        }
+        if (periodic) return q[cidx(wrapi(x, nlon))];
+        if (x < 0) return A.qw[((long long)sq1 * nlev + k) * nlat + j];
+        if (x >= nlon) return A.qe[((long long)sq1 * nlev + k) * nlat + j];
+        return q[cidx(x)];
+    };
+    const double alpha = A.alpha;
+    auto mu = [&](int x) -> double {          // x in [0, n)
+        const long long c = cidx(x);
+        const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
+        return xv / m[c];
+    };
+    auto flux = [&](int F) -> double {
+        if (DIR == 2) return A.Mz[(long long)F * plane + col];
+        if (DIR == 1) return A.My[((long long)k * (nlat + 1) + F) * nlon + i];
+        return A.Mx[((long long)k * nlat + j) * (nlon + 1) + F];
+    };
+    // divergence coefficient of face F in cell x (the face flux's factor)
+    auto coefR = [&](int F) -> double {       // cell F (face F is its lower face)
+        if (DIR == 2) return 1.0;
+        if (DIR == 1) return A.ce[F] * A.rdS[F];
+        return A.gx[j];
+    };
+    auto coefL = [&](int F) -> double {       // cell F-1 (face F is its upper face)
+        if (DIR == 2) return 1.0;
+        if (DIR == 1) return A.ce[F] * A.rdS[F - 1];
+        return A.gx[j];
+    };
+    const long long nrs = (long long)(nlev + 4) * plane, nre = (long long)(nlev + 3) * plane;
+    // sliding windows: v[d] = value of logical F-3+d, acc[d] = contributions
+    // gathered so far for logical F-3+d
+    double v[6], acc[6], early[3] = {0.0, 0.0, 0.0}, head[3] = {0.0, 0.0, 0.0}, bot = 0.0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) { v[d] = val(-3 + d); acc[d] = 0.0; }
+    double mul = 0.0, mur = mu(0);            // mu of cells F-1 and F (0 outside the line)
+    // the logical cell leaving the window: write, fold or drop it
+    auto emit = [&](int L, double a) {
+        if (DIR == 2) {
+            if (L < 0) { bot += a; return; }
+            if (L == 0) a = a + bot;
+        } else if (periodic) {
+            if (L < 0) { early[L + 3] = a; return; }
+            if (L < 3) { head[L] = a; return; }
+        } else if (L < 0 || L >= n) {
+            return;
+        }
+        const long long c = cidx(L);
+        if (DIR == 2) {
+            G[sb - nc + c] = mu(L) * divm[c] + a;
+        } else if (DIR == 1) {
+            G[sb - nc + c] = G[sb - nc + c] + a;
+        } else {
+            const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
+            double o = xv + A.h2 * (G[sb - nc + c] + a);
+            if (A.b) o = A.beta * A.b[sb + c] + o;
+            A.out[sb + c] = o;
+        }
+    };
+    for (int F = 0; F <= n; ++F) {
+        const double M = flux(F);
+        if (M != 0.0) {
+            const bool right = M >= 0.0;
+            double w[5], g[5];
+#pragma unroll
+            for (int d = 0; d < 5; ++d) w[d] = right ? v[d] : v[d + 1];
+            if (DIR == 2) {
+                // width factors of logical cells c-1..c+1 (rows L + 2) and
+                // edges (c-1, c), (c, c+1), c = F-1 (M >= 0) or F
+                const int c = right ? F - 1 : F;
+                double sR[3], sL[3], e1[3], e2[3];
+#pragma unroll
+                for (int q3 = 0; q3 < 3; ++q3) {
+                    const long long r = (long long)(c - 1 + q3 + 2) * plane + col;
+                    sR[q3] = A.vs[r];
+                    sL[q3] = A.vs[nrs + r];
+                }
+                const long long r1 = (long long)(c - 1 + 2) * plane + col, r2 = (long long)(c + 2) * plane + col;
+#pragma unroll
+                for (int q3 = 0; q3 < 3; ++q3) { e1[q3] = A.ve[q3 * nre + r1]; e2[q3] = A.ve[q3 * nre + r2]; }
+                face_grad5<true>(w, right, sR, sL, e1, e2, g);
+            } else {
+                face_grad5<false>(w, right, nullptr, nullptr, nullptr, nullptr, g);
+            }
+            const double W = M * ((F < n ? mur * coefR(F) : 0.0) - (F >= 1 ? mul * coefL(F) : 0.0));
+#pragma unroll
+            for (int d = 0; d < 5; ++d) {
+                const double t = W * g[d];
+                if (right) acc[d] += t;
+                else acc[d + 1] += t;
+            }
+        }
+        if (F == n) break;
+        emit(F - 3, acc[0]);
+#pragma unroll
+        for (int d = 0; d < 5; ++d) { v[d] = v[d + 1]; acc[d] = acc[d + 1]; }
+        v[5] = val(F + 3);
+        acc[5] = 0.0;
+        mul = mur;
+        mur = F + 1 < n ? mu(F + 1) : 0.0;
+    }
+    // the window holds logical n-3 .. n+2, all complete
+    if (DIR == 2) {
+        // clamped tops: logical n .. n+2 belong to the top cell
+        const double top = (acc[3] + acc[4]) + acc[5];
+        for (int d = 0; d < 3; ++d) {
+            const int L = n - 3 + d;
+            emit(L, d == 2 ? acc[d] + top : acc[d]);
+        }
+    } else if (periodic) {
+        for (int d = 0; d < 3; ++d) {
+            const int L = n - 3 + d;               // real n-3+d, its image L - n = d - 3
+            const long long c = cidx(L);
+            const double a = acc[d] + early[d];
+            const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
+            double o = xv + A.h2 * (G[sb - nc + c] + a);
+            if (A.b) o = A.beta * A.b[sb + c] + o;
+            A.out[sb + c] = o;
+        }
+        for (int d = 0; d < 3; ++d) {
+            const long long c = cidx(d);           // real d, its image n + d
+            const double a = head[d] + acc[3 + d];
+            const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
+            double o = xv + A.h2 * (G[sb - nc + c] + a);
+            if (A.b) o = A.beta * A.b[sb + c] + o;
+            A.out[sb + c] = o;
+        }
+    } else {
+        for (int d = 0; d < 6; ++d) emit(n - 3 + d, acc[d]);
+    }
+}
+
+// out_m = [beta b_m +] x_m - h2 sum_s x_s F_s / m, F the tendency at `at`.
+__global__ void __launch_bounds__(PB) rs_ppm_vjp_mrow(RsPPMVjpArgs A, const double* __restrict__ F) {
+    const long long cell = (long long)blockIdx.x * PB + threadIdx.x;
+    if (cell >= A.nc) return;
+    const long long nc = A.nc;
+    const double m = A.at[cell];
+    const double alpha = A.alpha;
+    double msum = 0.0;
+    for (int s = 1; s < A.nt; ++s) {
+        const long long g = (long long)s * nc + cell;
+        const double xs = alpha == 1.0 ? A.a[g] : alpha * A.a[g];
+        const double t = xs * F[g];
+        msum -= t / m;
+    }
+    const double xm = alpha == 1.0 ? A.a[cell] : alpha * A.a[cell];
+    double o = xm + A.h2 * msum;
+    if (A.b) o = A.beta * A.b[cell] + o;
+    A.out[cell] = o;
+}
+
 long long rs_ppm_nblocks(int nlon, int nlat, int nlev) {
     return (long long)((nlon + PTX - 1) / PTX) * ((nlat + PTY - 1) / PTY) * nlev;
 }
@@ -711,11 +1000,34 @@ hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st) {
                     (unsigned)a->nlev);
     if (kind == 1) hipLaunchKernelGGL(rs_ppm_stage<1>, grid, dim3(PB), 0, st, *a);
     else if (kind == 2) hipLaunchKernelGGL(rs_ppm_stage<2>, grid, dim3(PB), 0, st, *a);
+    else if (kind == 3) hipLaunchKernelGGL(rs_ppm_stage<3>, grid, dim3(PB), 0, st, *a);
     else hipLaunchKernelGGL(rs_ppm_stage<0>, grid, dim3(PB), 0, st, *a);
     return hipGetLastError();
 }
 
 hipError_t rs_launch_ppm_vjp(const RsPPMVjpArgs* a, hipStream_t st) {
     hipLaunchKernelGGL(rs_ppm_vjp, dim3((unsigned)((a->nc + PB - 1) / PB)), dim3(PB), 0, st, *a);
+    return hipGetLastError();
+}
+
+// The sweep form of rs_launch_ppm_vjp: F (NT*NC) receives the tendency at
+// a->at (computed by the caller with stage kind 3), G (NQ*NC) and divm (NC)
+// are scratch.  Periodic rows need nlon >= 6 (the gather kernel otherwise).
+hipError_t rs_launch_ppm_vjp_sweeps(const RsPPMVjpArgs* a, const double* F, double* G,
+                                    double* divm, hipStream_t st) {
+    const long long nq = a->nt - 1;
+    if (a->lon_periodic && a->nlon < 6) return hipErrorInvalidValue;
+    const long long plane = (long long)a->nlon * a->nlat;
+    auto blocks = [](long long n) { return dim3((unsigned)((n + PB - 1) / PB)); };
+    if (nq > 0) {
+        hipLaunchKernelGGL(rs_ppm_divm, blocks(a->nc), dim3(PB), 0, st, *a, divm);
+        hipLaunchKernelGGL(rs_ppm_vjp_sweep<2>, blocks(nq * plane), dim3(PB), 0, st, *a,
+                           (const double*)divm, G);
+        hipLaunchKernelGGL(rs_ppm_vjp_sweep<1>, blocks(nq * a->nlev * a->nlon), dim3(PB), 0, st, *a,
+                           (const double*)divm, G);
+        hipLaunchKernelGGL(rs_ppm_vjp_sweep<0>, blocks(nq * a->nlev * a->nlat), dim3(PB), 0, st, *a,
+                           (const double*)divm, G);
+    }
+    hipLaunchKernelGGL(rs_ppm_vjp_mrow, blocks(a->nc), dim3(PB), 0, st, *a, F);
     return hipGetLastError();
 }

@@ -66,6 +66,8 @@ hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlo
                                int nlev, hipStream_t st);
 hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st);
 hipError_t rs_launch_ppm_vjp(const RsPPMVjpArgs* a, hipStream_t st);
+hipError_t rs_launch_ppm_vjp_sweeps(const RsPPMVjpArgs* a, const double* F, double* G,
+                                    double* divm, hipStream_t st);
 hipError_t rs_launch_ppm_source(const double* Mx, const double* My, const double* dp,
                                 const double* gx, const double* ce, const double* rdS,
                                 const double* dPSdt, double* msrc, int nlon, int nlat, int nlev,
@@ -171,6 +173,7 @@ struct rs_ctx {
     int lon_periodic = 1;
     bool have_ppm_geom = false, have_ppm_flux = false;
     DevArray<double> pMx, pMy, pMz, pdp, pce, pgx, prdS, pvs, pve, pqw, pqe, pqs, pqn, pZ;
+    DevArray<double> pF, pG, pdivm;   // PPM adjoint scratch: tendency, partial sums, div(M)
     DevArray<double> pdPS, pmsrc;   // pressure fixer: dPS/dt per column, air-mass source per cell
     bool have_fixer = false;
     DevArray<double> red;
@@ -889,11 +892,28 @@ void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
     v.h2 = dt / 2;
     v.nlon = c->nlon; v.nlat = c->nlat; v.nlev = c->nlev; v.nt = c->nt;
     v.lon_periodic = c->lon_periodic;
+    // line sweeps (each face gradient once), or the per-cell gather on a
+    // periodic row shorter than a stencil
+    const bool sweeps = !(c->lon_periodic && c->nlon < 6);
+    if (sweeps) {
+        c->pF.ensure((size_t)c->nt * c->nc);
+        c->pG.ensure((size_t)std::max(c->nt - 1, 1) * c->nc);
+        c->pdivm.ensure((size_t)c->nc);
+    }
+    RsPPMArgs tf = ppm_args(c, dt);
+    tf.out = c->pF.p;
     auto apply = [&](const double* at, const double* in, double alpha, const double* b, double beta,
                      double* out) {
         v.at = at; v.a = in; v.alpha = alpha; v.b = b; v.beta = beta; v.out = out;
-        ProfScope ps(c, P_TPPMADJ);
-        HIPCHK(rs_launch_ppm_vjp(&v, c->stream));
+        if (sweeps) {
+            tf.a = at;
+            { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(3, &tf, c->stream)); }
+            ProfScope ps(c, P_TPPMADJ);
+            HIPCHK(rs_launch_ppm_vjp_sweeps(&v, c->pF.p, c->pG.p, c->pdivm.p, c->stream));
+        } else {
+            ProfScope ps(c, P_TPPMADJ);
+            HIPCHK(rs_launch_ppm_vjp(&v, c->stream));
+        }
     };
     apply(c->pZ.p, c->lam.p, 1.0, nullptr, 0.0, c->LB.p);
     apply(c->Y.p, c->LB.p, 1.0 / 3.0, nullptr, 0.0, c->LC.p);

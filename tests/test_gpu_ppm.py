@@ -96,14 +96,23 @@ def _relnorm(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("name", ["regional-20x12x9", "global-48x24x10"])
+VJP_CASES = {
+    "regional-20x12x9": regional_random,
+    "global-48x24x10": lambda: P.ppm_global(48, 24, 10, nq=3, seed=9),
+    # a periodic row shorter than a stencil: the per-cell gather kernel
+    "global-4x6x5": lambda: P.ppm_global(4, 6, 5, nq=2, seed=3),
+}
+
+
+@pytest.mark.parametrize("name", list(VJP_CASES))
 def test_ppm_step_vjp(hip, oracle, name):
     """One SSPRK43 step VJP of the PPM transport at the context's state: the
     stage states are the forward's bits, the adjoint sums run in another order
-    (gather on the GPU, scatter in the oracle)."""
+    (line sweeps or a per-cell gather on the GPU, a scatter in the oracle)."""
     print(f"ppm step vjp {name}", flush=True)
-    p = CASES[name]()
-    dt = 300.0 if name.startswith("global") else 0.03
+    p = VJP_CASES[name]()
+    # steps of a few tenths of the Courant limit, so J^T lam is not lost in lam
+    dt = {"regional-20x12x9": 0.03, "global-48x24x10": 3000.0, "global-4x6x5": 50000.0}[name]
     lam = np.random.default_rng(21).standard_normal(p.ns * p.nc)
     out = []
     for lib in (hip, oracle):
