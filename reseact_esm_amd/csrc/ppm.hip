@@ -369,6 +369,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
                 po[g] = A.clamp ? dmax(un, 0.0) : un;
             } else {
                 po[g] = un;
+                if (A.outf) A.outf[g] = f;
                 if (KIND == 2) {
                     const double a = C23 * pb[g];
                     const double b = C13 * un;
@@ -687,16 +688,16 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp(RsPPMVjpArgs A) {
 }
 
 // ---- adjoint of the tendency as three line sweeps --------------------------
-// The same J^T x as rs_ppm_vjp, with each face gradient computed once: a lane
-// walks one grid line (a longitude row, a latitude column or a vertical
-// column of one mixing ratio) face by face, and adds the face's contribution
-// W_f g_f[d] to the six cells of its stencil, held in a register window of
-// "logical" cells f-3 .. f+2 that slides by one per face.  A cell leaves the
-// window complete (no later face's stencil reaches back to it) and is
-// written once: deterministic, no atomics.  Logical cells outside the line
-// are inflow halos (dropped), periodic images (longitude: folded onto their
-// cell at the ends of the walk) or clamped vertical indices (added to the
-// end cell).  The sweeps run z, then y (adding), then x (finishing):
+// The same J^T x as rs_ppm_vjp, with each face gradient computed once.  In
+// the vertical and in latitude a lane walks one grid line of one mixing
+// ratio face by face, and adds the face's contribution W_f g_f[d] to the six
+// cells of its stencil, held in a register window of "logical" cells f-3 ..
+// f+2 that slides by one per face.  A cell leaves the window complete (no
+// later face's stencil reaches back to it) and is written once:
+// deterministic, no atomics.  Logical cells outside the line are inflow halos
+// (dropped) or clamped vertical indices (added to the end cell).  Longitude
+// rows (strided across lanes) run on LDS tiles instead (rs_ppm_vjp_xtile).
+// Order: z, then y (adding), then x (finishing):
 //   G   = mu divM + acc_z;   G += acc_y;   out = [beta b +] x + h2 (G + acc_x)
 // and the air-mass row from the tendency F = L(at):
 //   out_m = [beta b_m +] x_m - h2 sum_s x_s F_s / m.
@@ -786,45 +787,43 @@ __global__ void __launch_bounds__(PB) rs_ppm_divm(RsPPMVjpArgs A, double* __rest
 
 // One line sweep.  DIR 2: vertical columns (lane = species-1, column), writes
 // G = mu divM + acc; DIR 1: latitude columns (lane = species-1, level, lon),
-// adds to G; DIR 0: longitude rows (lane = species-1, level, lat), writes out.
+// adds to G.  Consecutive lanes are consecutive longitudes: coalesced.
 template <int DIR>
 __global__ void __launch_bounds__(PB) rs_ppm_vjp_sweep(RsPPMVjpArgs A, const double* __restrict__ divm,
                                                        double* __restrict__ G) {
+    static_assert(DIR == 1 || DIR == 2, "latitude or vertical sweep");
     const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev, nq = A.nt - 1;
     const long long nc = A.nc, plane = (long long)nlon * nlat;
     const long long lane = (long long)blockIdx.x * PB + threadIdx.x;
-    const int n = DIR == 0 ? nlon : (DIR == 1 ? nlat : nlev);          // cells on the line
-    const long long nlines = (long long)nq * (DIR == 0 ? (long long)nlev * nlat
-                                              : (DIR == 1 ? (long long)nlev * nlon : plane));
+    const int n = DIR == 1 ? nlat : nlev;                             // cells on the line
+    const long long nlines = (long long)nq * (DIR == 1 ? (long long)nlev * nlon : plane);
     if (lane >= nlines) return;
-    int s, k = 0, j = 0, i = 0;
+    int s, k = 0, i = 0;
     long long col = 0;
-    if (DIR == 2) { s = 1 + (int)(lane / plane); col = lane % plane; }
-    else if (DIR == 1) { s = 1 + (int)(lane / ((long long)nlev * nlon)); const long long r = lane % ((long long)nlev * nlon); k = (int)(r / nlon); i = (int)(r % nlon); }
-    else { s = 1 + (int)(lane / ((long long)nlev * nlat)); const long long r = lane % ((long long)nlev * nlat); k = (int)(r / nlat); j = (int)(r % nlat); }
+    if (DIR == 2) {
+        s = 1 + (int)(lane / plane);
+        col = lane % plane;
+    } else {
+        s = 1 + (int)(lane / ((long long)nlev * nlon));
+        const long long r = lane % ((long long)nlev * nlon);
+        k = (int)(r / nlon);
+        i = (int)(r % nlon);
+    }
     const long long sb = (long long)s * nc;
     const double* __restrict__ q = A.at + sb;
     const double* __restrict__ la = A.a + sb;
     const double* __restrict__ m = A.at;
     const int sq1 = s - 1;
-    const bool periodic = DIR == 0 && A.lon_periodic;
     // cell index of position x (0 <= x < n) on the line
     auto cidx = [&](int x) -> long long {
         if (DIR == 2) return (long long)x * plane + col;
-        if (DIR == 1) return (long long)k * plane + (long long)x * nlon + i;
-        return (long long)k * plane + (long long)j * nlon + x;
+        return (long long)k * plane + (long long)x * nlon + i;
     };
-    // value of logical position x (halo, wrap or clamp)
+    // value of logical position x (inflow halo or clamp)
     auto val = [&](int x) -> double {
         if (DIR == 2) return q[cidx(clampi(x, 0, nlev - 1))];
-        if (DIR == 1) {
-            if (x < 0) return A.qs[((long long)sq1 * nlev + k) * nlon + i];
-            if (x >= nlat) return A.qn[((long long)sq1 * nlev + k) * nlon + i];
-            return q[cidx(x)];
-        }
-        if (periodic) return q[cidx(wrapi(x, nlon))];
-        if (x < 0) return A.qw[((long long)sq1 * nlev + k) * nlat + j];
-        if (x >= nlon) return A.qe[((long long)sq1 * nlev + k) * nlat + j];
+        if (x < 0) return A.qs[((long long)sq1 * nlev + k) * nlon + i];
+        if (x >= nlat) return A.qn[((long long)sq1 * nlev + k) * nlon + i];
         return q[cidx(x)];
     };
     const double alpha = A.alpha;
@@ -835,49 +834,31 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_sweep(RsPPMVjpArgs A, const dou
     };
     auto flux = [&](int F) -> double {
         if (DIR == 2) return A.Mz[(long long)F * plane + col];
-        if (DIR == 1) return A.My[((long long)k * (nlat + 1) + F) * nlon + i];
-        return A.Mx[((long long)k * nlat + j) * (nlon + 1) + F];
+        return A.My[((long long)k * (nlat + 1) + F) * nlon + i];
     };
-    // divergence coefficient of face F in cell x (the face flux's factor)
-    auto coefR = [&](int F) -> double {       // cell F (face F is its lower face)
-        if (DIR == 2) return 1.0;
-        if (DIR == 1) return A.ce[F] * A.rdS[F];
-        return A.gx[j];
-    };
-    auto coefL = [&](int F) -> double {       // cell F-1 (face F is its upper face)
-        if (DIR == 2) return 1.0;
-        if (DIR == 1) return A.ce[F] * A.rdS[F - 1];
-        return A.gx[j];
-    };
+    // divergence coefficient of face F in cell F (its lower face) and in
+    // cell F-1 (its upper face)
+    auto coefR = [&](int F) -> double { return DIR == 2 ? 1.0 : A.ce[F] * A.rdS[F]; };
+    auto coefL = [&](int F) -> double { return DIR == 2 ? 1.0 : A.ce[F] * A.rdS[F - 1]; };
     const long long nrs = (long long)(nlev + 4) * plane, nre = (long long)(nlev + 3) * plane;
     // sliding windows: v[d] = value of logical F-3+d, acc[d] = contributions
     // gathered so far for logical F-3+d
-    double v[6], acc[6], early[3] = {0.0, 0.0, 0.0}, head[3] = {0.0, 0.0, 0.0}, bot = 0.0;
+    double v[6], acc[6], bot = 0.0;
 #pragma unroll
     for (int d = 0; d < 6; ++d) { v[d] = val(-3 + d); acc[d] = 0.0; }
     double mul = 0.0, mur = mu(0);            // mu of cells F-1 and F (0 outside the line)
-    // the logical cell leaving the window: write, fold or drop it
+    // the logical cell leaving the window: write it, fold it (clamped
+    // vertical indices below the surface) or drop it (inflow halo)
     auto emit = [&](int L, double a) {
         if (DIR == 2) {
             if (L < 0) { bot += a; return; }
             if (L == 0) a = a + bot;
-        } else if (periodic) {
-            if (L < 0) { early[L + 3] = a; return; }
-            if (L < 3) { head[L] = a; return; }
         } else if (L < 0 || L >= n) {
             return;
         }
         const long long c = cidx(L);
-        if (DIR == 2) {
-            G[sb - nc + c] = mu(L) * divm[c] + a;
-        } else if (DIR == 1) {
-            G[sb - nc + c] = G[sb - nc + c] + a;
-        } else {
-            const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
-            double o = xv + A.h2 * (G[sb - nc + c] + a);
-            if (A.b) o = A.beta * A.b[sb + c] + o;
-            A.out[sb + c] = o;
-        }
+        if (DIR == 2) G[sb - nc + c] = mu(L) * divm[c] + a;
+        else G[sb - nc + c] = G[sb - nc + c] + a;
     };
     for (int F = 0; F <= n; ++F) {
         const double M = flux(F);
@@ -925,31 +906,119 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_sweep(RsPPMVjpArgs A, const dou
     if (DIR == 2) {
         // clamped tops: logical n .. n+2 belong to the top cell
         const double top = (acc[3] + acc[4]) + acc[5];
-        for (int d = 0; d < 3; ++d) {
-            const int L = n - 3 + d;
-            emit(L, d == 2 ? acc[d] + top : acc[d]);
-        }
-    } else if (periodic) {
-        for (int d = 0; d < 3; ++d) {
-            const int L = n - 3 + d;               // real n-3+d, its image L - n = d - 3
-            const long long c = cidx(L);
-            const double a = acc[d] + early[d];
-            const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
-            double o = xv + A.h2 * (G[sb - nc + c] + a);
-            if (A.b) o = A.beta * A.b[sb + c] + o;
-            A.out[sb + c] = o;
-        }
-        for (int d = 0; d < 3; ++d) {
-            const long long c = cidx(d);           // real d, its image n + d
-            const double a = head[d] + acc[3 + d];
-            const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
-            double o = xv + A.h2 * (G[sb - nc + c] + a);
-            if (A.b) o = A.beta * A.b[sb + c] + o;
-            A.out[sb + c] = o;
-        }
+        for (int d = 0; d < 3; ++d) emit(n - 3 + d, d == 2 ? acc[d] + top : acc[d]);
     } else {
         for (int d = 0; d < 6; ++d) emit(n - 3 + d, acc[d]);
     }
+}
+
+// Longitude part of J^T x on LDS tiles, finishing the output: a workgroup is
+// a tile of PTX longitudes x PTY latitudes at one level of one mixing ratio.
+// It stages the tile's values (5-cell halo: wrap or inflow) and mu (3-cell
+// halo, 0 outside an open row), computes each longitude face of the logical
+// range i0-2 .. i0+PTX+2 once (W_f g_f over its six stencil cells, into LDS),
+// then each cell sums the slots of its six faces:
+//   out = [beta b +] x + h2 (G + acc_x).
+// Periodic rows: logical face FL is the real face FL mod nlon; face 0 (the
+// west face of cell 0, flux Mx[0]) also carries face nlon (the east face of
+// cell nlon-1, flux Mx[nlon]), whose stencil is the same modulo nlon, so
+// every (face, stencil cell) pair is counted once (nlon >= 6).
+constexpr int XFW = PTX + 5;                  // logical faces per tile row
+__global__ void __launch_bounds__(PB) rs_ppm_vjp_xtile(RsPPMVjpArgs A, double* __restrict__ G) {
+    __shared__ double Qx[PTY][PTX + 10];
+    __shared__ double MU[PTY][PTX + 6];
+    __shared__ double CF[PTY][XFW][6];
+    const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev;
+    const long long nc = A.nc, plane = (long long)nlon * nlat;
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * PTX, j0 = blockIdx.y * PTY;
+    const int k = (int)(blockIdx.z % nlev), s = 1 + (int)(blockIdx.z / nlev);
+    const int sq1 = s - 1;
+    const long long sb = (long long)s * nc;
+    const double* __restrict__ q = A.at + sb;
+    const double* __restrict__ la = A.a + sb;
+    const double alpha = A.alpha;
+    const bool periodic = A.lon_periodic;
+    // staging: values at columns i0-5 .. i0+PTX+4, mu at i0-3 .. i0+PTX+2
+    for (int x = tid; x < PTY * (PTX + 10); x += PB) {
+        const int r = x / (PTX + 10), cc = x - r * (PTX + 10);
+        const int gj = j0 + r, gi = i0 - 5 + cc;
+        double v = 0.0;
+        if (gj < nlat) {
+            const long long rowb = (long long)k * plane + (long long)gj * nlon;
+            if (periodic) v = q[rowb + wrapi(gi, nlon)];
+            else if (gi < 0) v = A.qw[((long long)sq1 * nlev + k) * nlat + gj];
+            else if (gi >= nlon) v = A.qe[((long long)sq1 * nlev + k) * nlat + gj];
+            else v = q[rowb + gi];
+        }
+        Qx[r][cc] = v;
+    }
+    for (int x = tid; x < PTY * (PTX + 6); x += PB) {
+        const int r = x / (PTX + 6), cc = x - r * (PTX + 6);
+        const int gj = j0 + r, gi = i0 - 3 + cc;
+        double v = 0.0;
+        if (gj < nlat && (periodic || (gi >= 0 && gi < nlon))) {
+            const long long c = (long long)k * plane + (long long)gj * nlon + (periodic ? wrapi(gi, nlon) : gi);
+            const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
+            v = xv / A.at[c];
+        }
+        MU[r][cc] = v;
+    }
+    __syncthreads();
+    const double* mxb = A.Mx + (long long)k * nlat * (nlon + 1);
+    for (int x = tid; x < PTY * XFW; x += PB) {
+        const int r = x / XFW, fo = x - r * XFW;
+        const int gj = j0 + r, FL = i0 - 2 + fo;         // logical face
+        double cf[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (gj < nlat) {
+            const double gx = A.gx[gj];
+            const double* mxr = mxb + (long long)gj * (nlon + 1);
+            // stencil cells FL-3 .. FL+2 sit at Qx[r][fo .. fo+5]; mu of
+            // cells FL-1, FL at MU[r][fo], MU[r][fo + 1]
+            const double* v = &Qx[r][fo];
+            auto add_face = [&](double M, double W) {
+                if (M == 0.0 || W == 0.0) return;
+                const bool right = M >= 0.0;
+                double w[5], g[5];
+#pragma unroll
+                for (int d = 0; d < 5; ++d) w[d] = right ? v[d] : v[d + 1];
+                face_grad5<false>(w, right, nullptr, nullptr, nullptr, nullptr, g);
+                // slot d of the six stencil cells: g[d] (M >= 0) or g[d-1]
+#pragma unroll
+                for (int d = 0; d < 6; ++d) {
+                    const double gd = right ? (d < 5 ? g[d] : 0.0) : (d > 0 ? g[d - 1] : 0.0);
+                    cf[d] += W * gd;
+                }
+            };
+            if (periodic) {
+                const int F = wrapi(FL, nlon);
+                const double M = mxr[F];
+                add_face(M, M * ((MU[r][fo + 1] * gx) - (F >= 1 ? MU[r][fo] * gx : 0.0)));
+                if (F == 0) {
+                    const double Mn = mxr[nlon];
+                    add_face(Mn, Mn * (0.0 - MU[r][fo] * gx));
+                }
+            } else if (FL >= 0 && FL <= nlon) {
+                const double M = mxr[FL];
+                add_face(M, M * ((MU[r][fo + 1] * gx) - (MU[r][fo] * gx)));
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 6; ++d) CF[r][fo][d] = cf[d];
+    }
+    __syncthreads();
+    const int tx = tid % PTX, ty = tid / PTX;
+    const int i = i0 + tx, j = j0 + ty;
+    if (i >= nlon || j >= nlat) return;
+    // faces FL = i-2 .. i+3 are fo = tx .. tx+5; this cell is slot i - FL + 3
+    double a = 0.0;
+#pragma unroll
+    for (int o = 0; o < 6; ++o) a += CF[ty][tx + o][5 - o];
+    const long long c = (long long)k * plane + (long long)j * nlon + i;
+    const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
+    double o = xv + A.h2 * (G[sb - nc + c] + a);
+    if (A.b) o = A.beta * A.b[sb + c] + o;
+    A.out[sb + c] = o;
 }
 
 // out_m = [beta b_m +] x_m - h2 sum_s x_s F_s / m, F the tendency at `at`.
@@ -1025,8 +1094,9 @@ hipError_t rs_launch_ppm_vjp_sweeps(const RsPPMVjpArgs* a, const double* F, doub
                            (const double*)divm, G);
         hipLaunchKernelGGL(rs_ppm_vjp_sweep<1>, blocks(nq * a->nlev * a->nlon), dim3(PB), 0, st, *a,
                            (const double*)divm, G);
-        hipLaunchKernelGGL(rs_ppm_vjp_sweep<0>, blocks(nq * a->nlev * a->nlat), dim3(PB), 0, st, *a,
-                           (const double*)divm, G);
+        const dim3 xg((unsigned)((a->nlon + PTX - 1) / PTX), (unsigned)((a->nlat + PTY - 1) / PTY),
+                      (unsigned)(nq * a->nlev));
+        hipLaunchKernelGGL(rs_ppm_vjp_xtile, xg, dim3(PB), 0, st, *a, G);
     }
     hipLaunchKernelGGL(rs_ppm_vjp_mrow, blocks(a->nc), dim3(PB), 0, st, *a, F);
     return hipGetLastError();

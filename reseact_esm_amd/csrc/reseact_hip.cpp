@@ -871,17 +871,31 @@ void enqueue_ssprk43_vjp(rs_ctx* c, double dt) {
 //   LA = LC + h/2 J(u1)^T LC                    (l1)
 //   lam = 2/3 LB + (LA + h/2 J(u)^T LA)
 void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
+    // line sweeps (each face gradient once), or the per-cell gather on a
+    // periodic row shorter than a stencil
+    const bool sweeps = !(c->lon_periodic && c->nlon < 6);
+    const size_t NT = (size_t)c->nt * c->nc;
+    if (sweeps) {
+        // the tendencies at u, u1, u2 (written by the recompute stages) and u4
+        c->pF.ensure(4 * NT);
+        c->pG.ensure((size_t)std::max(c->nt - 1, 1) * c->nc);
+        c->pdivm.ensure((size_t)c->nc);
+    }
+    auto Fslot = [&](int q) { return c->pF.p + (size_t)q * NT; };
     RsPPMArgs a = ppm_args(c, dt);
     a.a = u;
     a.out = c->X.p;
+    a.outf = sweeps ? Fslot(0) : nullptr;
     { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
     a.a = c->X.p;
     a.out = c->Y.p;
+    a.outf = sweeps ? Fslot(1) : nullptr;
     { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
     a.a = c->Y.p;
     a.out = c->LA.p;
     a.b = u;
     a.out2 = c->pZ.p;
+    a.outf = sweeps ? Fslot(2) : nullptr;
     { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(2, &a, c->stream)); }
     RsPPMVjpArgs v{};
     v.Mx = c->pMx.p; v.My = c->pMy.p; v.Mz = c->pMz.p;
@@ -892,33 +906,25 @@ void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
     v.h2 = dt / 2;
     v.nlon = c->nlon; v.nlat = c->nlat; v.nlev = c->nlev; v.nt = c->nt;
     v.lon_periodic = c->lon_periodic;
-    // line sweeps (each face gradient once), or the per-cell gather on a
-    // periodic row shorter than a stencil
-    const bool sweeps = !(c->lon_periodic && c->nlon < 6);
     if (sweeps) {
-        c->pF.ensure((size_t)c->nt * c->nc);
-        c->pG.ensure((size_t)std::max(c->nt - 1, 1) * c->nc);
-        c->pdivm.ensure((size_t)c->nc);
+        // the tendency at u4 (tendency-only stage)
+        RsPPMArgs tf = ppm_args(c, dt);
+        tf.a = c->pZ.p;
+        tf.out = Fslot(3);
+        ProfScope ps(c, P_TSTAGE);
+        HIPCHK(rs_launch_ppm_stage(3, &tf, c->stream));
     }
-    RsPPMArgs tf = ppm_args(c, dt);
-    tf.out = c->pF.p;
-    auto apply = [&](const double* at, const double* in, double alpha, const double* b, double beta,
-                     double* out) {
+    auto apply = [&](const double* at, int fq, const double* in, double alpha, const double* b,
+                     double beta, double* out) {
         v.at = at; v.a = in; v.alpha = alpha; v.b = b; v.beta = beta; v.out = out;
-        if (sweeps) {
-            tf.a = at;
-            { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(3, &tf, c->stream)); }
-            ProfScope ps(c, P_TPPMADJ);
-            HIPCHK(rs_launch_ppm_vjp_sweeps(&v, c->pF.p, c->pG.p, c->pdivm.p, c->stream));
-        } else {
-            ProfScope ps(c, P_TPPMADJ);
-            HIPCHK(rs_launch_ppm_vjp(&v, c->stream));
-        }
+        ProfScope ps(c, P_TPPMADJ);
+        if (sweeps) HIPCHK(rs_launch_ppm_vjp_sweeps(&v, Fslot(fq), c->pG.p, c->pdivm.p, c->stream));
+        else HIPCHK(rs_launch_ppm_vjp(&v, c->stream));
     };
-    apply(c->pZ.p, c->lam.p, 1.0, nullptr, 0.0, c->LB.p);
-    apply(c->Y.p, c->LB.p, 1.0 / 3.0, nullptr, 0.0, c->LC.p);
-    apply(c->X.p, c->LC.p, 1.0, nullptr, 0.0, c->LA.p);
-    apply(u, c->LA.p, 1.0, c->LB.p, 2.0 / 3.0, c->lam.p);
+    apply(c->pZ.p, 3, c->lam.p, 1.0, nullptr, 0.0, c->LB.p);
+    apply(c->Y.p, 2, c->LB.p, 1.0 / 3.0, nullptr, 0.0, c->LC.p);
+    apply(c->X.p, 1, c->LC.p, 1.0, nullptr, 0.0, c->LA.p);
+    apply(u, 0, c->LA.p, 1.0, c->LB.p, 2.0 / 3.0, c->lam.p);
 }
 
 void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt) {

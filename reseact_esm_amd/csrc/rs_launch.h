@@ -121,6 +121,7 @@ typedef struct RsPPMArgs {
     const double* qs;
     const double* qn;
     const double* msrc;     /* air-mass source of the pressure fixer, or 0 */
+    double* outf;           /* stages 1-3: also the tendency at a (adjoint), or 0 */
     long long nc;
     double dt_static, atol, rtol;
     int nlon, nlat, nlev, nt;
