@@ -185,11 +185,14 @@ def ppm_roofline(lib, nsteps: int = 10, nq: int = 20):
         ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)
     t1 = time.perf_counter()
     prof = ctx.profile_read()
+    ctx.profile(False)
     # the step VJP at the same state: three forward stages recompute the stage
-    # states, four adjoint applications (ppm_vjp) reverse them
+    # states (and the tendencies), a tendency-only stage covers u4, four
+    # adjoint applications (ppm_vjp: z and y sweeps, longitude tiles, air-mass
+    # row) reverse them
     ctx.set_lambda(np.ones(p.ns * p.nc))
     ctx.ssprk43_vjp(0.0, dt)                       # warm-up
-    ctx.profile(True)
+    ctx.profile(True)                              # counters restart here
     nvjp = max(2, nsteps // 2)
     t2 = time.perf_counter()
     for _ in range(nvjp):

@@ -842,10 +842,30 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_sweep(RsPPMVjpArgs A, const dou
     auto coefL = [&](int F) -> double { return DIR == 2 ? 1.0 : A.ce[F] * A.rdS[F - 1]; };
     const long long nrs = (long long)(nlev + 4) * plane, nre = (long long)(nlev + 3) * plane;
     // sliding windows: v[d] = value of logical F-3+d, acc[d] = contributions
-    // gathered so far for logical F-3+d
+    // gathered so far for logical F-3+d; vertical width factors (DIR 2) of the
+    // slopes of logical cells F-2+d (sw) and of the edges (F-2+d, F-1+d) (ew),
+    // rows L + 2 clamped to their tables (rows past the lid are never used)
     double v[6], acc[6], bot = 0.0;
+    double swR[5], swL[5], ew[4][3];
 #pragma unroll
     for (int d = 0; d < 6; ++d) { v[d] = val(-3 + d); acc[d] = 0.0; }
+    auto slope_row = [&](int L, double* r0, double* r1) {
+        const long long r = (long long)clampi(L + 2, 0, nlev + 3) * plane + col;
+        *r0 = A.vs[r];
+        *r1 = A.vs[nrs + r];
+    };
+    auto edge_row = [&](int L, double* e) {
+        const long long r = (long long)clampi(L + 2, 0, nlev + 2) * plane + col;
+        e[0] = A.ve[r];
+        e[1] = A.ve[nre + r];
+        e[2] = A.ve[2 * nre + r];
+    };
+    if (DIR == 2) {
+#pragma unroll
+        for (int d = 0; d < 5; ++d) slope_row(-2 + d, &swR[d], &swL[d]);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) edge_row(-2 + d, ew[d]);
+    }
     double mul = 0.0, mur = mu(0);            // mu of cells F-1 and F (0 outside the line)
     // the logical cell leaving the window: write it, fold it (clamped
     // vertical indices below the surface) or drop it (inflow halo)
@@ -868,19 +888,16 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_sweep(RsPPMVjpArgs A, const dou
 #pragma unroll
             for (int d = 0; d < 5; ++d) w[d] = right ? v[d] : v[d + 1];
             if (DIR == 2) {
-                // width factors of logical cells c-1..c+1 (rows L + 2) and
-                // edges (c-1, c), (c, c+1), c = F-1 (M >= 0) or F
-                const int c = right ? F - 1 : F;
+                // slopes of logical cells c-1..c+1 and edges (c-1, c), (c, c+1),
+                // c = F-1 (M >= 0: window positions 0..2, 0..1) or F (1..3, 1..2)
                 double sR[3], sL[3], e1[3], e2[3];
 #pragma unroll
                 for (int q3 = 0; q3 < 3; ++q3) {
-                    const long long r = (long long)(c - 1 + q3 + 2) * plane + col;
-                    sR[q3] = A.vs[r];
-                    sL[q3] = A.vs[nrs + r];
+                    sR[q3] = right ? swR[q3] : swR[q3 + 1];
+                    sL[q3] = right ? swL[q3] : swL[q3 + 1];
+                    e1[q3] = right ? ew[0][q3] : ew[1][q3];
+                    e2[q3] = right ? ew[1][q3] : ew[2][q3];
                 }
-                const long long r1 = (long long)(c - 1 + 2) * plane + col, r2 = (long long)(c + 2) * plane + col;
-#pragma unroll
-                for (int q3 = 0; q3 < 3; ++q3) { e1[q3] = A.ve[q3 * nre + r1]; e2[q3] = A.ve[q3 * nre + r2]; }
                 face_grad5<true>(w, right, sR, sL, e1, e2, g);
             } else {
                 face_grad5<false>(w, right, nullptr, nullptr, nullptr, nullptr, g);
@@ -901,6 +918,16 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_sweep(RsPPMVjpArgs A, const dou
         acc[5] = 0.0;
         mul = mur;
         mur = F + 1 < n ? mu(F + 1) : 0.0;
+        if (DIR == 2) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) { swR[d] = swR[d + 1]; swL[d] = swL[d + 1]; }
+            slope_row(F + 3, &swR[4], &swL[4]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int q3 = 0; q3 < 3; ++q3) ew[d][q3] = ew[d + 1][q3];
+            edge_row(F + 2, ew[3]);
+        }
     }
     // the window holds logical n-3 .. n+2, all complete
     if (DIR == 2) {
