@@ -259,13 +259,42 @@ def chemistry_roofline(lib):
     log(f"chemistry {p.name}: attempts {st.attempts_total} (min {st.attempts_min} max "
         f"{st.attempts_max}) kernel {ms:.1f} ms wall {t1 - t0:.2f} s {achieved:.2f} TFLOP/s "
         f"regs {info['regs_percell']} scratch {info['scratch_percell']}")
+    dropin = chemistry_dropin(lib)
     return {"workload": p.name, "bound": "mfma", "compute": "fp64 valu", "kernel": "chem_percell",
+            "dropin_global_norm": dropin,
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "kernel_ms": round(ms, 2),
             "attempts": st.attempts_total, "attempts_min": st.attempts_min,
             "attempts_max": st.attempts_max, "flops_per_attempt": info["flops_attempt"],
             "cell_steps_per_s": round(p.nc / (ms * 1e-3), 1)}
+
+
+def chemistry_dropin(lib, nlon: int = 24):
+    """The batched per-cell solve beside the plain drop-in on the same cells: a
+    24-longitude subset of the 50-species grid (day and night columns), the
+    persistent work-stealing kernel with per-cell error control against the
+    global-norm adaptive solve (one controller over all cells, the reference's
+    adaptive_solve), both over the 1800 s interval."""
+    p = P.config3(nlon=nlon)
+    res = {"sample": f"{p.grid.nlon}x{p.grid.nlat}x{p.grid.nlev} subset of the 50-species grid"}
+    with lib.context() as ctx:
+        ctx.load(p)
+        t0 = time.perf_counter()
+        st = ctx.chem_percell(0.0, 1800.0, p.dt0_C, None, None, p.atol_C, p.rtol_C, True)
+        res["percell_seconds"] = round(time.perf_counter() - t0, 3)
+        res["percell_attempts_mean"] = round(st.attempts_total / p.nc, 1)
+    with lib.context() as ctx:
+        ctx.load(p)
+        t0 = time.perf_counter()
+        _, c = ctx.adaptive_chem(0.0, 1800.0, p.dt0_C, None, p.atol_C, p.rtol_C, True)
+        res["global_seconds"] = round(time.perf_counter() - t0, 3)
+        res["global_attempts"] = c.naccept + c.nreject
+    res["percell_cell_steps_per_s"] = round(p.nc / res["percell_seconds"], 1)
+    res["global_cell_steps_per_s"] = round(p.nc / res["global_seconds"], 1)
+    res["percell_speedup"] = round(res["global_seconds"] / res["percell_seconds"], 3)
+    log(f"chemistry drop-in: {res}")
+    return res
 
 
 def ensemble_run(lib, ndom: int = 32, nmacro: int = 6, tape_budget_gib: float = 96.0):
@@ -306,6 +335,23 @@ def ensemble_run(lib, ndom: int = 32, nmacro: int = 6, tape_budget_gib: float = 
     for m in members:
         m.close()
     ens.close()
+    # the plain drop-in beside it: the same domains one after another through
+    # rs_adjoint_run, each in a fresh context (default kernels, no graphs)
+    fwd = bwd = 0.0
+    for p in probs:
+        with lib.context() as ctx:
+            ctx.load(p)
+            _, _, _, s1 = ctx.adjoint_run(p.tstops(), p.dt0_T, p.dt0_C, split_opts(p), 0,
+                                          want_grads=False)
+            fwd += s1.seconds_forward
+            bwd += s1.seconds_backward
+    out["dropin_sequential"] = {
+        "forward_cell_steps_per_s": round(cells * nmacro / fwd, 1),
+        "seconds_forward": round(fwd, 3), "seconds_backward": round(bwd, 3),
+        "adjoint_ratio": round((fwd + bwd) / fwd, 3),
+        "graph_speedup_forward": round(fwd / es.seconds_forward, 3),
+        "graph_speedup_forward_plus_adjoint": round((fwd + bwd) / (es.seconds_forward + es.seconds_backward), 3)}
+    log(f"ensemble drop-in (sequential): {out['dropin_sequential']}")
     return out
 
 

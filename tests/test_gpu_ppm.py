@@ -101,6 +101,11 @@ VJP_CASES = {
     "global-48x24x10": lambda: P.ppm_global(48, 24, 10, nq=3, seed=9),
     # a periodic row shorter than a stencil: the per-cell gather kernel
     "global-4x6x5": lambda: P.ppm_global(4, 6, 5, nq=2, seed=3),
+    # one level (the vertical sweep folds every clamped index onto it) and
+    # lines shorter than a stencil in latitude and the vertical
+    "thin-12x3x1": lambda: P.ppm_global(12, 3, 1, nq=2, seed=5),
+    "thin-8x2x2": lambda: P.ppm_global(8, 2, 2, nq=2, seed=5),
+    "regional-9x2x2": lambda: regional_random(9, 2, 2, nq=2, seed=6),
 }
 
 
@@ -112,7 +117,8 @@ def test_ppm_step_vjp(hip, oracle, name):
     print(f"ppm step vjp {name}", flush=True)
     p = VJP_CASES[name]()
     # steps of a few tenths of the Courant limit, so J^T lam is not lost in lam
-    dt = {"regional-20x12x9": 0.03, "global-48x24x10": 3000.0, "global-4x6x5": 50000.0}[name]
+    dt = {"regional-20x12x9": 0.03, "global-48x24x10": 3000.0, "global-4x6x5": 50000.0,
+          "thin-12x3x1": 20000.0, "thin-8x2x2": 20000.0, "regional-9x2x2": 0.05}[name]
     lam = np.random.default_rng(21).standard_normal(p.ns * p.nc)
     out = []
     for lib in (hip, oracle):
