@@ -127,12 +127,15 @@ def coupled(lib, steps: int, warmup: int):
 
 
 def transport_roofline(lib, nsteps: int = 60, nt: int = 20):
-    """C2 transport at Courant 0.8 with a fixed dt, fused kernel (one launch per
-    attempt) and per-stage kernels (four launches) on the same inputs."""
+    """C2 transport at Courant 0.8 with a fixed dt: the fused kernel (one launch
+    per attempt) at its default workgroup size (512 lanes on this grid's
+    longitude segments), the fused kernel forced to 1024-lane workgroups, and
+    the per-stage kernels (four launches), on the same inputs."""
     p = P.config2(nt=nt, nsteps=nsteps)
     dt = p.extra["dt_fixed"]
     res = {}
-    for mode, kernels in ((1, ("transport_fused",)), (0, ("transport_stage", "transport_last"))):
+    for mode, kernels in ((1, ("transport_fused",)), (2, ("transport_fused",)),
+                          (0, ("transport_stage", "transport_last"))):
         ctx = lib.context()
         ctx.set_transport_kernel(mode)
         ctx.load(p)
@@ -151,19 +154,21 @@ def transport_roofline(lib, nsteps: int = 60, nt: int = 20):
         res[mode] = dict(ms_step=ms / nsteps, wall=t1 - t0, launches=launches,
                          avg_launch_ms=ms / max(launches, 1))
     cells = nt * p.nc
-    f, s = res[1], res[0]
+    f, f1024, s = res[1], res[2], res[0]
     # algorithmic bytes: the fused kernel must read u and write unew (16 B per
     # tracer-cell); the per-stage chain moves 72 B (stages 1-3: 8 + 8, last
     # stage: u, u3 in, unew out)
     achieved = 16.0 * cells / (f["avg_launch_ms"] * 1e-3) / 1e9
-    log(f"transport {p.name}: fused {f['ms_step']:.3f} ms/step ({achieved:.0f} GB/s at 16 B), "
-        f"per-stage {s['ms_step']:.3f} ms/step ({72.0 * cells / (s['ms_step'] * 1e-3) / 1e9:.0f} GB/s "
-        f"at 72 B)")
+    log(f"transport {p.name}: fused {f['ms_step']:.3f} ms/step ({achieved:.0f} GB/s at 16 B; "
+        f"1024-lane workgroups {f1024['ms_step']:.3f} ms), per-stage {s['ms_step']:.3f} ms/step "
+        f"({72.0 * cells / (s['ms_step'] * 1e-3) / 1e9:.0f} GB/s at 72 B)")
     return {"workload": p.name, "kernel": "transport_fused", "bound": "hbm",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": 16.0 * cells,
             "avg_launch_ms": round(f["avg_launch_ms"], 4), "ms_per_step": round(f["ms_step"], 4),
             "tracer_cell_steps_per_s": round(cells / (f["ms_step"] * 1e-3), 1),
+            "fused_1024_lanes": {"ms_per_step": round(f1024["ms_step"], 4),
+                                 "achieved_GBs": round(16.0 * cells / (f1024["ms_step"] * 1e-3) / 1e9, 1)},
             "per_stage": {"ms_per_step": round(s["ms_step"], 4), "bytes_per_step": 72.0 * cells,
                           "achieved_GBs": round(72.0 * cells / (s["ms_step"] * 1e-3) / 1e9, 1),
                           "launches_per_step": round(s["launches"] / nsteps, 2)},

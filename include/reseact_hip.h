@@ -306,8 +306,12 @@ RS_API int rs_set_lu_ordering(rs_ctx* ctx, int ordering);
 /* Transport kernels for each SSPRK43 attempt and its VJP:
  * 1 = one fused launch per attempt (all four stages in LDS; default, used
  *     where the grid allows: nlon >= 4 and a longitude segment with an
- *     8-column halo fits in LDS at this nlev);
- * 0 = one launch per stage.  Both give the same bits. */
+ *     8-column halo fits in LDS at this nlev), with 1024-lane workgroups
+ *     where a whole periodic row fits one, else 512-lane workgroups
+ *     (narrower segments, two workgroups per CU);
+ * 2 = fused, 1024-lane workgroups;  3 = fused, 512-lane workgroups;
+ * 0 = one launch per stage.  All give the same bits.  rs_autotune times
+ * the per-stage tiles and both fused sizes and keeps the fastest. */
 RS_API int rs_set_transport_kernel(rs_ctx* ctx, int mode);
 
 /* Static work of the compiled mechanism kernels (roofline accounting). */
@@ -343,7 +347,7 @@ typedef struct rs_tune_result {
     double transport_ms[4];      /* per candidate, 0 if not timed */
     double chem_ms[3];
     int transport_fused;         /* 1 if the fused kernel was kept */
-    int pad;
+    int transport_fused_threads; /* its workgroup size (1024 or 512), 0 if not kept */
     double transport_fused_ms;   /* 0 if the grid does not allow it */
     int chem_kpointer;           /* 1: attempt reads rate constants through a pointer */
     int pad2;

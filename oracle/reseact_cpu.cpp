@@ -1930,7 +1930,7 @@ int rs_set_lu_ordering(rs_ctx* c, int ordering) {
 
 int rs_set_transport_kernel(rs_ctx* c, int mode) {
     if (!c) return RS_E_ARG;
-    if (mode != 0 && mode != 1) return fail(c, RS_E_ARG, "mode must be 0 or 1");
+    if (mode < 0 || mode > 3) return fail(c, RS_E_ARG, "mode must be 0, 1, 2 or 3");
     return RS_OK;   // one transport path on the CPU
 }
 

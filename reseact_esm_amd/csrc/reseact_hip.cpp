@@ -44,7 +44,7 @@
 extern "C" {
 hipError_t rs_launch_transport(int kind, int tk, const RsTransArgs* a, hipStream_t st);
 long long rs_transport_nblocks(int nlon, int nlat, int nlev, int nt, int tk);
-int rs_transport_fused_plan(int nlon, int nlev, long long lds_bytes, int* plan);
+int rs_transport_fused_plan(int nlon, int nlev, long long lds_bytes, int nth, int* plan);
 long long rs_transport_fused_nblocks(int nlat, int nt, int nseg);
 hipError_t rs_launch_transport_fused(int adj, const RsTransArgs* a, hipStream_t st);
 hipError_t rs_launch_ctrl(RsCtrl* c, const double* partial, int npart, double* seqlog,
@@ -163,8 +163,12 @@ struct rs_ctx {
     int lu_order = 1;
     int tk = 16;                 // transport tile height (levels per workgroup)
     int tmode = 1;               // 1: fused SSPRK43 kernels where the grid allows, 0: per stage
+    int fnth = 1024;             // lanes per fused workgroup (1024 or 512)
+    int fnth_req = 0;            // requested: 0 = by the grid's plan, else 1024 or 512
     int fplan[6] = {0, 0, 0, 0, 0, 0};  // fused plan: seg_w, seg_in, seg_halo, nseg, seg_lev, seg_chunks
     bool fplan_ok = false;
+    int fplans[2][6] = {};       // the plans for 1024 and 512 lanes
+    bool fplans_ok[2] = {false, false};
     unsigned chem_block = 64;    // chemistry threads per workgroup
     bool chem_kp = false;        // attempt kernel reads the rate constants through a pointer
     DevArray<double> S[2], X, Y, cx, wz, wl, K, jfac, E, lam, LA, LB, LC, gE, partial, negpart, seqlog;
@@ -409,6 +413,19 @@ void set_fused_plan(RsTransArgs& a, const rs_ctx* c) {
     a.nseg = c->fplan[3];
     a.seg_lev = c->fplan[4];
     a.seg_chunks = c->fplan[5];
+    a.fnth = c->fnth;
+}
+
+// Fused workgroup size (1024 or 512 lanes) and its plan for the grid; nth 0:
+// 1024 where a whole periodic row fits one workgroup (no halo recomputed),
+// otherwise 512 (two workgroups per CU on the segments: 8-9% faster on
+// 360x180x72, while 1024 is 7% faster on the whole rows of 144x91x47).
+void select_fused(rs_ctx* c, int nth) {
+    if (nth == 0) nth = (c->fplans_ok[0] && c->fplans[0][2] == 0) || !c->fplans_ok[1] ? 1024 : 512;
+    const int q = nth == 512 ? 1 : 0;
+    c->fnth = q ? 512 : 1024;
+    c->fplan_ok = c->fplans_ok[q];
+    for (int i = 0; i < 6; ++i) c->fplan[i] = c->fplans[q][i];
 }
 
 // Clamp flags of every step entry point: 1 = max(u, 0) on the accepted state
@@ -1146,11 +1163,15 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
         c->lam.resize(N);
         c->LA.resize(NT); c->LB.resize(NT); c->LC.resize(NT);
         c->gE.resize((size_t)ns * c->ncol);
-        c->fplan_ok = rs_transport_fused_plan(nlon, nlev, (long long)c->lds_max, c->fplan) != 0;
         long long np = std::max<long long>(rs_transport_nblocks(nlon, nlat, nlev, std::max(nt, 1), 4),
                                            (c->nc + 63) / 64);
-        if (c->fplan_ok)
-            np = std::max<long long>(np, rs_transport_fused_nblocks(nlat, std::max(nt, 1), c->fplan[3]));
+        for (int q = 0; q < 2; ++q) {
+            c->fplans_ok[q] = rs_transport_fused_plan(nlon, nlev, (long long)c->lds_max,
+                                                      q ? 512 : 1024, c->fplans[q]) != 0;
+            if (c->fplans_ok[q])
+                np = std::max<long long>(np, rs_transport_fused_nblocks(nlat, std::max(nt, 1), c->fplans[q][3]));
+        }
+        select_fused(c, c->fnth_req);
         // PPM stages: one workgroup per tile of a level
         np = std::max<long long>(np, rs_ppm_nblocks(nlon, nlat, nlev));
         c->partial.resize((size_t)np);
@@ -1205,8 +1226,10 @@ int rs_set_wind(rs_ctx* c, const double* cx, const double* wz, const double* wl)
 
 int rs_set_transport_kernel(rs_ctx* c, int mode) {
     if (!c) return RS_E_ARG;
-    if (mode != 0 && mode != 1) return fail(c, RS_E_ARG, "mode must be 0 or 1");
-    c->tmode = mode;
+    if (mode < 0 || mode > 3) return fail(c, RS_E_ARG, "mode must be 0, 1, 2 or 3");
+    c->tmode = mode == 0 ? 0 : 1;
+    c->fnth_req = mode == 2 ? 1024 : (mode == 3 ? 512 : 0);
+    select_fused(c, c->fnth_req);
     return RS_OK;
 }
 
@@ -1823,14 +1846,26 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
             if (r.transport_ms[q] < best) { best = r.transport_ms[q]; r.transport_tile_levels = tks[q]; }
         }
         c->tk = r.transport_tile_levels;
-        if (c->fplan_ok) {
+        // the fused kernel with 1024- and 512-lane workgroups (two of the
+        // smaller fit a CU, so one's barrier waits overlap the other's work)
+        int bestf = 0;
+        for (int nth : {1024, 512}) {
+            select_fused(c, nth);
+            if (!c->fplan_ok) continue;
             c->tmode = 1;
-            r.transport_fused_ms = time_ms(c, [&]() {
+            // more repetitions than the per-stage candidates: the two sizes
+            // are often within 10% of each other
+            const double ms = time_ms(c, [&]() {
                 enqueue_ssprk43(c, nullptr, in, cand, 60.0, 1e-6, 1e-3, 0);
-            }, 3);
-            if (r.transport_fused_ms >= best) c->tmode = 0;
+            }, 10);
+            if (bestf == 0 || ms < r.transport_fused_ms) { r.transport_fused_ms = ms; bestf = nth; }
         }
+        const bool fused = bestf != 0 && r.transport_fused_ms < best;
+        c->tmode = fused ? 1 : 0;
+        c->fnth_req = fused ? bestf : 0;
+        select_fused(c, c->fnth_req);
         r.transport_fused = trans_fused(c) ? 1 : 0;
+        r.transport_fused_threads = fused ? bestf : 0;
     }
     if (c->have_mech) {
         const unsigned bl[3] = {64, 128, 256};

@@ -91,7 +91,8 @@ typedef struct RsTransArgs {
      * per column */
     int seg_w, seg_in, seg_halo, nseg, seg_lev, seg_chunks;
     double* negpart;        /* positive domain: negative entries per block, or 0 */
-    int posdom, pad;
+    int posdom;
+    int fnth;               /* fused kernels: lanes per workgroup (1024 or 512) */
 } RsTransArgs;
 
 /* Monotone PPM transport stage (CWC mixing-ratio form; species 0 = air

@@ -240,8 +240,10 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
 }
 
 // ---- fused SSPRK43 attempt / adjoint: all four stages in one launch --------
-// A 1024-lane workgroup owns one (tracer, latitude) slab, or a longitude
-// segment of it, with all levels, in LDS.  With seg_halo = 0 the segment is
+// A workgroup of NTH lanes (1024, or 512 so that two share a CU and one's
+// barrier waits overlap the other's work, on narrower segments) owns one
+// (tracer, latitude) slab, or a longitude segment of it, with all levels, in
+// LDS.  With seg_halo = 0 the segment is
 // the whole periodic row and the zonal stencil wraps inside LDS; otherwise
 // FHALO = 8 columns on each side (2 per stage x 4 stages) are recomputed and
 // only the seg_in interior columns are written.
@@ -258,7 +260,7 @@ __global__ void __launch_bounds__(TB) rs_transport_stage(RsTransArgs A) {
 // per tracer-cell (times seg_w / seg_in for the read when segmented).  Every
 // face flux and every cell update is the same expression, in the same order,
 // as in the stage kernels above, so both paths give the same bits.
-#define FT 1024
+#define FT 1024      // lanes per workgroup (the default; 512 is the other instantiation)
 #define FL 8
 #define FHALO 8
 
@@ -389,10 +391,10 @@ __device__ __forceinline__ void fstage_adj(const FCol& C, int nk, double h2, Out
 
 // PD: positive-domain solves also count the negative entries of the trial
 // state per workgroup (compile-time, so other solves do not carry it).
-template <int ADJ, int PD>
-__global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
+template <int ADJ, int PD, int NTH>
+__global__ void __launch_bounds__(NTH) rs_transport_fused(RsTransArgs P) {
     extern __shared__ double lds[];
-    __shared__ double red[FT / 64];
+    __shared__ double red[NTH / 64];
     const RsCtrl* ctl = P.ctrl;
     if (ctl && ctl->done) return;
     const int cur = ctl ? ctl->cur : P.cur_static;
@@ -419,8 +421,8 @@ __global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
         gi = gi < 0 ? gi + nlon : gi;
         return gi >= nlon ? gi - nlon : gi;
     };
-    for (int x = tid; x < w; x += FT) Wl[x] = P.wl[gcol(x)];
-    for (int x = tid; x < nlev + 4; x += FT) Wz[x] = (x >= 1 && x <= nlev + 1) ? P.wz[x - 1] : 0.0;
+    for (int x = tid; x < w; x += NTH) Wl[x] = P.wl[gcol(x)];
+    for (int x = tid; x < nlev + 4; x += NTH) Wz[x] = (x >= 1 && x <= nlev + 1) ? P.wz[x - 1] : 0.0;
     const int il = tid % w, kc = tid / w;
     const int nk = kc < P.seg_chunks ? min(L, nlev - kc * L) : 0;
     const int k0 = nk > 0 ? kc * L : 0;
@@ -513,10 +515,10 @@ __global__ void __launch_bounds__(FT) rs_transport_fused(RsTransArgs P) {
     }
     if (!ADJ) {
         const long long bid = (long long)blockIdx.y * gridDim.x + blockIdx.x;
-        const double t = block_sum_fixed<FT>(sq, red, tid);
+        const double t = block_sum_fixed<NTH>(sq, red, tid);
         if (tid == 0) P.partial[bid] = t;
         if (PD) {
-            const double nt = block_sum_fixed<FT>(ng, red, tid);
+            const double nt = block_sum_fixed<NTH>(ng, red, tid);
             if (tid == 0) P.negpart[bid] = nt;
         }
     }
@@ -784,14 +786,15 @@ extern "C" {
 // Segment plan for the fused kernels on an nlon x nlev slab: plan[0..5] =
 // seg_w, seg_in, seg_halo, nseg, seg_lev, seg_chunks.  Returns 0 if the fused
 // kernels cannot take this grid (nlon < 4, or too many levels for a useful
-// segment: at most FL levels per lane and FT lanes per workgroup, and the
-// tile within lds_bytes of LDS per workgroup).
-int rs_transport_fused_plan(int nlon, int nlev, long long lds_bytes, int* plan) {
+// segment: at most FL levels per lane and nth (1024 or 512) lanes per
+// workgroup, and the tile within lds_bytes of LDS per workgroup).
+int rs_transport_fused_plan(int nlon, int nlev, long long lds_bytes, int nth, int* plan) {
+    if (nth != 1024 && nth != 512) return 0;
     const long long lds_doubles = (std::min<long long>(lds_bytes, FUSED_LDS_MAX) - 256) / 8;
     auto fits = [&](long long w, int* lev, int* chunks) {
-        if (w < 4 || w > FT) return false;
+        if (w < 4 || w > nth) return false;
         if (w * fpitch(nlev) + w + nlev + 4 > lds_doubles) return false;
-        const int per_col = (int)(FT / w);
+        const int per_col = (int)(nth / w);
         const int L = (nlev + per_col - 1) / per_col;
         if (L > FL) return false;
         *lev = L;
@@ -804,7 +807,7 @@ int rs_transport_fused_plan(int nlon, int nlev, long long lds_bytes, int* plan) 
         plan[0] = nlon; plan[1] = nlon; plan[2] = 0; plan[3] = 1; plan[4] = L; plan[5] = ch;
         return 1;
     }
-    long long wmax = std::min<long long>(FT, nlon + 2 * FHALO);
+    long long wmax = std::min<long long>(nth, nlon + 2 * FHALO);
     while (wmax >= 2 * FHALO + 16 && !fits(wmax, &L, &ch)) --wmax;
     if (wmax < 2 * FHALO + 16) return 0;
     const int nseg = (int)((nlon + (wmax - 2 * FHALO) - 1) / (wmax - 2 * FHALO));
@@ -822,15 +825,23 @@ long long rs_transport_fused_nblocks(int nlat, int nt, int nseg) {
 }
 
 hipError_t rs_launch_transport_fused(int adj, const RsTransArgs* a, hipStream_t st) {
-    if (a->seg_w < 4 || a->seg_w > FT || a->nseg < 1 || a->seg_lev < 1 || a->seg_lev > FL ||
-        (long long)a->seg_w * a->seg_chunks > FT || (long long)a->seg_lev * a->seg_chunks < a->nlev)
+    const int nth = a->fnth;
+    if ((nth != 1024 && nth != 512) || a->seg_w < 4 || a->seg_w > nth || a->nseg < 1 ||
+        a->seg_lev < 1 || a->seg_lev > FL || (long long)a->seg_w * a->seg_chunks > nth ||
+        (long long)a->seg_lev * a->seg_chunks < a->nlev)
         return hipErrorInvalidValue;
     const size_t lds = ((size_t)a->seg_w * fpitch(a->nlev) + a->seg_w + a->nlev + 4) * sizeof(double);
     if (lds > FUSED_LDS_MAX - 256) return hipErrorInvalidValue;
     dim3 grid((unsigned)a->nseg, (unsigned)(a->nlat * a->nt));
-    if (adj) hipLaunchKernelGGL((rs_transport_fused<1, 0>), grid, dim3(FT), lds, st, *a);
-    else if (a->negpart) hipLaunchKernelGGL((rs_transport_fused<0, 1>), grid, dim3(FT), lds, st, *a);
-    else hipLaunchKernelGGL((rs_transport_fused<0, 0>), grid, dim3(FT), lds, st, *a);
+    if (nth == 512) {
+        if (adj) hipLaunchKernelGGL((rs_transport_fused<1, 0, 512>), grid, dim3(512), lds, st, *a);
+        else if (a->negpart) hipLaunchKernelGGL((rs_transport_fused<0, 1, 512>), grid, dim3(512), lds, st, *a);
+        else hipLaunchKernelGGL((rs_transport_fused<0, 0, 512>), grid, dim3(512), lds, st, *a);
+    } else {
+        if (adj) hipLaunchKernelGGL((rs_transport_fused<1, 0, FT>), grid, dim3(FT), lds, st, *a);
+        else if (a->negpart) hipLaunchKernelGGL((rs_transport_fused<0, 1, FT>), grid, dim3(FT), lds, st, *a);
+        else hipLaunchKernelGGL((rs_transport_fused<0, 0, FT>), grid, dim3(FT), lds, st, *a);
+    }
     return hipGetLastError();
 }
 

@@ -82,7 +82,8 @@ class MechInfo(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("transport_tile_levels", C.c_int), ("chem_block", C.c_int),
                 ("transport_ms", C.c_double * 4), ("chem_ms", C.c_double * 3),
-                ("transport_fused", C.c_int), ("pad", C.c_int), ("transport_fused_ms", C.c_double),
+                ("transport_fused", C.c_int), ("transport_fused_threads", C.c_int),
+                ("transport_fused_ms", C.c_double),
                 ("chem_kpointer", C.c_int), ("pad2", C.c_int), ("chem_kp_ms", C.c_double)]
 
 
@@ -309,7 +310,9 @@ class Context:
         self._chk(self.lib.rs_set_lu_ordering(self.h, ordering))
 
     def set_transport_kernel(self, mode: int):
-        """1: fused SSPRK43 launch per attempt (default where the grid allows), 0: per stage."""
+        """1: fused SSPRK43 launch per attempt (default where the grid allows; 1024-lane
+        workgroups on whole periodic rows, else 512); 2: fused, 1024 lanes; 3: fused,
+        512 lanes; 0: per stage."""
         self._chk(self.lib.rs_set_transport_kernel(self.h, mode))
 
     def mechanism_info(self) -> dict:

@@ -4,7 +4,8 @@ of its VJP, in one launch) against the per-stage kernels and the oracle.
 Grids cover both plans: a whole periodic row per workgroup (nlon * nlev small
 enough for LDS) and longitude segments with an 8-column halo (360x72 and
 200x90 split into 4 segments, 97x90 into 2 uneven ones), winds of both signs.
-The arithmetic per cell is the same in both kernels, so the bar is bit-for-bit.
+Both workgroup sizes of the fused kernel (1024 and 512 lanes) are run.  The
+arithmetic per cell is the same in all kernels, so the bar is bit-for-bit.
 """
 from __future__ import annotations
 
@@ -29,9 +30,11 @@ def problem(dims, flip=False, nt=2, ns=3, seed=5):
                      u0=P.blob_state(g, ns, seed))
 
 
-def contexts(hip, p):
+def contexts(hip, p, mode=1):
+    """The fused kernels (mode 1: workgroup size by the plan, 2: 1024 lanes,
+    3: 512 lanes) and the per-stage kernels on the same problem."""
     cf = hip.context()
-    cf.set_transport_kernel(1)
+    cf.set_transport_kernel(mode)
     cf.load(p)
     cs = hip.context()
     cs.set_transport_kernel(0)
@@ -39,12 +42,13 @@ def contexts(hip, p):
     return cf, cs
 
 
+@pytest.mark.parametrize("mode", [2, 3])
 @pytest.mark.parametrize("flip", [False, True])
 @pytest.mark.parametrize("dims", GRIDS)
-def test_fused_step_bitwise(hip, oracle, dims, flip):
-    print(f"fused step dims={dims} flip={flip}", flush=True)
+def test_fused_step_bitwise(hip, oracle, dims, flip, mode):
+    print(f"fused step dims={dims} flip={flip} mode={mode}", flush=True)
     p = problem(dims, flip)
-    cf, cs = contexts(hip, p)
+    cf, cs = contexts(hip, p, mode)
     co = oracle.context().load(p)
     for dt in (60.0, 600.0):
         ef = cf.ssprk43_step(0.0, dt, 1e-6, 1e-3)
@@ -58,11 +62,12 @@ def test_fused_step_bitwise(hip, oracle, dims, flip):
         assert abs(ef - eo) <= 1e-12 * max(eo, 1.0)
 
 
+@pytest.mark.parametrize("mode", [2, 3])
 @pytest.mark.parametrize("dims", GRIDS)
-def test_fused_vjp_bitwise(hip, oracle, dims):
-    print(f"fused vjp dims={dims}", flush=True)
+def test_fused_vjp_bitwise(hip, oracle, dims, mode):
+    print(f"fused vjp dims={dims} mode={mode}", flush=True)
     p = problem(dims, flip=dims[0] % 2 == 1)
-    cf, cs = contexts(hip, p)
+    cf, cs = contexts(hip, p, mode)
     co = oracle.context().load(p)
     lam = np.random.default_rng(11).standard_normal(p.ns * p.nc)
     for c in (cf, cs, co):
@@ -75,11 +80,12 @@ def test_fused_vjp_bitwise(hip, oracle, dims):
     assert np.array_equal(lf[p.nt * p.nc:], lam[p.nt * p.nc:])
 
 
+@pytest.mark.parametrize("mode", [2, 3])
 @pytest.mark.parametrize("dims", [(360, 2, 72), (36, 4, 10)])
-def test_fused_adaptive_same_sequence(hip, oracle, dims):
-    print(f"fused adaptive dims={dims}", flush=True)
+def test_fused_adaptive_same_sequence(hip, oracle, dims, mode):
+    print(f"fused adaptive dims={dims} mode={mode}", flush=True)
     p = problem(dims, nt=3, ns=3)
-    cf, cs = contexts(hip, p)
+    cf, cs = contexts(hip, p, mode)
     co = oracle.context().load(p)
     runs = [c.adaptive_transport(0.0, 1800.0, 15.0, pictrl_ssprk43(), 1e-6, 1e-3, True)
             for c in (cf, cs, co)]
@@ -101,11 +107,12 @@ def test_fused_adaptive_same_sequence(hip, oracle, dims):
 # more slabs than workgroups: each persistent workgroup runs several slabs,
 # streaming the next one into LDS while it computes (whole rows: 64 x 5 = 320
 # slabs; segmented: 4 segments x 40 x 2 = 320 slabs)
+@pytest.mark.parametrize("mode", [2, 3])
 @pytest.mark.parametrize("dims,nt", [((48, 64, 10), 5), ((360, 40, 72), 2)])
-def test_fused_many_slabs_bitwise(hip, oracle, dims, nt):
-    print(f"fused many slabs dims={dims} nt={nt}", flush=True)
+def test_fused_many_slabs_bitwise(hip, oracle, dims, nt, mode):
+    print(f"fused many slabs dims={dims} nt={nt} mode={mode}", flush=True)
     p = problem(dims, flip=False, nt=nt, ns=nt + 1)
-    cf, cs = contexts(hip, p)
+    cf, cs = contexts(hip, p, mode)
     co = oracle.context().load(p)
     ef = cf.ssprk43_step(0.0, 600.0, 1e-6, 1e-3)
     es = cs.ssprk43_step(0.0, 600.0, 1e-6, 1e-3)
