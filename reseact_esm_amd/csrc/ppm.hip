@@ -424,85 +424,100 @@ __global__ void __launch_bounds__(PB) rs_ppm_source(const double* __restrict__ M
 // limiters switch on products of neighbour differences, so the tendency is
 // piecewise smooth; these are its derivatives almost everywhere).
 //
-// Local gradient g[d] = d(face value)/d v[d] of one face from its six stencil
-// values v[0..5] (cells f-3 .. f+2): the upwind cell c (2 for M >= 0, 3
-// otherwise), the slopes of cells c-1..c+1 and the two edges of c, then the
-// limiter's branch.  NU: width-weighted (vertical), with the slope factors
-// (cR, cL) of stencil cell x at sR[x], sL[x] and the edge factors of the edge
-// between stencil cells x and x+1 at ec[x][0..2]; values from the forward's
-// own expressions, so branch decisions are the forward's.
+// Local gradient of a face value over the five cells w[0..4] = the upwind
+// cell's neighbourhood c-2 .. c+2 (the six-cell stencil shifted by one for
+// M < 0); `right`: the value is the right edge of w[2] (M >= 0), else its left
+// edge.  Width factors (NU): slopes of w[1..3] (sR, sL), edges (w1, w2) and
+// (w2, w3) (e1, e2).  Branch-free in the flux sign.
 template <bool NU>
-__device__ __forceinline__ void face_grad(const double* v, double M, const double* sR,
-                                          const double* sL, const double (*ec)[3], double* g) {
-#pragma unroll
-    for (int d = 0; d < 6; ++d) g[d] = 0.0;
-    const int c = M >= 0.0 ? 2 : 3;
+__device__ __forceinline__ double face_grad5(const double* w, bool right, const double* sR,
+                                             const double* sL, const double* e1, const double* e2,
+                                             double* g) {
     double sv[3], sg[3][3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        const int x = c - 1 + q;
-        const double am = v[x - 1], a0 = v[x], ap = v[x + 1];
-        const double cR = NU ? sR[x] : 0.5, cL = NU ? sL[x] : 0.5;
+        const double am = w[q], a0 = w[q + 1], ap = w[q + 2];
+        const double cR = NU ? sR[q] : 0.5, cL = NU ? sL[q] : 0.5;
         sv[q] = NU ? slope_w(cR, cL, am, a0, ap) : slope_u(am, a0, ap);
-        sg[q][0] = sg[q][1] = sg[q][2] = 0.0;
         const double dl = a0 - am, dr = ap - a0;
-        if (dr * dl > 0.0) {
-            const double da = NU ? cR * dr + cL * dl : 0.5 * (ap - am);
-            const double ax = fabs(da), y = 2.0 * fabs(dl), z = 2.0 * fabs(dr);
-            const double in = (z < y) ? z : y;
-            if (in < ax) {
-                if (z < y) { sg[q][2] = 2.0; sg[q][1] = -2.0; }
-                else { sg[q][1] = 2.0; sg[q][0] = -2.0; }
-            } else {
-                sg[q][2] = cR; sg[q][1] = cL - cR; sg[q][0] = -cL;
-            }
-        }
+        const bool on = dr * dl > 0.0;
+        const double da = NU ? cR * dr + cL * dl : 0.5 * (ap - am);
+        const double ax = fabs(da), y = 2.0 * fabs(dl), z = 2.0 * fabs(dr);
+        const bool zl = z < y;
+        const double in = zl ? z : y;
+        const bool lim = in < ax;
+        // gradient of the chosen term: da (cR, cL), 2 dr or 2 dl
+        double g0 = -cL, g1 = cL - cR, g2 = cR;
+        if (lim) { g0 = zl ? 0.0 : -2.0; g1 = 2.0 * (zl ? -1.0 : 1.0); g2 = zl ? 2.0 : 0.0; }
+        sg[q][0] = on ? g0 : 0.0;
+        sg[q][1] = on ? g1 : 0.0;
+        sg[q][2] = on ? g2 : 0.0;
     }
-    double aL, aR, eL[4], eR[4];
+    double aL, aR, eL0, eL1, eL2, eL3, eR0, eR1, eR2, eR3;
     if (NU) {
-        aL = edge_w(ec[c - 1], v[c - 1], v[c], sv[0], sv[1]);
-        aR = edge_w(ec[c], v[c], v[c + 1], sv[1], sv[2]);
-        eL[0] = 1.0 - ec[c - 1][0]; eL[1] = ec[c - 1][0]; eL[2] = ec[c - 1][2]; eL[3] = -ec[c - 1][1];
-        eR[0] = 1.0 - ec[c][0]; eR[1] = ec[c][0]; eR[2] = ec[c][2]; eR[3] = -ec[c][1];
+        aL = edge_w(e1, w[1], w[2], sv[0], sv[1]);
+        aR = edge_w(e2, w[2], w[3], sv[1], sv[2]);
+        eL0 = 1.0 - e1[0]; eL1 = e1[0]; eL2 = e1[2]; eL3 = -e1[1];
+        eR0 = 1.0 - e2[0]; eR1 = e2[0]; eR2 = e2[2]; eR3 = -e2[1];
     } else {
-        aL = edge_u(v[c - 1], v[c], sv[0], sv[1]);
-        aR = edge_u(v[c], v[c + 1], sv[1], sv[2]);
-        eL[0] = eR[0] = 0.5; eL[1] = eR[1] = 0.5; eL[2] = eR[2] = C16; eL[3] = eR[3] = -C16;
+        aL = edge_u(w[1], w[2], sv[0], sv[1]);
+        aR = edge_u(w[2], w[3], sv[1], sv[2]);
+        eL0 = eR0 = 0.5; eL1 = eR1 = 0.5; eL2 = eR2 = C16; eL3 = eR3 = -C16;
     }
-    const bool right = M >= 0.0;
-    double bc = 0.0, bL = 0.0, bR = 0.0;
-    if ((aR - v[c]) * (v[c] - aL) <= 0.0) {
-        bc = 1.0;
-    } else {
-        const double d = aR - aL;
-        const double a6 = 6.0 * (v[c] - 0.5 * (aL + aR));
-        const double dd = d * d;
-        const double da6 = d * a6;
-        if (da6 > dd) {                 // aL <- 3 a_c - 2 aR
-            if (right) bR = 1.0;
-            else { bc = 3.0; bR = -2.0; }
-        } else if (-dd > da6) {         // aR <- 3 a_c - 2 aL
-            if (right) { bc = 3.0; bL = -2.0; }
-            else bL = 1.0;
-        } else {
-            if (right) bR = 1.0;
-            else bL = 1.0;
-        }
-    }
-    g[c] += bc;
-    g[c - 1] += bL * eL[0];
-    g[c] += bL * eL[1];
-    const double bsm = bL * eL[2];
-    const double bs0 = bL * eL[3] + bR * eR[2];
-    g[c] += bR * eR[0];
-    g[c + 1] += bR * eR[1];
-    const double bsp = bR * eR[3];
+    const double a0 = w[2];
+    // limiter branch -> adjoints (bc, bL, bR) of (a0, aL, aR) for the output
+    const bool flat = (aR - a0) * (a0 - aL) <= 0.0;
+    const double d = aR - aL;
+    const double a6 = 6.0 * (a0 - 0.5 * (aL + aR));
+    const double dd = d * d;
+    const double da6 = d * a6;
+    const bool resetL = da6 > dd, resetR = !resetL && (-dd > da6);
+    double bc, bL, bR;
+    if (flat) { bc = 1.0; bL = 0.0; bR = 0.0; }
+    else if (right) { bc = resetR ? 3.0 : 0.0; bL = resetR ? -2.0 : 0.0; bR = resetR ? 0.0 : 1.0; }
+    else { bc = resetL ? 3.0 : 0.0; bR = resetL ? -2.0 : 0.0; bL = resetL ? 0.0 : 1.0; }
+    // the face value (the forward's limiter)
+    double lL = aL, lR = aR;
+    limit(a0, lL, lR);
+    g[0] = 0.0; g[1] = 0.0; g[2] = bc; g[3] = 0.0; g[4] = 0.0;
+    g[1] += bL * eL0;
+    g[2] += bL * eL1 + bR * eR0;
+    g[3] += bR * eR1;
+    const double bs0 = bL * eL2, bs1 = bL * eL3 + bR * eR2, bs2 = bR * eR3;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        g[c - 2 + q] += bsm * sg[0][q];
-        g[c - 1 + q] += bs0 * sg[1][q];
-        g[c + q] += bsp * sg[2][q];
+        g[q] += bs0 * sg[0][q];
+        g[q + 1] += bs1 * sg[1][q];
+        g[q + 2] += bs2 * sg[2][q];
     }
+    return right ? lR : lL;
+}
+
+// Local gradient g[d] = d(face value)/d v[d] of one face from its six stencil
+// values v[0..5] (cells f-3 .. f+2), for the per-cell gather below: the five
+// cells around the upwind cell (v[0..4] for M >= 0, v[1..5] otherwise) go
+// through face_grad5.  NU: width-weighted (vertical), with the slope factors
+// of stencil cell x at sR[x], sL[x] and the factors of the edge between
+// stencil cells x and x+1 at ec[x][0..2].
+template <bool NU>
+__device__ __forceinline__ void face_grad6(const double* v, double M, const double* sR,
+                                           const double* sL, const double (*ec)[3], double* g) {
+    const bool right = M >= 0.0;
+    double w[5], g5[5], r[3], l[3], e1[3], e2[3];
+#pragma unroll
+    for (int d = 0; d < 5; ++d) w[d] = right ? v[d] : v[d + 1];
+    if (NU) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            r[q] = right ? sR[1 + q] : sR[2 + q];
+            l[q] = right ? sL[1 + q] : sL[2 + q];
+            e1[q] = right ? ec[1][q] : ec[2][q];
+            e2[q] = right ? ec[2][q] : ec[3][q];
+        }
+    }
+    face_grad5<NU>(w, right, r, l, e1, e2, g5);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) g[d] = right ? (d < 5 ? g5[d] : 0.0) : (d > 0 ? g5[d - 1] : 0.0);
 }
 
 // out = [beta b +] x + h2 J(at)^T x, x = alpha a, J the Jacobian of the PPM
@@ -591,7 +606,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp(RsPPMVjpArgs A) {
                     if (F == i) fw_ = M * face_u(v, M);
                     if (F == i + 1) fe_ = M * face_u(v, M);
                     if (!hit || M == 0.0) continue;
-                    face_grad<false>(v, M, nullptr, nullptr, nullptr, g);
+                    face_grad6<false>(v, M, nullptr, nullptr, nullptr, g);
                     double gs = 0.0;
 #pragma unroll
                     for (int d = 0; d < 6; ++d) if (in[d]) gs += g[d];
@@ -617,7 +632,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp(RsPPMVjpArgs A) {
             if (F == j) fs_ = M * face_u(v, M);
             if (F == j + 1) fn_ = M * face_u(v, M);
             if (!hit || M == 0.0) continue;
-            face_grad<false>(v, M, nullptr, nullptr, nullptr, g);
+            face_grad6<false>(v, M, nullptr, nullptr, nullptr, g);
             double gs = 0.0;
 #pragma unroll
             for (int d = 0; d < 6; ++d) if (in[d]) gs += g[d];
@@ -663,7 +678,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp(RsPPMVjpArgs A) {
                     else ft_ = fv;
                 }
                 if (!hit || M == 0.0) continue;
-                face_grad<true>(v, M, sR, sL, ec, g);
+                face_grad6<true>(v, M, sR, sL, ec, g);
                 double gs = 0.0;
 #pragma unroll
                 for (int d = 0; d < 6; ++d) if (in[d]) gs += g[d];
@@ -701,75 +716,6 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp(RsPPMVjpArgs A) {
 //   G   = mu divM + acc_z;   G += acc_y;   out = [beta b +] x + h2 (G + acc_x)
 // and the air-mass row from the tendency F = L(at):
 //   out_m = [beta b_m +] x_m - h2 sum_s x_s F_s / m.
-
-// Local gradient of a face value over the five cells w[0..4] = the upwind
-// cell's neighbourhood c-2 .. c+2 (the six-cell stencil shifted by one for
-// M < 0); `right`: the value is the right edge of w[2] (M >= 0), else its left
-// edge.  Width factors (NU): slopes of w[1..3] (sR, sL), edges (w1, w2) and
-// (w2, w3) (e1, e2).  Branch-free in the flux sign.
-template <bool NU>
-__device__ __forceinline__ double face_grad5(const double* w, bool right, const double* sR,
-                                             const double* sL, const double* e1, const double* e2,
-                                             double* g) {
-    double sv[3], sg[3][3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const double am = w[q], a0 = w[q + 1], ap = w[q + 2];
-        const double cR = NU ? sR[q] : 0.5, cL = NU ? sL[q] : 0.5;
-        sv[q] = NU ? slope_w(cR, cL, am, a0, ap) : slope_u(am, a0, ap);
-        const double dl = a0 - am, dr = ap - a0;
-        const bool on = dr * dl > 0.0;
-        const double da = NU ? cR * dr + cL * dl : 0.5 * (ap - am);
-        const double ax = fabs(da), y = 2.0 * fabs(dl), z = 2.0 * fabs(dr);
-        const bool zl = z < y;
-        const double in = zl ? z : y;
-        const bool lim = in < ax;
-        // gradient of the chosen term: da (cR, cL), 2 dr or 2 dl
-        double g0 = -cL, g1 = cL - cR, g2 = cR;
-        if (lim) { g0 = zl ? 0.0 : -2.0; g1 = 2.0 * (zl ? -1.0 : 1.0); g2 = zl ? 2.0 : 0.0; }
-        sg[q][0] = on ? g0 : 0.0;
-        sg[q][1] = on ? g1 : 0.0;
-        sg[q][2] = on ? g2 : 0.0;
-    }
-    double aL, aR, eL0, eL1, eL2, eL3, eR0, eR1, eR2, eR3;
-    if (NU) {
-        aL = edge_w(e1, w[1], w[2], sv[0], sv[1]);
-        aR = edge_w(e2, w[2], w[3], sv[1], sv[2]);
-        eL0 = 1.0 - e1[0]; eL1 = e1[0]; eL2 = e1[2]; eL3 = -e1[1];
-        eR0 = 1.0 - e2[0]; eR1 = e2[0]; eR2 = e2[2]; eR3 = -e2[1];
-    } else {
-        aL = edge_u(w[1], w[2], sv[0], sv[1]);
-        aR = edge_u(w[2], w[3], sv[1], sv[2]);
-        eL0 = eR0 = 0.5; eL1 = eR1 = 0.5; eL2 = eR2 = C16; eL3 = eR3 = -C16;
-    }
-    const double a0 = w[2];
-    // limiter branch -> adjoints (bc, bL, bR) of (a0, aL, aR) for the output
-    const bool flat = (aR - a0) * (a0 - aL) <= 0.0;
-    const double d = aR - aL;
-    const double a6 = 6.0 * (a0 - 0.5 * (aL + aR));
-    const double dd = d * d;
-    const double da6 = d * a6;
-    const bool resetL = da6 > dd, resetR = !resetL && (-dd > da6);
-    double bc, bL, bR;
-    if (flat) { bc = 1.0; bL = 0.0; bR = 0.0; }
-    else if (right) { bc = resetR ? 3.0 : 0.0; bL = resetR ? -2.0 : 0.0; bR = resetR ? 0.0 : 1.0; }
-    else { bc = resetL ? 3.0 : 0.0; bR = resetL ? -2.0 : 0.0; bL = resetL ? 0.0 : 1.0; }
-    // the face value (the forward's limiter)
-    double lL = aL, lR = aR;
-    limit(a0, lL, lR);
-    g[0] = 0.0; g[1] = 0.0; g[2] = bc; g[3] = 0.0; g[4] = 0.0;
-    g[1] += bL * eL0;
-    g[2] += bL * eL1 + bR * eR0;
-    g[3] += bR * eR1;
-    const double bs0 = bL * eL2, bs1 = bL * eL3 + bR * eR2, bs2 = bR * eR3;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        g[q] += bs0 * sg[0][q];
-        g[q + 1] += bs1 * sg[1][q];
-        g[q + 2] += bs2 * sg[2][q];
-    }
-    return right ? lR : lL;
-}
 
 // Face divergence per cell, div(M) in the operator's order (species-independent).
 __global__ void __launch_bounds__(PB) rs_ppm_divm(RsPPMVjpArgs A, double* __restrict__ divm) {
