@@ -42,10 +42,8 @@ from reseact_esm_amd import hip_library, problems as P, split_opts  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # PPM transport: flops per mixing-ratio species-cell and stage, counted from the
-# operator's statements (ppm_roofline); the counter pass in the PPM kernel
-# profile reads 381 FP64 VALU lane operations for the same work (min/max and
-# the final division's Newton steps included there)
-PPM_FLOPS_SPECIES_CELL_STAGE = 330.0
+# operator's statements with each face reconstructed once (ppm_roofline)
+PPM_FLOPS_SPECIES_CELL_STAGE = 172.0
 
 
 def log(msg: str) -> None:
@@ -223,11 +221,12 @@ def ppm_roofline(lib, nsteps: int = 10, nq: int = 20):
     bytes_attempt = p.nc * (96.0 * nt + 4.0 * per_cell_stage)
     gbs = bytes_attempt / (ms * 1e-3) / 1e9
     # algorithmic flops per mixing-ratio species-cell and stage, counted from
-    # the operator's statements (min/max and compares not counted): four
-    # uniform faces x 52 (3 slopes, 2 edges, limiter), two width-weighted
-    # vertical faces x 51, the six face fluxes and the divergence 15, the CWC
-    # tendency 3, the stage update 2; the last stage adds 9 for the error
-    # norm; the air mass takes 11 per stage (divergence and update)
+    # the operator's statements (min/max and compares not counted), each face
+    # reconstructed once (a cell owns one face per direction): two uniform
+    # faces x 52 (3 slopes, 2 edges, limiter), one width-weighted vertical
+    # face 51, three face fluxes and the divergence 12, the CWC tendency 3,
+    # the stage update 2; the last stage adds 9 for the error norm; the air
+    # mass takes 11 per stage (divergence and update)
     flops_attempt = p.nc * ((nt - 1) * (4.0 * PPM_FLOPS_SPECIES_CELL_STAGE + 9.0) + 4.0 * 11.0 + 9.0)
     tf = flops_attempt / (ms * 1e-3) / 1e12
     log(f"ppm {p.name}: {ms:.3f} ms/attempt (stage {ms_s / max(ns_, 1):.3f} ms, last "

@@ -11,16 +11,16 @@
 // Lateral open walls read 3 halo cells of prescribed inflow mixing ratio;
 // the vertical clamps its stencil at the surface and the lid (Mz = 0 there).
 //
-// One lane per cell, all species of that cell: the air-mass fluxes, metrics,
-// the divergence of M and the vertical width factors are loaded once and
-// serve every species.  The horizontal faces of a workgroup's tile are
-// reconstructed once each from an LDS copy of the tile; the vertical ones per
-// cell from the column.  The width-only vertical factors are tabulated per
-// column when the fluxes are set (rs_ppm_coefs), so a stage does no divisions
-// besides the final one by m.  Built with -ffp-contract=off: every statement
-// rounds as the oracle's, so stages agree with it bit for bit.
+// A stage workgroup walks one species up a tile of columns: the horizontal
+// faces of each level are reconstructed once each from an LDS copy of the
+// tile, the vertical ones once each from the lane's sliding column window.
+// The width-only vertical factors are tabulated per column when the fluxes
+// are set (rs_ppm_coefs), so a stage does no divisions besides the final one
+// by m.  Built with -ffp-contract=off: every statement rounds as the
+// oracle's, so stages agree with it bit for bit.
 
 #include <hip/hip_runtime.h>
+
 
 #include "rs_launch.h"
 
@@ -38,6 +38,9 @@ __device__ __forceinline__ int wrapi(int i, int n) {
     i %= n;
     return i < 0 ? i + n : i;
 }
+
+// wrapi for -n <= i < 2n, without the integer division
+__device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
@@ -176,34 +179,41 @@ __global__ void __launch_bounds__(PB) rs_ppm_coefs(const double* __restrict__ dp
     }
 }
 
-// One SSPRK43 stage over the transported block (KIND 3: out = L a, the
-// tendency alone, for the adjoint).  KIND 0: out = a + h/2 L a;
-// KIND 2 (third stage): the same, and out2 = a4 = 2/3 u + 1/3 out (u = b);
-// KIND 1 (last): unew = a4 + h/2 L a4 (a4 = c4), the error partial from
-// u = a and u3 = b, and (positive domain) the count of negative entries.
-//
-// A workgroup is a tile of PTX longitudes x PTY latitudes at one level.  Per
-// mixing-ratio species the tile and its 3-cell halo are staged in LDS, every
-// longitude and latitude face of the tile is reconstructed once (by one lane)
-// into a face-flux array, and each lane then takes its cell's divergence
-// from the four horizontal face fluxes and its own two vertical faces (the
-// vertical stencil comes from global memory, clamped to the column).
 constexpr int PTX = 64, PTY = 4;
 constexpr int PWX = PTX + 6, PWY = PTY + 6;                 // tile + halo
 constexpr int NFX = (PTX + 1) * PTY, NFY = PTX * (PTY + 1);  // faces per tile
 
+// One SSPRK43 stage over the transported block.  KIND 0: out = a + h/2 L a
+// (and, with outf, the tendency L a beside it, for the adjoint); KIND 2
+// (third stage): the same, and out2 = a4 = 2/3 u + 1/3 out (u = b); KIND 1
+// (last): unew = a4 + h/2 L a4 (a4 = c4), the error partial from u = a and
+// u3 = b, and (positive domain) the count of negative entries; KIND 3:
+// out = L a, the tendency alone.
+//
+// A workgroup is one species on a tile of PTX longitudes x PTY latitudes over
+// a block of A.kblk levels, which it walks upwards.  Per level the tile and
+// its 3-cell halo are staged in LDS (double-buffered, so two barriers per
+// level) and every longitude and latitude face of the tile is reconstructed
+// once, by one lane, into a face-flux array.  In the vertical each lane
+// carries its column's stencil values, the slope of the cell above, the
+// unlimited edge above and the limited parabola of its cell, so a level adds
+// one value, one slope, one edge and one limited parabola: each vertical face
+// is reconstructed once too.  The lane then takes its cell's divergence from
+// the six face fluxes.
+// Grid: x = species (the species of a tile run side by side and share its
+// fluxes and width factors in cache), y = tile, z = level block.
 template <int KIND>
 __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
     __shared__ double red[PB / 64];
-    __shared__ double Q[PWY][PWX];
-    __shared__ double FX[PTY][PTX + 1];
-    __shared__ double FY[PTY + 1][PTX];
+    __shared__ double Q[2][PWY][PWX];
+    __shared__ double FX[2][PTY][PTX + 1];
+    __shared__ double FY[2][PTY + 1][PTX];
     const RsCtrl* ctl = A.ctrl;
     if (ctl && ctl->done) return;
     const int cur = ctl ? ctl->cur : A.cur_static;
     const double dt = ctl ? ctl->dtc : A.dt_static;
     const double h2 = dt / 2;
-    const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev, nt = A.nt;
+    const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev;
     const long long nc = A.nc, plane = (long long)nlon * nlat;
     const double* __restrict__ pa = A.a_is_state ? (cur ? A.S1 : A.S0) : A.a;
     const double* __restrict__ pb = A.b_is_state ? (cur ? A.S1 : A.S0) : A.b;
@@ -211,54 +221,69 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
     double* __restrict__ po = A.out_is_state ? (cur ? A.S0 : A.S1) : A.out;
     const int tid = threadIdx.x;
     const int tx = tid % PTX, ty = tid / PTX;
-    const int i0 = blockIdx.x * PTX, j0 = blockIdx.y * PTY, k = blockIdx.z;
+    const int ntx = (nlon + PTX - 1) / PTX;
+    const int s = blockIdx.x;
+    const int i0 = (int)(blockIdx.y % ntx) * PTX, j0 = (int)(blockIdx.y / ntx) * PTY;
+    const int k0 = blockIdx.z * A.kblk, k1 = min(nlev, k0 + A.kblk);
     const int i = i0 + tx, j = j0 + ty;
     const bool valid = (i < nlon) && (j < nlat);
     const long long col = (long long)j * nlon + i;
-    const long long cell = (long long)k * plane + col;
-    // per-cell quantities shared by every species
-    double mw = 0.0, me = 0.0, ms = 0.0, mn = 0.0, mb = 0.0, mt = 0.0;
-    double gx = 0.0, ces = 0.0, cen = 0.0, rdS = 0.0, divM = 0.0, mval = 1.0;
-    double sR[5], sL[5], ec[4][3];
+    const long long sb = (long long)s * nc;
+    const int sq1 = s - 1;
+    const long long nrs = (long long)(nlev + 4) * plane, nre = (long long)(nlev + 3) * plane;
+    double gx = 0.0, ces = 0.0, cen = 0.0, rdS = 0.0;
     if (valid) {
-        const double* mxr = A.Mx + ((long long)k * nlat + j) * (nlon + 1);
-        mw = mxr[i];
-        me = mxr[i + 1];
-        ms = A.My[((long long)k * (nlat + 1) + j) * nlon + i];
-        mn = A.My[((long long)k * (nlat + 1) + j + 1) * nlon + i];
-        mb = A.Mz[(long long)k * plane + col];
-        mt = A.Mz[(long long)(k + 1) * plane + col];
         gx = A.gx[j];
         ces = A.ce[j];
         cen = A.ce[j + 1];
         rdS = A.rdS[j];
-        divM = div6(mw, me, ms, mn, mb, mt, gx, ces, cen, rdS);
-        mval = ps[cell];
-        // vertical width factors of the faces k (bottom) and k+1 (top):
-        // slopes of cells k-2..k+2, edges k-2..k+1
-        const long long nrs = (long long)(nlev + 4) * plane, nre = (long long)(nlev + 3) * plane;
+    }
+    // value of the lane's column at level x, clamped to the column
+    auto V = [&](int x) { return ps[sb + (long long)clampi(x, 0, nlev - 1) * plane + col]; };
+    // slope of cell x (factors at row x + 2) and edge between x and x+1 (row x + 2)
+    auto SL = [&](int x, double am, double a0, double ap) {
+        const long long r = (long long)(x + 2) * plane + col;
+        return slope_w(A.vs[r], A.vs[nrs + r], am, a0, ap);
+    };
+    auto ED = [&](int x, double a0, double a1, double d0, double d1) {
+        const long long r = (long long)(x + 2) * plane + col;
+        const double c[3] = {A.ve[r], A.ve[nre + r], A.ve[2 * nre + r]};
+        return edge_w(c, a0, a1, d0, d1);
+    };
+    // vertical walk state at level k: flux of face k (fb), limited parabola
+    // of cell k (pL, pR), unlimited edge (k, k+1) (e1), slope of k+1 (d1),
+    // values of k+1, k+2 (v1, v2), air-mass flux of face k (mb)
+    double fb = 0.0, pL = 0.0, pR = 0.0, e1 = 0.0, d1 = 0.0, v1 = 0.0, v2 = 0.0, mb = 0.0;
+    if (valid) {
+        mb = A.Mz[(long long)k0 * plane + col];
+        if (s > 0) {
+            double w[6];
 #pragma unroll
-        for (int d = 0; d < 5; ++d) {
-            const long long r = (long long)(k + d) * plane + col;   // row x + 2, x = k - 2 + d
-            sR[d] = A.vs[r];
-            sL[d] = A.vs[nrs + r];
-        }
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            const long long r = (long long)(k + d) * plane + col;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) ec[d][q] = A.ve[q * nre + r];
+            for (int d = 0; d < 6; ++d) w[d] = V(k0 - 3 + d);
+            const double sm2 = SL(k0 - 2, w[0], w[1], w[2]);
+            const double sm1 = SL(k0 - 1, w[1], w[2], w[3]);
+            const double s0 = SL(k0, w[2], w[3], w[4]);
+            const double sp1 = SL(k0 + 1, w[3], w[4], w[5]);
+            double qL = ED(k0 - 2, w[1], w[2], sm2, sm1);
+            double qR = ED(k0 - 1, w[2], w[3], sm1, s0);
+            pL = qR;
+            pR = ED(k0, w[3], w[4], s0, sp1);
+            e1 = pR;
+            limit(w[2], qL, qR);     // cell k0 - 1
+            limit(w[3], pL, pR);     // cell k0
+            fb = mb * (mb >= 0.0 ? qR : pL);
+            d1 = sp1;
+            v1 = w[4];
+            v2 = w[5];
         }
     }
     double sq = 0.0, ng = 0.0;
-    for (int s = 0; s < nt; ++s) {
-        const long long sb = (long long)s * nc;
+    for (int k = k0; k < k1; ++k) {
+        const int bf = (k - k0) & 1;
+        const long long cell = (long long)k * plane + col;
         const long long g = sb + cell;
-        double f = 0.0;
-        if (s == 0) {
-            if (valid) f = A.msrc ? -divM + A.msrc[cell] : -divM;
-        } else {
-            const int sq1 = s - 1;
+        double f = 0.0, ac = 0.0, ft = 0.0, mt = 0.0;
+        if (s > 0) {
             // ---- stage the tile and its halo (corners are never read)
             for (int x = tid; x < PWX * PWY; x += PB) {
                 const int r = x / PWX, q = x - r * PWX;
@@ -269,28 +294,29 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
                 if (inlat) {
                     const long long rowb = sb + (long long)k * plane + (long long)gj * nlon;
                     if (inlon) v = ps[rowb + gi];
-                    else if (A.lon_periodic) v = ps[rowb + wrapi(gi, nlon)];
+                    else if (A.lon_periodic) {
+                        // faces reach cells <= nlon + 2; columns past them stay 0
+                        if (gi <= nlon + 2) v = ps[rowb + (nlon >= 3 ? wrap1(gi, nlon) : wrapi(gi, nlon))];
+                    }
                     else if (gi < 0) v = A.qw[((long long)sq1 * nlev + k) * nlat + gj];
                     else v = A.qe[((long long)sq1 * nlev + k) * nlat + gj];
                 } else if (inlon) {
                     if (gj < 0) v = A.qs[((long long)sq1 * nlev + k) * nlon + gi];
                     else v = A.qn[((long long)sq1 * nlev + k) * nlon + gi];
                 }
-                Q[r][q] = v;
+                Q[bf][r][q] = v;
             }
             __syncthreads();
-            // ---- longitude faces i0 + fx (fx = 0..PTX) of rows j0 + fr
             for (int x = tid; x < NFX; x += PB) {
                 const int fr = x / (PTX + 1), fx = x - fr * (PTX + 1);
                 const int gj = j0 + fr, gi = i0 + fx;
                 double F = 0.0;
                 if (gj < nlat && gi <= nlon) {
                     const double M = A.Mx[((long long)k * nlat + gj) * (nlon + 1) + gi];
-                    F = M * face_u(&Q[fr + 3][fx], M);
+                    F = M * face_u(&Q[bf][fr + 3][fx], M);
                 }
-                FX[fr][fx] = F;
+                FX[bf][fr][fx] = F;
             }
-            // ---- latitude faces j0 + fy (fy = 0..PTY) of columns i0 + fc
             for (int x = tid; x < NFY; x += PB) {
                 const int fy = x / PTX, fc = x - fy * PTX;
                 const int gj = j0 + fy, gi = i0 + fc;
@@ -299,60 +325,50 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
                     const double M = A.My[((long long)k * (nlat + 1) + gj) * nlon + gi];
                     double v[6];
 #pragma unroll
-                    for (int d = 0; d < 6; ++d) v[d] = Q[fy + d][fc + 3];
+                    for (int d = 0; d < 6; ++d) v[d] = Q[bf][fy + d][fc + 3];
                     F = M * face_u(v, M);
                 }
-                FY[fy][fc] = F;
+                FY[bf][fy][fc] = F;
             }
-            // ---- this cell's vertical faces
-            double fb = 0.0, ft = 0.0;
             if (valid) {
-                double v[7];
-#pragma unroll
-                for (int d = 0; d < 7; ++d)
-                    v[d] = ps[sb + (long long)clampi(k - 3 + d, 0, nlev - 1) * plane + col];
-                // physical cell x is v[x - k + 3]; its slope factors and the
-                // edge between x and x+1 sit at index x - k + 2
-                auto sl_ = [&](int x) {
-                    const int d = x - k + 3, c = x - k + 2;
-                    return slope_w(sR[c], sL[c], v[d - 1], v[d], v[d + 1]);
-                };
-                auto fz = [&](int fface, double M) -> double {
-                    if (M >= 0.0) {
-                        const double dm2 = sl_(fface - 2), dm1 = sl_(fface - 1), d0 = sl_(fface);
-                        const int e1 = fface - 2 - k + 2, e2 = fface - 1 - k + 2;
-                        const int c1 = fface - 2 - k + 3;   // v index of cell fface-2
-                        double aL = edge_w(ec[e1], v[c1], v[c1 + 1], dm2, dm1);
-                        double aR = edge_w(ec[e2], v[c1 + 1], v[c1 + 2], dm1, d0);
-                        limit(v[c1 + 1], aL, aR);
-                        return aR;
-                    }
-                    const double dm1 = sl_(fface - 1), d0 = sl_(fface), dp1 = sl_(fface + 1);
-                    const int e1 = fface - 1 - k + 2, e2 = fface - k + 2;
-                    const int c1 = fface - 1 - k + 3;       // v index of cell fface-1
-                    double aL = edge_w(ec[e1], v[c1], v[c1 + 1], dm1, d0);
-                    double aR = edge_w(ec[e2], v[c1 + 1], v[c1 + 2], d0, dp1);
-                    limit(v[c1 + 1], aL, aR);
-                    return aL;
-                };
-                fb = mb * fz(k, mb);
-                ft = mt * fz(k + 1, mt);
+                // ---- face k+1: one new value, slope, edge and parabola
+                mt = A.Mz[(long long)(k + 1) * plane + col];
+                const double v3 = V(k + 3);
+                const double d2 = SL(k + 2, v1, v2, v3);
+                const double e2 = ED(k + 1, v1, v2, d1, d2);
+                double nL = e1, nR = e2;
+                limit(v1, nL, nR);       // cell k + 1
+                ft = mt * (mt >= 0.0 ? pR : nL);
+                pL = nL;
+                pR = nR;
+                e1 = e2;
+                d1 = d2;
+                v1 = v2;
+                v2 = v3;
             }
             __syncthreads();
-            if (valid) {
-                const double divF = div6(FX[ty][tx], FX[ty][tx + 1], FY[ty][tx], FY[ty + 1][tx],
-                                         fb, ft, gx, ces, cen, rdS);
-                const double qc = Q[ty + 3][tx + 3];
-                const double num = qc * divM - divF;
-                f = num / mval;
-            }
+        } else if (valid) {
+            mt = A.Mz[(long long)(k + 1) * plane + col];
         }
         if (valid) {
-            const double ac = (s == 0) ? mval : Q[ty + 3][tx + 3];
+            const double* mxr = A.Mx + ((long long)k * nlat + j) * (nlon + 1);
+            const double* myr = A.My + ((long long)k * (nlat + 1) + j) * nlon + i;
+            const double divM = div6(mxr[i], mxr[i + 1], myr[0], myr[nlon], mb, mt, gx, ces, cen, rdS);
+            if (s == 0) {
+                f = A.msrc ? -divM + A.msrc[cell] : -divM;
+                ac = ps[cell];
+            } else {
+                const double divF = div6(FX[bf][ty][tx], FX[bf][ty][tx + 1], FY[bf][ty][tx],
+                                         FY[bf][ty + 1][tx], fb, ft, gx, ces, cen, rdS);
+                const double qc = Q[bf][ty + 3][tx + 3];
+                const double num = qc * divM - divF;
+                f = num / ps[cell];
+                ac = qc;
+            }
             const double du = h2 * f;
             const double un = ac + du;
             if (KIND == 3) {
-                po[g] = f;                 // the tendency itself (adjoint's m row)
+                po[g] = f;
             } else if (KIND == 1) {
                 const double u = pa[g], u3 = pb[g];
                 const double a = C13 * u;
@@ -376,11 +392,13 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
                     A.out2[g] = a + b;
                 }
             }
+            fb = ft;
+            mb = mt;
         }
     }
     if (KIND == 1) {
-        const long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         const double t = block_sum(sq, red, tid);
+        const long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         if (tid == 0) A.partial[bid] = t;
         if (A.negpart) {
             const double n = block_sum(ng, red, tid);
@@ -742,18 +760,21 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_sweep(RsPPMVjpArgs A, const dou
     const long long nc = A.nc, plane = (long long)nlon * nlat;
     const long long lane = (long long)blockIdx.x * PB + threadIdx.x;
     const int n = DIR == 1 ? nlat : nlev;                             // cells on the line
-    const long long nlines = (long long)nq * (DIR == 1 ? (long long)nlev * nlon : plane);
-    if (lane >= nlines) return;
-    int s, k = 0, i = 0;
+    // lines per species; a wave takes 64 consecutive lines of one species and
+    // the next waves the same lines of the next species, so the species of a
+    // line run side by side and share its fluxes and width factors in cache
+    const long long per = DIR == 1 ? (long long)nlev * nlon : plane;
+    const long long blk = lane / 64;
+    const long long idx = (blk / nq) * 64 + lane % 64;
+    if (idx >= per) return;
+    const int s = 1 + (int)(blk % nq);
+    int k = 0, i = 0;
     long long col = 0;
     if (DIR == 2) {
-        s = 1 + (int)(lane / plane);
-        col = lane % plane;
+        col = idx;
     } else {
-        s = 1 + (int)(lane / ((long long)nlev * nlon));
-        const long long r = lane % ((long long)nlev * nlon);
-        k = (int)(r / nlon);
-        i = (int)(r % nlon);
+        k = (int)(idx / nlon);
+        i = (int)(idx % nlon);
     }
     const long long sb = (long long)s * nc;
     const double* __restrict__ q = A.at + sb;
@@ -900,7 +921,7 @@ constexpr int XFW = PTX + 5;                  // logical faces per tile row
 __global__ void __launch_bounds__(PB) rs_ppm_vjp_xtile(RsPPMVjpArgs A, double* __restrict__ G) {
     __shared__ double Qx[PTY][PTX + 10];
     __shared__ double MU[PTY][PTX + 6];
-    __shared__ double CF[PTY][XFW][6];
+    __shared__ double CF[6][PTY][XFW];      // slot-major: lanes read consecutive words
     const int nlon = A.nlon, nlat = A.nlat, nlev = A.nlev;
     const long long nc = A.nc, plane = (long long)nlon * nlat;
     const int tid = threadIdx.x;
@@ -919,7 +940,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_xtile(RsPPMVjpArgs A, double* _
         double v = 0.0;
         if (gj < nlat) {
             const long long rowb = (long long)k * plane + (long long)gj * nlon;
-            if (periodic) v = q[rowb + wrapi(gi, nlon)];
+            if (periodic) { if (gi <= nlon + 4) v = q[rowb + wrap1(gi, nlon)]; }   // faces <= nlon + 2 reach cells <= nlon + 4
             else if (gi < 0) v = A.qw[((long long)sq1 * nlev + k) * nlat + gj];
             else if (gi >= nlon) v = A.qe[((long long)sq1 * nlev + k) * nlat + gj];
             else v = q[rowb + gi];
@@ -930,8 +951,8 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_xtile(RsPPMVjpArgs A, double* _
         const int r = x / (PTX + 6), cc = x - r * (PTX + 6);
         const int gj = j0 + r, gi = i0 - 3 + cc;
         double v = 0.0;
-        if (gj < nlat && (periodic || (gi >= 0 && gi < nlon))) {
-            const long long c = (long long)k * plane + (long long)gj * nlon + (periodic ? wrapi(gi, nlon) : gi);
+        if (gj < nlat && (periodic ? gi <= nlon + 2 : (gi >= 0 && gi < nlon))) {
+            const long long c = (long long)k * plane + (long long)gj * nlon + (periodic ? wrap1(gi, nlon) : gi);
             const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
             v = xv / A.at[c];
         }
@@ -964,12 +985,14 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_xtile(RsPPMVjpArgs A, double* _
                 }
             };
             if (periodic) {
-                const int F = wrapi(FL, nlon);
-                const double M = mxr[F];
-                add_face(M, M * ((MU[r][fo + 1] * gx) - (F >= 1 ? MU[r][fo] * gx : 0.0)));
-                if (F == 0) {
-                    const double Mn = mxr[nlon];
-                    add_face(Mn, Mn * (0.0 - MU[r][fo] * gx));
+                if (FL <= nlon + 2) {           // cells i < nlon gather faces i-2 .. i+3
+                    const int F = wrap1(FL, nlon);
+                    const double M = mxr[F];
+                    add_face(M, M * ((MU[r][fo + 1] * gx) - (F >= 1 ? MU[r][fo] * gx : 0.0)));
+                    if (F == 0) {
+                        const double Mn = mxr[nlon];
+                        add_face(Mn, Mn * (0.0 - MU[r][fo] * gx));
+                    }
                 }
             } else if (FL >= 0 && FL <= nlon) {
                 const double M = mxr[FL];
@@ -977,7 +1000,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_xtile(RsPPMVjpArgs A, double* _
             }
         }
 #pragma unroll
-        for (int d = 0; d < 6; ++d) CF[r][fo][d] = cf[d];
+        for (int d = 0; d < 6; ++d) CF[d][r][fo] = cf[d];
     }
     __syncthreads();
     const int tx = tid % PTX, ty = tid / PTX;
@@ -986,7 +1009,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_xtile(RsPPMVjpArgs A, double* _
     // faces FL = i-2 .. i+3 are fo = tx .. tx+5; this cell is slot i - FL + 3
     double a = 0.0;
 #pragma unroll
-    for (int o = 0; o < 6; ++o) a += CF[ty][tx + o][5 - o];
+    for (int o = 0; o < 6; ++o) a += CF[5 - o][ty][tx + o];
     const long long c = (long long)k * plane + (long long)j * nlon + i;
     const double xv = alpha == 1.0 ? la[c] : alpha * la[c];
     double o = xv + A.h2 * (G[sb - nc + c] + a);
@@ -1014,8 +1037,12 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_mrow(RsPPMVjpArgs A, const doub
     A.out[cell] = o;
 }
 
-long long rs_ppm_nblocks(int nlon, int nlat, int nlev) {
-    return (long long)((nlon + PTX - 1) / PTX) * ((nlat + PTY - 1) / PTY) * nlev;
+// Level blocks of the column stage: about PKB levels each.
+constexpr int PKB = 24;
+static int ppm_nkb(int nlev) { return (nlev + PKB - 1) / PKB; }
+
+long long rs_ppm_nblocks(int nlon, int nlat, int nlev, int nt) {
+    return (long long)((nlon + PTX - 1) / PTX) * ((nlat + PTY - 1) / PTY) * nt * ppm_nkb(nlev);
 }
 
 hipError_t rs_launch_ppm_source(const double* Mx, const double* My, const double* dp,
@@ -1038,12 +1065,16 @@ hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlo
 
 // kind 0: first and second stages; 2: third stage (also a4); 1: last stage
 hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st) {
-    const dim3 grid((unsigned)((a->nlon + PTX - 1) / PTX), (unsigned)((a->nlat + PTY - 1) / PTY),
-                    (unsigned)a->nlev);
-    if (kind == 1) hipLaunchKernelGGL(rs_ppm_stage<1>, grid, dim3(PB), 0, st, *a);
-    else if (kind == 2) hipLaunchKernelGGL(rs_ppm_stage<2>, grid, dim3(PB), 0, st, *a);
-    else if (kind == 3) hipLaunchKernelGGL(rs_ppm_stage<3>, grid, dim3(PB), 0, st, *a);
-    else hipLaunchKernelGGL(rs_ppm_stage<0>, grid, dim3(PB), 0, st, *a);
+    if (a->nt < 1 || a->nlev < 1) return hipErrorInvalidValue;
+    RsPPMArgs b = *a;
+    const int nkb = ppm_nkb(a->nlev);
+    b.kblk = (a->nlev + nkb - 1) / nkb;
+    const long long ntile = (long long)((a->nlon + PTX - 1) / PTX) * ((a->nlat + PTY - 1) / PTY);
+    const dim3 grid((unsigned)a->nt, (unsigned)ntile, (unsigned)nkb);
+    if (kind == 1) hipLaunchKernelGGL(rs_ppm_stage<1>, grid, dim3(PB), 0, st, b);
+    else if (kind == 2) hipLaunchKernelGGL(rs_ppm_stage<2>, grid, dim3(PB), 0, st, b);
+    else if (kind == 3) hipLaunchKernelGGL(rs_ppm_stage<3>, grid, dim3(PB), 0, st, b);
+    else hipLaunchKernelGGL(rs_ppm_stage<0>, grid, dim3(PB), 0, st, b);
     return hipGetLastError();
 }
 
@@ -1063,10 +1094,12 @@ hipError_t rs_launch_ppm_vjp_sweeps(const RsPPMVjpArgs* a, const double* F, doub
     auto blocks = [](long long n) { return dim3((unsigned)((n + PB - 1) / PB)); };
     if (nq > 0) {
         hipLaunchKernelGGL(rs_ppm_divm, blocks(a->nc), dim3(PB), 0, st, *a, divm);
-        hipLaunchKernelGGL(rs_ppm_vjp_sweep<2>, blocks(nq * plane), dim3(PB), 0, st, *a,
+        // 64-line blocks per species, interleaved over the species
+        auto lines = [&](long long per) { return nq * ((per + 63) / 64) * 64; };
+        hipLaunchKernelGGL(rs_ppm_vjp_sweep<2>, blocks(lines(plane)), dim3(PB), 0, st, *a,
                            (const double*)divm, G);
-        hipLaunchKernelGGL(rs_ppm_vjp_sweep<1>, blocks(nq * a->nlev * a->nlon), dim3(PB), 0, st, *a,
-                           (const double*)divm, G);
+        hipLaunchKernelGGL(rs_ppm_vjp_sweep<1>, blocks(lines((long long)a->nlev * a->nlon)), dim3(PB),
+                           0, st, *a, (const double*)divm, G);
         const dim3 xg((unsigned)((a->nlon + PTX - 1) / PTX), (unsigned)((a->nlat + PTY - 1) / PTY),
                       (unsigned)(nq * a->nlev));
         hipLaunchKernelGGL(rs_ppm_vjp_xtile, xg, dim3(PB), 0, st, *a, G);

@@ -61,7 +61,7 @@ hipError_t rs_launch_loss_seed(double* lam, const double* u, long long ntot, lon
                                long long ncol, hipStream_t st);
 }
 // from ppm.hip
-long long rs_ppm_nblocks(int nlon, int nlat, int nlev);
+long long rs_ppm_nblocks(int nlon, int nlat, int nlev, int nt);
 hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlon, int nlat,
                                int nlev, hipStream_t st);
 hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st);
@@ -401,7 +401,7 @@ bool trans_ppm(const rs_ctx* c) { return c->scheme == RS_TRANSPORT_PPM; }
 bool trans_fused(const rs_ctx* c) { return c->tmode == 1 && c->fplan_ok && !trans_ppm(c); }
 
 long long trans_nblocks(rs_ctx* c) {
-    if (trans_ppm(c)) return rs_ppm_nblocks(c->nlon, c->nlat, c->nlev);
+    if (trans_ppm(c)) return rs_ppm_nblocks(c->nlon, c->nlat, c->nlev, c->nt);
     if (trans_fused(c)) return rs_transport_fused_nblocks(c->nlat, c->nt, c->fplan[3]);
     return rs_transport_nblocks(c->nlon, c->nlat, c->nlev, c->nt, c->tk);
 }
@@ -1173,7 +1173,7 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
         }
         select_fused(c, c->fnth_req);
         // PPM stages: one workgroup per tile of a level
-        np = std::max<long long>(np, rs_ppm_nblocks(nlon, nlat, nlev));
+        np = std::max<long long>(np, rs_ppm_nblocks(nlon, nlat, nlev, std::max(nt, 1)));
         c->partial.resize((size_t)np);
         c->negpart.resize((size_t)np);
         c->red.resize(1024);

@@ -127,7 +127,8 @@ typedef struct RsPPMArgs {
     double dt_static, atol, rtol;
     int nlon, nlat, nlev, nt;
     int cur_static, a_is_state, out_is_state, clamp;
-    int lon_periodic, posdom, b_is_state, pad;
+    int lon_periodic, posdom, b_is_state;
+    int kblk;               /* levels per workgroup (set by the launcher) */
 } RsPPMArgs;
 
 /* Adjoint of the PPM tendency at the state `at` (species 0 = air mass):
