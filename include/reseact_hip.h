@@ -363,6 +363,7 @@ RS_API int rs_autotune(rs_ctx* ctx, rs_tune_result* out);
  * backward sweep (replays + VJPs over all split steps) is one graph launch.
  * The groups' tapes are slices of one device pool, sized once for the
  * largest group and kept by the ensemble.
+ * Members may use either transport scheme (upwind or PPM).
  * Each member's gradients stay in its context (rs_get_lambda = dJ/du0,
  * rs_get_grad_emis).  No counterpart in the reference, which runs one domain
  * per process. */

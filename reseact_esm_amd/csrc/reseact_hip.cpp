@@ -887,17 +887,24 @@ void enqueue_ssprk43_vjp(rs_ctx* c, double dt) {
 //   LC = x + h/2 J(u2)^T x,   x = 1/3 LB        (l2)
 //   LA = LC + h/2 J(u1)^T LC                    (l1)
 //   lam = 2/3 LB + (LA + h/2 J(u)^T LA)
+// The PPM VJP uses line sweeps (each face gradient once), or the per-cell
+// gather on a periodic row shorter than a stencil.
+bool ppm_vjp_sweeps(const rs_ctx* c) { return !(c->lon_periodic && c->nlon < 6); }
+
+// Scratch of the PPM VJP sweeps: the tendencies at u, u1, u2 (written by the
+// recompute stages) and u4, the partial sums, div(M).  Sized before a graph
+// capture, so that the enqueue below allocates nothing there.
+void ppm_vjp_scratch(rs_ctx* c) {
+    if (!ppm_vjp_sweeps(c)) return;
+    c->pF.ensure(4 * (size_t)c->nt * c->nc);
+    c->pG.ensure((size_t)std::max(c->nt - 1, 1) * c->nc);
+    c->pdivm.ensure((size_t)c->nc);
+}
+
 void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
-    // line sweeps (each face gradient once), or the per-cell gather on a
-    // periodic row shorter than a stencil
-    const bool sweeps = !(c->lon_periodic && c->nlon < 6);
+    const bool sweeps = ppm_vjp_sweeps(c);
     const size_t NT = (size_t)c->nt * c->nc;
-    if (sweeps) {
-        // the tendencies at u, u1, u2 (written by the recompute stages) and u4
-        c->pF.ensure(4 * NT);
-        c->pG.ensure((size_t)std::max(c->nt - 1, 1) * c->nc);
-        c->pdivm.ensure((size_t)c->nc);
-    }
+    ppm_vjp_scratch(c);
     auto Fslot = [&](int q) { return c->pF.p + (size_t)q * NT; };
     RsPPMArgs a = ppm_args(c, dt);
     a.a = u;
@@ -2012,8 +2019,6 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
         int rc = check_ready(e->m[d], true, true);
         if (rc) return ens_fail(e, rc, "member " + std::to_string(d) + ": " + e->m[d]->err);
         if (loss_species < 0 || loss_species >= e->m[d]->ns) return ens_fail(e, RS_E_ARG, "bad loss species");
-        if (trans_ppm(e->m[d]) && e->m[d]->nt > 0)
-            return ens_fail(e, RS_E_ARG, "member " + std::to_string(d) + ": no adjoint of the PPM transport");
     }
     using clk = std::chrono::steady_clock;
     rs_ensemble_stats ES{};
@@ -2036,6 +2041,7 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
             const long long cap = std::min<long long>(o->maxiters, 1 << 22);
             c->seqlog.ensure((size_t)(2 * std::max<long long>(cap, 1)));
             c->relpart.ensure((size_t)nmacro * 1024);
+            if (trans_ppm(c) && c->nt > 0) ppm_vjp_scratch(c);
             HIPCHK(hipStreamSynchronize(c->stream));
         }
         const int CH = 8;   // attempts per member per graph launch
