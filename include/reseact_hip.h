@@ -186,6 +186,20 @@ RS_API int rs_set_inflow(rs_ctx* ctx, const double* qbc_w, const double* qbc_e,
  * set again; call after both. */
 RS_API int rs_set_pressure_fixer(rs_ctx* ctx, const double* dPSdt);
 
+/* PBL mixing of the mixing ratios (reseact.esm:1563-1690, the tracer
+ * tendency's GEOS-Chem-style full PBL mixing; weights :7536 pbl_f, :7698
+ * pbl_wt, :7747 pbl_sum_X, :8455 pbl_mean_X; tau_pblmix :1170): each mixing
+ * ratio q gains
+ *   pbl_f (pbl_mean - q) / tau_pblmix,
+ *   pbl_mean = sum_k (dp pbl_f)_k q_k / max(sum_k (dp pbl_f)_k, 1)
+ * per column (sums in level order).  pbl_f: NC layer fractions inside the PBL
+ * (cell order), computed by the caller from its forcing (reseact.esm:7455
+ * p_pbl_top, :7536); tau_pblmix > 0 in seconds.  The weights use the dp of
+ * rs_set_mass_fluxes and follow it when the fluxes are set again.  NULL
+ * switches it off.  Conserves sum_k dp q per column exactly in exact
+ * arithmetic. */
+RS_API int rs_set_pbl_mixing(rs_ctx* ctx, const double* pbl_f, double tau_pblmix);
+
 /* Mass-action mechanism, NR reactions: reactants[2r], reactants[2r+1]
  * (second = -1 for unimolecular; equal entries = 2A), products[2r],
  * products[2r+1] (-1 = none), rate constants k[r] (1/s or 1/(conc s)),

@@ -72,6 +72,10 @@ struct rs_ctx {
     std::vector<double> Mx, My, Mz, dp;               // faces / cells, lon fastest
     std::vector<double> qw, qe, qs, qn;               // inflow halos, (nt-1) species
     std::vector<double> dPSdt, msrc;                  // pressure fixer (ncol), air-mass source (nc)
+    // PBL mixing of the mixing ratios: layer fraction in the PBL (nc), its
+    // weight dp * pbl_f (nc), max(column weight, 1) (ncol), relaxation time
+    std::vector<double> pblf, pblw, pblWp;
+    double pbl_tau = 0.0;
     bool have_ppm_geom = false, have_ppm_flux = false;
     // mechanism
     int nr = 0;
@@ -380,11 +384,38 @@ inline double ppm_div(double fw, double fe, double fs, double fn, double fb, dou
     return (dx + dy) + dz;
 }
 
-// f = L_ppm u over the transported block (species 0 = air mass).
-void ppm_rhs_all(const rs_ctx& c, const double* u, double* f) {
+// PBL means of the mixing ratios of u, per species and column:
+// pbl_mean = sum_k (dp pbl_f)_k q_k / max(sum_k (dp pbl_f)_k, 1), sums in level
+// order (reseact.esm:7698-7760 pbl_wt, pbl_sum_X; :8455 pbl_mean_X).
+static std::vector<double> ppm_pbl_means(const rs_ctx& c, const double* u) {
+    const size_t nc = c.nc, plane = (size_t)c.nlon * c.nlat;
+    const int nq = c.nt - 1;
+    std::vector<double> pm((size_t)std::max(nq, 0) * plane);
+    for (int s = 1; s < c.nt; ++s) {
+        const double* q = u + (size_t)s * nc;
+        for (size_t col = 0; col < plane; ++col) {
+            double S = 0.0;
+            for (int k = 0; k < c.nlev; ++k) {
+                const size_t x = (size_t)k * plane + col;
+                S = S + c.pblw[x] * q[x];
+            }
+            pm[(size_t)(s - 1) * plane + col] = S / c.pblWp[col];
+        }
+    }
+    return pm;
+}
+
+// f = L_ppm u over the transported block (species 0 = air mass).  The
+// mixing-ratio tendency is the CWC advection [q div(M) - div(M q_f)] / m plus,
+// when set, the PBL mixing pbl_f (pbl_mean - q) / tau (reseact.esm:1563-1690);
+// with_pbl = false leaves the latter out (the derivatives take the advective
+// part for the air-mass row).
+void ppm_rhs_all(const rs_ctx& c, const double* u, double* f, bool with_pbl) {
     const int nlon = c.nlon, nlat = c.nlat, nlev = c.nlev, nt = c.nt;
     const size_t nc = c.nc, plane = (size_t)nlon * nlat;
     const double* m = u;
+    const bool pbl = with_pbl && !c.pblf.empty();
+    const std::vector<double> pm = pbl ? ppm_pbl_means(c, u) : std::vector<double>();
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < nlev; ++k) {
         for (int j = 0; j < nlat; ++j) {
@@ -445,7 +476,13 @@ void ppm_rhs_all(const rs_ctx& c, const double* u, double* f) {
                                                 mt * qt_, gx, ces, cen, rdS);
                     const double qc = q[cell];
                     const double num = qc * divM - divF;
-                    f[(size_t)s * nc + cell] = num / m[cell];
+                    const double fa = num / m[cell];
+                    double fo = fa;
+                    if (pbl) {
+                        const double t = c.pblf[cell] * (pm[(size_t)sq * plane + (size_t)j * nlon + i] - qc);
+                        fo = fa + t / c.pbl_tau;
+                    }
+                    f[(size_t)s * nc + cell] = fo;
                 }
             }
         }
@@ -637,12 +674,54 @@ inline void ppm_face_grad(const double* v, const double* h, double M, double g[6
 }
 
 
+// The PBL mixing term is linear in the mixing ratios (pbl_f fixed):
+//   T_k = pbl_f_k (sum_j w_j q_j / W' - q_k) / tau,  w = dp pbl_f,
+//   W' = max(sum_j w_j, 1), per column.
+// Its tangent, added to out (mixing ratios only): out_k += T_k(v).
+static void ppm_pbl_jvp(const rs_ctx& c, const double* v, double* out) {
+    if (c.pblf.empty()) return;
+    const size_t nc = c.nc, plane = (size_t)c.nlon * c.nlat;
+    for (int s = 1; s < c.nt; ++s) {
+        const double* vs = v + (size_t)s * nc;
+        double* os = out + (size_t)s * nc;
+        for (size_t col = 0; col < plane; ++col) {
+            double V = 0.0;
+            for (int k = 0; k < c.nlev; ++k) V = V + c.pblw[(size_t)k * plane + col] * vs[(size_t)k * plane + col];
+            const double vm = V / c.pblWp[col];
+            for (int k = 0; k < c.nlev; ++k) {
+                const size_t x = (size_t)k * plane + col;
+                os[x] += (c.pblf[x] * (vm - vs[x])) / c.pbl_tau;
+            }
+        }
+    }
+}
+
+// Its transpose, added to out: out_j += w_j L / (W' tau) - pbl_f_j lam_j / tau,
+// L = sum_k pbl_f_k lam_k over the column.
+static void ppm_pbl_vjp(const rs_ctx& c, const double* lam, double* out) {
+    if (c.pblf.empty()) return;
+    const size_t nc = c.nc, plane = (size_t)c.nlon * c.nlat;
+    for (int s = 1; s < c.nt; ++s) {
+        const double* ls = lam + (size_t)s * nc;
+        double* os = out + (size_t)s * nc;
+        for (size_t col = 0; col < plane; ++col) {
+            double L = 0.0;
+            for (int k = 0; k < c.nlev; ++k) L = L + c.pblf[(size_t)k * plane + col] * ls[(size_t)k * plane + col];
+            const double g = L / (c.pblWp[col] * c.pbl_tau);
+            for (int k = 0; k < c.nlev; ++k) {
+                const size_t x = (size_t)k * plane + col;
+                os[x] += c.pblw[x] * g - (c.pblf[x] * ls[x]) / c.pbl_tau;
+            }
+        }
+    }
+}
+
 // out = J(u) v over the transported block (tangent of ppm_rhs_all).
 void ppm_rhs_jvp_all(const rs_ctx& c, const double* u, const double* vt, double* out) {
     const int nlon = c.nlon, nlat = c.nlat, nlev = c.nlev, nt = c.nt;
     const size_t nc = c.nc, plane = (size_t)nlon * nlat;
     std::vector<double> f((size_t)nt * nc);
-    ppm_rhs_all(c, u, f.data());
+    ppm_rhs_all(c, u, f.data(), false);
     const double* m = u;
     const double* vm = vt;
     for (size_t x = 0; x < nc; ++x) out[x] = 0.0;
@@ -718,6 +797,7 @@ void ppm_rhs_jvp_all(const rs_ctx& c, const double* u, const double* vt, double*
             }
         }
     }
+    ppm_pbl_jvp(c, vt, out);
 }
 
 // out = J(u)^T lam over the transported block: the air mass gathers
@@ -730,7 +810,7 @@ void ppm_rhs_vjp_all(const rs_ctx& c, const double* u, const double* lam, double
     const int nlon = c.nlon, nlat = c.nlat, nlev = c.nlev, nt = c.nt;
     const size_t nc = c.nc, plane = (size_t)nlon * nlat;
     std::vector<double> f((size_t)nt * nc);
-    ppm_rhs_all(c, u, f.data());
+    ppm_rhs_all(c, u, f.data(), false);
     const double* m = u;
     for (size_t x = 0; x < nc; ++x) {
         double a = 0.0;
@@ -815,12 +895,13 @@ void ppm_rhs_vjp_all(const rs_ctx& c, const double* u, const double* lam, double
             }
         }
     }
+    ppm_pbl_vjp(c, lam, out);
 }
 
 // f = L u over the transported block, for either scheme.
 void transport_rhs_all(const rs_ctx& c, const double* u, double* f) {
     if (c.scheme == RS_TRANSPORT_PPM) {
-        ppm_rhs_all(c, u, f);
+        ppm_rhs_all(c, u, f, true);
         return;
     }
     for (int s = 0; s < c.nt; ++s)
@@ -1432,6 +1513,24 @@ void ros23_vjp_all(rs_ctx* c, const std::vector<double>& u, double dt) {
 // ============================================================================
 // C ABI
 // ============================================================================
+// Weights of the PBL mixing from pbl_f and dp (level order, as the GPU).
+static void ppm_pbl_weights(rs_ctx* c) {
+    if (c->pblf.empty()) return;
+    const size_t plane = (size_t)c->nlon * c->nlat;
+    c->pblw.assign(c->nc, 0.0);
+    c->pblWp.assign(plane, 1.0);
+    for (size_t col = 0; col < plane; ++col) {
+        double W = 0.0;
+        for (int k = 0; k < c->nlev; ++k) {
+            const size_t x = (size_t)k * plane + col;
+            const double w = c->dp[x] * c->pblf[x];
+            c->pblw[x] = w;
+            W = W + w;
+        }
+        c->pblWp[col] = std::max(W, 1.0);
+    }
+}
+
 extern "C" {
 
 int rs_create(rs_ctx** out, int device) {
@@ -1460,6 +1559,7 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
     c->have_ppm_geom = c->have_ppm_flux = false;
     c->dPSdt.clear();
     c->msrc.clear();
+    c->pblf.clear(); c->pblw.clear(); c->pblWp.clear();
     const size_t nq = nt > 1 ? (size_t)(nt - 1) : 0;
     c->qw.assign(nq * nlev * nlat, 0.0); c->qe = c->qw;
     c->qs.assign(nq * nlev * nlon, 0.0); c->qn = c->qs;
@@ -1529,6 +1629,7 @@ int rs_set_mass_fluxes(rs_ctx* c, const double* Mx, const double* My, const doub
         if (c->Mz[x] != 0.0 || c->Mz[nlev * c->ncol + x] != 0.0)
             return fail(c, RS_E_ARG, "Mz must vanish at the surface and the lid");
     c->have_ppm_flux = true;
+    ppm_pbl_weights(c);
     ppm_update_source(c);
     return RS_OK;
 }
@@ -1575,6 +1676,21 @@ int rs_set_pressure_fixer(rs_ctx* c, const double* dPSdt) {
     c->dPSdt.assign(dPSdt, dPSdt + c->ncol);
     if (!all_finite(c->dPSdt)) return fail(c, RS_E_NONFINITE, "dPSdt");
     ppm_update_source(c);
+    return RS_OK;
+}
+
+int rs_set_pbl_mixing(rs_ctx* c, const double* pbl_f, double tau_pblmix) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!pbl_f) {
+        c->pblf.clear(); c->pblw.clear(); c->pblWp.clear();
+        return RS_OK;
+    }
+    if (!c->have_ppm_flux) return fail(c, RS_E_STATE, "set the PPM mass fluxes (dp) first");
+    if (!(tau_pblmix > 0.0) || !std::isfinite(tau_pblmix)) return fail(c, RS_E_ARG, "tau_pblmix must be positive");
+    c->pblf.assign(pbl_f, pbl_f + c->nc);
+    if (!all_finite(c->pblf)) { c->pblf.clear(); return fail(c, RS_E_NONFINITE, "pbl_f"); }
+    c->pbl_tau = tau_pblmix;
+    ppm_pbl_weights(c);
     return RS_OK;
 }
 

@@ -277,12 +277,14 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
             v2 = w[5];
         }
     }
+    // PBL mean of this species in this column (PBL mixing on)
+    const double pm = (A.pblf && s > 0 && valid) ? A.pblm[(long long)sq1 * plane + col] : 0.0;
     double sq = 0.0, ng = 0.0;
     for (int k = k0; k < k1; ++k) {
         const int bf = (k - k0) & 1;
         const long long cell = (long long)k * plane + col;
         const long long g = sb + cell;
-        double f = 0.0, ac = 0.0, ft = 0.0, mt = 0.0;
+        double f = 0.0, fa = 0.0, ac = 0.0, ft = 0.0, mt = 0.0;
         if (s > 0) {
             // ---- stage the tile and its halo (corners are never read)
             for (int x = tid; x < PWX * PWY; x += PB) {
@@ -356,19 +358,26 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
             const double divM = div6(mxr[i], mxr[i + 1], myr[0], myr[nlon], mb, mt, gx, ces, cen, rdS);
             if (s == 0) {
                 f = A.msrc ? -divM + A.msrc[cell] : -divM;
+                fa = f;
                 ac = ps[cell];
             } else {
                 const double divF = div6(FX[bf][ty][tx], FX[bf][ty][tx + 1], FY[bf][ty][tx],
                                          FY[bf][ty + 1][tx], fb, ft, gx, ces, cen, rdS);
                 const double qc = Q[bf][ty + 3][tx + 3];
                 const double num = qc * divM - divF;
-                f = num / ps[cell];
+                fa = num / ps[cell];
+                f = fa;
+                if (A.pblf) {
+                    // PBL mixing, pbl_f (pbl_mean - q) / tau (reseact.esm:1563-1690)
+                    const double t = A.pblf[cell] * (pm - qc);
+                    f = fa + t / A.pbl_tau;
+                }
                 ac = qc;
             }
             const double du = h2 * f;
             const double un = ac + du;
             if (KIND == 3) {
-                po[g] = f;
+                po[g] = fa;                // the advective tendency (the adjoint's m row)
             } else if (KIND == 1) {
                 const double u = pa[g], u3 = pb[g];
                 const double a = C13 * u;
@@ -385,7 +394,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
                 po[g] = A.clamp ? dmax(un, 0.0) : un;
             } else {
                 po[g] = un;
-                if (A.outf) A.outf[g] = f;
+                if (A.outf) A.outf[g] = fa;
                 if (KIND == 2) {
                     const double a = C23 * pb[g];
                     const double b = C13 * un;
@@ -1037,6 +1046,75 @@ __global__ void __launch_bounds__(PB) rs_ppm_vjp_mrow(RsPPMVjpArgs A, const doub
     A.out[cell] = o;
 }
 
+// ---- PBL mixing ----------------------------------------------------------
+// Weights per column, level order: w = dp pbl_f, Wp = max(sum_k w, 1).
+__global__ void __launch_bounds__(PB) rs_ppm_pbl_setup(const double* __restrict__ dp,
+                                                       const double* __restrict__ pf,
+                                                       double* __restrict__ w, double* __restrict__ Wp,
+                                                       int nlev, long long plane) {
+    const long long col = (long long)blockIdx.x * PB + threadIdx.x;
+    if (col >= plane) return;
+    double W = 0.0;
+    for (int k = 0; k < nlev; ++k) {
+        const long long x = (long long)k * plane + col;
+        const double wk = dp[x] * pf[x];
+        w[x] = wk;
+        W = W + wk;
+    }
+    Wp[col] = dmax(W, 1.0);
+}
+
+// PBL means of a stage's stencil source, per mixing ratio and column:
+// pblm = sum_k w_k q_k / Wp (level order).  last: the source is c4.
+__global__ void __launch_bounds__(PB) rs_ppm_pbl_mean(RsPPMArgs A, int last) {
+    const RsCtrl* ctl = A.ctrl;
+    if (ctl && ctl->done) return;
+    const int cur = ctl ? ctl->cur : A.cur_static;
+    const double* __restrict__ ps = last ? A.c4 : (A.a_is_state ? (cur ? A.S1 : A.S0) : A.a);
+    const long long plane = (long long)A.nlon * A.nlat;
+    const long long lane = (long long)blockIdx.x * PB + threadIdx.x;
+    if (lane >= (long long)(A.nt - 1) * plane) return;
+    const long long sq = lane / plane, col = lane % plane;
+    const double* q = ps + (sq + 1) * A.nc;
+    double S = 0.0;
+    for (int k = 0; k < A.nlev; ++k) {
+        const long long x = (long long)k * plane + col;
+        S = S + A.pblw[x] * q[x];
+    }
+    A.pblm[lane] = S / A.pblWp[col];
+}
+
+// The PBL part of J^T x, added to out (mixing ratios): with x = alpha a,
+//   out_j += h2 (w_j L / (Wp tau) - pbl_f_j x_j / tau),  L = sum_k pbl_f_k x_k.
+__global__ void __launch_bounds__(PB) rs_ppm_vjp_pbl(RsPPMVjpArgs A) {
+    const long long plane = (long long)A.nlon * A.nlat;
+    const long long lane = (long long)blockIdx.x * PB + threadIdx.x;
+    if (lane >= (long long)(A.nt - 1) * plane) return;
+    const long long sb = (lane / plane + 1) * A.nc, col = lane % plane;
+    const double alpha = A.alpha;
+    double L = 0.0;
+    for (int k = 0; k < A.nlev; ++k) {
+        const long long x = (long long)k * plane + col;
+        const double xv = alpha == 1.0 ? A.a[sb + x] : alpha * A.a[sb + x];
+        L = L + A.pblf[x] * xv;
+    }
+    const double g = L / (A.pblWp[col] * A.pbl_tau);
+    for (int k = 0; k < A.nlev; ++k) {
+        const long long x = (long long)k * plane + col;
+        const double xv = alpha == 1.0 ? A.a[sb + x] : alpha * A.a[sb + x];
+        const double t = A.pblw[x] * g - (A.pblf[x] * xv) / A.pbl_tau;
+        A.out[sb + x] = A.out[sb + x] + A.h2 * t;
+    }
+}
+
+hipError_t rs_launch_ppm_pbl_setup(const double* dp, const double* pf, double* w, double* Wp,
+                                   int nlon, int nlat, int nlev, hipStream_t st) {
+    const long long plane = (long long)nlon * nlat;
+    hipLaunchKernelGGL(rs_ppm_pbl_setup, dim3((unsigned)((plane + PB - 1) / PB)), dim3(PB), 0, st,
+                       dp, pf, w, Wp, nlev, plane);
+    return hipGetLastError();
+}
+
 // Level blocks of the column stage: about PKB levels each.
 constexpr int PKB = 24;
 static int ppm_nkb(int nlev) { return (nlev + PKB - 1) / PKB; }
@@ -1066,6 +1144,11 @@ hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlo
 // kind 0: first and second stages; 2: third stage (also a4); 1: last stage
 hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st) {
     if (a->nt < 1 || a->nlev < 1) return hipErrorInvalidValue;
+    if (a->pblf && a->nt > 1) {
+        const long long n = (long long)(a->nt - 1) * a->nlon * a->nlat;
+        hipLaunchKernelGGL(rs_ppm_pbl_mean, dim3((unsigned)((n + PB - 1) / PB)), dim3(PB), 0, st, *a,
+                           kind == 1 ? 1 : 0);
+    }
     RsPPMArgs b = *a;
     const int nkb = ppm_nkb(a->nlev);
     b.kblk = (a->nlev + nkb - 1) / nkb;
@@ -1078,8 +1161,15 @@ hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st) {
     return hipGetLastError();
 }
 
+static void launch_vjp_pbl(const RsPPMVjpArgs* a, hipStream_t st) {
+    if (!a->pblf || a->nt < 2) return;
+    const long long n = (long long)(a->nt - 1) * a->nlon * a->nlat;
+    hipLaunchKernelGGL(rs_ppm_vjp_pbl, dim3((unsigned)((n + PB - 1) / PB)), dim3(PB), 0, st, *a);
+}
+
 hipError_t rs_launch_ppm_vjp(const RsPPMVjpArgs* a, hipStream_t st) {
     hipLaunchKernelGGL(rs_ppm_vjp, dim3((unsigned)((a->nc + PB - 1) / PB)), dim3(PB), 0, st, *a);
+    launch_vjp_pbl(a, st);
     return hipGetLastError();
 }
 
@@ -1105,5 +1195,6 @@ hipError_t rs_launch_ppm_vjp_sweeps(const RsPPMVjpArgs* a, const double* F, doub
         hipLaunchKernelGGL(rs_ppm_vjp_xtile, xg, dim3(PB), 0, st, *a, G);
     }
     hipLaunchKernelGGL(rs_ppm_vjp_mrow, blocks(a->nc), dim3(PB), 0, st, *a, F);
+    launch_vjp_pbl(a, st);
     return hipGetLastError();
 }

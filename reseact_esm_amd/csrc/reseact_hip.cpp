@@ -65,6 +65,8 @@ long long rs_ppm_nblocks(int nlon, int nlat, int nlev, int nt);
 hipError_t rs_launch_ppm_coefs(const double* dp, double* vs, double* ve, int nlon, int nlat,
                                int nlev, hipStream_t st);
 hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st);
+hipError_t rs_launch_ppm_pbl_setup(const double* dp, const double* pf, double* w, double* Wp,
+                                   int nlon, int nlat, int nlev, hipStream_t st);
 hipError_t rs_launch_ppm_vjp(const RsPPMVjpArgs* a, hipStream_t st);
 hipError_t rs_launch_ppm_vjp_sweeps(const RsPPMVjpArgs* a, const double* F, double* G,
                                     double* divm, hipStream_t st);
@@ -180,6 +182,10 @@ struct rs_ctx {
     DevArray<double> pF, pG, pdivm;   // PPM adjoint scratch: tendency, partial sums, div(M)
     DevArray<double> pdPS, pmsrc;   // pressure fixer: dPS/dt per column, air-mass source per cell
     bool have_fixer = false;
+    // PBL mixing: layer fraction, weights dp*pbl_f, max(column weight, 1), stage means
+    DevArray<double> ppblf, ppblw, ppblWp, ppblm;
+    double pbl_tau = 0.0;
+    bool have_pbl = false;
     DevArray<double> red;
     DevArray<unsigned long long> head;
     DevArray<long long> stats;
@@ -449,6 +455,10 @@ RsPPMArgs ppm_args(rs_ctx* c, double dt) {
     a.vs = c->pvs.p; a.ve = c->pve.p;
     a.qw = c->pqw.p; a.qe = c->pqe.p; a.qs = c->pqs.p; a.qn = c->pqn.p;
     a.msrc = c->pmsrc.n ? c->pmsrc.p : nullptr;
+    if (c->have_pbl) {
+        a.pblf = c->ppblf.p; a.pblw = c->ppblw.p; a.pblWp = c->ppblWp.p; a.pblm = c->ppblm.p;
+        a.pbl_tau = c->pbl_tau;
+    }
     a.nc = c->nc;
     a.dt_static = dt;
     a.nlon = c->nlon; a.nlat = c->nlat; a.nlev = c->nlev; a.nt = c->nt;
@@ -930,6 +940,9 @@ void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
     v.h2 = dt / 2;
     v.nlon = c->nlon; v.nlat = c->nlat; v.nlev = c->nlev; v.nt = c->nt;
     v.lon_periodic = c->lon_periodic;
+    if (c->have_pbl) {
+        v.pblf = c->ppblf.p; v.pblw = c->ppblw.p; v.pblWp = c->ppblWp.p; v.pbl_tau = c->pbl_tau;
+    }
     if (sweeps) {
         // the tendency at u4 (tendency-only stage)
         RsPPMArgs tf = ppm_args(c, dt);
@@ -1190,6 +1203,7 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
         c->scheme = RS_TRANSPORT_UPWIND3;
         c->pZ.resize(0);
         c->have_ppm_geom = c->have_ppm_flux = false;
+        c->have_pbl = false;
         c->have_fixer = false;
         c->pmsrc.resize(0);
         {
@@ -1268,6 +1282,37 @@ void ppm_update_source(rs_ctx* c) {
     HIPCHK(hipStreamSynchronize(c->stream));
 }
 
+// Weights of the PBL mixing (recomputed with dp) and the stage-mean scratch.
+void ppm_update_pbl(rs_ctx* c) {
+    if (!c->have_pbl || !c->have_ppm_flux) return;
+    c->ppblw.resize(c->nc);
+    c->ppblWp.resize(c->ncol);
+    c->ppblm.resize((size_t)std::max(c->nt - 1, 1) * c->ncol);
+    HIPCHK(rs_launch_ppm_pbl_setup(c->pdp.p, c->ppblf.p, c->ppblw.p, c->ppblWp.p, c->nlon, c->nlat,
+                                   c->nlev, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+int rs_set_pbl_mixing(rs_ctx* c, const double* pbl_f, double tau_pblmix) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!pbl_f) {
+        c->have_pbl = false;
+        return RS_OK;
+    }
+    if (!c->have_ppm_flux) return fail(c, RS_E_STATE, "set the PPM mass fluxes (dp) first");
+    if (!(tau_pblmix > 0.0) || !std::isfinite(tau_pblmix)) return fail(c, RS_E_ARG, "tau_pblmix must be positive");
+    for (long long x = 0; x < c->nc; ++x)
+        if (!std::isfinite(pbl_f[x])) return fail(c, RS_E_NONFINITE, "pbl_f");
+    return guarded(c, [&]() {
+        c->ppblf.resize(c->nc);
+        h2d(c->ppblf.p, pbl_f, c->nc, c->stream);
+        c->pbl_tau = tau_pblmix;
+        c->have_pbl = true;
+        ppm_update_pbl(c);
+        return RS_OK;
+    });
+}
+
 int rs_set_pressure_fixer(rs_ctx* c, const double* dPSdt) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
     if (!dPSdt) {
@@ -1343,6 +1388,7 @@ int rs_set_mass_fluxes(rs_ctx* c, const double* Mx, const double* My, const doub
         HIPCHK(hipStreamSynchronize(c->stream));
         c->have_ppm_flux = true;
         ppm_update_source(c);
+        ppm_update_pbl(c);
         return RS_OK;
     });
 }

@@ -122,7 +122,12 @@ typedef struct RsPPMArgs {
     const double* qs;
     const double* qn;
     const double* msrc;     /* air-mass source of the pressure fixer, or 0 */
-    double* outf;           /* stages 1-3: also the tendency at a (adjoint), or 0 */
+    double* outf;           /* stages 1-3: also the advective tendency at a (adjoint), or 0 */
+    const double* pblf;     /* PBL mixing: layer fraction in the PBL (nc), or 0 (off) */
+    const double* pblw;     /* dp * pbl_f (nc) */
+    const double* pblWp;    /* max(column sum of pblw, 1) (ncol) */
+    double* pblm;           /* PBL means of the stencil source ((nt-1) x ncol) */
+    double pbl_tau;
     long long nc;
     double dt_static, atol, rtol;
     int nlon, nlat, nlev, nt;
@@ -151,8 +156,11 @@ typedef struct RsPPMVjpArgs {
     const double* qe;
     const double* qs;
     const double* qn;
+    const double* pblf;     /* PBL mixing (as RsPPMArgs), or 0 */
+    const double* pblw;
+    const double* pblWp;
     long long nc;
-    double h2, alpha, beta;
+    double h2, alpha, beta, pbl_tau;
     int nlon, nlat, nlev, nt, lon_periodic, pad;
 } RsPPMVjpArgs;
 

@@ -150,6 +150,7 @@ ABI = {
     "rs_set_mass_fluxes": [_P, _D, _D, _D, _D],
     "rs_set_inflow": [_P, _D, _D, _D, _D],
     "rs_set_pressure_fixer": [_P, _D],
+    "rs_set_pbl_mixing": [_P, _D, C.c_double],
     "rs_set_mechanism": [_P, C.c_int, _I, _I, _D, _I],
     "rs_set_mechanism_stoich": [_P, C.c_int, _I, _I, _I, _D, _D, _I],
     "rs_set_photolysis": [_P, _D],
@@ -360,6 +361,13 @@ class Context:
         fix = None if ppm.dPSdt is None else np.ascontiguousarray(ppm.dPSdt, np.float64)
         self._ppm_keep.append(fix)
         self._chk(self.lib.rs_set_pressure_fixer(self.h, _dp(fix)))
+        self.set_pbl_mixing(ppm.pbl_f, ppm.tau_pbl)
+
+    def set_pbl_mixing(self, pbl_f, tau=900.0):
+        """PBL mixing of the mixing ratios (pbl_f per cell, None: off)."""
+        a = None if pbl_f is None else np.ascontiguousarray(pbl_f, np.float64).reshape(-1)
+        self._pbl_keep = a
+        self._chk(self.lib.rs_set_pbl_mixing(self.h, _dp(a), float(tau)))
 
     def set_state(self, u):
         u = np.ascontiguousarray(u, np.float64)

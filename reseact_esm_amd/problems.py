@@ -362,6 +362,41 @@ class PPMSetup:
     qbc_s: np.ndarray | None = None   # (nq, nlev, nlon)
     qbc_n: np.ndarray | None = None
     dPSdt: np.ndarray | None = None   # (nlat, nlon): column pressure fixer of the air mass
+    pbl_f: np.ndarray | None = None   # (nlev, nlat, nlon): layer fraction inside the PBL
+    tau_pbl: float = 900.0            # s, PBL mixing relaxation time
+
+
+G_ACC = 9.80665        # m/s^2 (reseact.esm:1164)
+RD_AIR = 287.05        # J/(kg K) (reseact.esm:1158)
+
+
+def pbl_fraction(dp: np.ndarray, pblh: np.ndarray, temp: float | np.ndarray = 290.0,
+                 p_lid: float = 0.0) -> np.ndarray:
+    """Fraction of each layer inside the PBL, the reference's F_OF_PBL
+    (reseact.esm:7455 p_pbl_top = PS exp(-PBLH g / (Rd T)); :7536 pbl_f =
+    clamp((min(PS, p_bot) - max(p_pbl_top, p_top)) / dp, 0, 1)), for layers
+    dp (nlev, nlat, nlon) stacked from the surface (level 0) on a lid at
+    p_lid: PS = p_lid + sum dp, p_bot/p_top the layer's edge pressures.
+    pblh (nlat, nlon) in m, temp the surface-layer temperature."""
+    dp = np.asarray(dp, np.float64)
+    ps = p_lid + dp.sum(axis=0)
+    p_bot = ps[None] - np.concatenate([np.zeros_like(ps)[None], np.cumsum(dp, axis=0)[:-1]])
+    p_top = p_bot - dp
+    p_pbl = ps * np.exp(-(np.asarray(pblh, np.float64) * G_ACC) / (RD_AIR * np.asarray(temp, np.float64)))
+    f = (np.minimum(ps[None], p_bot) - np.maximum(p_pbl[None], p_top)) / dp
+    return np.clip(f, 0.0, 1.0)
+
+
+def with_pbl_mixing(p: Problem, pblh_mean: float = 1000.0, pblh_amp: float = 600.0,
+                    tau: float = 900.0) -> Problem:
+    """The PPM workload with PBL mixing: a boundary layer of pblh_mean +-
+    pblh_amp metres varying with longitude (a day side and a night side)."""
+    g = p.grid
+    lam = 2.0 * math.pi * (np.arange(g.nlon) + 0.5) / g.nlon
+    pblh = np.broadcast_to(pblh_mean + pblh_amp * np.cos(lam)[None, :], (g.nlat, g.nlon))
+    p.ppm.pbl_f = pbl_fraction(p.ppm.dp, pblh, p_lid=5000.0)
+    p.ppm.tau_pbl = tau
+    return p
 
 
 # GEOS-FP hybrid coefficients at the lowest 8 level edges (surface first),

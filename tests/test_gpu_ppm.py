@@ -47,6 +47,12 @@ CASES = {
     # levels 18 and 36 (53 levels: blocks of 18, 18, 17), or 15 (30 levels)
     "global-40x12x53": lambda: P.ppm_global(40, 12, 53, nq=3, seed=11),
     "regional-20x12x30": lambda: regional_random(20, 12, 30, seed=12),
+    # with the PBL mixing term (reseact.esm:1563-1690): a 1-4 km boundary
+    # layer over many of 40 levels; the regional patch relaxes in 0.05 s
+    "global-pbl-40x12x40": lambda: P.with_pbl_mixing(P.ppm_global(40, 12, 40, nq=3, seed=13),
+                                                     pblh_mean=2500.0, pblh_amp=1500.0),
+    "regional-pbl-20x12x30": lambda: P.with_pbl_mixing(regional_random(20, 12, 30, seed=14),
+                                                       pblh_mean=800.0, pblh_amp=400.0, tau=0.05),
 }
 
 
@@ -115,6 +121,11 @@ VJP_CASES = {
     "regional-9x2x2": lambda: regional_random(9, 2, 2, nq=2, seed=6),
     # several level blocks per column in the recompute stages
     "global-40x12x53": lambda: P.ppm_global(40, 12, 53, nq=3, seed=11),
+    # the PBL mixing term, on the sweeps and on the per-cell gather
+    "global-pbl-40x12x40": lambda: P.with_pbl_mixing(P.ppm_global(40, 12, 40, nq=3, seed=13),
+                                                     pblh_mean=2500.0, pblh_amp=1500.0),
+    "global-pbl-4x6x12": lambda: P.with_pbl_mixing(P.ppm_global(4, 6, 12, nq=2, seed=15),
+                                                   pblh_mean=6000.0, pblh_amp=3000.0),
 }
 
 
@@ -128,7 +139,8 @@ def test_ppm_step_vjp(hip, oracle, name):
     # steps of a few tenths of the Courant limit, so J^T lam is not lost in lam
     dt = {"regional-20x12x9": 0.03, "global-48x24x10": 3000.0, "global-4x6x5": 50000.0,
           "thin-12x3x1": 20000.0, "thin-8x2x2": 20000.0, "regional-9x2x2": 0.05,
-          "global-40x12x53": 2000.0}[name]
+          "global-40x12x53": 2000.0, "global-pbl-40x12x40": 2000.0,
+          "global-pbl-4x6x12": 50000.0}[name]
     lam = np.random.default_rng(21).standard_normal(p.ns * p.nc)
     out = []
     for lib in (hip, oracle):

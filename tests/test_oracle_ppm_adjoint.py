@@ -95,7 +95,20 @@ def global_grid(seed=4):
     return p, 20000.0
 
 
-CASES = {"regional": regional, "global": global_grid}
+def regional_pbl():
+    p, dt = regional()
+    return P.with_pbl_mixing(p, pblh_mean=800.0, pblh_amp=400.0, tau=0.5), dt
+
+
+def global_pbl():
+    p, dt = global_grid()
+    # a deep boundary layer (3-9 km), so that it spans several of the 6 layers
+    return P.with_pbl_mixing(p, pblh_mean=6000.0, pblh_amp=3000.0, tau=3600.0), dt
+
+
+# the PBL mixing term (reseact.esm:1563-1690) on both grids
+CASES = {"regional": regional, "global": global_grid, "regional-pbl": regional_pbl,
+         "global-pbl": global_pbl}
 
 
 @pytest.mark.parametrize("case", list(CASES))
