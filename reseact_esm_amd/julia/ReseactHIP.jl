@@ -104,6 +104,35 @@ set_photolysis!(c::Context, jfac::Union{Nothing,Vector{Float64}}) =
 set_emissions!(c::Context, e::Union{Nothing,Vector{Float64}}) =
     check(c, ccall((:rs_set_emissions, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr,
                    e === nothing ? C_NULL : e))
+# The reference's own transport: monotone PPM in CWC mixing-ratio form
+# (species 0 of the transported block is the air mass).  Arrays are flat,
+# longitude fastest, as in include/reseact_hip.h.
+const TRANSPORT_UPWIND3 = Cint(0)
+const TRANSPORT_PPM = Cint(1)
+set_transport_scheme!(c::Context, scheme::Integer) =
+    check(c, ccall((:rs_set_transport_scheme, LIB), Cint, (Ptr{Cvoid}, Cint), c.ptr, scheme))
+set_ppm_geometry!(c::Context, lon_periodic::Bool, dlon::Float64, coslat_e::Vector{Float64},
+                  dS_lat::Vector{Float64}, dphi_lat::Vector{Float64}) =
+    check(c, ccall((:rs_set_ppm_geometry, LIB), Cint,
+                   (Ptr{Cvoid}, Cint, Cdouble, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                   c.ptr, lon_periodic ? 1 : 0, dlon, coslat_e, dS_lat, dphi_lat))
+set_mass_fluxes!(c::Context, Mx::Vector{Float64}, My::Vector{Float64}, Mz::Vector{Float64},
+                 dp::Vector{Float64}) =
+    check(c, ccall((:rs_set_mass_fluxes, LIB), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                   c.ptr, Mx, My, Mz, dp))
+_ornull(a) = a === nothing ? C_NULL : a
+set_inflow!(c::Context, qw, qe, qs, qn) =
+    check(c, ccall((:rs_set_inflow, LIB), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                   c.ptr, _ornull(qw), _ornull(qe), _ornull(qs), _ornull(qn)))
+set_pressure_fixer!(c::Context, dPSdt::Union{Nothing,Vector{Float64}}) =
+    check(c, ccall((:rs_set_pressure_fixer, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr,
+                   _ornull(dPSdt)))
+# pbl_f: layer fraction inside the PBL per cell (nothing: off); tau_pblmix in s
+set_pbl_mixing!(c::Context, pbl_f::Union{Nothing,Vector{Float64}}, tau_pblmix::Float64=900.0) =
+    check(c, ccall((:rs_set_pbl_mixing, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble), c.ptr,
+                   _ornull(pbl_f), tau_pblmix))
 set_state!(c::Context, u::Vector{Float64}) =
     check(c, ccall((:rs_set_state, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, u))
 function get_state(c::Context)
