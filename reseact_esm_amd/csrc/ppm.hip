@@ -202,7 +202,9 @@ constexpr int NFX = (PTX + 1) * PTY, NFY = PTX * (PTY + 1);  // faces per tile
 // the six face fluxes.
 // Grid: x = species (the species of a tile run side by side and share its
 // fluxes and width factors in cache), y = tile, z = level block.
-template <int KIND>
+// PBL: the PBL mixing term is on (A.pblf set), a separate instantiation so
+// that the stage without it is unchanged.
+template <int KIND, bool PBL>
 __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
     __shared__ double red[PB / 64];
     __shared__ double Q[2][PWY][PWX];
@@ -278,7 +280,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
         }
     }
     // PBL mean of this species in this column (PBL mixing on)
-    const double pm = (A.pblf && s > 0 && valid) ? A.pblm[(long long)sq1 * plane + col] : 0.0;
+    const double pm = (PBL && s > 0 && valid) ? A.pblm[(long long)sq1 * plane + col] : 0.0;
     double sq = 0.0, ng = 0.0;
     for (int k = k0; k < k1; ++k) {
         const int bf = (k - k0) & 1;
@@ -367,7 +369,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
                 const double num = qc * divM - divF;
                 fa = num / ps[cell];
                 f = fa;
-                if (A.pblf) {
+                if (PBL) {
                     // PBL mixing, pbl_f (pbl_mean - q) / tau (reseact.esm:1563-1690)
                     const double t = A.pblf[cell] * (pm - qc);
                     f = fa + t / A.pbl_tau;
@@ -1154,10 +1156,17 @@ hipError_t rs_launch_ppm_stage(int kind, const RsPPMArgs* a, hipStream_t st) {
     b.kblk = (a->nlev + nkb - 1) / nkb;
     const long long ntile = (long long)((a->nlon + PTX - 1) / PTX) * ((a->nlat + PTY - 1) / PTY);
     const dim3 grid((unsigned)a->nt, (unsigned)ntile, (unsigned)nkb);
-    if (kind == 1) hipLaunchKernelGGL(rs_ppm_stage<1>, grid, dim3(PB), 0, st, b);
-    else if (kind == 2) hipLaunchKernelGGL(rs_ppm_stage<2>, grid, dim3(PB), 0, st, b);
-    else if (kind == 3) hipLaunchKernelGGL(rs_ppm_stage<3>, grid, dim3(PB), 0, st, b);
-    else hipLaunchKernelGGL(rs_ppm_stage<0>, grid, dim3(PB), 0, st, b);
+    if (b.pblf) {
+        if (kind == 1) hipLaunchKernelGGL((rs_ppm_stage<1, true>), grid, dim3(PB), 0, st, b);
+        else if (kind == 2) hipLaunchKernelGGL((rs_ppm_stage<2, true>), grid, dim3(PB), 0, st, b);
+        else if (kind == 3) hipLaunchKernelGGL((rs_ppm_stage<3, true>), grid, dim3(PB), 0, st, b);
+        else hipLaunchKernelGGL((rs_ppm_stage<0, true>), grid, dim3(PB), 0, st, b);
+    } else {
+        if (kind == 1) hipLaunchKernelGGL((rs_ppm_stage<1, false>), grid, dim3(PB), 0, st, b);
+        else if (kind == 2) hipLaunchKernelGGL((rs_ppm_stage<2, false>), grid, dim3(PB), 0, st, b);
+        else if (kind == 3) hipLaunchKernelGGL((rs_ppm_stage<3, false>), grid, dim3(PB), 0, st, b);
+        else hipLaunchKernelGGL((rs_ppm_stage<0, false>), grid, dim3(PB), 0, st, b);
+    }
     return hipGetLastError();
 }
 
