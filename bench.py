@@ -203,7 +203,20 @@ def ppm_roofline(lib, nsteps: int = 10, nq: int = 20):
     t3 = time.perf_counter()
     profv = ctx.profile_read()
     ctx.profile(False)
+    # the same attempts with the PBL mixing term on (a 400-1600 m boundary
+    # layer; its column means are a pre-pass inside each stage's timing)
+    P.with_pbl_mixing(p)
+    ctx.set_pbl_mixing(p.ppm.pbl_f, p.ppm.tau_pbl)
+    ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+    ctx.profile(True)
+    for _ in range(nsteps):
+        ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+    profp = ctx.profile_read()
+    ctx.profile(False)
     ctx.close()
+    _, ms_ps = kernel_ms(profp, "transport_stage")
+    _, ms_pl = kernel_ms(profp, "transport_last")
+    pbl_ms = (ms_ps + ms_pl) / nsteps
     nv_, ms_v = kernel_ms(profv, "ppm_vjp")
     nvs_, ms_vs = kernel_ms(profv, "transport_stage")
     vjp_ms = (ms_v + ms_vs) / nvjp
@@ -242,6 +255,7 @@ def ppm_roofline(lib, nsteps: int = 10, nq: int = 20):
             "ms_per_attempt": round(ms, 4),
             "avg_stage_ms": round(ms_s / max(ns_, 1), 4), "avg_last_ms": round(ms_l / max(nl_, 1), 4),
             "tracer_cell_attempts_per_s": round(nt * p.nc / (ms * 1e-3), 1),
+            "pbl_mixing": {"ms_per_attempt": round(pbl_ms, 4), "over_plain": round(pbl_ms / ms, 3)},
             "vjp": {"ms_per_step_vjp": round(vjp_ms, 4), "avg_ppm_vjp_ms": round(ms_v / max(nv_, 1), 4),
                     "ppm_vjp_launches_per_vjp": round(nv_ / nvjp, 2),
                     "vjp_over_attempt": round(vjp_ms / ms, 3), "wall_s": round(t3 - t2, 3)}}
