@@ -65,11 +65,14 @@ def test_ensemble_adjoint_matches_oracle(hip, oracle, budget_gib):
 
 def test_ensemble_adjoint_ppm(hip, oracle):
     """Members on the PPM transport (regional patches of three sizes with open
-    walls, random inflow halos and the three-reaction mechanism): the
+    walls, random inflow halos and the three-reaction mechanism; one with the
+    PBL mixing term): the
     graph-captured forward and backward sweeps against each member's oracle
     adjoint, loss on the first mixing ratio."""
     probs = [P.ppm_coupled_regional(nlon, nlat, nlev, seed=s)
              for nlon, nlat, nlev, s in ((10, 8, 6, 5), (12, 6, 7, 6), (9, 9, 5, 7))]
+    # the second member also mixes its boundary layer
+    P.with_pbl_mixing(probs[1], pblh_mean=800.0, pblh_amp=400.0, tau=0.05)
     members = [hip.context().load(p) for p in probs]
     ens = Ensemble(members)
     p0 = probs[0]

@@ -154,12 +154,16 @@ def test_ppm_step_vjp(hip, oracle, name):
     assert float(np.max(np.abs(lg - lo))) <= 1e-11 * float(np.max(np.abs(lo)))
 
 
-def test_ppm_adjoint_run(hip, oracle):
+@pytest.mark.parametrize("pbl", [False, True])
+def test_ppm_adjoint_run(hip, oracle, pbl):
     """Checkpointed time-loop adjoint of split steps with the PPM transport
-    and a small mechanism: same accepted sequences, gradient to 1e-10."""
+    (with and without the PBL mixing term) and a small mechanism: same
+    accepted sequences, gradient to 1e-10."""
     from reseact_esm_amd.host import split_opts
-    print("ppm adjoint run", flush=True)
+    print(f"ppm adjoint run pbl={pbl}", flush=True)
     p = P.ppm_coupled_regional()
+    if pbl:
+        P.with_pbl_mixing(p, pblh_mean=800.0, pblh_amp=400.0, tau=0.05)
     res = []
     for lib in (hip, oracle):
         with lib.context() as c:

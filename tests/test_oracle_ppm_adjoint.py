@@ -231,13 +231,17 @@ def test_kink_at_uniform_field(oracle):
     ctx.close()
 
 
-def test_time_loop_adjoint_against_central_fd(oracle):
+@pytest.mark.parametrize("pbl", [False, True])
+def test_time_loop_adjoint_against_central_fd(oracle, pbl):
     """The discrete adjoint of split steps with the PPM transport (checkpoint,
     fixed-sequence replay, reverse sweep) against central differences of the
     frozen-sequence loss (tools/adjoint_gradient.jl:1183-1244), at a random
-    (off-kink) state."""
+    (off-kink) state; also with the PBL mixing term (relaxation time of the
+    patch's own time scale)."""
     from reseact_esm_amd.host import split_opts
     p = P.ppm_coupled_regional()
+    if pbl:
+        P.with_pbl_mixing(p, pblh_mean=800.0, pblh_amp=400.0, tau=0.05)
     ctx = oracle.context().load(p)
     J, gu, ge, st = ctx.adjoint_run(p.tstops(), p.dt0_T, p.dt0_C, split_opts(p), loss_species=1)
     assert st.naccept_T >= 2 and st.naccept_C >= 2 and st.replay_maxrel == 0.0
