@@ -100,6 +100,17 @@ typedef struct rs_adjoint_stats {
     long long nvjp_T, nvjp_C;
     double seconds_forward, seconds_backward, seconds_replay;
     double replay_maxrel;    /* replayed end state vs checkpoint (adjoint_gradient.jl:802-808) */
+    /* The first VJP of the backward sweep that made lambda or its gradient
+     * non-finite (adjoint_gradient.jl:763-779: more non-finite entries in
+     * lambda after it than before, or any in its gradient contribution):
+     * bad_op -1 none, 0 transport, 1 chemistry; bad_macro the macro step
+     * (0-based); bad_step the step (0-based, forward order) of bad_nsteps of
+     * that operator in it, with its t and dt; non-finite counts of lambda
+     * before and after it and of its gradient contribution. */
+    int bad_op, bad_macro;
+    long long bad_step, bad_nsteps;
+    double bad_t, bad_dt;
+    long long bad_lam_in, bad_lam_out, bad_grad;
 } rs_adjoint_stats;
 
 /* ---- lifecycle ------------------------------------------------------------ */
@@ -254,6 +265,14 @@ RS_API int rs_adaptive_chem(rs_ctx* ctx, double t0, double t1, double* dt_io,
                      const rs_pictrl* ctrl, double atol, double rtol,
                      int clamp_nonneg, long long maxiters, rs_counts* out);
 
+/* Accepted steps of the last rs_adaptive_transport (op 0) or rs_adaptive_chem
+ * (op 1) call on this context: *n_out pairs (t, dt), t the time the step
+ * started from, written to t_dt[2i], t_dt[2i+1] when t_dt is not NULL (cap =
+ * pairs it can hold).  The sequence the reference's adaptive_solve takes
+ * (rx_traced_integrator.jl:480-505) and its adjoint replays
+ * (tools/adjoint_gradient.jl:647-661 replay_fixed). */
+RS_API int rs_get_step_log(rs_ctx* ctx, int op, double* t_dt, long long cap, long long* n_out);
+
 /* Per-cell adaptive chemistry: every cell integrates [t0, t1] as its own ODE
  * (the reference loop applied with NC = 1), all cells in one persistent
  * work-stealing launch.  dt_cells (NC values, may be NULL = dt0 everywhere)
@@ -290,17 +309,33 @@ RS_API int rs_ros23_vjp(rs_ctx* ctx, double t, double dt);
  * state and the accepted (t, dt) sequences at every macro step, then the
  * backward sweep: per macro step (last first) a fixed-sequence replay from
  * its checkpoint, chemistry VJPs (last step first), then transport VJPs, with
- * the clamp mask lambda *= 1[u_raw > 0] before every step.
- * Loss (loss_kind 0): J = sum over level-0 cells of u[loss_species]^2.
+ * the clamp mask lambda .*= 1[u_raw > 0] before every step.
+ * Loss RS_LOSS_SURFACE_SQ (0): J = sum over level-0 cells of
+ * u[loss_species]^2.  RS_LOSS_WEIGHTS (1): J = <w, u(t_end)> with the weights
+ * of rs_set_objective_weights (loss_species unused).
  * Outputs: *J_out, grad_u0 (NS*NC), grad_emis (NS*NLAT*NLON); either
  * gradient pointer may be NULL.
+ * Errors, after the outputs and stats are written: RS_E_STATE when the
+ * fixed-sequence replay of a macro step lands >= 1e-6 (relative, or
+ * non-finite) from the next checkpoint (adjoint_gradient.jl:802-808);
+ * RS_E_NONFINITE when a VJP made lambda or its gradient non-finite, with the
+ * first such VJP in stats->bad_* and in rs_last_error.
  * Reference: forward_pass / replay_fixed / backward_stage! / backward_sweep,
  * tools/adjoint_gradient.jl:647-834.                                        */
+#define RS_LOSS_SURFACE_SQ 0
+#define RS_LOSS_WEIGHTS    1
 RS_API int rs_adjoint_run(rs_ctx* ctx, int nmacro, const double* tstops,
                    double dtT0, double dtC0, const rs_split_opts* opts,
                    int loss_kind, int loss_species,
                    double* J_out, double* grad_u0, double* grad_emis,
                    rs_adjoint_stats* stats);
+
+/* Weights w (NS*NC, species-major) of the linear objective J = <w, u(t_end)>
+ * (loss kind RS_LOSS_WEIGHTS); NULL clears them.  Replaces the reference's
+ * objective as a weight vector, objective_weights(var_map, spec),
+ * tools/adjoint_gradient.jl:404-426 (the mean of a species over all cells,
+ * the surface or one level: w = 1/count on the matched states). */
+RS_API int rs_set_objective_weights(rs_ctx* ctx, const double* w);
 
 /* Frozen-sequence forward map used by the finite-difference checks of the
  * time-loop adjoint (adjoint_gradient.jl:1183-1244 "fdtape"): re-runs the

@@ -16,6 +16,10 @@ int rs_oracle_transport_rhs_adj(rs_ctx* c, const double* lam, double* g);
 int rs_oracle_transport_rhs_jvp(rs_ctx* c, const double* v, double* out);
 int rs_oracle_chem_rhs(rs_ctx* c, const double* u, double* f);
 int rs_oracle_chem_jac(rs_ctx* c, const double* u, size_t cell, double* J);
+/* Every attempt of the last rs_adaptive_transport / rs_adaptive_chem call:
+ * rec[4i..4i+3] = (t, dtc, EEst, outcome 1 accept / 0 reject / -1 out of
+ * domain); *n_out attempts; rec may be NULL to query the count. */
+int rs_oracle_attempt_log(rs_ctx* c, double* rec, long long cap, long long* n_out);
 int rs_oracle_threads(void);
 #ifdef __cplusplus
 }

@@ -37,6 +37,7 @@
 #include "../include/reseact_hip.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -88,8 +89,14 @@ struct rs_ctx {
     std::vector<double> u, cand;
     bool have_cand = false;
     std::vector<double> lam, gemis;
+    std::vector<double> wobj;   // linear objective weights (loss kind 1), empty = unset
     // time-loop adjoint records (last rs_adjoint_run)
     std::vector<Seq> seqT, seqC;
+    // the last rs_adaptive_transport / rs_adaptive_chem call: every attempt as
+    // (t, dtc, EEst, outcome: 1 accept, 0 reject, -1 out-of-domain reject),
+    // and the accepted (t, dt) per operator (rs_get_step_log)
+    std::vector<std::array<double, 4>> alog;
+    Seq lastseq[2];
     rs_split_opts last_opts{};
     bool have_record = false;
     std::string err;
@@ -1371,8 +1378,9 @@ enum class Op { T, C };
 
 int adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs_pictrl& ctrl,
              double atol, double rtol, int clamp, long long maxiters, rs_counts* out,
-             Seq* seq) {
+             Seq* seq, std::vector<std::array<double, 4>>* alog = nullptr) {
     PIState s{t0, *dt_io, ctrl.qoldinit};
+    if (alog) alog->clear();
     const double tlim = t1 - 1.0e-9;                              // :479
     std::vector<double> raw;
     while (s.t < tlim && s.iters < maxiters) {                    // :480
@@ -1384,9 +1392,12 @@ int adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs_pic
         const size_t nown = (op == Op::T) ? (size_t)c->nt * c->nc : raw.size();
         if ((clamp & 2) && out_of_domain(raw, nown)) {
             reject_out_of_domain(s, dtc);
+            if (alog) alog->push_back({tbefore, dtc, ee, -1.0});
             continue;
         }
-        if (pi_update(ctrl, s, dtc, ee)) {
+        const bool acc = pi_update(ctrl, s, dtc, ee);
+        if (alog) alog->push_back({tbefore, dtc, ee, acc ? 1.0 : 0.0});
+        if (acc) {
             if (clamp & 1) for (double& x : raw) x = std::max(x, 0.0);
             c->u.swap(raw);
             if (seq) seq->ts.emplace_back(tbefore, dtc);
@@ -1461,11 +1472,23 @@ int check_ready(rs_ctx* c, bool need_mech, bool need_wind) {
     return RS_OK;
 }
 
-double loss_value(const rs_ctx& c, int species) {
+// Loss kind 0: J = sum over level-0 cells of u[species]^2; kind 1: J = <w, u>
+// (the reference's linear objective, adjoint_gradient.jl:404-426).
+double loss_value(const rs_ctx& c, int kind, int species) {
     double J = 0.0;
+    if (kind == 1) {
+        for (size_t x = 0; x < c.u.size(); ++x) J += c.wobj[x] * c.u[x];
+        return J;
+    }
     const double* q = c.u.data() + (size_t)species * c.nc;
     for (size_t x = 0; x < c.ncol; ++x) J += q[x] * q[x];
     return J;
+}
+
+int check_loss(rs_ctx* c, int kind, int species) {
+    if (kind == 0) return (species < 0 || species >= c->ns) ? fail(c, RS_E_ARG, "bad loss species") : RS_OK;
+    if (kind == 1) return c->wobj.empty() ? fail(c, RS_E_STATE, "objective weights not set") : RS_OK;
+    return fail(c, RS_E_ARG, "bad loss kind");
 }
 
 // One macro step: transport then chemistry, recording the accepted sequences.
@@ -1487,9 +1510,11 @@ int macro(rs_ctx* c, double t0, double t1, double* dtT, double* dtC, const rs_sp
                     o.maxiters, cC, sC);
 }
 
-void ros23_vjp_all(rs_ctx* c, const std::vector<double>& u, double dt) {
+// Returns the number of non-finite emission-gradient contributions.
+long long ros23_vjp_all(rs_ctx* c, const std::vector<double>& u, double dt) {
     std::vector<double> out(c->lam.size());
-#pragma omp parallel
+    long long nbad = 0;
+#pragma omp parallel reduction(+ : nbad)
     {
         std::vector<double> uc(c->ns), lc(c->ns), lo(c->ns), eb(c->ns);
 #pragma omp for schedule(static)
@@ -1502,10 +1527,20 @@ void ros23_vjp_all(rs_ctx* c, const std::vector<double>& u, double dt) {
             ros23_vjp_cell(cc, uc.data(), dt, lc.data(), lo.data(), surf ? eb.data() : nullptr);
             scatter(*c, lo.data(), cell, out.data());
             if (surf)
-                for (int s = 0; s < c->ns; ++s) c->gemis[(size_t)s * c->ncol + cell] += eb[s];
+                for (int s = 0; s < c->ns; ++s) {
+                    nbad += !std::isfinite(eb[s]);
+                    c->gemis[(size_t)s * c->ncol + cell] += eb[s];
+                }
         }
     }
     c->lam.swap(out);
+    return nbad;
+}
+
+long long count_nonfinite(const std::vector<double>& v) {
+    long long n = 0;
+    for (double x : v) n += !std::isfinite(x);
+    return n;
 }
 
 }  // namespace
@@ -1555,6 +1590,7 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
     c->gemis.assign((size_t)ns * c->ncol, 0.0);
     c->jfac.clear(); c->emis.clear();
     c->have_state = c->have_mech = c->have_wind = false;
+    c->wobj.clear();
     c->scheme = RS_TRANSPORT_UPWIND3;
     c->have_ppm_geom = c->have_ppm_flux = false;
     c->dPSdt.clear();
@@ -1852,7 +1888,9 @@ int rs_adaptive_transport(rs_ctx* c, double t0, double t1, double* dt_io, const 
     int rc = check_ready(c, false, true);
     if (rc) return rc;
     if (!dt_io || !ctrl || !(*dt_io > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
-    return adaptive(c, Op::T, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out, nullptr);
+    c->lastseq[0].ts.clear();
+    return adaptive(c, Op::T, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out,
+                    &c->lastseq[0], &c->alog);
 }
 
 int rs_adaptive_chem(rs_ctx* c, double t0, double t1, double* dt_io, const rs_pictrl* ctrl,
@@ -1860,7 +1898,20 @@ int rs_adaptive_chem(rs_ctx* c, double t0, double t1, double* dt_io, const rs_pi
     int rc = check_ready(c, true, false);
     if (rc) return rc;
     if (!dt_io || !ctrl || !(*dt_io > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
-    return adaptive(c, Op::C, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out, nullptr);
+    c->lastseq[1].ts.clear();
+    return adaptive(c, Op::C, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out,
+                    &c->lastseq[1], &c->alog);
+}
+
+int rs_get_step_log(rs_ctx* c, int op, double* t_dt, long long cap, long long* n_out) {
+    if (!c || (op != 0 && op != 1) || !n_out) return fail(c, RS_E_ARG, "bad step-log arguments");
+    const auto& ts = c->lastseq[op].ts;
+    *n_out = (long long)ts.size();
+    if (t_dt) {
+        if (cap < (long long)ts.size()) return fail(c, RS_E_ARG, "step-log buffer too small");
+        for (size_t i = 0; i < ts.size(); ++i) { t_dt[2 * i] = ts[i].first; t_dt[2 * i + 1] = ts[i].second; }
+    }
+    return RS_OK;
 }
 
 int rs_chem_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells,
@@ -1928,10 +1979,12 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
     if (rc) return rc;
     if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
     if (o->chem_percell) return fail(c, RS_E_ARG, "adjoint needs the global-norm chemistry");
-    if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
-        return fail(c, RS_E_ARG, "bad loss");
+    rc = check_loss(c, loss_kind, loss_species);
+    if (rc) return rc;
     using clk = std::chrono::steady_clock;
     rs_adjoint_stats S{};
+    S.bad_op = S.bad_macro = -1;
+    S.bad_step = S.bad_nsteps = -1;
     auto tf0 = clk::now();
     std::vector<std::vector<double>> ck(nmacro);
     c->seqT.assign(nmacro, {}); c->seqC.assign(nmacro, {});
@@ -1945,7 +1998,7 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         S.naccept_C += cC.naccept; S.nreject_C += cC.nreject;
     }
     std::vector<double> uend = c->u;
-    if (J_out) *J_out = loss_value(*c, loss_species);
+    if (J_out) *J_out = loss_value(*c, loss_kind, loss_species);
     S.seconds_forward = std::chrono::duration<double>(clk::now() - tf0).count();
     c->last_opts = *o;
     c->have_record = true;
@@ -1954,11 +2007,28 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
     double trep = 0.0;
     std::fill(c->lam.begin(), c->lam.end(), 0.0);
     std::fill(c->gemis.begin(), c->gemis.end(), 0.0);
-    {
+    if (loss_kind == 1) {
+        c->lam = c->wobj;
+    } else {
         const double* q = uend.data() + (size_t)loss_species * c->nc;
         double* L = c->lam.data() + (size_t)loss_species * c->nc;
         for (size_t x = 0; x < c->ncol; ++x) L[x] = 2.0 * q[x];
     }
+    // first VJP that made lambda or its gradient non-finite
+    // (adjoint_gradient.jl:763-779)
+    long long nprev = count_nonfinite(c->lam);
+    bool bad = false;
+    auto note = [&](int op, int m, int j, const Seq& sq, long long ngrad) {
+        const long long nout = count_nonfinite(c->lam);
+        if (!bad && (nout > nprev || ngrad > 0)) {
+            bad = true;
+            S.bad_op = op; S.bad_macro = m; S.bad_step = j; S.bad_nsteps = (long long)sq.ts.size();
+            S.bad_t = sq.ts[j].first; S.bad_dt = sq.ts[j].second;
+            S.bad_lam_in = nprev; S.bad_lam_out = nout; S.bad_grad = ngrad;
+        }
+        nprev = nout;
+    };
+    int worst = 0;
     const int clampf = o->clamp_nonneg & 1;
     for (int m = nmacro - 1; m >= 0; --m) {
         auto tr0 = clk::now();
@@ -1980,19 +2050,23 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         }
         const std::vector<double>& ref = (m + 1 < nmacro) ? ck[m + 1] : uend;
         double d = 0.0;
-        for (size_t x = 0; x < u.size(); ++x)
-            d = std::max(d, std::fabs(u[x] - ref[x]) / std::max(std::fabs(ref[x]), 1e-30));
-        S.replay_maxrel = std::max(S.replay_maxrel, d);
+        for (size_t x = 0; x < u.size(); ++x) {
+            const double e = std::fabs(u[x] - ref[x]) / std::max(std::fabs(ref[x]), 1e-30);
+            if (!(e <= d)) d = e;   // a NaN deviation wins
+        }
+        if (!(d <= S.replay_maxrel)) { S.replay_maxrel = d; worst = m; }
         trep += std::chrono::duration<double>(clk::now() - tr0).count();
-        // chemistry backwards (adjoint_gradient.jl:821)
+        // chemistry backwards (adjoint_gradient.jl:821); the clamp mask is a
+        // multiplication (lam .* mask, :736), so a non-finite entry stays
         auto mask = [&](const std::vector<double>& out) {
             if (!clampf) return;
-            for (size_t x = 0; x < c->lam.size(); ++x) if (!(out[x] > 0.0)) c->lam[x] = 0.0;
+            for (size_t x = 0; x < c->lam.size(); ++x) c->lam[x] = c->lam[x] * ((out[x] > 0.0) ? 1.0 : 0.0);
         };
         for (int jx = (int)tapeC.size() - 1; jx >= 0; --jx) {
             const std::vector<double>& outj = (jx + 1 < (int)tapeC.size()) ? tapeC[jx + 1] : u;
             mask(outj);
-            ros23_vjp_all(c, tapeC[jx], c->seqC[m].ts[jx].second);
+            const long long ng = ros23_vjp_all(c, tapeC[jx], c->seqC[m].ts[jx].second);
+            note(1, m, jx, c->seqC[m], ng);
             S.nvjp_C += 1;
         }
         for (int jx = (int)tapeT.size() - 1; jx >= 0; --jx) {
@@ -2002,6 +2076,7 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
                 ssprk43_vjp_ppm(*c, tapeT[jx].data(), c->lam, c->seqT[m].ts[jx].second);
             else
                 ssprk43_vjp(*c, c->lam, c->seqT[m].ts[jx].second);
+            note(0, m, jx, c->seqT[m], 0);
             S.nvjp_T += 1;
         }
     }
@@ -2010,6 +2085,29 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
     if (grad_u0) std::copy(c->lam.begin(), c->lam.end(), grad_u0);
     if (grad_emis) std::copy(c->gemis.begin(), c->gemis.end(), grad_emis);
     if (st) *st = S;
+    if (!(S.replay_maxrel < 1e-6)) {   // adjoint_gradient.jl:802-808
+        char b[200];
+        std::snprintf(b, sizeof(b), "fixed-sequence replay of macro step %d lands %.3g relative "
+                      "away from the checkpointed end state: the tape is not the forward "
+                      "trajectory", worst + 1, S.replay_maxrel);
+        return fail(c, RS_E_STATE, b);
+    }
+    if (bad) {
+        char b[320];
+        std::snprintf(b, sizeof(b),
+                      "non-finite adjoint: first at %s step %lld of %lld in macro step %d "
+                      "(t=%.9g dt=%.9g): lambda non-finite %lld -> %lld, gradient non-finite %lld",
+                      S.bad_op ? "chemistry" : "transport", S.bad_step + 1, S.bad_nsteps,
+                      S.bad_macro + 1, S.bad_t, S.bad_dt, S.bad_lam_in, S.bad_lam_out, S.bad_grad);
+        return fail(c, RS_E_NONFINITE, b);
+    }
+    return RS_OK;
+}
+
+int rs_set_objective_weights(rs_ctx* c, const double* w) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!w) { c->wobj.clear(); return RS_OK; }
+    c->wobj.assign(w, w + (size_t)c->ns * c->nc);
     return RS_OK;
 }
 
@@ -2017,8 +2115,8 @@ int rs_replay_loss(rs_ctx* c, int loss_kind, int loss_species, double* J_out) {
     int rc = check_ready(c, true, true);
     if (rc) return rc;
     if (!c->have_record) return fail(c, RS_E_STATE, "no recorded sequence");
-    if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
-        return fail(c, RS_E_ARG, "bad loss");
+    rc = check_loss(c, loss_kind, loss_species);
+    if (rc) return rc;
     const rs_split_opts& o = c->last_opts;
     std::vector<double> raw;
     for (size_t m = 0; m < c->seqT.size(); ++m) {
@@ -2033,7 +2131,7 @@ int rs_replay_loss(rs_ctx* c, int loss_kind, int loss_species, double* J_out) {
             c->u.swap(raw);
         }
     }
-    if (J_out) *J_out = loss_value(*c, loss_species);
+    if (J_out) *J_out = loss_value(*c, loss_kind, loss_species);
     return RS_OK;
 }
 
@@ -2161,6 +2259,17 @@ int rs_oracle_chem_jac(rs_ctx* c, const double* u, size_t cell, double* J) {
     cc.jac(uc.data(), J);
     return RS_OK;
 }
+int rs_oracle_attempt_log(rs_ctx* c, double* rec, long long cap, long long* n_out) {
+    if (!c || !n_out) return RS_E_ARG;
+    *n_out = (long long)c->alog.size();
+    if (rec) {
+        if (cap < (long long)c->alog.size()) return fail(c, RS_E_ARG, "attempt-log buffer too small");
+        for (size_t i = 0; i < c->alog.size(); ++i)
+            for (int q = 0; q < 4; ++q) rec[4 * i + q] = c->alog[i][q];
+    }
+    return RS_OK;
+}
+
 int rs_oracle_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

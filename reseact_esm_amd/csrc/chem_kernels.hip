@@ -125,7 +125,8 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
  *   f0 = f(u); k1 = W\f0; u1 = u + h/2 k1; f1 = f(u1); k2 = W\(f1-k1) + k1;
  *   unew = u + h k2,   W = I - h*gamma*J(u).
  * Given lam = dL/dunew, returns dL/du in lam and adds dL/dE to G[i*es] (G
- * may be 0; E enters f additively on the cells that have it).  Reverse:
+ * may be 0; E enters f additively on the cells that have it); returns the
+ * number of non-finite values among those dL/dE contributions.  Reverse:
  *   z2 = W^-T (h lam)             = f1bar;  k1bar = h lam - z2
  *   u1bar = J(u1)^T z2;  ubar = lam + u1bar;  k1bar += h/2 u1bar
  *   z1 = W^-T k1bar               = f0bar;  ubar += J(u)^T z1
@@ -139,7 +140,7 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
 #define RS_SB(i) sv[(RS_NS + (i)) * ST]
 #define RS_SC(i) sv[(2 * RS_NS + (i)) * ST]
 template <int ST, int WS, class LIn, class LOut>
-RS_FN void rs_ros23_vjp(const double* __restrict__ K, double jf, const double* __restrict__ E,
+RS_FN int rs_ros23_vjp(const double* __restrict__ K, double jf, const double* __restrict__ E,
                         long long es, double* __restrict__ sv, double* __restrict__ W, double dt,
                         LIn lam_in, LOut lam_out, double* __restrict__ G) {
     double f[RS_NS], x[RS_NS], y[RS_NS];
@@ -187,10 +188,16 @@ RS_FN void rs_ros23_vjp(const double* __restrict__ K, double jf, const double* _
     rs_hess<ST>(K, jf, x, &RS_SC(0), hg, ub);
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) lam_out(i, ub[i]);
+    int nbad = 0;
     if (G) {
 #pragma unroll
-        for (int i = 0; i < RS_NS; ++i) G[i * es] += x[i] + y[i];
+        for (int i = 0; i < RS_NS; ++i) {
+            const double g = x[i] + y[i];
+            nbad += !RS_ISFINITE(g);
+            G[i * es] += g;
+        }
     }
+    return nbad;
 }
 #undef RS_SA
 #undef RS_SB
@@ -425,7 +432,10 @@ rs_chem_percell(RsPercellArgs a RS_KPARAM) {
 #undef RS_U
 
 /* One ROS23 step VJP for every cell, with the clamp mask of that step
- * (lam *= 1[uout > 0], adjoint_gradient.jl:736) applied first. */
+ * (lam .*= 1[uout > 0], adjoint_gradient.jl:736; a multiplication, so a
+ * non-finite entry stays non-finite) applied first.  With a.nfc the step's
+ * non-finite outputs are counted (adjoint_gradient.jl:763-779): a lane with
+ * none issues no atomic. */
 extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK)
 rs_chem_vjp(RsChemVjpArgs a RS_KPARAM) {
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -438,14 +448,22 @@ rs_chem_vjp(RsChemVjpArgs a RS_KPARAM) {
     const bool surf = cell < a.ncol;
     const double* Ec = (surf && a.E) ? a.E + cell : nullptr;
     double* Gc = (surf && a.gE) ? a.gE + cell : nullptr;
-    rs_ros23_vjp<RS_SLAB_ST, RS_W_ST>(
+    unsigned nlam = 0;
+    const int ngrad = rs_ros23_vjp<RS_SLAB_ST, RS_W_ST>(
         RS_KPTR(a), jf, Ec, a.ncol, sv, W, a.dt,
         [&](int s) {
             double l = a.lam[(long long)s * a.nc + cell];
-            if (a.uout && !(a.uout[(long long)s * a.nc + cell] > 0.0)) l = 0.0;
+            if (a.uout) l = l * ((a.uout[(long long)s * a.nc + cell] > 0.0) ? 1.0 : 0.0);
             return l;
         },
-        [&](int s, double v) { a.lam[(long long)s * a.nc + cell] = v; }, Gc);
+        [&](int s, double v) {
+            nlam += !RS_ISFINITE(v);
+            a.lam[(long long)s * a.nc + cell] = v;
+        }, Gc);
+    if (a.nfc) {
+        if (nlam) atomicAdd(&a.nfc[0], (unsigned long long)nlam);
+        if (ngrad) atomicAdd(&a.nfc[1], (unsigned long long)ngrad);
+    }
 }
 
 #endif /* RS_HOST_BUILD */

@@ -9,11 +9,13 @@
 #define RS_SQRT std::sqrt
 #define RS_FABS std::fabs
 #define RS_POW std::pow
+#define RS_ISFINITE std::isfinite
 #else
 #define RS_FN __device__ __attribute__((always_inline)) inline
 #define RS_SQRT sqrt
 #define RS_FABS fabs
 #define RS_POW pow
+#define RS_ISFINITE __builtin_isfinite
 #endif
 /* Pivots.  RS_PIVOT(w) gives 1/U_kk for the elimination (the reference's
  * piv = 1 ./ W[k,k], rx_traced_integrator.jl:114); RS_DIAG_SOLVE(acc, w)

@@ -57,6 +57,9 @@ int rs_reduce_blocks(long long n);
 hipError_t rs_launch_relmax(const double* a, const double* b, long long n, double* part, hipStream_t st);
 hipError_t rs_launch_sumsq(const double* q, long long n, double* part, hipStream_t st);
 hipError_t rs_launch_sum(const double* x, long long n, double* part, hipStream_t st);
+hipError_t rs_launch_nonfinite(const double* x, long long n, unsigned long long* count,
+                               hipStream_t st);
+hipError_t rs_launch_dot(const double* w, const double* u, long long n, double* part, hipStream_t st);
 hipError_t rs_launch_loss_seed(double* lam, const double* u, long long ntot, long long sbase,
                                long long ncol, hipStream_t st);
 }
@@ -199,9 +202,13 @@ struct rs_ctx {
     std::unique_ptr<ChemModule> chem;
     // time-loop adjoint records
     std::vector<std::vector<std::pair<double, double>>> seqT, seqC;
+    std::vector<std::pair<double, double>> lastseq[2];   // rs_get_step_log
     rs_split_opts last_opts{};
     bool have_record = false;
     DevArray<double> ckpt, tape, relpart;
+    DevArray<unsigned long long> nfc;   // non-finite counts of the backward sweep's VJPs
+    DevArray<double> wobj;              // weights of the linear objective (loss kind 1)
+    bool have_wobj = false;
     // profiling
     bool prof = false;
     struct EvPair { hipEvent_t a, b; int kind; };
@@ -633,12 +640,14 @@ double host_sum(rs_ctx* c, const double* dpart, int n) {
     return s;
 }
 
+// max of n device partials; a NaN partial makes the result NaN
 double host_max(rs_ctx* c, const double* dpart, int n) {
     if (n > 1024) throw ArgError{"host_max: more than 1024 partials", RS_E_ARG};
     HIPCHK(hipMemcpyAsync(c->hred, dpart, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     double s = 0.0;
-    for (int i = 0; i < n; ++i) s = std::max(s, c->hred[i]);
+    for (int i = 0; i < n; ++i)
+        if (!(c->hred[i] <= s)) s = c->hred[i];
     return s;
 }
 
@@ -964,8 +973,10 @@ void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
     apply(u, 0, c->LA.p, 1.0, c->LB.p, 2.0 / 3.0, c->lam.p);
 }
 
-void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt) {
+void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt,
+                       unsigned long long* nfc = nullptr) {
     RsChemVjpArgs a{};
+    a.nfc = nfc;
     a.u = u;
     a.uout = uout;
     a.lam = c->lam.p;
@@ -979,12 +990,33 @@ void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt
     chem_launch(c, c->chem->vjp, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEMVJP);
 }
 
-double loss_on_device(rs_ctx* c, const double* u, int species) {
+// The objective at state u: loss kind 0, J = sum over level-0 cells of
+// u[species]^2 (the workloads' quadratic loss); kind 1, J = <w, u> with the
+// weights of rs_set_objective_weights (the reference's linear objective,
+// adjoint_gradient.jl:408-426).
+double loss_on_device(rs_ctx* c, const double* u, int kind, int species) {
+    c->red.ensure(1024);
+    if (kind == 1) {
+        const long long n = (long long)c->N();
+        HIPCHK(rs_launch_dot(c->wobj.p, u, n, c->red.p, c->stream));
+        return host_sum(c, c->red.p, rs_reduce_blocks(n));
+    }
     const double* q = u + (size_t)species * c->nc;
     const int nb = rs_reduce_blocks(c->ncol);
-    c->red.ensure(1024);
     HIPCHK(rs_launch_sumsq(q, c->ncol, c->red.p, c->stream));
     return host_sum(c, c->red.p, nb);
+}
+
+int check_loss(rs_ctx* c, int kind, int species) {
+    if (kind == 0) {
+        if (species < 0 || species >= c->ns) return fail(c, RS_E_ARG, "bad loss species");
+        return RS_OK;
+    }
+    if (kind == 1) {
+        if (!c->have_wobj) return fail(c, RS_E_STATE, "objective weights not set");
+        return RS_OK;
+    }
+    return fail(c, RS_E_ARG, "bad loss kind");
 }
 
 // The replay tape: the ensemble's pool slice when one is lent, else the
@@ -1043,14 +1075,81 @@ size_t tape_bytes(rs_ctx* c, int nmacro) {
     return maxslots * c->N() * sizeof(double);
 }
 
-// lambda = dJ/du(t_end) for J = sum over level-0 cells of u[species]^2, from the
-// final checkpoint; emission gradient cleared.
-void backward_seed(rs_ctx* c, int species) {
+// lambda = dJ/du(t_end) from the final checkpoint: 2 u[species] on level-0
+// cells (kind 0) or the weights (kind 1); emission gradient cleared.  The
+// non-finite counters of the sweep are cleared (a fill kernel, so that the
+// graph-captured ensemble records no memset) and slot 0 gets the seed's count.
+void backward_seed(rs_ctx* c, int kind, int species, int nvjp) {
     const size_t N = c->N();
     const int nm = (int)c->seqT.size();
-    HIPCHK(rs_launch_loss_seed(c->lam.p, c->ckpt.p + (size_t)nm * N, (long long)N,
-                               (long long)species * c->nc, c->ncol, c->stream));
+    if (kind == 1) d2d(c->lam.p, c->wobj.p, N, c->stream);
+    else HIPCHK(rs_launch_loss_seed(c->lam.p, c->ckpt.p + (size_t)nm * N, (long long)N,
+                                    (long long)species * c->nc, c->ncol, c->stream));
     HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
+    HIPCHK(rs_launch_fill_u64(c->nfc.p, 2 * (nvjp + 1), 0ULL, c->stream));
+    HIPCHK(rs_launch_nonfinite(c->lam.p, (long long)N, c->nfc.p, c->stream));
+}
+
+// VJPs of the backward sweep, and the counter slots before macro step m's
+// (slot 0 is the seed; macro steps are swept last first).
+int sweep_vjps(const rs_ctx* c) {
+    size_t n = 0;
+    for (size_t m = 0; m < c->seqT.size(); ++m) n += c->seqT[m].size() + c->seqC[m].size();
+    if (n > (size_t)(INT_MAX / 2 - 2)) throw ArgError{"too many steps for the adjoint", RS_E_ARG};
+    return (int)n;
+}
+size_t sweep_slot(const rs_ctx* c, int m) {
+    size_t k = 1;
+    for (size_t q = (size_t)m + 1; q < c->seqT.size(); ++q) k += c->seqT[q].size() + c->seqC[q].size();
+    return k;
+}
+
+// The first VJP of the sweep that made lambda or its gradient non-finite
+// (adjoint_gradient.jl:763-779: lambda_out has more non-finite entries than
+// lambda_in, or the step's gradient contribution has any), from the counters
+// read back: cnt[2 i], cnt[2 i + 1] for slot i.  Fills the bad_* fields of S;
+// returns false when the sweep stayed finite.
+bool first_nonfinite(const rs_ctx* c, const std::vector<unsigned long long>& cnt,
+                     rs_adjoint_stats& S) {
+    S.bad_op = -1;
+    S.bad_macro = -1;
+    S.bad_step = S.bad_nsteps = -1;
+    S.bad_t = S.bad_dt = 0.0;
+    S.bad_lam_in = S.bad_lam_out = S.bad_grad = 0;
+    unsigned long long prev = cnt[0];
+    size_t i = 1;
+    for (int m = (int)c->seqT.size() - 1; m >= 0; --m) {
+        for (int op = 1; op >= 0; --op) {
+            const auto& sq = op ? c->seqC[m] : c->seqT[m];
+            for (size_t q = sq.size(); q-- > 0; ++i) {
+                const unsigned long long nl = cnt[2 * i], ng = cnt[2 * i + 1];
+                if (nl > prev || ng > 0) {
+                    S.bad_op = op;
+                    S.bad_macro = m;
+                    S.bad_step = (long long)q;
+                    S.bad_nsteps = (long long)sq.size();
+                    S.bad_t = sq[q].first;
+                    S.bad_dt = sq[q].second;
+                    S.bad_lam_in = (long long)prev;
+                    S.bad_lam_out = (long long)nl;
+                    S.bad_grad = (long long)ng;
+                    return true;
+                }
+                prev = nl;
+            }
+        }
+    }
+    return false;
+}
+
+std::string nonfinite_message(const rs_adjoint_stats& S) {
+    char b[320];
+    std::snprintf(b, sizeof(b),
+                  "non-finite adjoint: first at %s step %lld of %lld in macro step %d (t=%.9g "
+                  "dt=%.9g): lambda non-finite %lld -> %lld, gradient non-finite %lld",
+                  S.bad_op ? "chemistry" : "transport", S.bad_step + 1, S.bad_nsteps,
+                  S.bad_macro + 1, S.bad_t, S.bad_dt, S.bad_lam_in, S.bad_lam_out, S.bad_grad);
+    return b;
 }
 
 // Backward work of macro step m, enqueued on c->stream without host
@@ -1062,6 +1161,7 @@ void backward_seed(rs_ctx* c, int species) {
 // its own step.  Optional events bracket the replay.
 void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* relpart,
                             hipEvent_t ev0, hipEvent_t ev1) {
+    unsigned long long* slot = c->nfc.p + 2 * sweep_slot(c, m);
     const size_t N = c->N();
     const auto& sT = c->seqT[m];
     const auto& sC = c->seqC[m];
@@ -1072,26 +1172,47 @@ void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* re
                             relpart, c->stream));
     if (ev1) HIPCHK(hipEventRecord(ev1, c->stream));
     const size_t nT = sT.size();
-    for (size_t q = sC.size(); q-- > 0;) {
-        const size_t slot = nT + q;
-        enqueue_ros23_vjp(c, tp + slot * N, clamp_nonneg_of(o.clamp_nonneg) ? tp + (slot + 1) * N : nullptr,
-                          sC[q].second);
+    for (size_t q = sC.size(); q-- > 0; slot += 2) {
+        const size_t ts = nT + q;
+        enqueue_ros23_vjp(c, tp + ts * N, clamp_nonneg_of(o.clamp_nonneg) ? tp + (ts + 1) * N : nullptr,
+                          sC[q].second, slot);
     }
-    for (size_t q = nT; q-- > 0;) {
+    for (size_t q = nT; q-- > 0; slot += 2) {
         if (clamp_nonneg_of(o.clamp_nonneg)) {
             ProfScope ps(c, P_UTIL);
             HIPCHK(rs_launch_mask(c->lam.p, tp + (q + 1) * N, (long long)N, c->stream));
         }
         if (trans_ppm(c)) enqueue_ssprk43_vjp_ppm(c, tp + q * N, sT[q].second);
         else enqueue_ssprk43_vjp(c, sT[q].second);
+        ProfScope ps(c, P_UTIL);
+        HIPCHK(rs_launch_nonfinite(c->lam.p, (long long)N, slot, c->stream));
     }
 }
 
-// max over `nsets` groups of `n` device partials (each group <= 1024)
-double host_max_many(rs_ctx* c, const double* dpart, int nsets, int n) {
+// max over `nsets` groups of `n` device partials (each group <= 1024); *worst
+// gets the group of the largest (a NaN wins)
+double host_max_many(rs_ctx* c, const double* dpart, int nsets, int n, int* worst = nullptr) {
     double mx = 0.0;
-    for (int s = 0; s < nsets; ++s) mx = std::max(mx, host_max(c, dpart + (size_t)s * 1024, n));
+    for (int s = 0; s < nsets; ++s) {
+        const double v = host_max(c, dpart + (size_t)s * 1024, n);
+        if (!(v <= mx)) {
+            mx = v;
+            if (worst) *worst = s;
+        }
+    }
     return mx;
+}
+
+// The replay check of the reference (adjoint_gradient.jl:802-808): the
+// fixed-sequence replay of every macro step must land within 1e-6 (relative,
+// entrywise) of the next checkpoint.
+std::string replay_failure(double d, int m) {
+    if (d < 1e-6) return "";
+    char b[200];
+    std::snprintf(b, sizeof(b),
+                  "fixed-sequence replay of macro step %d lands %.3g relative away from the "
+                  "checkpointed end state: the tape is not the forward trajectory", m + 1, d);
+    return b;
 }
 
 }  // namespace
@@ -1224,6 +1345,7 @@ int rs_set_grid(rs_ctx* c, int nlon, int nlat, int nlev, int ns, int nt) {
         c->has_jfac = c->has_E = false;
         c->chem.reset();
         c->have_record = false;
+        c->have_wobj = false;
         return RS_OK;
     });
 }
@@ -1439,12 +1561,16 @@ int install_mechanism(rs_ctx* c, rs::MechSpec& m, const double* k) {
         auto cm = std::make_unique<ChemModule>();
         cm->plan = rs::plan_mechanism(m, c->lu_order);
         std::string log;
-        // Exact arithmetic (no contraction, true division) is opt-in with
-        // RS_JIT_EXACT=1: on gfx950 that build has returned wrong values for
-        // some lanes on repeated launches, so it is experimental and off.
-        bool exact = false;
-        if (const char* ev = std::getenv("RS_JIT_EXACT")) exact = (ev[0] == '1');
-        std::vector<char> code = jit_compile(chem_source(m, cm->plan), exact, log);
+        // The exact-arithmetic build (no contraction, true division) is
+        // refused: on gfx950 it has returned wrong values for some lanes on a
+        // repeated launch, with no cause shown, so RS_JIT_EXACT=1 is an error
+        // rather than a silently different result.
+        if (const char* ev = std::getenv("RS_JIT_EXACT"))
+            if (ev[0] == '1')
+                return fail(c, RS_E_ARG, "RS_JIT_EXACT=1: the exact-arithmetic chemistry build "
+                                         "is disabled (wrong lanes on repeated launches, cause "
+                                         "not found)");
+        std::vector<char> code = jit_compile(chem_source(m, cm->plan), false, log);
         if (code.empty()) return fail(c, RS_E_JIT, "hipRTC compile failed: " + log.substr(0, 2000));
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipModuleLoadData(&cm->mod, code.data()));
@@ -1616,7 +1742,9 @@ int rs_adaptive_transport(rs_ctx* c, double t0, double t1, double* dt_io, const 
     if (c->nt == 0) return fail(c, RS_E_STATE, "no transported species");
     c->have_cand = false;
     return guarded(c, [&]() {
-        return run_adaptive(c, Op::T, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out, nullptr);
+        c->lastseq[0].clear();
+        return run_adaptive(c, Op::T, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out,
+                            &c->lastseq[0]);
     });
 }
 
@@ -1627,8 +1755,21 @@ int rs_adaptive_chem(rs_ctx* c, double t0, double t1, double* dt_io, const rs_pi
     if (!dt_io || !ctrl || !(*dt_io > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
     c->have_cand = false;
     return guarded(c, [&]() {
-        return run_adaptive(c, Op::C, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out, nullptr);
+        c->lastseq[1].clear();
+        return run_adaptive(c, Op::C, t0, t1, dt_io, *ctrl, atol, rtol, clamp, maxiters, out,
+                            &c->lastseq[1]);
     });
+}
+
+int rs_get_step_log(rs_ctx* c, int op, double* t_dt, long long cap, long long* n_out) {
+    if (!c || (op != 0 && op != 1) || !n_out) return fail(c, RS_E_ARG, "bad step-log arguments");
+    const auto& ts = c->lastseq[op];
+    *n_out = (long long)ts.size();
+    if (t_dt) {
+        if (cap < (long long)ts.size()) return fail(c, RS_E_ARG, "step-log buffer too small");
+        for (size_t i = 0; i < ts.size(); ++i) { t_dt[2 * i] = ts[i].first; t_dt[2 * i + 1] = ts[i].second; }
+    }
+    return RS_OK;
 }
 
 int rs_chem_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells,
@@ -1713,12 +1854,13 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
     if (rc) return rc;
     if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
     if (o->chem_percell) return fail(c, RS_E_ARG, "adjoint needs the global-norm chemistry");
-    if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
-        return fail(c, RS_E_ARG, "bad loss");
+    rc = check_loss(c, loss_kind, loss_species);
+    if (rc) return rc;
     c->have_cand = false;
     return guarded(c, [&]() -> int {
         using clk = std::chrono::steady_clock;
         rs_adjoint_stats S{};
+        S.bad_op = S.bad_macro = -1;
         const size_t N = c->N();
         c->ckpt.ensure((size_t)(nmacro + 1) * N);
         c->seqT.assign(nmacro, {});
@@ -1736,7 +1878,9 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         }
         d2d(c->ckpt.p + (size_t)nmacro * N, c->S[c->cur].p, N, c->stream);
         size_tape(c, nmacro);
-        const double J = loss_on_device(c, c->S[c->cur].p, loss_species);
+        const int nvjp = sweep_vjps(c);
+        c->nfc.ensure(2 * (size_t)(nvjp + 1));
+        const double J = loss_on_device(c, c->S[c->cur].p, loss_kind, loss_species);
         if (J_out) *J_out = J;
         HIPCHK(hipStreamSynchronize(c->stream));
         S.seconds_forward = std::chrono::duration<double>(clk::now() - tf0).count();
@@ -1744,7 +1888,7 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         c->have_record = true;
 
         auto tb0 = clk::now();
-        backward_seed(c, loss_species);
+        backward_seed(c, loss_kind, loss_species, nvjp);
         c->relpart.ensure((size_t)nmacro * 1024);
         std::vector<hipEvent_t> ev(2 * (size_t)nmacro);
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -1760,14 +1904,39 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         }
         for (auto& e : ev) (void)hipEventDestroy(e);
         S.seconds_replay = trep;
-        S.replay_maxrel = host_max_many(c, c->relpart.p, nmacro, rs_reduce_blocks((long long)N));
+        int worst = 0;
+        S.replay_maxrel = host_max_many(c, c->relpart.p, nmacro, rs_reduce_blocks((long long)N),
+                                        &worst);
         for (int m = 0; m < nmacro; ++m) {
             S.nvjp_T += (long long)c->seqT[m].size();
             S.nvjp_C += (long long)c->seqC[m].size();
         }
+        std::vector<unsigned long long> cnt(2 * (size_t)(nvjp + 1));
+        HIPCHK(hipMemcpyAsync(cnt.data(), c->nfc.p, cnt.size() * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c->stream));
         if (grad_u0) d2h(grad_u0, c->lam.p, N, c->stream);
         if (grad_emis) d2h(grad_emis, c->gE.p, c->gE.n, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const bool bad = first_nonfinite(c, cnt, S);
         if (st) *st = S;
+        const std::string rf = replay_failure(S.replay_maxrel, worst);
+        if (!rf.empty()) return fail(c, RS_E_STATE, rf);
+        if (bad) return fail(c, RS_E_NONFINITE, nonfinite_message(S));
+        return RS_OK;
+    });
+}
+
+int rs_set_objective_weights(rs_ctx* c, const double* w) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    return guarded(c, [&]() {
+        if (!w) {
+            c->have_wobj = false;
+            return RS_OK;
+        }
+        c->wobj.resize(c->N());
+        h2d(c->wobj.p, w, c->N(), c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->have_wobj = true;
         return RS_OK;
     });
 }
@@ -1776,8 +1945,8 @@ int rs_replay_loss(rs_ctx* c, int loss_kind, int loss_species, double* J_out) {
     int rc = check_ready(c, true, true);
     if (rc) return rc;
     if (!c->have_record) return fail(c, RS_E_STATE, "no recorded sequence");
-    if (loss_kind != 0 || loss_species < 0 || loss_species >= c->ns)
-        return fail(c, RS_E_ARG, "bad loss");
+    rc = check_loss(c, loss_kind, loss_species);
+    if (rc) return rc;
     return guarded(c, [&]() {
         const size_t N = c->N();
         const double* u = c->S[c->cur].p;
@@ -1789,7 +1958,7 @@ int rs_replay_loss(rs_ctx* c, int loss_kind, int loss_species, double* J_out) {
             const size_t ns = replay_into_tape(c, tmp.p, c->seqT[m], c->seqC[m], c->last_opts);
             d2d(tmp.p, tape_ptr(c, ns * N) + (ns - 1) * N, N, c->stream);
         }
-        const double J = loss_on_device(c, tmp.p, loss_species);
+        const double J = loss_on_device(c, tmp.p, loss_kind, loss_species);
         if (J_out) *J_out = J;
         d2d(c->S[c->cur].p, tmp.p, N, c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -2059,17 +2228,22 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
                             rs_ensemble_stats* es) {
     if (!e || !o || !tstops || nmacro < 1) return ens_fail(e, RS_E_ARG, "bad arguments");
     if (o->chem_percell) return ens_fail(e, RS_E_ARG, "adjoint needs the global-norm chemistry");
-    if (loss_kind != 0) return ens_fail(e, RS_E_ARG, "bad loss");
     const int n = (int)e->m.size();
     for (int d = 0; d < n; ++d) {
         int rc = check_ready(e->m[d], true, true);
+        if (!rc) rc = check_loss(e->m[d], loss_kind, loss_species);
         if (rc) return ens_fail(e, rc, "member " + std::to_string(d) + ": " + e->m[d]->err);
-        if (loss_species < 0 || loss_species >= e->m[d]->ns) return ens_fail(e, RS_E_ARG, "bad loss species");
     }
     using clk = std::chrono::steady_clock;
     rs_ensemble_stats ES{};
     std::vector<rs_adjoint_stats> S(n);
     std::vector<int> profs(n);
+    std::vector<int> nvjp(n, 0), worst(n, 0);
+    // graph execs of the forward phases, destroyed on every path out
+    struct ExecGuard {
+        hipGraphExec_t x = nullptr;
+        ~ExecGuard() { if (x) (void)hipGraphExecDestroy(x); }
+    } gT, gC;
     try {
         HIPCHK(hipSetDevice(e->device));
         std::vector<int> all(n), withT;
@@ -2094,9 +2268,9 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
         AdaptiveRun proto_T{Op::T, o->atol_T, o->rtol_T, o->clamp_nonneg & RS_CLAMP_NONNEG,
                             o->maxiters, 0, true};
         AdaptiveRun proto_C{Op::C, o->atol_C, o->rtol_C, o->clamp_nonneg, o->maxiters, 0, true};
-        hipGraphExec_t gT = withT.empty() ? nullptr
-                                          : ens_capture(e, withT, [&](int d) { adaptive_chunk(e->m[d], proto_T, CH); }, nullptr);
-        hipGraphExec_t gC = ens_capture(e, all, [&](int d) { adaptive_chunk(e->m[d], proto_C, CH); }, nullptr);
+        if (!withT.empty())
+            gT.x = ens_capture(e, withT, [&](int d) { adaptive_chunk(e->m[d], proto_T, CH); }, nullptr);
+        gC.x = ens_capture(e, all, [&](int d) { adaptive_chunk(e->m[d], proto_C, CH); }, nullptr);
         std::vector<double> dtT(n, dtT0), dtC(n, dtC0);
         auto tf0 = clk::now();
         auto phase = [&](Op op, hipGraphExec_t g, const std::vector<int>& idx, int mstep) {
@@ -2139,25 +2313,28 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
                 d2d(c->ckpt.p + (size_t)mstep * c->N(), c->S[c->cur].p, c->N(), c->stream);
                 HIPCHK(hipStreamSynchronize(c->stream));
             }
-            if (gT) phase(Op::T, gT, withT, mstep);
-            phase(Op::C, gC, all, mstep);
+            if (gT.x) phase(Op::T, gT.x, withT, mstep);
+            phase(Op::C, gC.x, all, mstep);
         }
         for (int d = 0; d < n; ++d) {
             rs_ctx* c = e->m[d];
             d2d(c->ckpt.p + (size_t)nmacro * c->N(), c->S[c->cur].p, c->N(), c->stream);
-            const double J = loss_on_device(c, c->S[c->cur].p, loss_species);
+            const double J = loss_on_device(c, c->S[c->cur].p, loss_kind, loss_species);
             if (J_out) J_out[d] = J;
             c->last_opts = *o;
             c->have_record = true;
+            nvjp[d] = sweep_vjps(c);
+            c->nfc.ensure(2 * (size_t)(nvjp[d] + 1));
         }
         ES.seconds_forward = std::chrono::duration<double>(clk::now() - tf0).count();
-        if (gT) (void)hipGraphExecDestroy(gT);
-        (void)hipGraphExecDestroy(gC);
+        if (gT.x) (void)hipGraphExecDestroy(gT.x);
+        (void)hipGraphExecDestroy(gC.x);
+        gT.x = gC.x = nullptr;
 
         // backward: groups of members whose tapes fit the budget
         auto tb0 = clk::now();
         for (int d = 0; d < n; ++d) {
-            backward_seed(e->m[d], loss_species);
+            backward_seed(e->m[d], loss_kind, loss_species, nvjp[d]);
             HIPCHK(hipStreamSynchronize(e->m[d]->stream));
         }
         const double budget = (tape_budget_gib > 0 ? tape_budget_gib : 64.0) * 1073741824.0;
@@ -2194,22 +2371,23 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
                 off += need;
             }
             size_t nn = 0;
-            hipGraphExec_t gB = ens_capture(e, grp, [&](int d) {
+            ExecGuard gB;
+            gB.x = ens_capture(e, grp, [&](int d) {
                 rs_ctx* c = e->m[d];
                 for (int mstep = nmacro - 1; mstep >= 0; --mstep)
                     enqueue_backward_macro(c, mstep, *o, c->relpart.p + (size_t)mstep * 1024, nullptr, nullptr);
             }, &nn);
             auto tg1 = clk::now();
             ES.seconds_backward_setup += std::chrono::duration<double>(tg1 - tg0).count();
-            HIPCHK(hipGraphLaunch(gB, e->origin));
+            HIPCHK(hipGraphLaunch(gB.x, e->origin));
             HIPCHK(hipStreamSynchronize(e->origin));
             ES.seconds_backward_run += std::chrono::duration<double>(clk::now() - tg1).count();
-            (void)hipGraphExecDestroy(gB);
             ES.graph_launches_backward += 1;
             ES.graph_nodes_backward += (long long)nn;
             for (int d : grp) {
                 rs_ctx* c = e->m[d];
-                S[d].replay_maxrel = host_max_many(c, c->relpart.p, nmacro, rs_reduce_blocks((long long)c->N()));
+                S[d].replay_maxrel = host_max_many(c, c->relpart.p, nmacro,
+                                                   rs_reduce_blocks((long long)c->N()), &worst[d]);
                 for (int mstep = 0; mstep < nmacro; ++mstep) {
                     S[d].nvjp_T += (long long)c->seqT[mstep].size();
                     S[d].nvjp_C += (long long)c->seqC[mstep].size();
@@ -2227,13 +2405,35 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
         for (int d = 0; d < n; ++d) { e->m[d]->prof = profs[d]; e->m[d]->tape_view = nullptr; }
         return ens_fail(e, x.code, x.msg);
     }
+    int rc = RS_OK;
+    std::string msg;
     for (int d = 0; d < n; ++d) {
-        e->m[d]->prof = profs[d];
+        rs_ctx* c = e->m[d];
+        c->prof = profs[d];
         S[d].seconds_forward = ES.seconds_forward;
         S[d].seconds_backward = ES.seconds_backward;
+        bool bad = false;
+        try {
+            std::vector<unsigned long long> cnt(2 * (size_t)(nvjp[d] + 1));
+            HIPCHK(hipMemcpy(cnt.data(), c->nfc.p, cnt.size() * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost));
+            bad = first_nonfinite(c, cnt, S[d]);
+        } catch (const HipError& x) {
+            return ens_fail(e, RS_E_HIP, std::string(x.what) + ": " + hipGetErrorString(x.e));
+        }
         if (stats) stats[d] = S[d];
+        if (rc) continue;
+        const std::string rf = replay_failure(S[d].replay_maxrel, worst[d]);
+        if (!rf.empty()) {
+            rc = RS_E_STATE;
+            msg = "member " + std::to_string(d) + ": " + rf;
+        } else if (bad) {
+            rc = RS_E_NONFINITE;
+            msg = "member " + std::to_string(d) + ": " + nonfinite_message(S[d]);
+        }
     }
     if (es) *es = ES;
+    if (rc) return ens_fail(e, rc, msg);
     return RS_OK;
 }
 

@@ -67,6 +67,8 @@ typedef struct RsChemVjpArgs {
     const double* E;
     long long nc, ncol;
     double dt;
+    unsigned long long* nfc;    /* or 0: [0] += non-finite lam outputs, [1] += non-finite
+                                   emission-gradient contributions of this step */
 } RsChemVjpArgs;
 
 /* One transport stage over all transported species (see transport.hip). */

@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
+#include <cmath>
 
 #include "rs_launch.h"
 
@@ -618,11 +620,23 @@ __global__ void rs_fill_u64(unsigned long long* x, int n, unsigned long long v) 
     if (i < n) x[i] = v;
 }
 
-// lam *= 1[uout > 0]
+// lam .*= 1[uout > 0] (a multiplication: a non-finite entry stays non-finite,
+// adjoint_gradient.jl:736)
 __global__ void rs_mask(double* lam, const double* uout, long long n) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
-        if (!(uout[i] > 0.0)) lam[i] = 0.0;
+        lam[i] = lam[i] * ((uout[i] > 0.0) ? 1.0 : 0.0);
+}
+
+// *count += number of non-finite entries of x (a lane with none issues no
+// atomic)
+__global__ void __launch_bounds__(256) rs_nonfinite(const double* x, long long n,
+                                                    unsigned long long* count) {
+    unsigned k = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        k += !__builtin_isfinite(x[i]);
+    if (k) atomicAdd(count, (unsigned long long)k);
 }
 
 // x = max(x, 0)
@@ -639,7 +653,8 @@ __global__ void __launch_bounds__(256) rs_relmax(const double* a, const double* 
     double m = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
-        const double d = fabs(a[i] - b[i]) / dmax(fabs(b[i]), 1e-30);
+        double d = fabs(a[i] - b[i]) / dmax(fabs(b[i]), 1e-30);
+        if (!(d <= DBL_MAX)) d = HUGE_VAL;   // a NaN deviation must not vanish in the max
         m = dmax(m, d);
     }
     sh[threadIdx.x] = m;
@@ -678,6 +693,23 @@ __global__ void __launch_bounds__(256) rs_sumsq(const double* q, long long n, do
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// per-block sum of w * u (fixed order): the linear objective <w, u>
+__global__ void __launch_bounds__(256) rs_dot(const double* w, const double* u, long long n,
+                                              double* part) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        s += w[i] * u[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w2 = 128; w2 > 0; w2 >>= 1) {
+        if ((int)threadIdx.x < w2) sh[threadIdx.x] += sh[threadIdx.x + w2];
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
@@ -758,6 +790,16 @@ hipError_t rs_launch_clamp(double* x, long long n, hipStream_t st) {
     return hipGetLastError();
 }
 int rs_reduce_blocks(long long n) { return (int)util_grid(n) > 1024 ? 1024 : (int)util_grid(n); }
+hipError_t rs_launch_nonfinite(const double* x, long long n, unsigned long long* count,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(rs_nonfinite, dim3(rs_reduce_blocks(n)), dim3(256), 0, st, x, n, count);
+    return hipGetLastError();
+}
+hipError_t rs_launch_dot(const double* w, const double* u, long long n, double* part,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(rs_dot, dim3(rs_reduce_blocks(n)), dim3(256), 0, st, w, u, n, part);
+    return hipGetLastError();
+}
 hipError_t rs_launch_relmax(const double* a, const double* b, long long n, double* part,
                             hipStream_t st) {
     hipLaunchKernelGGL(rs_relmax, dim3(rs_reduce_blocks(n)), dim3(256), 0, st, a, b, n, part);

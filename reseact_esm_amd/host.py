@@ -67,7 +67,12 @@ class AdjointStats(C.Structure):
                 ("nreject_T", C.c_longlong), ("nreject_C", C.c_longlong),
                 ("nvjp_T", C.c_longlong), ("nvjp_C", C.c_longlong),
                 ("seconds_forward", C.c_double), ("seconds_backward", C.c_double),
-                ("seconds_replay", C.c_double), ("replay_maxrel", C.c_double)]
+                ("seconds_replay", C.c_double), ("replay_maxrel", C.c_double),
+                ("bad_op", C.c_int), ("bad_macro", C.c_int),
+                ("bad_step", C.c_longlong), ("bad_nsteps", C.c_longlong),
+                ("bad_t", C.c_double), ("bad_dt", C.c_double),
+                ("bad_lam_in", C.c_longlong), ("bad_lam_out", C.c_longlong),
+                ("bad_grad", C.c_longlong)]
 
 
 class MechInfo(C.Structure):
@@ -166,6 +171,7 @@ ABI = {
                               C.c_double, C.c_int, C.c_longlong, C.POINTER(Counts)],
     "rs_adaptive_chem": [_P, C.c_double, C.c_double, _D, C.POINTER(PICtrl), C.c_double,
                          C.c_double, C.c_int, C.c_longlong, C.POINTER(Counts)],
+    "rs_get_step_log": [_P, C.c_int, _D, C.c_longlong, C.POINTER(C.c_longlong)],
     "rs_chem_percell": [_P, C.c_double, C.c_double, C.c_double, _D, C.POINTER(PICtrl),
                         C.c_double, C.c_double, C.c_int, C.c_longlong, C.POINTER(PercellStats)],
     "rs_macro_step": [_P, C.c_double, C.c_double, _D, _D, C.POINTER(SplitOpts),
@@ -179,6 +185,7 @@ ABI = {
     "rs_adjoint_run": [_P, C.c_int, _D, C.c_double, C.c_double, C.POINTER(SplitOpts), C.c_int,
                        C.c_int, _D, _D, _D, C.POINTER(AdjointStats)],
     "rs_replay_loss": [_P, C.c_int, C.c_int, _D],
+    "rs_set_objective_weights": [_P, _D],
     "rs_set_lu_ordering": [_P, C.c_int],
     "rs_set_transport_kernel": [_P, C.c_int],
     "rs_mechanism_info": [_P, C.c_void_p],
@@ -434,6 +441,15 @@ class Context:
                                             rtol, _flags(clamp), maxiters, C.byref(c)))
         return d.value, c
 
+    def step_log(self, op: int) -> np.ndarray:
+        """Accepted (t, dt) pairs of the last adaptive transport (op 0) or
+        chemistry (op 1) solve, shape (n, 2)."""
+        n = C.c_longlong(0)
+        self._chk(self.lib.rs_get_step_log(self.h, op, None, 0, C.byref(n)))
+        out = np.empty((n.value, 2))
+        self._chk(self.lib.rs_get_step_log(self.h, op, _dp(out), n.value, C.byref(n)))
+        return out
+
     def chem_percell(self, t0, t1, dt0, dt_cells=None, ctrl=None, atol=1e-6, rtol=1e-3,
                      clamp=True, maxiters=20000):
         st = PercellStats()
@@ -474,21 +490,35 @@ class Context:
     def ros23_vjp(self, t, dt):
         self._chk(self.lib.rs_ros23_vjp(self.h, t, dt))
 
+    def set_objective_weights(self, w):
+        """Weights of the linear objective J = <w, u(t_end)> (loss kind 1);
+        None clears them."""
+        w = None if w is None else np.ascontiguousarray(w, np.float64)
+        self._chk(self.lib.rs_set_objective_weights(self.h, _dp(w)))
+
     def adjoint_run(self, tstops, dtT0, dtC0, opts: SplitOpts, loss_species=0,
-                    want_grads=True):
+                    want_grads=True, loss_kind=0):
+        """Forward + discrete adjoint.  A failure raises ReseactError; for
+        RS_E_NONFINITE / RS_E_STATE the error carries .stats (with the first
+        non-finite VJP in its bad_* fields) and .result = (J, gu, ge)."""
         ts = np.ascontiguousarray(tstops, np.float64)
         J = C.c_double()
         gu = np.empty(self.ns * self.nc) if want_grads else None
         ge = np.empty(self.ns * self.grid.ncol) if want_grads else None
         st = AdjointStats()
-        self._chk(self.lib.rs_adjoint_run(self.h, len(ts) - 1, _dp(ts), dtT0, dtC0,
-                                          C.byref(opts), 0, loss_species, C.byref(J), _dp(gu),
-                                          _dp(ge), C.byref(st)))
+        rc = self.lib.rs_adjoint_run(self.h, len(ts) - 1, _dp(ts), dtT0, dtC0, C.byref(opts),
+                                     loss_kind, loss_species, C.byref(J), _dp(gu), _dp(ge),
+                                     C.byref(st))
+        if rc != RS_OK:
+            err = ReseactError(rc, self.lib.rs_last_error(self.h).decode())
+            err.stats = st
+            err.result = (J.value, gu, ge)
+            raise err
         return J.value, gu, ge, st
 
-    def replay_loss(self, loss_species=0) -> float:
+    def replay_loss(self, loss_species=0, loss_kind=0) -> float:
         J = C.c_double()
-        self._chk(self.lib.rs_replay_loss(self.h, 0, loss_species, C.byref(J)))
+        self._chk(self.lib.rs_replay_loss(self.h, loss_kind, loss_species, C.byref(J)))
         return J.value
 
 
@@ -525,15 +555,19 @@ class Ensemble:
         return list(arr)
 
     def adjoint_run(self, tstops, dtT0, dtC0, opts: SplitOpts, loss_species=0,
-                    tape_budget_gib=64.0):
+                    tape_budget_gib=64.0, loss_kind=0):
         ts = np.ascontiguousarray(tstops, np.float64)
         n = len(self.members)
         J = np.empty(n)
         st = (AdjointStats * n)()
         es = EnsembleStats()
-        self._chk(self.lib.rs_ensemble_adjoint_run(self.h, len(ts) - 1, _dp(ts), dtT0, dtC0,
-                                                   C.byref(opts), 0, loss_species,
-                                                   tape_budget_gib, _dp(J), st, C.byref(es)))
+        rc = self.lib.rs_ensemble_adjoint_run(self.h, len(ts) - 1, _dp(ts), dtT0, dtC0,
+                                              C.byref(opts), loss_kind, loss_species,
+                                              tape_budget_gib, _dp(J), st, C.byref(es))
+        if rc != RS_OK:
+            err = ReseactError(rc, self.lib.rs_ensemble_last_error(self.h).decode())
+            err.stats = list(st)
+            raise err
         return J, list(st), es
 
 

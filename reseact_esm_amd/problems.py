@@ -256,6 +256,33 @@ class Problem:
         return self.t0 + self.dt_macro * np.arange(self.nmacro + 1, dtype=np.float64)
 
 
+def objective_weights(grid: Grid, ns: int, species: int, scope: str) -> tuple[np.ndarray, int]:
+    """Weights of the linear objective J = <w, u>: the mean of one species
+    over a scope, the reference's objective_weights(var_map, "<prefix>:<scope>")
+    (tools/adjoint_gradient.jl:408-426).  scope is "all", "surf" (the lowest
+    level, the reference's k == 1) or "levK" (level K counted from 1 at the
+    surface, as the reference's state names).  Returns (w, number of states
+    selected); w = 1/count on those, 0 elsewhere (species-major, NS*NC)."""
+    if not 0 <= species < ns:
+        raise ValueError(f"species {species} out of range")
+    if scope == "all":
+        levels = range(grid.nlev)
+    elif scope == "surf":
+        levels = [0]
+    elif scope.startswith("lev"):
+        kk = int(scope[3:])
+        if not 1 <= kk <= grid.nlev:
+            raise ValueError(f'objective "{scope}" selected no states')
+        levels = [kk - 1]
+    else:
+        raise ValueError(f'unknown objective scope "{scope}"')
+    w = np.zeros(ns * grid.nc)
+    plane = grid.nlon * grid.nlat
+    idx = np.concatenate([species * grid.nc + k * plane + np.arange(plane) for k in levels])
+    w[idx] = 1.0 / len(idx)
+    return w, len(idx)
+
+
 def config1(nlon: int = 72, nlat: int = 46, nlev: int = 10, seed: int = 1001,
             nmacro: int = 24) -> Problem:
     """Plain coupled forward: 72x46x10, 4 tracers, 10 species / 20 reactions,

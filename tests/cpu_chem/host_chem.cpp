@@ -44,3 +44,38 @@ extern "C" void hc_vjp(long long nc, long long ncol, const double* K, const doub
         for (int s = 0; s < RS_NS; ++s) lam[s * nc + c] = lc[s];
     }
 }
+
+// The device's storage layout on the CPU: the lanes of one wave share a slab
+// laid out [3][NS][64] (stride 64 between a lane's species, lane offset
+// within), as the LDS slab of the chemistry kernels.  Lanes run one after the
+// other on a slab whose other lanes' slots hold a sentinel, and after each lane
+// every slot that is not its own must still hold the sentinel: a slab index
+// that strays into a neighbour's slots is reported (returns the number of
+// clobbered slots; 0 when clean).  Cells are taken 64 at a time.
+extern "C" long long hc_attempt_wave(long long nc, long long ncol, const double* K,
+                                     const double* jfac, const double* E, const double* u,
+                                     double* unew, double dt, double atol, double rtol,
+                                     double* sse_out) {
+    static double slab[3 * RS_NS * 64];
+    const double sentinel = -1234.5678;
+    long long bad = 0;
+    double tot = 0.0;
+    for (long long c0 = 0; c0 < nc; c0 += 64) {
+        for (int lane = 0; lane < 64; ++lane) {
+            const long long c = c0 + lane;
+            if (c >= nc) break;
+            for (int x = 0; x < 3 * RS_NS * 64; ++x) slab[x] = sentinel;
+            double* sv = slab + lane;
+            for (int s = 0; s < RS_NS; ++s) sv[(2 * RS_NS + s) * 64] = u[s * nc + c];
+            const double jf = jfac ? jfac[c] : 1.0;
+            const double* Ec = (E && c < ncol) ? E + c : nullptr;
+            double W[RS_NNZ];
+            tot += rs_ros23_attempt<64, 1, false>(K, jf, Ec, ncol, sv, W, dt, atol, rtol,
+                                                  [&](int s, double un) { unew[s * nc + c] = un; });
+            for (int x = 0; x < 3 * RS_NS * 64; ++x)
+                if ((x & 63) != lane && slab[x] != sentinel) ++bad;
+        }
+    }
+    *sse_out = tot;
+    return bad;
+}

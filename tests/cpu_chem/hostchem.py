@@ -50,13 +50,13 @@ def generate(mech, ordering: int) -> tuple[str, str]:
 
 
 class HostChem:
-    def __init__(self, mech, ordering: int = 1, exact_div: bool = False):
+    def __init__(self, mech, ordering: int = 1, exact_div: bool = False, sanitize: bool = False):
         src, info = generate(mech, ordering)
         self.info = dict(zip(info.split()[0::2], map(int, info.split()[1::2])))
         deps = "".join(open(os.path.join(d_, f)).read() for d_, f in
                        ((HERE, "host_chem.cpp"), (CSRC, "chem_kernels.hip"),
                         (CSRC, "chem_prelude.h"), (CSRC, "rs_launch.h")))
-        key = hashlib.sha1((src + deps + str(exact_div)).encode()).hexdigest()[:16]
+        key = hashlib.sha1((src + deps + str(exact_div) + str(sanitize)).encode()).hexdigest()[:16]
         d = os.path.join(WORK, key)
         os.makedirs(d, exist_ok=True)
         so = os.path.join(d, "hostchem.so")
@@ -68,12 +68,18 @@ class HostChem:
                    "-o", so + ".tmp"]
             if exact_div:
                 cmd.insert(1, "-DRS_EXACT_DIV")
+            if sanitize:   # host code only (g++): address and undefined-behaviour checks
+                cmd[1:1] = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                            "-fno-omit-frame-pointer"]
             subprocess.run(cmd, check=True)
             os.replace(so + ".tmp", so)
         self.lib = C.CDLL(so)
         self.lib.hc_attempt.restype = C.c_double
         self.lib.hc_attempt.argtypes = [C.c_longlong, C.c_longlong, _D, _D, _D, _D, _D,
                                         C.c_double, C.c_double, C.c_double]
+        self.lib.hc_attempt_wave.restype = C.c_longlong
+        self.lib.hc_attempt_wave.argtypes = [C.c_longlong, C.c_longlong, _D, _D, _D, _D, _D,
+                                             C.c_double, C.c_double, C.c_double, _D]
         self.lib.hc_vjp.restype = None
         self.lib.hc_vjp.argtypes = [C.c_longlong, C.c_longlong, _D, _D, _D, _D, C.c_double, _D, _D]
         self.mech = mech
@@ -89,6 +95,18 @@ class HostChem:
                 for x in (self.mech.k, p.jfac, p.emis, u)]
         sse = self.lib.hc_attempt(nc, ncol, *(self._p(x) for x in keep), self._p(out), dt, atol, rtol)
         return out, float(np.sqrt(sse / (p.ns * nc)))
+
+    def attempt_wave(self, p, u, dt, atol, rtol):
+        """The attempt with the device's [3][NS][64] slab layout; returns
+        (unew, EEst, number of slab slots of other lanes that were written)."""
+        nc, ncol = p.grid.nc, p.grid.ncol
+        out = np.empty_like(u)
+        sse = C.c_double(0.0)
+        keep = [np.ascontiguousarray(x, np.float64) if x is not None else None
+                for x in (self.mech.k, p.jfac, p.emis, u)]
+        bad = self.lib.hc_attempt_wave(nc, ncol, *(self._p(x) for x in keep), self._p(out), dt,
+                                       atol, rtol, C.byref(sse))
+        return out, float(np.sqrt(sse.value / (p.ns * nc))), int(bad)
 
     def vjp(self, p, u, dt, lam):
         nc, ncol = p.grid.nc, p.grid.ncol

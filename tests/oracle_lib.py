@@ -20,6 +20,7 @@ EXTRA = {
     "rs_oracle_transport_rhs_jvp": [_P, _D, _D],
     "rs_oracle_chem_rhs": [_P, _D, _D],
     "rs_oracle_chem_jac": [_P, _D, C.c_size_t, _D],
+    "rs_oracle_attempt_log": [_P, _D, C.c_longlong, C.POINTER(C.c_longlong)],
     "rs_oracle_threads": [],
 }
 

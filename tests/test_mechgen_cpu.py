@@ -81,3 +81,42 @@ def test_sparse_plan_is_cheaper_than_dense():
     assert mk["nnz_LU"] < nat["nnz_LU"] <= p.ns * p.ns
     assert mk["flops_attempt"] < nat["flops_attempt"]
     assert mk["nnz_J"] == nat["nnz_J"]
+
+
+_SANITIZED_RUN = r"""
+import sys, numpy as np
+sys.path[:0] = sys.argv[1:3]
+from cpu_chem.hostchem import HostChem
+from oracle_lib import oracle_library
+from test_mechgen_cpu import CASES
+for name in ("c1", "c4"):
+    p = CASES[name]
+    ctx = oracle_library().context().load(p)
+    hc = HostChem(p.mech, ordering=0, exact_div=True, sanitize=True)
+    for dt in (5.0, 0.5, 5.0, 40.0):
+        eo = ctx.ros23_step(0.0, dt, 1e-6, 1e-3)
+        un, e, bad = hc.attempt_wave(p, p.u0, dt, 1e-6, 1e-3)
+        assert bad == 0, (name, dt, bad)
+        assert np.array_equal(un, ctx.get_candidate()), (name, dt)
+        assert abs(e - eo) <= 1e-15 * eo
+print("clean")
+"""
+
+
+def test_exact_build_wave_slab_clean_under_sanitizers():
+    """The exact build's attempt with the device's [3][NS][64] slab layout,
+    built for the CPU with AddressSanitizer and UndefinedBehaviorSanitizer,
+    launched repeatedly with changing steps: no out-of-bounds or undefined
+    operation, no write into another lane's slab slots, and every launch
+    bit-for-bit the oracle's natural-order elimination."""
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    asan = subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True,
+                          text=True, check=True).stdout.strip()
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    r = subprocess.run([sys.executable, "-c", _SANITIZED_RUN, here, os.path.dirname(here)],
+                       capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0 and "clean" in r.stdout, r.stdout[-2000:] + r.stderr[-4000:]
