@@ -253,6 +253,26 @@ RS_API int rs_ros23_step(rs_ctx* ctx, double t, double dt, double atol, double r
 RS_API int rs_get_candidate(rs_ctx* ctx, double* u);
 RS_API int rs_accept_candidate(rs_ctx* ctx, int clamp_nonneg);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: one ROS23 attempt, as rs_ros23_step, on the first ncells
+ * cells of the context's grid (with their photolysis factors and emissions),
+ * from the caller's DEVICE buffer u_dev into the caller's DEVICE buffer
+ * unew_dev (both NS x ncells doubles, species-major with stride ncells),
+ * *eest its error norm.  The context's state is not touched.  Needs the grid
+ * and the mechanism (set up beforehand); allocates nothing: the
+ * per-workgroup error partials live in a fixed device array of the library,
+ * RS_NOALLOC_PARTIALS doubles, one copy per GPU kept until the process ends,
+ * so ncells is limited to RS_NOALLOC_PARTIALS times the chemistry workgroup
+ * size (64 by default: 8,388,608 cells); larger is RS_E_ARG.  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (other threads,
+ * other contexts) race; each call synchronises its context's stream before
+ * it returns. */
+#define RS_NOALLOC_PARTIALS 131072
+RS_API int rs_ros23_step_noalloc(rs_ctx* ctx, const double* u_dev, double* unew_dev,
+                                 long long ncells, double dt, double atol, double rtol,
+                                 double* eest);
+
 /* ---- adaptive solves ------------------------------------------------------
  * The PI-controlled accept/reject loop over [t0, t1] with a single global
  * error norm.  *dt_io: initial dt in, proposed next dt out.

@@ -1869,6 +1869,35 @@ int rs_ros23_step(rs_ctx* c, double t, double dt, double atol, double rtol, doub
     return RS_OK;
 }
 
+// The no-allocation attempt of the HIP library on the CPU: the caller's
+// buffers are host memory here.  One attempt on the first ncells cells,
+// species-major with stride ncells; the context's state is not touched.
+int rs_ros23_step_noalloc(rs_ctx* c, const double* u, double* unew, long long ncells, double dt,
+                          double atol, double rtol, double* eest) {
+    if (!c) return RS_E_ARG;
+    if (c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!c->have_mech) return fail(c, RS_E_STATE, "mechanism not set");
+    if (!u || !unew) return fail(c, RS_E_ARG, "null buffer");
+    if (ncells < 1 || ncells > (long long)c->nc)
+        return fail(c, RS_E_ARG, "ncells must be in [1, NC] of the context's grid");
+    const int ns = c->ns;
+    std::vector<double> sq((size_t)ncells);
+#pragma omp parallel
+    {
+        Ros23Work w(ns);
+        std::vector<double> uc(ns), un(ns);
+#pragma omp for schedule(static)
+        for (long long cell = 0; cell < ncells; ++cell) {
+            CellChem cc(*c, (size_t)cell);
+            for (int s = 0; s < ns; ++s) uc[s] = u[(size_t)s * ncells + cell];
+            sq[cell] = ros23_cell(cc, uc.data(), un.data(), dt, atol, rtol, w);
+            for (int s = 0; s < ns; ++s) unew[(size_t)s * ncells + cell] = un[s];
+        }
+    }
+    if (eest) *eest = std::sqrt(fixed_sum(sq) / ((double)ns * (double)ncells));
+    return RS_OK;
+}
+
 int rs_get_candidate(rs_ctx* c, double* u) {
     if (!c || !c->have_cand) return fail(c, RS_E_STATE, "no candidate");
     std::copy(c->cand.begin(), c->cand.end(), u);

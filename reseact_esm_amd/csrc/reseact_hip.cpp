@@ -140,6 +140,34 @@ struct ChemModule {
     ~ChemModule() { if (mod) (void)hipModuleUnload(mod); }
 };
 
+// Device memory of rs_ros23_step_noalloc, for callers that cannot allocate:
+// the error partials of one attempt (one per chemistry workgroup) and the
+// fold of more than 1024 of them.  Every GPU the library runs on holds its
+// own copy until the process ends; host code never passes `scratch` itself
+// (a host-side stand-in), only the device address that hipGetSymbolAddress
+// returns for the current device, kept per device in g_scratch_ptr.
+#define RS_NOALLOC_FOLD 1024   /* RS_NOALLOC_PARTIALS: include/reseact_hip.h */
+#define RS_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[RS_NOALLOC_PARTIALS + RS_NOALLOC_FOLD];
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[RS_NOALLOC_MAX_DEVICES] = {};
+
+// Looks up this device's copy of `scratch` once (the device must be current).
+void scratch_setup(int device) {
+    if (device < 0 || device >= RS_NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+}
+
+double* scratch_ptr(int device) {
+    if (device < 0 || device >= RS_NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 // Compiled code objects, shared by every context of the process.
 std::mutex g_jit_mu;
 std::map<std::string, std::vector<char>> g_jit_cache;
@@ -1258,6 +1286,7 @@ int rs_create(rs_ctx** out, int device) {
         HIPCHK(hipHostMalloc((void**)&c->hstats, 8 * sizeof(long long), hipHostMallocDefault));
         HIPCHK(hipHostMalloc((void**)&c->hred, 1024 * sizeof(double), hipHostMallocDefault));
         std::memset(c->hctrl, 0, sizeof(RsCtrl));
+        scratch_setup(device);
         return RS_OK;
     });
     if (rc) {
@@ -1714,6 +1743,70 @@ int rs_ros23_step(rs_ctx* c, double t, double dt, double atol, double rtol, doub
         double sse = host_sum(c, c->partial.p, (int)chem_grid(c));
         if (eest) *eest = std::sqrt(sse / ((double)c->ns * (double)c->nc));
         c->have_cand = true;
+        return RS_OK;
+    });
+}
+
+int rs_ros23_step_noalloc(rs_ctx* c, const double* u_dev, double* unew_dev, long long ncells,
+                          double dt, double atol, double rtol, double* eest) {
+    if (!c) return RS_E_ARG;
+    if (c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!c->have_mech) return fail(c, RS_E_STATE, "mechanism not set");
+    if (!u_dev || !unew_dev) return fail(c, RS_E_ARG, "null device buffer");
+    if (ncells < 1 || ncells > c->nc)
+        return fail(c, RS_E_ARG, "ncells must be in [1, NC] of the context's grid");
+    const unsigned block = chem_threads(c);
+    const long long grid = (ncells + block - 1) / block;
+    if (grid > RS_NOALLOC_PARTIALS)
+        return fail(c, RS_E_ARG, "ncells " + std::to_string(ncells) + " needs " +
+                                     std::to_string(grid) + " error partials; the no-allocation "
+                                     "entry point holds at most " +
+                                     std::to_string(RS_NOALLOC_PARTIALS) + " (ncells <= " +
+                                     std::to_string((long long)RS_NOALLOC_PARTIALS * block) +
+                                     " at " + std::to_string(block) + " lanes per workgroup)");
+    double* ptr = scratch_ptr(c->device);
+    if (!ptr) return fail(c, RS_E_STATE, "no scratch on this device");
+    return guarded(c, [&]() {
+        RsChemArgs a{};
+        a.S0 = const_cast<double*>(u_dev);   // read only: cur_static = 0 reads S0, writes S1
+        a.S1 = unew_dev;
+        a.ctrl = nullptr;
+        a.K = c->K.p;
+        a.jfac = c->has_jfac ? c->jfac.p : nullptr;
+        a.E = c->has_E ? c->E.p : nullptr;
+        a.partial = ptr;
+        a.nc = ncells;
+        a.ncol = c->ncol;
+        a.dt_static = dt;
+        a.atol = atol;
+        a.rtol = rtol;
+        // argument block on the stack: the struct, then the rate constants when
+        // the kernel takes them inline (as chem_launch lays it out)
+        const bool kin = !c->chem_kp && c->nr > 0 && c->nr <= RS_K_INLINE_MAX;
+        alignas(8) char args[((sizeof(RsChemArgs) + 7) & ~(size_t)7) + RS_K_INLINE_MAX * sizeof(double)];
+        const size_t off = (sizeof(RsChemArgs) + 7) & ~(size_t)7;
+        size_t total = kin ? off + (size_t)c->nr * sizeof(double) : sizeof(RsChemArgs);
+        std::memcpy(args, &a, sizeof(a));
+        if (kin) std::memcpy(args + off, c->hK.data(), (size_t)c->nr * sizeof(double));
+        void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &total,
+                       HIP_LAUNCH_PARAM_END};
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipModuleLaunchKernel(kin ? c->chem->attempt : c->chem->attempt_kp, (unsigned)grid,
+                                     1, 1, block, 1, 1, (unsigned)chem_slab_bytes(c, block),
+                                     c->stream, nullptr, cfg));
+        // the sum of the partials in host_sum's order, folded inside the scratch
+        const double* part = ptr;
+        int n = (int)grid;
+        if (n > 1024) {
+            HIPCHK(rs_launch_sum(ptr, n, ptr + RS_NOALLOC_PARTIALS, c->stream));
+            part = ptr + RS_NOALLOC_PARTIALS;
+            n = rs_reduce_blocks(n);
+        }
+        HIPCHK(hipMemcpyAsync(c->hred, part, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double sse = 0.0;
+        for (int i = 0; i < n; ++i) sse += c->hred[i];
+        if (eest) *eest = std::sqrt(sse / ((double)c->ns * (double)ncells));
         return RS_OK;
     });
 }

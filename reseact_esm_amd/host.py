@@ -166,6 +166,8 @@ ABI = {
     "rs_ssprk43_step": [_P, C.c_double, C.c_double, C.c_double, C.c_double, _D],
     "rs_ros23_step": [_P, C.c_double, C.c_double, C.c_double, C.c_double, _D],
     "rs_get_candidate": [_P, _D],
+    "rs_ros23_step_noalloc": [_P, C.c_void_p, C.c_void_p, C.c_longlong, C.c_double, C.c_double,
+                              C.c_double, _D],
     "rs_accept_candidate": [_P, C.c_int],
     "rs_adaptive_transport": [_P, C.c_double, C.c_double, _D, C.POINTER(PICtrl), C.c_double,
                               C.c_double, C.c_int, C.c_longlong, C.POINTER(Counts)],
