@@ -14,8 +14,13 @@ Also reported on the same run:
     tracer-cell algorithmically (u in, unew out), the per-stage chain 72 B;
   * chemistry FP64 roofline: the 360x180x72, 50-species / 120-reaction
     per-cell adaptive chemistry over 1800 s in one persistent launch;
-  * cpu_baseline: the CPU oracle (C++/OpenMP, same algorithm) on a bounded
-    sub-grid of the coupled workload, on this host's cores.
+  * the plain coupled forward of 72x46x10 (4 tracers, 10 species, 24 split
+    steps of 600 s), whole run;
+  * cpu_baseline: the CPU oracle (C++/OpenMP, same algorithm) on 7 of the 91
+    latitude rows of the same 144x91x47 grid, one warm-up split step then one
+    timed, on this host's cores; cpu_baselines: the same for the other
+    workloads (72x46x10 whole run; transport, batched chemistry and ensemble
+    on labelled subsets).
 Kernel durations come from HIP events recorded by the library on its own
 stream around every launch (rs_profile).
 
@@ -109,9 +114,9 @@ def coupled(lib, steps: int, warmup: int):
     flops = info["flops_attempt"] * p.nc
     avg_ms = ms_att / max(n_att, 1)
     achieved = flops / (avg_ms * 1e-3) / 1e12 if n_att else 0.0
-    # compute roof: FP64 vector (VALU) issue; on MI355X the FP64 vector and
-    # FP64 matrix peaks are both 78.6 TFLOP/s
-    roof = {"bound": "mfma", "compute": "fp64 valu", "kernel": "chem_attempt",
+    # compute roof: FP64 vector (VALU) issue, 78.6 TFLOP/s on MI355X; no
+    # matrix instructions are used
+    roof = {"bound": "fp64-valu", "kernel": "chem_attempt",
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
@@ -246,8 +251,8 @@ def ppm_roofline(lib, nsteps: int = 10, nq: int = 20):
         f"{ms_l / max(nl_, 1):.3f} ms), {tf:.2f} TFLOP/s, {gbs:.0f} GB/s algorithmic, "
         f"wall {t1 - t0:.2f} s; step VJP {vjp_ms:.3f} ms ({ms_v / max(nv_, 1):.3f} ms per "
         f"ppm_vjp launch), VJP/attempt {vjp_ms / ms:.2f}")
-    return {"workload": p.name, "kernels": ["transport_stage", "transport_last"], "bound": "mfma",
-            "compute": "fp64 valu", "achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS,
+    return {"workload": p.name, "kernels": ["transport_stage", "transport_last"],
+            "bound": "fp64-valu", "achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 4),
             "flops_per_attempt": flops_attempt,
             "hbm": {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -278,7 +283,7 @@ def chemistry_roofline(lib):
         f"{st.attempts_max}) kernel {ms:.1f} ms wall {t1 - t0:.2f} s {achieved:.2f} TFLOP/s "
         f"regs {info['regs_percell']} scratch {info['scratch_percell']}")
     dropin = chemistry_dropin(lib)
-    return {"workload": p.name, "bound": "mfma", "compute": "fp64 valu", "kernel": "chem_percell",
+    return {"workload": p.name, "bound": "fp64-valu", "kernel": "chem_percell",
             "dropin_global_norm": dropin,
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -373,28 +378,165 @@ def ensemble_run(lib, ndom: int = 32, nmacro: int = 6, tape_budget_gib: float = 
     return out
 
 
-def cpu_baseline(sample_dims=(72, 46, 24), steps: int = 1):
-    sys.path.insert(0, os.path.join(REPO, "tests"))
-    from oracle_lib import oracle_library
-    lib = oracle_library()
-    p = P.config4(nlon=sample_dims[0], nlat=sample_dims[1], nlev=sample_dims[2], nmacro=steps)
+def config1_leg(lib, warmup: int = 1):
+    """The plain coupled forward of 72x46x10 (4 tracers, 10 species / 20
+    reactions, 24 split steps of 600 s), whole run, after `warmup` untimed
+    split steps; chemistry and transport kernel times from rs_profile."""
+    p = P.config1()
     opts = split_opts(p)
     ctx = lib.context().load(p)
-    threads = int(lib.lib.rs_oracle_threads())
+    ctx.autotune()
+    info = ctx.mechanism_info()
     dtT, dtC = p.dt0_T, p.dt0_C
-    t0 = time.perf_counter()
+    for m in range(warmup):
+        dtT, dtC, _, _ = ctx.macro_step(m * p.dt_macro, (m + 1) * p.dt_macro, dtT, dtC, opts)
+    ts = p.tstops()
+
+    def run():
+        ctx.set_state(p.u0)
+        dT, dC = p.dt0_T, p.dt0_C
+        nC = 0
+        t0 = time.perf_counter()
+        for m in range(p.nmacro):
+            dT, dC, cT, cC = ctx.macro_step(ts[m], ts[m + 1], dT, dC, opts)
+            nC += cC.naccept + cC.nreject
+        return time.perf_counter() - t0, nC
+    sec, nC = run()
+    ctx.profile(True)
+    run()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ctx.close()
+    _, ms_att = kernel_ms(prof, "chem_attempt")
+    avg = ms_att / max(nC, 1)
+    tf = info["flops_attempt"] * p.nc / (avg * 1e-3) / 1e12 if nC else 0.0
+    out = {"workload": p.name, "split_steps": p.nmacro,
+           "cell_steps_per_s": round(p.nc * p.nmacro / sec, 1),
+           "seconds": round(sec, 4), "ms_per_step": round(sec / p.nmacro * 1e3, 3),
+           "chem_attempt": {"bound": "fp64-valu", "avg_ms": round(avg, 5),
+                            "achieved": round(tf, 3), "unit": "TFLOP/s",
+                            "frac": round(tf / FP64_PEAK_TFLOPS, 4), "attempts": nC,
+                            "flops_per_attempt_cell": info["flops_attempt"]}}
+    log(f"config1 {p.name}: {out}")
+    return out
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from oracle_lib import oracle_library
+    return oracle_library()
+
+
+def host_info(lib) -> dict:
+    return {"cores": int(lib.lib.rs_oracle_threads()), "nproc": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": cpu_model()}
+
+
+def _macro_run(ctx, p, steps):
+    opts = split_opts(p)
+    dtT, dtC = p.dt0_T, p.dt0_C
     for m in range(steps):
         dtT, dtC, _, _ = ctx.macro_step(m * p.dt_macro, (m + 1) * p.dt_macro, dtT, dtC, opts)
-    t1 = time.perf_counter()
-    rate = p.nc * steps / (t1 - t0)
-    log(f"cpu oracle sample {sample_dims} x {steps} steps: {t1 - t0:.2f} s, {rate:.1f} cell-steps/s, "
-        f"{threads} threads")
-    return {"value": round(rate, 2), "unit": "coupled cell-steps/s", "cores": threads,
+
+
+def cpu_baseline(rows_step: int = 13):
+    """The oracle on latitude rows 0, 13, ..., 78 (7 of 91) of the same
+    144x91x47 coupled workload: one warm-up split step, then one timed."""
+    lib = _oracle()
+    p = P.latitude_rows(P.config4(nmacro=1), range(0, 91, rows_step))
+    ctx = lib.context().load(p)
+    _macro_run(ctx, p, 1)                                # warm-up
+    ctx.set_state(p.u0)
+    t0 = time.perf_counter()
+    _macro_run(ctx, p, 1)
+    sec = time.perf_counter() - t0
+    ctx.close()
+    rate = p.nc / sec
+    h = host_info(lib)
+    log(f"cpu oracle {p.name}: {sec:.2f} s, {rate:.1f} cell-steps/s, {h}")
+    return {"value": round(rate, 2), "unit": "coupled cell-steps/s", "cores": h["cores"],
             "kind": "port",
-            "sample": f"coupled split step of the same workload on a "
-                      f"{sample_dims[0]}x{sample_dims[1]}x{sample_dims[2]} sub-grid, {steps} step(s) "
-                      f"of 900 s, {t1 - t0:.1f} s of CPU time",
-            "cpu_model": cpu_model()}
+            "sample": f"latitude rows 0, {rows_step}, ..., {rows_step * (p.grid.nlat - 1)} "
+                      f"({p.grid.nlat} of 91) of the same 144x91x47 workload, all longitudes and "
+                      f"levels ({p.nc} cells): one warm-up split step of 900 s, then one timed, "
+                      f"{sec:.1f} s",
+            **{k: v for k, v in h.items() if k != "cores"}}
+
+
+def cpu_baselines() -> dict:
+    """The oracle beside every other GPU leg, on this host's cores, each after
+    one warm-up step: the 72x46x10 coupled run whole; the 360x180x72
+    transport, the 50-species batched chemistry and the ensemble on labelled
+    subsets."""
+    lib = _oracle()
+    h = host_info(lib)
+    out = {"host": h}
+    # C1: whole run
+    p = P.config1()
+    ctx = lib.context().load(p)
+    _macro_run(ctx, p, 1)
+    ctx.set_state(p.u0)
+    t0 = time.perf_counter()
+    _macro_run(ctx, p, p.nmacro)
+    sec = time.perf_counter() - t0
+    ctx.close()
+    out["config1"] = {"value": round(p.nc * p.nmacro / sec, 2), "unit": "coupled cell-steps/s",
+                      "cores": h["cores"], "kind": "port", "seconds": round(sec, 3),
+                      "sample": f"the whole run: {p.grid} x {p.nmacro} split steps, after one "
+                                f"warm-up step"}
+    # C2: every 10th latitude row of the 20-tracer transport, 2 fixed steps
+    p = P.latitude_rows(P.config2(), range(0, 180, 10))
+    dt = p.extra["dt_fixed"]
+    ctx = lib.context().load(p)
+    ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        ctx.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+        ctx.accept_candidate(False)
+    sec = time.perf_counter() - t0
+    ctx.close()
+    out["config2"] = {"value": round(p.ns * p.nc * 2 / sec, 1), "unit": "tracer-cell-steps/s",
+                      "cores": h["cores"], "kind": "port", "seconds": round(sec, 3),
+                      "sample": f"latitude rows 0, 10, ..., 170 (18 of 180) of 360x180x72, 20 "
+                                f"tracers, 2 steps at Courant 0.8 after one warm-up step"}
+    # C3: stratified cells of the batched chemistry, per cell over 1800 s
+    p3 = P.config3()
+    cells = P.stratified_cells(p3.grid)
+    q = P.cell_subset(p3, cells)
+    w = P.cell_subset(p3, cells[:512])
+    ctx = lib.context().load(w)
+    ctx.chem_percell(0.0, 1800.0, w.dt0_C, None, None, w.atol_C, w.rtol_C, True)   # warm-up
+    ctx.close()
+    ctx = lib.context().load(q)
+    t0 = time.perf_counter()
+    st = ctx.chem_percell(0.0, 1800.0, q.dt0_C, None, None, q.atol_C, q.rtol_C, True)
+    sec = time.perf_counter() - t0
+    ctx.close()
+    out["config3"] = {"value": round(cells.size / sec, 2), "unit": "cell-steps/s (1800 s interval)",
+                      "attempts_per_s": round(st.attempts_total / sec, 1),
+                      "cores": h["cores"], "kind": "port", "seconds": round(sec, 3),
+                      "sample": f"{cells.size} cells of 360x180x72: every 16th longitude, 6th "
+                                f"latitude and 6th level, per-cell ROS23 over 1800 s (after a "
+                                f"warm-up run on 512 of them)"}
+    # C5: every 6th latitude row of the smallest of the 32 domains, one split
+    # step forward
+    probs = P.config5(nmacro=1)
+    p0 = min(probs, key=lambda q_: q_.nc)
+    p = P.latitude_rows(p0, range(0, p0.grid.nlat, 6))
+    ctx = lib.context().load(p)
+    _macro_run(ctx, p, 1)
+    ctx.set_state(p.u0)
+    t0 = time.perf_counter()
+    _macro_run(ctx, p, 1)
+    sec = time.perf_counter() - t0
+    ctx.close()
+    out["ensemble"] = {"value": round(p.nc / sec, 2), "unit": "coupled cell-steps/s (forward)",
+                       "cores": h["cores"], "kind": "port", "seconds": round(sec, 3),
+                       "sample": f"every 6th latitude row ({p.grid.nlat} rows) of the smallest of "
+                                 f"the 32 domains ({p0.grid.nlon}x{p0.grid.nlat}x{p0.grid.nlev}), one "
+                                 f"split step of 900 s forward after one warm-up step"}
+    log(f"cpu baselines: {out}")
+    return out
 
 
 def cpu_model() -> str:
@@ -517,12 +659,15 @@ def main() -> int:
     c = coupled(lib, a.steps, a.warmup)
     extra = {}
     if not a.quick:
+        extra["config1"] = config1_leg(lib)
         extra["transport_hbm"] = transport_roofline(lib)
         extra["chemistry_fp64"] = chemistry_roofline(lib)
         extra["ppm_transport"] = ppm_roofline(lib)
     if not a.quick and not a.no_ensemble:
         extra["ensemble"] = ensemble_run(lib)
     cpu = None if a.no_cpu else cpu_baseline()
+    if not a.no_cpu and not a.quick:
+        extra["cpu_baselines"] = cpu_baselines()
     tr = None
     if not a.no_traffic:
         try:
@@ -530,6 +675,15 @@ def main() -> int:
             log(f"traffic: {tr}")
         except Exception as e:  # the counters are a report, not the measurement
             log(f"traffic counters unavailable: {type(e).__name__} {e}")
+    if "cpu_baselines" in extra:
+        # GPU over CPU on each leg, in the leg's own unit
+        cb = extra["cpu_baselines"]
+        for leg, key, cpu_key in (("config1", "cell_steps_per_s", "config1"),
+                                  ("transport_hbm", "tracer_cell_steps_per_s", "config2"),
+                                  ("chemistry_fp64", "cell_steps_per_s", "config3"),
+                                  ("ensemble", "forward_cell_steps_per_s", "ensemble")):
+            if leg in extra and cb.get(cpu_key, {}).get("value"):
+                extra[leg]["vs_cpu"] = round(extra[leg][key] / cb[cpu_key]["value"], 1)
     p = c["p"]
     if tr:
         c["roof"]["traffic"] = tr["chem_attempt"]["bytes"]

@@ -350,6 +350,13 @@ RS_API int rs_adjoint_run(rs_ctx* ctx, int nmacro, const double* tstops,
                    double* J_out, double* grad_u0, double* grad_emis,
                    rs_adjoint_stats* stats);
 
+/* Accepted steps of operator op (0 transport, 1 chemistry) in macro step
+ * `macro_step` (0-based) of the last rs_adjoint_run: *n_out pairs (t, dt)
+ * into t_dt when not NULL (cap pairs).  The recorded sequence the backward
+ * sweep replays (Ckpt seqT / seqC, tools/adjoint_gradient.jl:621-629). */
+RS_API int rs_get_recorded_steps(rs_ctx* ctx, int macro_step, int op, double* t_dt,
+                                 long long cap, long long* n_out);
+
 /* Weights w (NS*NC, species-major) of the linear objective J = <w, u(t_end)>
  * (loss kind RS_LOSS_WEIGHTS); NULL clears them.  Replaces the reference's
  * objective as a weight vector, objective_weights(var_map, spec),

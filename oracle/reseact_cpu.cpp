@@ -2001,9 +2001,12 @@ int rs_ros23_vjp(rs_ctx* c, double t, double dt) {
     return RS_OK;
 }
 
-int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, double dtC0,
-                   const rs_split_opts* o, int loss_kind, int loss_species, double* J_out,
-                   double* grad_u0, double* grad_emis, rs_adjoint_stats* st) {
+// The time-loop adjoint; with fixedT/fixedC the forward pass takes those
+// accepted sequences (one per macro step) instead of running the controllers.
+static int adjoint_impl(rs_ctx* c, int nmacro, const double* tstops, double dtT0, double dtC0,
+                        const rs_split_opts* o, int loss_kind, int loss_species, double* J_out,
+                        double* grad_u0, double* grad_emis, rs_adjoint_stats* st,
+                        const std::vector<Seq>* fixedT, const std::vector<Seq>* fixedC) {
     int rc = check_ready(c, true, true);
     if (rc) return rc;
     if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
@@ -2020,6 +2023,25 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
     double dtT = dtT0, dtC = dtC0;
     for (int m = 0; m < nmacro; ++m) {
         ck[m] = c->u;
+        if (fixedT) {
+            // fixed-sequence forward (replay_fixed, adjoint_gradient.jl:647-661)
+            std::vector<double> raw;
+            c->seqT[m] = (*fixedT)[m];
+            c->seqC[m] = (*fixedC)[m];
+            for (auto& pq : c->seqT[m].ts) {
+                ssprk43_attempt(*c, c->u, raw, pq.second, o->atol_T, o->rtol_T);
+                if (o->clamp_nonneg & 1) for (double& x : raw) x = std::max(x, 0.0);
+                c->u.swap(raw);
+            }
+            for (auto& pq : c->seqC[m].ts) {
+                ros23_attempt(*c, c->u, raw, pq.second, o->atol_C, o->rtol_C);
+                if (o->clamp_nonneg & 1) for (double& x : raw) x = std::max(x, 0.0);
+                c->u.swap(raw);
+            }
+            S.naccept_T += (long long)c->seqT[m].ts.size();
+            S.naccept_C += (long long)c->seqC[m].ts.size();
+            continue;
+        }
         rs_counts cT{}, cC{};
         rc = macro(c, tstops[m], tstops[m + 1], &dtT, &dtC, *o, &cT, &cC, &c->seqT[m], &c->seqC[m]);
         if (rc) return rc;
@@ -2131,6 +2153,43 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         return fail(c, RS_E_NONFINITE, b);
     }
     return RS_OK;
+}
+
+int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, double dtC0,
+                   const rs_split_opts* o, int loss_kind, int loss_species, double* J_out,
+                   double* grad_u0, double* grad_emis, rs_adjoint_stats* st) {
+    return adjoint_impl(c, nmacro, tstops, dtT0, dtC0, o, loss_kind, loss_species, J_out, grad_u0,
+                        grad_emis, st, nullptr, nullptr);
+}
+
+int rs_get_recorded_steps(rs_ctx* c, int macro_step, int op, double* t_dt, long long cap,
+                          long long* n_out) {
+    if (!c || !n_out || (op != 0 && op != 1)) return fail(c, RS_E_ARG, "bad arguments");
+    if (!c->have_record || macro_step < 0 || macro_step >= (int)c->seqT.size())
+        return fail(c, RS_E_STATE, "no recorded macro step " + std::to_string(macro_step));
+    const auto& ts = (op ? c->seqC : c->seqT)[macro_step].ts;
+    *n_out = (long long)ts.size();
+    if (t_dt) {
+        if (cap < (long long)ts.size()) return fail(c, RS_E_ARG, "buffer too small");
+        for (size_t i = 0; i < ts.size(); ++i) { t_dt[2 * i] = ts[i].first; t_dt[2 * i + 1] = ts[i].second; }
+    }
+    return RS_OK;
+}
+
+int rs_oracle_adjoint_on_sequence(rs_ctx* c, int nmacro, const double* tstops,
+                                  const long long* nT, const long long* nC, const double* t_dt_T,
+                                  const double* t_dt_C, const rs_split_opts* o, int loss_kind,
+                                  int loss_species, double* J_out, double* grad_u0,
+                                  double* grad_emis, rs_adjoint_stats* st) {
+    if (!c || !nT || !nC || !t_dt_T || !t_dt_C || nmacro < 1) return RS_E_ARG;
+    std::vector<Seq> fT(nmacro), fC(nmacro);
+    size_t oT = 0, oC = 0;
+    for (int m = 0; m < nmacro; ++m) {
+        for (long long q = 0; q < nT[m]; ++q, ++oT) fT[m].ts.emplace_back(t_dt_T[2 * oT], t_dt_T[2 * oT + 1]);
+        for (long long q = 0; q < nC[m]; ++q, ++oC) fC[m].ts.emplace_back(t_dt_C[2 * oC], t_dt_C[2 * oC + 1]);
+    }
+    return adjoint_impl(c, nmacro, tstops, 0.0, 0.0, o, loss_kind, loss_species, J_out, grad_u0,
+                        grad_emis, st, &fT, &fC);
 }
 
 int rs_set_objective_weights(rs_ctx* c, const double* w) {

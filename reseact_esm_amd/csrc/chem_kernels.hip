@@ -33,6 +33,21 @@
 #define RS_PHASE() __asm__ volatile("" ::: "memory")
 #endif
 
+/* Scheduling fences between the attempt's phases (no instruction), selected
+ * by the bits of RS_AFENCE_MASK: 1 after f0, 2 after the LU, 4 after k1, 8
+ * around f1, 16 after k2, 32 around f2, 64 after k3.  A fence keeps the
+ * compiler from interleaving the phases on either side.  Measured on the
+ * 20-species mechanism (one attempt over 616 k cells): no fence 0.107 ms
+ * (196 B/lane of scratch), all fences 0.117 ms (no scratch, but the phases
+ * no longer overlap), the fence after the LU alone 0.103 ms. */
+#ifndef RS_AFENCE_MASK
+#define RS_AFENCE_MASK 2
+#endif
+#ifdef RS_HOST_BUILD
+#define RS_AFENCE(bit)
+#else
+#define RS_AFENCE(bit) do { if (RS_AFENCE_MASK & (bit)) __builtin_amdgcn_sched_barrier(0); } while (0)
+#endif
 #define RS_SA(i) sv[(i) * ST]
 #define RS_SB(i) sv[(RS_NS + (i)) * ST]
 #define RS_SC(i) sv[(2 * RS_NS + (i)) * ST]
@@ -41,32 +56,44 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
                               const double* __restrict__ E, long long es,
                               double* __restrict__ sv, double* __restrict__ W, double dt,
                               double atol, double rtol, Out out) {
-    double f[RS_NS], x[RS_NS], y[RS_NS];
+    /* two short-lived register vectors; the solves work in place */
+    double f[RS_NS], y[RS_NS];
     rs_rhs<ST>(K, jf, E, es, &RS_SC(0), f);                        /* f0, :282 */
+    RS_AFENCE(1);
     const double hg = dt * RS_ROS23_D;                             /* :291 */
     rs_jac_W<ST, WS>(K, jf, &RS_SC(0), hg, W);                     /* :287-296 */
     rs_lu<WS>(W);
-    rs_solve<WS>(W, f, x);                                         /* k1, :298 */
+    RS_AFENCE(2);
+#pragma unroll
+    for (int i = 0; i < RS_NS; ++i) RS_SA(i) = f[i];               /* f0, until k1 exists */
+    rs_solve_ip<WS>(W, f);                                         /* k1, :298 */
+    RS_AFENCE(4);
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) {                              /* :323-324 */
-        const double c = x[i] - f[i];
+        const double c = f[i] - RS_SA(i);
         RS_SA(i) = 2.0 * c;
-        RS_SB(i) = x[i];
+        RS_SB(i) = f[i];
     }
     const double dto2 = dt / 2;
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * x[i]; y[i] = RS_SC(i) + du1; }
+    for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * f[i]; y[i] = RS_SC(i) + du1; }
+    RS_AFENCE(8);
     rs_rhs(K, jf, E, es, y, f);                                    /* f1, :303 */
+    RS_AFENCE(8);
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) y[i] = f[i] - x[i];            /* :306-309 */
-    rs_solve<WS>(W, y, x);                                         /* :310 */
+    for (int i = 0; i < RS_NS; ++i) y[i] = f[i] - RS_SB(i);        /* :306-309 */
+    rs_solve_ip<WS>(W, y);                                         /* :310 */
+    RS_AFENCE(16);
 #pragma unroll
-    for (int i = 0; i < RS_NS; ++i) x[i] = x[i] + RS_SB(i);        /* k2, :311-313 */
+    for (int i = 0; i < RS_NS; ++i) y[i] = y[i] + RS_SB(i);        /* k2, :311-313 */
+#ifdef RS_EXACT_DIV
+    double c3[RS_NS];
+#endif
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) {                              /* :321-322 */
-        const double a = x[i] - f[i];
+        const double a = y[i] - f[i];
 #ifdef RS_EXACT_DIV
-        y[i] = RS_ROS23_C32 * a;
+        c3[i] = RS_ROS23_C32 * a;
 #else
         const double ca = RS_ROS23_C32 * a;
         RS_SA(i) = ca + RS_SA(i);
@@ -79,38 +106,39 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) {
         const double u = RS_SC(i);
-        const double du2 = dt * x[i];
+        const double du2 = dt * y[i];
         const double un = u + du2;
-        f[i] = un;
         out(i, un);
         const double am = rs_max(RS_FABS(u), RS_FABS(un));        /* :335-346 */
         double d = am * rtol;
         d = d + atol;
         if (KEEP_U) RS_DEN(i) = d;
         else RS_SC(i) = d;
-        const double w2 = 2.0 * x[i];                              /* :333-334 */
+        const double w2 = 2.0 * y[i];                              /* :333-334 */
         RS_SB(i) = RS_SB(i) - w2;
+        y[i] = un;
     }
-#pragma unroll
-    for (int i = 0; i < RS_NS; ++i) x[i] = f[i];
-    rs_rhs(K, jf, E, es, x, f);                                    /* f2, :318 */
+    RS_AFENCE(32);
+    rs_rhs(K, jf, E, es, y, f);                                    /* f2, :318 */
+    RS_AFENCE(32);
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) {                              /* :325-326 */
 #ifdef RS_EXACT_DIV
-        const double d3 = f[i] - y[i];
+        const double d3 = f[i] - c3[i];
         f[i] = d3 - RS_SA(i);
 #else
         f[i] = f[i] - RS_SA(i);
 #endif
     }
-    rs_solve<WS>(W, f, x);                                         /* k3, :328 */
+    rs_solve_ip<WS>(W, f);                                         /* k3, :328 */
+    RS_AFENCE(64);
     const double dto6 = dt / 6;
     double sse = 0.0;
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) {
-        double ut = RS_SB(i) + x[i];
+        double ut = RS_SB(i) + f[i];
         ut = dto6 * ut;
-        const double at = ut / (KEEP_U ? RS_DEN(i) : RS_SC(i));
+        const double at = RS_DIV(ut, (KEEP_U ? RS_DEN(i) : RS_SC(i)));
         sse += at * at;
     }
     return sse;

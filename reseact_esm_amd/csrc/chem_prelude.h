@@ -25,9 +25,12 @@
 #ifdef RS_EXACT_DIV
 #define RS_PIVOT(w) (1.0 / (w))
 #define RS_DIAG_SOLVE(acc, w) ((acc) / (w))
+#define RS_DIV(a, b) ((a) / (b))
 #else
 #define RS_PIVOT(w) ((w) = rs_rcp(w))
 #define RS_DIAG_SOLVE(acc, w) ((acc) * (w))
+/* the error norm's scaled residuals: a times the refined reciprocal of b */
+#define RS_DIV(a, b) ((a) * rs_rcp(b))
 #endif
 /* 1/x from the hardware reciprocal estimate refined by two Newton steps
  * (~1 ulp; a zero pivot gives a NaN, which the controller rejects). */

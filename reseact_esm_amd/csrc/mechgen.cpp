@@ -294,6 +294,25 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
         e << " x[" << p.perm[q] << "] = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << " * WS]); }\n";
     }
     e << "}\n";
+    // ---- W x = b in place (b becomes x; the same operations as rs_solve)
+    e << "template <int WS = 1>\n";
+    e << "RS_FN void rs_solve_ip(const double* __restrict__ W, double* __restrict__ b) {\n";
+    for (int q = 0; q < n; ++q) {
+        bool any = false;
+        for (int k = 0; k < q; ++k) any = any || p.idx[q][k] >= 0;
+        if (!any) continue;
+        e << "  { double y = b[" << p.perm[q] << "];";
+        for (int k = 0; k < q; ++k)
+            if (p.idx[q][k] >= 0) e << " y = y - W[" << p.idx[q][k] << " * WS] * b[" << p.perm[k] << "];";
+        e << " b[" << p.perm[q] << "] = y; }\n";
+    }
+    for (int q = n - 1; q >= 0; --q) {
+        e << "  { double acc = b[" << p.perm[q] << "];";
+        for (int j = q + 1; j < n; ++j)
+            if (p.idx[q][j] >= 0) e << " acc = acc - W[" << p.idx[q][j] << " * WS] * b[" << p.perm[j] << "];";
+        e << " b[" << p.perm[q] << "] = RS_DIAG_SOLVE(acc, W[" << p.idx[q][q] << " * WS]); }\n";
+    }
+    e << "}\n";
     // ---- W^T x = b: U^T z = b (forward), then L^T x = z (backward)
     e << "template <int WS = 1>\n";
     e << "RS_FN void rs_solveT(const double* __restrict__ W,"

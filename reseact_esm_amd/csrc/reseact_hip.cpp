@@ -628,6 +628,7 @@ void chem_launch(rs_ctx* c, hipFunction_t f, const void* argp, size_t argsz, uns
 
 unsigned chem_grid(rs_ctx* c) { return (unsigned)((c->nc + chem_threads(c) - 1) / chem_threads(c)); }
 
+
 void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cur_static,
                    double dt, double atol, double rtol, int flags) {
     const int clamp = clamp_nonneg_of(flags);
@@ -2017,6 +2018,20 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         if (bad) return fail(c, RS_E_NONFINITE, nonfinite_message(S));
         return RS_OK;
     });
+}
+
+int rs_get_recorded_steps(rs_ctx* c, int macro_step, int op, double* t_dt, long long cap,
+                          long long* n_out) {
+    if (!c || !n_out || (op != 0 && op != 1)) return fail(c, RS_E_ARG, "bad arguments");
+    if (!c->have_record || macro_step < 0 || macro_step >= (int)c->seqT.size())
+        return fail(c, RS_E_STATE, "no recorded macro step " + std::to_string(macro_step));
+    const auto& ts = (op ? c->seqC : c->seqT)[macro_step];
+    *n_out = (long long)ts.size();
+    if (t_dt) {
+        if (cap < (long long)ts.size()) return fail(c, RS_E_ARG, "buffer too small");
+        for (size_t i = 0; i < ts.size(); ++i) { t_dt[2 * i] = ts[i].first; t_dt[2 * i + 1] = ts[i].second; }
+    }
+    return RS_OK;
 }
 
 int rs_set_objective_weights(rs_ctx* c, const double* w) {

@@ -188,6 +188,7 @@ ABI = {
                        C.c_int, _D, _D, _D, C.POINTER(AdjointStats)],
     "rs_replay_loss": [_P, C.c_int, C.c_int, _D],
     "rs_set_objective_weights": [_P, _D],
+    "rs_get_recorded_steps": [_P, C.c_int, C.c_int, _D, C.c_longlong, C.POINTER(C.c_longlong)],
     "rs_set_lu_ordering": [_P, C.c_int],
     "rs_set_transport_kernel": [_P, C.c_int],
     "rs_mechanism_info": [_P, C.c_void_p],
@@ -517,6 +518,16 @@ class Context:
             err.result = (J.value, gu, ge)
             raise err
         return J.value, gu, ge, st
+
+    def recorded_steps(self, macro_step: int, op: int) -> np.ndarray:
+        """Accepted (t, dt) of operator op (0 transport, 1 chemistry) in one
+        macro step of the last adjoint run, shape (n, 2)."""
+        n = C.c_longlong(0)
+        self._chk(self.lib.rs_get_recorded_steps(self.h, macro_step, op, None, 0, C.byref(n)))
+        out = np.empty((n.value, 2))
+        self._chk(self.lib.rs_get_recorded_steps(self.h, macro_step, op, _dp(out), n.value,
+                                                 C.byref(n)))
+        return out
 
     def replay_loss(self, loss_species=0, loss_kind=0) -> float:
         J = C.c_double()

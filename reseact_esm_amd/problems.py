@@ -283,6 +283,54 @@ def objective_weights(grid: Grid, ns: int, species: int, scope: str) -> tuple[np
     return w, len(idx)
 
 
+def latitude_rows(p: Problem, rows) -> Problem:
+    """The workload restricted to a set of latitude rows of its own grid, for
+    a bounded CPU sample of the same cells: the same longitudes, levels,
+    latitudes' zonal rates, state, emissions and photolysis factors on those
+    rows.  With the synthetic wind (v = 0) latitude rows do not interact in
+    the transport; the chemistry is per cell.  Only the global error norm sees
+    fewer cells.  Upwind-transport workloads only."""
+    if p.ppm is not None:
+        raise ValueError("latitude_rows: upwind-transport workloads only")
+    rows = np.asarray(rows, dtype=np.int64)
+    g = p.grid
+    sub = Grid(g.nlon, rows.size, g.nlev)
+    u0 = p.u0.reshape(p.ns, g.nlev, g.nlat, g.nlon)[:, :, rows, :].reshape(-1).copy()
+    emis = None if p.emis is None else \
+        p.emis.reshape(p.ns, g.nlat, g.nlon)[:, rows, :].reshape(-1).copy()
+    jfac = None if p.jfac is None else \
+        p.jfac.reshape(g.nlev, g.nlat, g.nlon)[:, rows, :].reshape(-1).copy()
+    wind = Wind(cx=np.ascontiguousarray(p.wind.cx[rows]), wz=p.wind.wz, wl=p.wind.wl,
+                w0=p.wind.w0)
+    return dataclasses.replace(p, name=f"{p.name}-rows{rows.size}", grid=sub, u0=u0, emis=emis,
+                               jfac=jfac, wind=wind, extra=dict(p.extra, rows=rows.tolist()))
+
+
+def cell_subset(p: Problem, cells) -> Problem:
+    """Chemistry-only workload on a set of cells of p (cells are independent
+    in per-cell chemistry): their states and photolysis factors as a grid of
+    len(cells) x 1 x 1.  Workloads without emissions only (emissions sit on
+    level 0 of the grid)."""
+    if p.emis is not None:
+        raise ValueError("cell_subset: workloads without emissions only")
+    cells = np.asarray(cells, dtype=np.int64)
+    sub = Grid(cells.size, 1, 1)
+    u0 = p.u0.reshape(p.ns, p.nc)[:, cells].reshape(-1).copy()
+    jfac = None if p.jfac is None else p.jfac[cells].copy()
+    return dataclasses.replace(p, name=f"{p.name}-cells{cells.size}", grid=sub, nt=0, u0=u0,
+                               jfac=jfac, wind=solid_body_wind(sub, 1.0, 0.0))
+
+
+def stratified_cells(g: Grid, di: int = 16, dj: int = 6, dk: int = 6) -> np.ndarray:
+    """Every di-th longitude, dj-th latitude and dk-th level (both hemispheres,
+    day and night columns), as cell indices c = i + NLON (j + NLAT k)."""
+    i = np.arange(0, g.nlon, di)
+    j = np.arange(0, g.nlat, dj)
+    k = np.arange(0, g.nlev, dk)
+    K, J, I = np.meshgrid(k, j, i, indexing="ij")
+    return (I + g.nlon * (J + g.nlat * K)).reshape(-1)
+
+
 def config1(nlon: int = 72, nlat: int = 46, nlev: int = 10, seed: int = 1001,
             nmacro: int = 24) -> Problem:
     """Plain coupled forward: 72x46x10, 4 tracers, 10 species / 20 reactions,

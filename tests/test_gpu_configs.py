@@ -1,0 +1,195 @@
+"""Every workload at its stated size or length, on the GPU, against the
+oracle (run live, on the GPU's own accepted sequence where the global error
+norm's summation order can part the two), and through size-independent
+properties at full size.
+
+* coupled forward 72x46x10, 4 tracers, 10 species, 24 split steps of 600 s:
+  the same accepted/rejected counts every split step, states to 1e-8 after
+  the first, and at the end within the oracle's own round-off sensitivity;
+* transport 360x180x72, 20 tracers, 60 steps at Courant 0.8: tracer mass
+  conserved to 1e-13 (relative, each tracer);
+* batched chemistry 360x180x72 (4.67 M cells, 50 species), adaptive per cell
+  over 1800 s in one persistent launch: a stratified subset of its cells
+  (every 16th longitude, every 6th latitude and level) solved by the oracle
+  matches, states to 1e-8 and each cell's final step size to 1e-8;
+* coupled adjoint 36x24x12 over all 12 split steps: loss and full gradients
+  against the oracle's adjoint on the GPU's accepted sequence;
+* ensemble: two regional domains inside 64x64x20 .. 256x192x47 run as HIP
+  graphs, bit for bit the domains run alone, and against the oracle's adjoint
+  on the GPU's accepted sequence.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from reseact_esm_amd import problems as P
+from reseact_esm_amd.host import Ensemble, pictrl_ros23, split_opts
+
+pytestmark = pytest.mark.gpu
+
+def relchem(a, b, ncell, atol=1e-6):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    cmax = np.max(np.abs(b.reshape(-1, ncell)), axis=0)
+    floor = np.maximum(atol, 1e-4 * np.tile(cmax, b.size // ncell))
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+def _c1_run(lib, p, steps=None):
+    c = lib.context().load(p)
+    opts = split_opts(p)
+    ts = p.tstops()
+    d = [p.dt0_T, p.dt0_C]
+    counts, first = [], None
+    for m in range(steps or p.nmacro):
+        r = c.macro_step(ts[m], ts[m + 1], d[0], d[1], opts)
+        d = [r[0], r[1]]
+        counts.append((r[2].naccept, r[2].nreject, r[3].naccept, r[3].nreject))
+        if m == 0:
+            first = c.get_state()
+    u = c.get_state()
+    c.close()
+    return counts, first, u
+
+
+def test_config1_full_run(hip, oracle):
+    """All 24 split steps: the same accepted/rejected counts in every split
+    step, states to 1e-8 after the first.  Over the whole run the trajectory
+    amplifies round-off: the oracle itself, started from u0 perturbed by
+    1e-13 (relative), ends ~1e-3 away (at the scale max(atol, 1e-4 of the
+    cell's largest species), on species near the solver's absolute
+    tolerance) with identical step counts.  The GPU's end state must lie
+    within 10x the oracle's own sensitivity, and within 1e-6 of the largest
+    entry normwise."""
+    p = P.config1()
+    print(f"C1 {p.grid} x {p.nmacro} split steps", flush=True)
+    cg, ug1, ug = _c1_run(hip, p)
+    co, uo1, uo = _c1_run(oracle, p)
+    assert cg == co, [(m, a, b) for m, (a, b) in enumerate(zip(cg, co)) if a != b][:3]
+    r1 = relchem(ug1, uo1, p.nc)
+    q = P.config1()
+    q.u0 = q.u0 * (1.0 + 1e-13 * np.random.default_rng(5).standard_normal(q.u0.size))
+    cq, _, uq = _c1_run(oracle, q)
+    sens = relchem(uq, uo, p.nc)
+    r = relchem(ug, uo, p.nc)
+    rn = float(np.max(np.abs(ug - uo)) / np.max(np.abs(uo)))
+    print(f"C1: after step 1 max rel {r1:.2e}; after {p.nmacro}: GPU {r:.2e}, oracle's own "
+          f"sensitivity to 1e-13 {sens:.2e} (counts equal: {cq == co}), normwise {rn:.2e}",
+          flush=True)
+    assert r1 <= 1e-8
+    assert r <= max(10.0 * sens, 1e-8)
+    assert rn <= 1e-6
+
+
+def test_config2_full_mass(hip):
+    p = P.config2()
+    dt = p.extra["dt_fixed"]
+    w = p.grid.cell_volume_weights().reshape(-1)
+    m0 = p.u0.reshape(p.ns, p.nc) @ w
+    with hip.context() as c:
+        c.load(p)
+        for s in range(p.nmacro):
+            c.ssprk43_step(s * dt, dt, 1e-6, 1e-3)
+            c.accept_candidate(False)
+        u = c.get_state()
+    m1 = u.reshape(p.ns, p.nc) @ w
+    rel = np.abs(m1 - m0) / m0
+    print(f"C2 {p.grid} x {p.ns} tracers x {p.nmacro} steps: mass drift max {rel.max():.2e}",
+          flush=True)
+    assert np.all(rel <= 1e-13)
+
+
+def test_config3_full_percell_against_oracle_subset(hip, oracle):
+    p = P.config3()
+    cells = P.stratified_cells(p.grid)
+    print(f"C3 {p.grid}: {p.nc} cells on the GPU, {cells.size} on the oracle", flush=True)
+    dt_g = np.full(p.nc, 0.5)
+    with hip.context() as c:
+        c.load(p)
+        st = c.chem_percell(0.0, 1800.0, 0.5, dt_g, pictrl_ros23(), 1e-6, 1e-3, True)
+        ug = c.get_state().reshape(p.ns, p.nc)[:, cells]
+    print(f"C3 GPU: {st.attempts_total} attempts, per cell {st.attempts_min}..{st.attempts_max}",
+          flush=True)
+    assert st.n_failed == 0 and st.attempts_max > st.attempts_min
+    q = P.cell_subset(p, cells)
+    dt_o = np.full(cells.size, 0.5)
+    with oracle.context() as c:
+        c.load(q)
+        c.chem_percell(0.0, 1800.0, 0.5, dt_o, pictrl_ros23(), 1e-6, 1e-3, True)
+        uo = c.get_state()
+    r = relchem(ug.reshape(-1), uo, cells.size)
+    rdt = float(np.max(np.abs(dt_g[cells] - dt_o) / dt_o))
+    print(f"C3 subset: state max rel {r:.2e}, final dt max rel {rdt:.2e}", flush=True)
+    assert r <= 1e-8
+    assert rdt <= 1e-8
+
+
+def test_config4_mid_grid_12_steps_against_oracle(hip, oracle):
+    """The coupled adjoint workload on a 36x24x12 grid over all 12 split
+    steps.  The global error norms are summed in another order on the GPU, so
+    a controller decision at the edge of a threshold may go the other way and
+    the accepted sequences part (the oracle itself takes another sequence
+    from u0 perturbed by 1e-13).  The oracle therefore runs its adjoint on
+    the GPU's recorded sequence (the fixed-sequence forward of replay_fixed,
+    adjoint_gradient.jl:647-661): the same discrete map, so loss and full
+    gradients must agree (1e-10 / 1e-6)."""
+    from oracle_lib import adjoint_on_sequence
+    p = P.config4(nlon=36, nlat=24, nlev=12)
+    assert p.nmacro == 12
+    opts = split_opts(p)
+    with hip.context() as c:
+        c.load(p)
+        J, gu, ge, st = c.adjoint_run(p.tstops(), p.dt0_T, p.dt0_C, opts)
+        seqT = [c.recorded_steps(m, 0) for m in range(p.nmacro)]
+        seqC = [c.recorded_steps(m, 1) for m in range(p.nmacro)]
+    print(f"C4 {p.grid} x {p.nmacro}: GPU J {J:.12e}, accepted T {st.naccept_T} C {st.naccept_C}",
+          flush=True)
+    with oracle.context() as c:
+        c.load(p)
+        Jo, guo, geo, so = adjoint_on_sequence(c, p, opts, seqT, seqC)
+    ru = float(np.linalg.norm(gu - guo) / np.linalg.norm(guo))
+    re_ = float(np.linalg.norm(ge - geo) / np.linalg.norm(geo))
+    print(f"C4 oracle on the GPU's sequence: J rel {abs(J - Jo) / Jo:.2e}, grad u0 {ru:.2e}, "
+          f"grad E {re_:.2e}", flush=True)
+    assert (so.naccept_T, so.naccept_C) == (st.naccept_T, st.naccept_C)
+    assert abs(J - Jo) <= 1e-10 * abs(Jo)
+    assert ru <= 1e-6 and re_ <= 1e-6
+
+
+def test_config5_two_domains_against_oracle(hip, oracle):
+    """Two regional domains at sizes inside the ensemble's range (64x64x20 and
+    70x66x21), one split step forward plus adjoint, run together as HIP
+    graphs: each member's loss and gradients are bit-for-bit those of the
+    same domain run alone through rs_adjoint_run, and the first domain's
+    match the oracle's adjoint on the GPU's accepted sequence."""
+    from oracle_lib import adjoint_on_sequence
+    ps = P.config5(sizes=[(64, 64, 20), (70, 66, 21)], nmacro=1)
+    opts = split_opts(ps[0])
+    ctxs = [hip.context().load(q) for q in ps]
+    ens = Ensemble(ctxs)
+    J, stats, es = ens.adjoint_run(ps[0].tstops(), ps[0].dt0_T, ps[0].dt0_C, opts)
+    print(f"C5 two domains as graphs: {es.graph_launches_forward} forward launches", flush=True)
+    got = [(c.get_lambda(), c.get_grad_emis()) for c in ctxs]
+    ens.close()
+    for c in ctxs:
+        c.close()
+    for d, q in enumerate(ps):
+        with hip.context() as c:
+            c.load(q)
+            Jd, gud, ged, sd = c.adjoint_run(q.tstops(), q.dt0_T, q.dt0_C, opts)
+            if d == 0:
+                seqT = [c.recorded_steps(m, 0) for m in range(q.nmacro)]
+                seqC = [c.recorded_steps(m, 1) for m in range(q.nmacro)]
+        assert Jd == J[d] and np.array_equal(gud, got[d][0]) and np.array_equal(ged, got[d][1])
+        assert (sd.naccept_T, sd.naccept_C) == (stats[d].naccept_T, stats[d].naccept_C)
+    q = ps[0]
+    with oracle.context() as c:
+        c.load(q)
+        Jo, guo, geo, so = adjoint_on_sequence(c, q, opts, seqT, seqC)
+    ru = float(np.linalg.norm(got[0][0] - guo) / np.linalg.norm(guo))
+    re_ = float(np.linalg.norm(got[0][1] - geo) / np.linalg.norm(geo))
+    print(f"C5 domain 0 {q.grid}: oracle on the GPU's sequence: J rel {abs(J[0] - Jo) / Jo:.2e}, "
+          f"grad u0 {ru:.2e}, grad E {re_:.2e}", flush=True)
+    assert abs(J[0] - Jo) <= 1e-10 * abs(Jo)
+    assert ru <= 1e-6 and re_ <= 1e-6

@@ -10,7 +10,11 @@ oracle would take minutes.
   in E, seeded 1004), best over a step sweep, to 1e-6 relative.
 * The structural identity s dJ/ds = <E, dJ/dE> for emissions scaled by s
   (:962-970).
-Three split steps of 900 s keep each frozen-sequence evaluation near 50 ms.
+Three split steps of 900 s, with the clamp max(u, 0) on.  The clamp makes the
+step map piecewise smooth: over more steps some directions move entries across
+a kink within the difference step, where a central difference is not the
+derivative the adjoint takes (the reference lists the same confounder of its
+fd check, adjoint_gradient.jl:1168-1181).
 """
 from __future__ import annotations
 

@@ -172,3 +172,20 @@ def test_nonfinite_lambda_is_reported_at_its_first_vjp(oracle):
     # t and dt are that step's: it ends the run
     assert abs(s.bad_t + s.bad_dt - p.tstops()[-1]) <= 1e-9 * p.tstops()[-1]
 
+
+
+def test_adjoint_on_a_given_sequence_is_the_adjoint(oracle):
+    """The oracle's adjoint with the forward forced onto a recorded accepted
+    sequence equals the free-running adjoint that recorded it (bit for bit):
+    the check GPU parity tests use when the GPU's global error norm, summed in
+    another order, takes another sequence."""
+    from oracle_lib import adjoint_on_sequence
+    p = small_c4()
+    ctx, J, gu, ge, st = run(oracle, p)
+    seqT = [ctx.recorded_steps(m, 0) for m in range(p.nmacro)]
+    seqC = [ctx.recorded_steps(m, 1) for m in range(p.nmacro)]
+    assert sum(len(s) for s in seqC) == st.naccept_C
+    ctx.set_state(p.u0)
+    J2, gu2, ge2, st2 = adjoint_on_sequence(ctx, p, split_opts(p), seqT, seqC)
+    assert J2 == J and np.array_equal(gu2, gu) and np.array_equal(ge2, ge)
+    assert st2.replay_maxrel == 0.0 and st2.nreject_C == 0
