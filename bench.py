@@ -585,6 +585,39 @@ def probe(lib) -> dict:
             "chemistry": {"workload": p4.name, "cells": p4.nc}}
 
 
+def probe_chem(lib) -> dict:
+    """Short run for the FP64 counter passes of the chemistry kernels: two
+    ROS23 attempts and two step VJPs over the 144x91x47 coupled grid, then the
+    per-cell work-stealing solve of the 50-species mechanism over 1800 s on an
+    8-longitude band of the 360x180x72 grid (every longitude index i with
+    i % 45 == 0: day and night columns)."""
+    p4 = P.config4(nmacro=1)
+    ctx = lib.context().load(p4)
+    for _ in range(2):
+        ctx.ros23_step(0.0, 1.0, p4.atol_C, p4.rtol_C)
+    ctx.set_lambda(np.ones(p4.ns * p4.nc))
+    ctx.zero_grad()
+    for _ in range(2):
+        ctx.ros23_vjp(0.0, 1.0)
+    info4 = ctx.mechanism_info()
+    ctx.close()
+    p3 = P.config3()
+    band = np.arange(0, p3.grid.nlon, 45)
+    cells = P.stratified_cells(p3.grid, di=45, dj=1, dk=1)
+    q = P.cell_subset(p3, cells)
+    ctx = lib.context().load(q)
+    st = ctx.chem_percell(0.0, 1800.0, q.dt0_C, None, None, q.atol_C, q.rtol_C, True)
+    info3 = ctx.mechanism_info()
+    ctx.close()
+    return {"chem_attempt": {"workload": p4.name, "cells": p4.nc, "launches": 2,
+                             "flops_attempt_cell": info4["flops_attempt"]},
+            "chem_vjp": {"workload": p4.name, "cells": p4.nc, "launches": 2,
+                         "flops_vjp_cell": info4["flops_vjp"]},
+            "chem_percell": {"workload": p3.name, "cells": int(cells.size),
+                             "longitudes": band.tolist(), "attempts": st.attempts_total,
+                             "flops_attempt_cell": info3["flops_attempt"]}}
+
+
 def _pmc_pass(counter: str, outdir: str, timeout_s: int = 150) -> dict:
     """One rocprofv3 counter pass over `bench.py --probe` in a child process;
     returns {kernel name: [per-dispatch KiB]}."""
@@ -640,6 +673,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 32-domain ensemble")
     ap.add_argument("--probe", action="store_true",
                     help="only the short hardware-counter run (copy + one transport and chemistry)")
+    ap.add_argument("--probe-chem", action="store_true",
+                    help="only the short chemistry run for the FP64 counter passes")
     ap.add_argument("--ppm-only", action="store_true",
                     help="only the PPM transport leg (for profiling it)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -652,6 +687,9 @@ def main() -> int:
     lib = hip_library()
     if a.probe:
         print(json.dumps({"probe": probe(lib)}), flush=True)
+        return 0
+    if a.probe_chem:
+        print(json.dumps({"probe_chem": probe_chem(lib)}), flush=True)
         return 0
     if a.ppm_only:
         print(json.dumps({"ppm_transport": ppm_roofline(lib, nsteps=a.steps)}), flush=True)
