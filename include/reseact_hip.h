@@ -301,6 +301,21 @@ RS_API int rs_chem_percell(rs_ctx* ctx, double t0, double t1, double dt0, double
                     const rs_pictrl* ctrl, double atol, double rtol,
                     int clamp_nonneg, long long maxiters, rs_percell_stats* out);
 
+/* VJP of the per-cell adaptive chemistry over [t0, t1] from the current state
+ * (unchanged), every cell starting at dt0: the reference's backward sweep
+ * (tools/adjoint_gradient.jl:731-834) over each cell's own accepted step
+ * sequence (adaptive_solve with NC = 1, rx_traced_integrator.jl:455-507).
+ * Each cell is integrated again with its accepted steps recorded, then swept
+ * last step first: lambda .*= 1[u_{j+1} > 0] when clamping (the clamp of the
+ * accepted state, adjoint_gradient.jl:736), lambda <- VJP of ros23_step j at
+ * u_j.  The lambda of rs_set_lambda is dJ/du(t1) on entry and dJ/du(t0) on
+ * return; rs_get_grad_emis accumulates dJ/dE (rs_zero_grad clears it).
+ * Device memory: (NS + 1) doubles per resident lane and recorded state, the
+ * longest cell's accepted steps (found by a forward pass on scratch). */
+RS_API int rs_chem_percell_vjp(rs_ctx* ctx, double t0, double t1, double dt0,
+                               const rs_pictrl* ctrl, double atol, double rtol,
+                               int clamp_nonneg, long long maxiters, rs_percell_stats* out);
+
 /* One Lie-Trotter macro step: transport over [t0,t1] then chemistry over the
  * same interval from the transported state.
  * Reference: macro_step tools/adjoint_gradient.jl:609-619;
@@ -340,6 +355,13 @@ RS_API int rs_ros23_vjp(rs_ctx* ctx, double t, double dt);
  * non-finite) from the next checkpoint (adjoint_gradient.jl:802-808);
  * RS_E_NONFINITE when a VJP made lambda or its gradient non-finite, with the
  * first such VJP in stats->bad_* and in rs_last_error.
+ * With opts->chem_percell = 1 the chemistry of every macro step is the
+ * per-cell solve of rs_chem_percell (each cell from dtC0); its backward is
+ * that of rs_chem_percell_vjp, which replays each cell from the transported
+ * state (the replay check compares the cells' end states), and it is
+ * reported as one chemistry step spanning the macro step (bad_step 0 of 1,
+ * t = the macro start, dt = its length); stats->nvjp_C counts the per-cell
+ * step VJPs; rs_get_recorded_steps(op 1) returns no steps for it.
  * Reference: forward_pass / replay_fixed / backward_stage! / backward_sweep,
  * tools/adjoint_gradient.jl:647-834.                                        */
 #define RS_LOSS_SURFACE_SQ 0

@@ -99,6 +99,8 @@ struct rs_ctx {
     Seq lastseq[2];
     rs_split_opts last_opts{};
     bool have_record = false;
+    std::vector<double> rec_tstops;   // macro stops of the last adjoint run
+    double rec_dtC0 = 0.0;            // its per-cell chemistry start dt
     std::string err;
 };
 
@@ -1410,6 +1412,36 @@ int adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs_pic
     return RS_OK;
 }
 
+// One cell's adaptive ROS23 solve over [t0, t1] from uc (in place): the
+// reference loop with NC = 1 (rx_traced_integrator.jl:455-507).  With rec,
+// the accepted steps are recorded: rec->st gets the state before each
+// accepted step and, last, the end state (ns values each), rec->dt the steps.
+struct CellRec { std::vector<double> st, dt; };
+PIState percell_cell(const CellChem& cc, double* uc, double* un, double t0, double t1, double dt0,
+                     const rs_pictrl& ctrl, double atol, double rtol, int clamp,
+                     long long maxiters, Ros23Work& w, CellRec* rec) {
+    const int ns = cc.c.ns;
+    PIState s{t0, dt0, ctrl.qoldinit};
+    const double tlim = t1 - 1.0e-9;
+    if (rec) { rec->st.assign(uc, uc + ns); rec->dt.clear(); }
+    std::vector<double> unv(ns);
+    while (s.t < tlim && s.iters < maxiters) {
+        double dtc = std::min(s.dt, t1 - s.t);
+        double sse = ros23_cell(cc, uc, un, dtc, atol, rtol, w);
+        double ee = std::sqrt(sse / (double)ns);
+        unv.assign(un, un + ns);
+        if ((clamp & 2) && out_of_domain(unv, (size_t)ns)) {
+            reject_out_of_domain(s, dtc);
+            continue;
+        }
+        if (pi_update(ctrl, s, dtc, ee)) {
+            for (int i = 0; i < ns; ++i) uc[i] = (clamp & 1) ? std::max(un[i], 0.0) : un[i];
+            if (rec) { rec->dt.push_back(dtc); rec->st.insert(rec->st.end(), uc, uc + ns); }
+        }
+    }
+    return s;
+}
+
 // Per-cell adaptive chemistry (the reference loop with NC = 1).
 int percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells, const rs_pictrl& ctrl,
             double atol, double rtol, int clamp, long long maxiters, rs_percell_stats* st) {
@@ -1424,21 +1456,10 @@ int percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells, const
         for (long long cell = 0; cell < (long long)c->nc; ++cell) {
             CellChem cc(*c, (size_t)cell);
             gather(*c, c->u.data(), cell, uc.data());
-            PIState s{t0, dt_cells ? dt_cells[cell] : dt0, ctrl.qoldinit};
-            const double tlim = t1 - 1.0e-9;
-            while (s.t < tlim && s.iters < maxiters) {
-                double dtc = std::min(s.dt, t1 - s.t);
-                double sse = ros23_cell(cc, uc.data(), un.data(), dtc, atol, rtol, w);
-                double ee = std::sqrt(sse / (double)ns);
-                if ((clamp & 2) && out_of_domain(un, (size_t)ns)) {
-                    reject_out_of_domain(s, dtc);
-                    continue;
-                }
-                if (pi_update(ctrl, s, dtc, ee)) {
-                    for (int i = 0; i < ns; ++i) uc[i] = (clamp & 1) ? std::max(un[i], 0.0) : un[i];
-                }
-            }
-            if (s.iters >= maxiters && s.t < tlim) nfail += 1;
+            PIState s = percell_cell(cc, uc.data(), un.data(), t0, t1,
+                                     dt_cells ? dt_cells[cell] : dt0, ctrl, atol, rtol, clamp,
+                                     maxiters, w, nullptr);
+            if (s.iters >= maxiters && s.t < t1 - 1.0e-9) nfail += 1;
             scatter(*c, uc.data(), cell, c->u.data());
             if (dt_cells) dt_cells[cell] = s.dt;
             att += s.iters; acc += s.nacc; rej += s.nrej;
@@ -1448,6 +1469,72 @@ int percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells, const
 #pragma omp critical
         { if (lmin >= 0) amin = (amin < 0) ? lmin : std::min(amin, lmin); }
     }
+    if (st) {
+        st->attempts_total = att; st->accepts_total = acc; st->rejects_total = rej;
+        st->attempts_max = amax; st->attempts_min = amin < 0 ? 0 : amin; st->n_failed = nfail;
+    }
+    if (nfail) return fail(c, RS_E_MAXITERS, std::to_string(nfail) + " cells hit maxiters");
+    return RS_OK;
+}
+
+// Adjoint of the per-cell adaptive chemistry: the reference's backward sweep
+// (adjoint_gradient.jl:731-834) over each cell's own accepted sequence
+// (adaptive_solve with NC = 1, rx_traced_integrator.jl:455-507).  Each cell
+// is integrated again from u0 with its accepted steps recorded (the same
+// operations as percell, so the same trajectory), then swept backwards, last
+// step first: lambda .*= 1[u_{j+1} > 0] when clamping (the accepted state is
+// max(u, 0), :736), lambda <- VJP of step j at u_j.  c->lam: dJ/du(t1) in,
+// dJ/du(t0) out; c->gemis accumulates dJ/dE.  *uend gets the end states,
+// *nvjp the step VJPs taken, *ngrad the non-finite emission-gradient
+// contributions.
+int percell_vjp(rs_ctx* c, const std::vector<double>& u0, double t0, double t1, double dt0,
+                const rs_pictrl& ctrl, double atol, double rtol, int clamp, long long maxiters,
+                std::vector<double>* uend, long long* nvjp, long long* ngrad,
+                rs_percell_stats* st) {
+    const int ns = c->ns;
+    if (uend) uend->assign(u0.size(), 0.0);
+    long long att = 0, acc = 0, rej = 0, amax = 0, amin = -1, nfail = 0, nv = 0, ng = 0;
+#pragma omp parallel reduction(+ : att, acc, rej, nfail, nv, ng) reduction(max : amax)
+    {
+        Ros23Work w(ns);
+        CellRec rec;
+        std::vector<double> uc(ns), un(ns), lc(ns), lo(ns), eb(ns);
+        long long lmin = -1;
+#pragma omp for schedule(dynamic, 16)
+        for (long long cell = 0; cell < (long long)c->nc; ++cell) {
+            CellChem cc(*c, (size_t)cell);
+            gather(*c, u0.data(), cell, uc.data());
+            PIState s = percell_cell(cc, uc.data(), un.data(), t0, t1, dt0, ctrl, atol, rtol,
+                                     clamp, maxiters, w, &rec);
+            if (s.iters >= maxiters && s.t < t1 - 1.0e-9) nfail += 1;
+            if (uend) scatter(*c, uc.data(), cell, uend->data());
+            att += s.iters; acc += s.nacc; rej += s.nrej;
+            amax = std::max(amax, s.iters);
+            lmin = (lmin < 0) ? s.iters : std::min(lmin, s.iters);
+            const bool surf = (size_t)cell < c->ncol;
+            gather(*c, c->lam.data(), cell, lc.data());
+            for (long long q = (long long)rec.dt.size() - 1; q >= 0; --q) {
+                const double* uout = rec.st.data() + (size_t)(q + 1) * ns;
+                if (clamp & 1)
+                    for (int i = 0; i < ns; ++i) lc[i] = lc[i] * ((uout[i] > 0.0) ? 1.0 : 0.0);
+                std::fill(eb.begin(), eb.end(), 0.0);
+                ros23_vjp_cell(cc, rec.st.data() + (size_t)q * ns, rec.dt[q], lc.data(), lo.data(),
+                               surf ? eb.data() : nullptr);
+                lc.swap(lo);
+                nv += 1;
+                if (surf)
+                    for (int i = 0; i < ns; ++i) {
+                        ng += !std::isfinite(eb[i]);
+                        c->gemis[(size_t)i * c->ncol + cell] += eb[i];
+                    }
+            }
+            scatter(*c, lc.data(), cell, c->lam.data());
+        }
+#pragma omp critical
+        { if (lmin >= 0) amin = (amin < 0) ? lmin : std::min(amin, lmin); }
+    }
+    if (nvjp) *nvjp = nv;
+    if (ngrad) *ngrad = ng;
     if (st) {
         st->attempts_total = att; st->accepts_total = acc; st->rejects_total = rej;
         st->attempts_max = amax; st->attempts_min = amin < 0 ? 0 : amin; st->n_failed = nfail;
@@ -1503,8 +1590,10 @@ int macro(rs_ctx* c, double t0, double t1, double* dtT, double* dtC, const rs_sp
     }
     if (o.chem_percell) {
         rs_percell_stats st{};
-        return percell(c, t0, t1, *dtC, nullptr, o.ctrl_C, o.atol_C, o.rtol_C, o.clamp_nonneg,
-                       o.maxiters, &st);
+        rc = percell(c, t0, t1, *dtC, nullptr, o.ctrl_C, o.atol_C, o.rtol_C, o.clamp_nonneg,
+                     o.maxiters, &st);
+        if (cC) { cC->naccept = st.accepts_total; cC->nreject = st.rejects_total; cC->t_end = t1; cC->dt_end = *dtC; }
+        return rc;
     }
     return adaptive(c, Op::C, t0, t1, dtC, o.ctrl_C, o.atol_C, o.rtol_C, o.clamp_nonneg,
                     o.maxiters, cC, sC);
@@ -1952,6 +2041,16 @@ int rs_chem_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cell
     return percell(c, t0, t1, dt0, dt_cells, *ctrl, atol, rtol, clamp, maxiters, out);
 }
 
+int rs_chem_percell_vjp(rs_ctx* c, double t0, double t1, double dt0, const rs_pictrl* ctrl,
+                        double atol, double rtol, int clamp, long long maxiters,
+                        rs_percell_stats* out) {
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    if (!ctrl || !(dt0 > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
+    return percell_vjp(c, c->u, t0, t1, dt0, *ctrl, atol, rtol, clamp, maxiters, nullptr, nullptr,
+                       nullptr, out);
+}
+
 int rs_macro_step(rs_ctx* c, double t0, double t1, double* dtT, double* dtC,
                   const rs_split_opts* o, rs_counts* cT, rs_counts* cC) {
     int rc = check_ready(c, true, true);
@@ -2010,10 +2109,10 @@ static int adjoint_impl(rs_ctx* c, int nmacro, const double* tstops, double dtT0
     int rc = check_ready(c, true, true);
     if (rc) return rc;
     if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
-    if (o->chem_percell) return fail(c, RS_E_ARG, "adjoint needs the global-norm chemistry");
     rc = check_loss(c, loss_kind, loss_species);
     if (rc) return rc;
     using clk = std::chrono::steady_clock;
+    const bool pcell = o->chem_percell != 0;
     rs_adjoint_stats S{};
     S.bad_op = S.bad_macro = -1;
     S.bad_step = S.bad_nsteps = -1;
@@ -2032,6 +2131,15 @@ static int adjoint_impl(rs_ctx* c, int nmacro, const double* tstops, double dtT0
                 ssprk43_attempt(*c, c->u, raw, pq.second, o->atol_T, o->rtol_T);
                 if (o->clamp_nonneg & 1) for (double& x : raw) x = std::max(x, 0.0);
                 c->u.swap(raw);
+            }
+            if (pcell) {
+                // per-cell chemistry has no global sequence: each cell runs free
+                rs_percell_stats pst{};
+                rc = percell(c, tstops[m], tstops[m + 1], dtC0, nullptr, o->ctrl_C, o->atol_C,
+                             o->rtol_C, o->clamp_nonneg, o->maxiters, &pst);
+                if (rc) return rc;
+                c->seqC[m].ts.clear();
+                S.naccept_C += pst.accepts_total; S.nreject_C += pst.rejects_total;
             }
             for (auto& pq : c->seqC[m].ts) {
                 ros23_attempt(*c, c->u, raw, pq.second, o->atol_C, o->rtol_C);
@@ -2053,6 +2161,8 @@ static int adjoint_impl(rs_ctx* c, int nmacro, const double* tstops, double dtT0
     S.seconds_forward = std::chrono::duration<double>(clk::now() - tf0).count();
     c->last_opts = *o;
     c->have_record = true;
+    c->rec_tstops.assign(tstops, tstops + nmacro + 1);
+    c->rec_dtC0 = dtC0;
 
     auto tb0 = clk::now();
     double trep = 0.0;
@@ -2099,6 +2209,14 @@ static int adjoint_impl(rs_ctx* c, int nmacro, const double* tstops, double dtT0
             if (clampf) for (double& x : raw) x = std::max(x, 0.0);
             u.swap(raw);
         }
+        // per-cell chemistry: replayed and swept cell by cell (percell_vjp);
+        // u gets its end state for the replay check, lambda its adjoint
+        long long pc_nv = 0, pc_ng = 0;
+        if (pcell) {
+            rc = percell_vjp(c, uT, tstops[m], tstops[m + 1], dtC0, o->ctrl_C, o->atol_C,
+                             o->rtol_C, o->clamp_nonneg, o->maxiters, &u, &pc_nv, &pc_ng, nullptr);
+            if (rc) return rc;
+        }
         const std::vector<double>& ref = (m + 1 < nmacro) ? ck[m + 1] : uend;
         double d = 0.0;
         for (size_t x = 0; x < u.size(); ++x) {
@@ -2119,6 +2237,13 @@ static int adjoint_impl(rs_ctx* c, int nmacro, const double* tstops, double dtT0
             const long long ng = ros23_vjp_all(c, tapeC[jx], c->seqC[m].ts[jx].second);
             note(1, m, jx, c->seqC[m], ng);
             S.nvjp_C += 1;
+        }
+        if (pcell) {
+            // reported as one step spanning the macro step
+            Seq whole;
+            whole.ts.emplace_back(tstops[m], tstops[m + 1] - tstops[m]);
+            note(1, m, 0, whole, pc_ng);
+            S.nvjp_C += pc_nv;
         }
         for (int jx = (int)tapeT.size() - 1; jx >= 0; --jx) {
             const std::vector<double>& outj = (jx + 1 < (int)tapeT.size()) ? tapeT[jx + 1] : uT;
@@ -2217,6 +2342,14 @@ int rs_replay_loss(rs_ctx* c, int loss_kind, int loss_species, double* J_out) {
             ros23_attempt(*c, c->u, raw, p.second, o.atol_C, o.rtol_C);
             if (o.clamp_nonneg & 1) for (double& x : raw) x = std::max(x, 0.0);
             c->u.swap(raw);
+        }
+        if (o.chem_percell) {
+            // per-cell chemistry records no steps: it runs free, each cell
+            // from the run's start dt
+            rs_percell_stats pst{};
+            rc = percell(c, c->rec_tstops[m], c->rec_tstops[m + 1], c->rec_dtC0, nullptr, o.ctrl_C,
+                         o.atol_C, o.rtol_C, o.clamp_nonneg, o.maxiters, &pst);
+            if (rc) return rc;
         }
     }
     if (J_out) *J_out = loss_value(*c, loss_kind, loss_species);

@@ -459,6 +459,175 @@ rs_chem_percell(RsPercellArgs a RS_KPARAM) {
 
 #undef RS_U
 
+/* Adjoint of the per-cell solve (the reference's backward sweep,
+ * adjoint_gradient.jl:731-834, over each cell's own accepted sequence).
+ * Persistent with the same wave-aggregated queue as rs_chem_percell; a lane
+ * takes a cell, integrates it again from u with the same attempt code (so
+ * the same trajectory), recording in its own slice of a.ws the state before
+ * every accepted step, the step, and the end state; then sweeps the record
+ * backwards: lam .*= 1[u_{j+1} > 0] when clamping (the accepted state is
+ * max(u, 0)), lam <- VJP of step j at u_j (rs_ros23_vjp; lam lives in a.lam,
+ * read and written per step).  Lanes of a wave move through the two phases
+ * independently.  A cell that would need more than a.nmax recorded states
+ * is counted in stats[6] and left without its sweep (the host fails the
+ * call). */
+extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK) RS_PERCELL_ATTR
+rs_chem_percell_adj(RsPercellAdjArgs a RS_KPARAM) {
+    const int lane = threadIdx.x & 63;
+    const long long NL = a.nlanes;
+    double* __restrict__ ws = a.ws + ((long long)blockIdx.x * blockDim.x + threadIdx.x);
+    long long cell = -1;     /* -1: needs work, -2: queue drained */
+    int mode = 0;            /* 1: forward with record, 2: backward sweep */
+    RS_SLAB(sv);
+    RS_WDECL(W);
+#define RS_U(s) sv[(2 * RS_NS + (s)) * RS_SLAB_ST]
+#define RS_REC(j, s) ws[((j) * (RS_NS + 1) + (s)) * NL]
+    const double* Ec = nullptr;
+    double* Gc = nullptr;
+    double t = 0.0, dt = 0.0, qold = 0.0, jf = 1.0;
+    long long iters = 0, nacc = 0, nrej = 0, nst = 0, q = -1;
+    unsigned long long s_att = 0, s_acc = 0, s_rej = 0, s_max = 0, s_fail = 0, s_vjp = 0,
+                       s_ovf = 0, s_nl = 0, s_ng = 0;
+    bool ovf = false;
+    const double tlim = a.t1 - 1.0e-9;
+    for (;;) {
+        const bool need = (cell == -1);
+        const unsigned long long m = __ballot(need);
+        if (m) {
+            const int leader = __builtin_ctzll(m);
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(a.head, (unsigned long long)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const unsigned long long rank = __popcll(m & ((1ull << lane) - 1ull));
+                const unsigned long long idx = base + rank;
+                if (idx < (unsigned long long)a.nc) {
+                    cell = (long long)idx;
+#pragma unroll
+                    for (int s = 0; s < RS_NS; ++s) {
+                        const double v = a.u[(long long)s * a.nc + cell];
+                        RS_U(s) = v;
+                        RS_REC(0, s) = v;
+                    }
+                    jf = a.jfac ? a.jfac[cell] : 1.0;
+                    Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;
+                    Gc = (a.gE && cell < a.ncol) ? a.gE + cell : nullptr;
+                    t = a.t0;
+                    dt = a.dt0;
+                    qold = a.qoldinit;
+                    iters = nacc = nrej = nst = 0;
+                    ovf = false;
+                    mode = 1;
+                } else {
+                    cell = -2;
+                }
+            }
+        }
+        if (!__any(cell >= 0)) break;
+        if (cell >= 0 && mode == 1) {
+            if ((t < tlim) && (iters < a.maxiters)) {
+                const double dtc = (dt < a.t1 - t) ? dt : a.t1 - t;
+                double un[RS_NS];
+                const double sse = rs_ros23_attempt<RS_SLAB_ST, RS_W_ST, true>(
+                    RS_KPTR(a), jf, Ec, a.ncol, sv, W, dtc, a.atol, a.rtol,
+                    [&](int s, double v) { un[s] = v; });
+                const double eest = RS_SQRT(sse / (double)RS_NS);
+                bool isout = false;
+                if (a.posdom) {
+#pragma unroll
+                    for (int s = 0; s < RS_NS; ++s) isout = isout || un[s] < 0.0;
+                }
+                if (isout) {
+                    dt = dtc / 2.0;
+                    nrej += 1;
+                } else if (rs_pi_update(a.beta1, a.beta2, a.invqmin, a.invqmax, a.gamma, a.qsmin,
+                                        a.qsmax, a.qoldinit, dtc, eest, &t, &dt, &qold, &nacc, &nrej)) {
+#pragma unroll
+                    for (int s = 0; s < RS_NS; ++s) RS_U(s) = a.clamp ? rs_max(un[s], 0.0) : un[s];
+                    if (nst + 1 < a.nmax) {
+                        RS_REC(nst, RS_NS) = dtc;
+                        nst += 1;
+#pragma unroll
+                        for (int s = 0; s < RS_NS; ++s) RS_REC(nst, s) = RS_U(s);
+                    } else {
+                        ovf = true;
+                    }
+                }
+                iters += 1;
+            }
+            if (ovf || !((t < tlim) && (iters < a.maxiters))) {
+                if (a.uend) {
+#pragma unroll
+                    for (int s = 0; s < RS_NS; ++s) a.uend[(long long)s * a.nc + cell] = RS_U(s);
+                }
+                s_att += iters; s_acc += nacc; s_rej += nrej;
+                s_max = (unsigned long long)iters > s_max ? (unsigned long long)iters : s_max;
+                if (ovf) {
+                    s_ovf += 1;
+                    cell = -1;
+                    mode = 0;
+                } else {
+                    if (t < tlim) s_fail += 1;
+                    mode = 2;
+                    q = nst - 1;
+                }
+            }
+        } else if (cell >= 0 && mode == 2) {
+            if (q >= 0) {
+#pragma unroll
+                for (int s = 0; s < RS_NS; ++s) sv[s * RS_SLAB_ST] = RS_REC(q, s);   /* slot A: u_q */
+                const double dq = RS_REC(q, RS_NS);
+                const long long qn = q + 1;
+                const long long cb = cell;
+                s_ng += (unsigned long long)rs_ros23_vjp<RS_SLAB_ST, RS_W_ST>(
+                    RS_KPTR(a), jf, Ec, a.ncol, sv, W, dq,
+                    [&](int s) {
+                        double l = a.lam[(long long)s * a.nc + cb];
+                        if (a.clamp) l = l * ((RS_REC(qn, s) > 0.0) ? 1.0 : 0.0);
+                        return l;
+                    },
+                    [&](int s, double v) { a.lam[(long long)s * a.nc + cb] = v; }, Gc);
+                s_vjp += 1;
+                q -= 1;
+            }
+            if (q < 0) {
+#pragma unroll
+                for (int s = 0; s < RS_NS; ++s) s_nl += !RS_ISFINITE(a.lam[(long long)s * a.nc + cell]);
+                cell = -1;
+                mode = 0;
+            }
+        }
+    }
+#undef RS_REC
+    /* wave reduction of the statistics, one atomic per counter and wave */
+    for (int off = 32; off > 0; off >>= 1) {
+        s_att += __shfl_down(s_att, off, 64);
+        s_acc += __shfl_down(s_acc, off, 64);
+        s_rej += __shfl_down(s_rej, off, 64);
+        s_fail += __shfl_down(s_fail, off, 64);
+        s_vjp += __shfl_down(s_vjp, off, 64);
+        s_ovf += __shfl_down(s_ovf, off, 64);
+        s_nl += __shfl_down(s_nl, off, 64);
+        s_ng += __shfl_down(s_ng, off, 64);
+        const unsigned long long om = __shfl_down(s_max, off, 64);
+        s_max = om > s_max ? om : s_max;
+    }
+    if (lane == 0) {
+        atomicAdd(&a.stats[0], s_att);
+        atomicAdd(&a.stats[1], s_acc);
+        atomicAdd(&a.stats[2], s_rej);
+        atomicMax(&a.stats[3], s_max);
+        atomicAdd(&a.stats[4], s_fail);
+        atomicAdd(&a.stats[5], s_vjp);
+        atomicAdd(&a.stats[6], s_ovf);
+        atomicAdd(&a.stats[7], s_nl);
+        atomicAdd(&a.stats[8], s_ng);
+        if (a.nfc && s_ng) atomicAdd(&a.nfc[1], s_ng);
+    }
+}
+
+#undef RS_U
+
 /* One ROS23 step VJP for every cell, with the clamp mask of that step
  * (lam .*= 1[uout > 0], adjoint_gradient.jl:736; a multiplication, so a
  * non-finite entry stays non-finite) applied first.  With a.nfc the step's

@@ -135,7 +135,7 @@ struct DevArray {
 struct ChemModule {
     hipModule_t mod = nullptr;
     hipFunction_t attempt = nullptr, attempt_kp = nullptr, attempt_pd = nullptr, percell = nullptr,
-                  vjp = nullptr;
+                  vjp = nullptr, percell_adj = nullptr;
     rs::MechPlan plan;
     ~ChemModule() { if (mod) (void)hipModuleUnload(mod); }
 };
@@ -235,6 +235,15 @@ struct rs_ctx {
     bool have_record = false;
     DevArray<double> ckpt, tape, relpart;
     DevArray<unsigned long long> nfc;   // non-finite counts of the backward sweep's VJPs
+    // per-cell chemistry in the last adjoint run: every macro step's seqC holds
+    // one entry (t0, t1 - t0) for it; each cell started from rec_dtC0 and took
+    // at most rec_pc_amax attempts
+    bool rec_percell = false;
+    double rec_dtC0 = 0.0;
+    long long rec_pc_amax = 0;
+    long long last_pc_amax = 0;         // attempts_max of the last per-cell solve
+    DevArray<double> pcws, pcu;         // per-cell adjoint: lane records, state copy
+    DevArray<unsigned long long> pcstats;
     DevArray<double> wobj;              // weights of the linear objective (loss kind 1)
     bool have_wobj = false;
     // profiling
@@ -785,20 +794,42 @@ int run_adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs
     return adaptive_finish(c, ar, dt_io, out, seq);
 }
 
-int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_host,
-                const rs_pictrl& pc, double atol, double rtol, int clamp, long long maxiters,
-                rs_percell_stats* out) {
+// Lanes of a persistent chemistry launch: as many workgroups as fit on the
+// device at once, no more than the cells need.
+long long percell_grid(rs_ctx* c, hipFunction_t f) {
+    int per_cu = 0;
+    const unsigned thr = chem_threads(c);
+    HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, (int)thr,
+                                                              chem_slab_bytes(c, thr)));
+    per_cu = std::max(per_cu, 1);
+    long long grid = (long long)per_cu * c->ncu;
+    grid = std::min<long long>(grid, (c->nc + thr - 1) / thr);
+    return std::max<long long>(grid, 1);
+}
+
+// The controller fields of a per-cell argument block.
+template <class A>
+void percell_ctrl(A& a, const rs_pictrl& pc) {
+    a.beta1 = pc.beta1;
+    a.beta2 = pc.beta2;
+    a.invqmin = 1.0 / pc.qmin;
+    a.invqmax = 1.0 / pc.qmax;
+    a.gamma = pc.gamma;
+    a.qsmin = pc.qsteady_min;
+    a.qsmax = pc.qsteady_max;
+    a.qoldinit = pc.qoldinit;
+}
+
+// The per-cell solve on the state u (in place), statistics into c->stats;
+// no host synchronisation.
+void enqueue_percell(rs_ctx* c, double* u, double t0, double t1, double dt0, double* dtcell,
+                     const rs_pictrl& pc, double atol, double rtol, int clamp, long long maxiters) {
     RsPercellArgs a{};
-    a.u = c->S[c->cur].p;
+    a.u = u;
     a.K = c->K.p;
     a.jfac = c->has_jfac ? c->jfac.p : nullptr;
     a.E = c->has_E ? c->E.p : nullptr;
-    a.dtcell = nullptr;
-    if (dt_cells_host) {
-        c->dtcell.ensure((size_t)c->nc);
-        h2d(c->dtcell.p, dt_cells_host, (size_t)c->nc, c->stream);
-        a.dtcell = c->dtcell.p;
-    }
+    a.dtcell = dtcell;
     c->head.ensure(1);
     c->stats.ensure(8);
     a.head = c->head.p;
@@ -811,14 +842,7 @@ int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_ho
     a.dt0 = dt0;
     a.atol = atol;
     a.rtol = rtol;
-    a.beta1 = pc.beta1;
-    a.beta2 = pc.beta2;
-    a.invqmin = 1.0 / pc.qmin;
-    a.invqmax = 1.0 / pc.qmax;
-    a.gamma = pc.gamma;
-    a.qsmin = pc.qsteady_min;
-    a.qsmax = pc.qsteady_max;
-    a.qoldinit = pc.qoldinit;
+    percell_ctrl(a, pc);
     a.clamp = clamp_nonneg_of(clamp);
     a.posdom = posdom_of(clamp);
     // queue head and statistics (min starts at the largest value)
@@ -826,20 +850,26 @@ int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_ho
     std::memcpy(c->hstats, init, sizeof(init));
     HIPCHK(hipMemcpyAsync(c->stats.p, c->hstats, sizeof(init), hipMemcpyHostToDevice, c->stream));
     HIPCHK(rs_launch_fill_u64(c->head.p, 1, 0ULL, c->stream));
-    int per_cu = 0;
-    const unsigned thr = chem_threads(c);
-    const size_t slab = chem_slab_bytes(c, thr);
-    HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c->chem->percell,
-                                                              (int)thr, slab));
-    per_cu = std::max(per_cu, 1);
-    long long grid = (long long)per_cu * c->ncu;
-    grid = std::min<long long>(grid, (c->nc + thr - 1) / thr);
-    grid = std::max<long long>(grid, 1);
-    chem_launch(c, c->chem->percell, &a, sizeof(a), (unsigned)grid, thr, P_PERCELL);
+    const long long grid = percell_grid(c, c->chem->percell);
+    chem_launch(c, c->chem->percell, &a, sizeof(a), (unsigned)grid, chem_threads(c), P_PERCELL);
+}
+
+int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_host,
+                const rs_pictrl& pc, double atol, double rtol, int clamp, long long maxiters,
+                rs_percell_stats* out, double* u = nullptr) {
+    double* dtcell = nullptr;
+    if (dt_cells_host) {
+        c->dtcell.ensure((size_t)c->nc);
+        h2d(c->dtcell.p, dt_cells_host, (size_t)c->nc, c->stream);
+        dtcell = c->dtcell.p;
+    }
+    enqueue_percell(c, u ? u : c->S[c->cur].p, t0, t1, dt0, dtcell, pc, atol, rtol, clamp,
+                    maxiters);
     HIPCHK(hipMemcpyAsync(c->hstats, c->stats.p, 8 * sizeof(long long), hipMemcpyDeviceToHost,
                           c->stream));
     if (dt_cells_host) d2h(dt_cells_host, c->dtcell.p, (size_t)c->nc, c->stream);
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->last_pc_amax = c->hstats[3];
     if (out) {
         out->attempts_total = c->hstats[0];
         out->accepts_total = c->hstats[1];
@@ -851,6 +881,66 @@ int run_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cells_ho
     if (c->hstats[5] > 0)
         return fail(c, RS_E_MAXITERS, std::to_string(c->hstats[5]) + " cells hit maxiters");
     return RS_OK;
+}
+
+// Lane records of the per-cell adjoint for cells of at most nmax recorded
+// states (allocated here, before any graph capture).
+long long percell_adj_lanes(rs_ctx* c) {
+    return percell_grid(c, c->chem->percell_adj) * (long long)chem_threads(c);
+}
+void ensure_percell_record(rs_ctx* c, long long nmax) {
+    const double need = (double)percell_adj_lanes(c) * (double)nmax * (double)(c->ns + 1) * 8.0;
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    if (need > 0.85 * (double)(fr + c->pcws.n * sizeof(double)))
+        throw ArgError{"per-cell adjoint record needs " + std::to_string(need / (1 << 30)) +
+                       " GiB of device memory (" + std::to_string(nmax) + " states per cell)",
+                       RS_E_ARG};
+    c->pcws.ensure((size_t)percell_adj_lanes(c) * (size_t)nmax * (size_t)(c->ns + 1));
+}
+
+// The adjoint of the per-cell solve from the state u: lam (c->lam) from
+// dJ/du(t1) to dJ/du(t0), c->gE accumulated, each cell's end state into uend
+// (or not), nine counters into stats (zeroed here) and the non-finite
+// emission-gradient contributions into nfc[1] (when nfc is given); no host
+// synchronisation.
+void enqueue_percell_adj(rs_ctx* c, const double* u, double* uend, double t0, double t1,
+                         double dt0, const rs_pictrl& pc, double atol, double rtol, int clamp,
+                         long long maxiters, long long nmax, unsigned long long* stats,
+                         unsigned long long* nfc = nullptr) {
+    RsPercellAdjArgs a{};
+    a.nfc = nfc;
+    a.u = u;
+    a.uend = uend;
+    a.lam = c->lam.p;
+    a.gE = c->has_E ? c->gE.p : nullptr;
+    a.K = c->K.p;
+    a.jfac = c->has_jfac ? c->jfac.p : nullptr;
+    a.E = c->has_E ? c->E.p : nullptr;
+    a.ws = c->pcws.p;
+    c->head.ensure(1);
+    a.head = c->head.p;
+    a.stats = stats;
+    a.nc = c->nc;
+    a.ncol = c->ncol;
+    a.maxiters = maxiters;
+    a.nmax = nmax;
+    const long long grid = percell_grid(c, c->chem->percell_adj);
+    a.nlanes = grid * (long long)chem_threads(c);
+    if ((size_t)a.nlanes * (size_t)nmax * (size_t)(c->ns + 1) > c->pcws.n)
+        throw ArgError{"per-cell adjoint record not sized", RS_E_STATE};
+    a.t0 = t0;
+    a.t1 = t1;
+    a.dt0 = dt0;
+    a.atol = atol;
+    a.rtol = rtol;
+    percell_ctrl(a, pc);
+    a.clamp = clamp_nonneg_of(clamp);
+    a.posdom = posdom_of(clamp);
+    HIPCHK(rs_launch_fill_u64(stats, 9, 0ULL, c->stream));
+    HIPCHK(rs_launch_fill_u64(c->head.p, 1, 0ULL, c->stream));
+    chem_launch(c, c->chem->percell_adj, &a, sizeof(a), (unsigned)grid, chem_threads(c),
+                P_PERCELL);
 }
 
 int check_ready(rs_ctx* c, bool need_mech, bool need_wind) {
@@ -884,6 +974,7 @@ int macro(rs_ctx* c, double t0, double t1, double* dtT, double* dtC, const rs_sp
         rc = run_percell(c, t0, t1, *dtC, nullptr, o.ctrl_C, o.atol_C, o.rtol_C, o.clamp_nonneg,
                          o.maxiters, &st);
         if (cC) { cC->naccept = st.accepts_total; cC->nreject = st.rejects_total; cC->t_end = t1; cC->dt_end = *dtC; }
+        if (sC) sC->assign(1, {t0, t1 - t0});   // the whole per-cell solve, as one entry
         return rc;
     }
     return run_adaptive(c, Op::C, t0, t1, dtC, o.ctrl_C, o.atol_C, o.rtol_C, o.clamp_nonneg,
@@ -1062,8 +1153,14 @@ double* tape_ptr(rs_ctx* c, size_t n) {
 // Fixed-sequence replay of one macro step from `u0` into the tape; tape slot j
 // holds the input of step j (transport steps first, then chemistry), the last
 // slot the end state.  Returns the number of slots written.
+//
+// Per-cell chemistry (c->rec_percell; sC is its one entry (t0, t1 - t0)):
+// with pc_forward the per-cell solve runs from slot nT into the last slot;
+// without, the last slot is left to the per-cell adjoint (which writes the
+// cells' end states there).
 size_t replay_into_tape(rs_ctx* c, const double* u0, const std::vector<std::pair<double, double>>& sT,
-                        const std::vector<std::pair<double, double>>& sC, const rs_split_opts& o) {
+                        const std::vector<std::pair<double, double>>& sC, const rs_split_opts& o,
+                        bool pc_forward = true) {
     const size_t N = c->N();
     const size_t nslots = sT.size() + sC.size() + 1;
     double* tp = tape_ptr(c, nslots * N);
@@ -1079,6 +1176,15 @@ size_t replay_into_tape(rs_ctx* c, const double* u0, const std::vector<std::pair
             d2d(outp + off, in + off, (size_t)(c->ns - c->nt) * c->nc, c->stream);
         }
         ++j;
+    }
+    if (c->rec_percell) {
+        if (pc_forward && !sC.empty()) {
+            d2d(tp + (j + 1) * N, tp + j * N, N, c->stream);
+            enqueue_percell(c, tp + (j + 1) * N, sC[0].first, sC[0].first + sC[0].second,
+                            c->rec_dtC0, nullptr, o.ctrl_C, o.atol_C, o.rtol_C, o.clamp_nonneg,
+                            o.maxiters);
+        }
+        return nslots;
     }
     for (auto& p : sC) {
         enqueue_ros23(c, nullptr, tp + j * N, tp + (j + 1) * N, 0, p.second, o.atol_C, o.rtol_C,
@@ -1195,13 +1301,28 @@ void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* re
     const auto& sT = c->seqT[m];
     const auto& sC = c->seqC[m];
     if (ev0) HIPCHK(hipEventRecord(ev0, c->stream));
-    const size_t ns = replay_into_tape(c, c->ckpt.p + (size_t)m * N, sT, sC, o);
+    const size_t ns = replay_into_tape(c, c->ckpt.p + (size_t)m * N, sT, sC, o, false);
     double* tp = tape_ptr(c, ns * N);
-    HIPCHK(rs_launch_relmax(tp + (ns - 1) * N, c->ckpt.p + (size_t)(m + 1) * N, (long long)N,
-                            relpart, c->stream));
-    if (ev1) HIPCHK(hipEventRecord(ev1, c->stream));
     const size_t nT = sT.size();
-    for (size_t q = sC.size(); q-- > 0; slot += 2) {
+    if (c->rec_percell) {
+        // per-cell chemistry: each cell replayed and swept by the per-cell
+        // adjoint, which writes the end states for the replay check
+        if (ev1) HIPCHK(hipEventRecord(ev1, c->stream));
+        enqueue_percell_adj(c, tp + nT * N, tp + (nT + 1) * N, sC[0].first,
+                            sC[0].first + sC[0].second, c->rec_dtC0, o.ctrl_C, o.atol_C, o.rtol_C,
+                            o.clamp_nonneg, o.maxiters, c->rec_pc_amax + 1,
+                            c->pcstats.p + 9 * (size_t)m, slot);
+        HIPCHK(rs_launch_relmax(tp + (ns - 1) * N, c->ckpt.p + (size_t)(m + 1) * N, (long long)N,
+                                relpart, c->stream));
+        ProfScope ps(c, P_UTIL);
+        HIPCHK(rs_launch_nonfinite(c->lam.p, (long long)N, slot, c->stream));
+        slot += 2;
+    } else {
+        HIPCHK(rs_launch_relmax(tp + (ns - 1) * N, c->ckpt.p + (size_t)(m + 1) * N, (long long)N,
+                                relpart, c->stream));
+        if (ev1) HIPCHK(hipEventRecord(ev1, c->stream));
+    }
+    for (size_t q = c->rec_percell ? 0 : sC.size(); q-- > 0; slot += 2) {
         const size_t ts = nT + q;
         enqueue_ros23_vjp(c, tp + ts * N, clamp_nonneg_of(o.clamp_nonneg) ? tp + (ts + 1) * N : nullptr,
                           sC[q].second, slot);
@@ -1609,6 +1730,7 @@ int install_mechanism(rs_ctx* c, rs::MechSpec& m, const double* k) {
         HIPCHK(hipModuleGetFunction(&cm->attempt_pd, cm->mod, "rs_chem_attempt_pd"));
         HIPCHK(hipModuleGetFunction(&cm->percell, cm->mod, "rs_chem_percell"));
         HIPCHK(hipModuleGetFunction(&cm->vjp, cm->mod, "rs_chem_vjp"));
+        HIPCHK(hipModuleGetFunction(&cm->percell_adj, cm->mod, "rs_chem_percell_adj"));
         c->K.resize(std::max(nr, 1));
         if (nr) h2d(c->K.p, k, nr, c->stream);
         c->hK.assign(k, k + nr);
@@ -1878,6 +2000,47 @@ int rs_chem_percell(rs_ctx* c, double t0, double t1, double dt0, double* dt_cell
     });
 }
 
+int rs_chem_percell_vjp(rs_ctx* c, double t0, double t1, double dt0, const rs_pictrl* ctrl,
+                        double atol, double rtol, int clamp, long long maxiters,
+                        rs_percell_stats* out) {
+    int rc = check_ready(c, true, false);
+    if (rc) return rc;
+    if (!ctrl || !(dt0 > 0)) return fail(c, RS_E_ARG, "bad dt / ctrl");
+    c->have_cand = false;
+    return guarded(c, [&]() {
+        const size_t N = c->N();
+        // a forward pass on a copy of the state finds the longest cell's
+        // step count, which sizes the lane records
+        c->pcu.ensure(N);
+        d2d(c->pcu.p, c->S[c->cur].p, N, c->stream);
+        rs_percell_stats fs{};
+        const int r = run_percell(c, t0, t1, dt0, nullptr, *ctrl, atol, rtol, clamp, maxiters, &fs,
+                                  c->pcu.p);
+        if (r) return r;
+        const long long nmax = fs.attempts_max + 1;
+        ensure_percell_record(c, nmax);
+        c->pcstats.ensure(9);
+        enqueue_percell_adj(c, c->S[c->cur].p, nullptr, t0, t1, dt0, *ctrl, atol, rtol, clamp,
+                            maxiters, nmax, c->pcstats.p);
+        unsigned long long h[9];
+        HIPCHK(hipMemcpyAsync(h, c->pcstats.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (out) {
+            out->attempts_total = (long long)h[0];
+            out->accepts_total = (long long)h[1];
+            out->rejects_total = (long long)h[2];
+            out->attempts_max = (long long)h[3];
+            out->attempts_min = fs.attempts_min;
+            out->n_failed = (long long)h[4];
+        }
+        if (h[6])
+            return fail(c, RS_E_STATE, std::to_string(h[6]) + " cells took more steps than their "
+                                       "forward pass (record overflow)");
+        if (h[4]) return fail(c, RS_E_MAXITERS, std::to_string(h[4]) + " cells hit maxiters");
+        return RS_OK;
+    });
+}
+
 int rs_macro_step(rs_ctx* c, double t0, double t1, double* dtT, double* dtC,
                   const rs_split_opts* o, rs_counts* cT, rs_counts* cC) {
     int rc = check_ready(c, true, true);
@@ -1947,7 +2110,6 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
     int rc = check_ready(c, true, true);
     if (rc) return rc;
     if (!o || !tstops || nmacro < 1) return fail(c, RS_E_ARG, "bad adjoint arguments");
-    if (o->chem_percell) return fail(c, RS_E_ARG, "adjoint needs the global-norm chemistry");
     rc = check_loss(c, loss_kind, loss_species);
     if (rc) return rc;
     c->have_cand = false;
@@ -1959,6 +2121,9 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         c->ckpt.ensure((size_t)(nmacro + 1) * N);
         c->seqT.assign(nmacro, {});
         c->seqC.assign(nmacro, {});
+        c->rec_percell = o->chem_percell != 0;
+        c->rec_dtC0 = dtC0;
+        c->rec_pc_amax = 0;
         HIPCHK(hipStreamSynchronize(c->stream));
         auto tf0 = clk::now();
         double dtT = dtT0, dtC = dtC0;
@@ -1967,6 +2132,7 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
             rs_counts cT{}, cC{};
             int r = macro(c, tstops[m], tstops[m + 1], &dtT, &dtC, *o, &cT, &cC, &c->seqT[m], &c->seqC[m]);
             if (r) return r;
+            if (c->rec_percell) c->rec_pc_amax = std::max(c->rec_pc_amax, c->last_pc_amax);
             S.naccept_T += cT.naccept; S.nreject_T += cT.nreject;
             S.naccept_C += cC.naccept; S.nreject_C += cC.nreject;
         }
@@ -1981,6 +2147,10 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         c->last_opts = *o;
         c->have_record = true;
 
+        if (c->rec_percell) {
+            ensure_percell_record(c, c->rec_pc_amax + 1);
+            c->pcstats.ensure(9 * (size_t)nmacro);
+        }
         auto tb0 = clk::now();
         backward_seed(c, loss_kind, loss_species, nvjp);
         c->relpart.ensure((size_t)nmacro * 1024);
@@ -2001,9 +2171,21 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         int worst = 0;
         S.replay_maxrel = host_max_many(c, c->relpart.p, nmacro, rs_reduce_blocks((long long)N),
                                         &worst);
+        unsigned long long pc_ovf = 0, pc_fail = 0;
+        if (c->rec_percell) {
+            std::vector<unsigned long long> ps(9 * (size_t)nmacro);
+            HIPCHK(hipMemcpyAsync(ps.data(), c->pcstats.p, ps.size() * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int m = 0; m < nmacro; ++m) {
+                S.nvjp_C += (long long)ps[9 * (size_t)m + 5];
+                pc_ovf += ps[9 * (size_t)m + 6];
+                pc_fail += ps[9 * (size_t)m + 4];
+            }
+        }
         for (int m = 0; m < nmacro; ++m) {
             S.nvjp_T += (long long)c->seqT[m].size();
-            S.nvjp_C += (long long)c->seqC[m].size();
+            if (!c->rec_percell) S.nvjp_C += (long long)c->seqC[m].size();
         }
         std::vector<unsigned long long> cnt(2 * (size_t)(nvjp + 1));
         HIPCHK(hipMemcpyAsync(cnt.data(), c->nfc.p, cnt.size() * sizeof(unsigned long long),
@@ -2013,6 +2195,10 @@ int rs_adjoint_run(rs_ctx* c, int nmacro, const double* tstops, double dtT0, dou
         HIPCHK(hipStreamSynchronize(c->stream));
         const bool bad = first_nonfinite(c, cnt, S);
         if (st) *st = S;
+        if (pc_ovf || pc_fail)
+            return fail(c, RS_E_STATE, "per-cell chemistry replay left its record (" +
+                                       std::to_string(pc_ovf) + " cells past the recorded step "
+                                       "count, " + std::to_string(pc_fail) + " at maxiters)");
         const std::string rf = replay_failure(S.replay_maxrel, worst);
         if (!rf.empty()) return fail(c, RS_E_STATE, rf);
         if (bad) return fail(c, RS_E_NONFINITE, nonfinite_message(S));
@@ -2026,6 +2212,10 @@ int rs_get_recorded_steps(rs_ctx* c, int macro_step, int op, double* t_dt, long 
     if (!c->have_record || macro_step < 0 || macro_step >= (int)c->seqT.size())
         return fail(c, RS_E_STATE, "no recorded macro step " + std::to_string(macro_step));
     const auto& ts = (op ? c->seqC : c->seqT)[macro_step];
+    if (op == 1 && c->rec_percell) {   // per-cell chemistry: no global step sequence
+        *n_out = 0;
+        return RS_OK;
+    }
     *n_out = (long long)ts.size();
     if (t_dt) {
         if (cap < (long long)ts.size()) return fail(c, RS_E_ARG, "buffer too small");
@@ -2364,6 +2554,7 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
             c->have_cand = false;
             const size_t N = c->N();
             c->ckpt.ensure((size_t)(nmacro + 1) * N);
+            c->rec_percell = false;   // the ensemble runs the global-norm chemistry
             c->seqT.assign(nmacro, {});
             c->seqC.assign(nmacro, {});
             const long long cap = std::min<long long>(o->maxiters, 1 << 22);

@@ -56,6 +56,31 @@ typedef struct RsPercellArgs {
     int clamp, posdom;      /* posdom: reject a trial state with a negative entry */
 } RsPercellArgs;
 
+/* Adjoint of the per-cell adaptive chemistry, persistent work-stealing
+ * launch: each lane integrates its cell again from u recording the accepted
+ * steps in its own slice of ws (lane stride nlanes; per recorded state j:
+ * the NS values at [j (NS+1) + s] and the step that leaves it at
+ * [j (NS+1) + NS]), then sweeps them backwards. */
+typedef struct RsPercellAdjArgs {
+    const double* u;        /* chemistry input state       */
+    double* uend;           /* end state of every cell, or 0 */
+    double* lam;            /* in/out                      */
+    double* gE;             /* emission gradient, ns*ncol, or 0 */
+    const double* K;
+    const double* jfac;
+    const double* E;
+    double* ws;             /* nmax * (ns + 1) * nlanes doubles */
+    unsigned long long* head;   /* work-queue head (zeroed before launch) */
+    unsigned long long* stats;  /* attempts, accepts, rejects, max attempts, failed cells,
+                                   step VJPs, record overflows, non-finite lam outputs,
+                                   non-finite emission-gradient contributions (zeroed) */
+    unsigned long long* nfc;    /* or 0: [1] += non-finite emission-gradient contributions */
+    long long nc, ncol, maxiters, nmax, nlanes;
+    double t0, t1, dt0, atol, rtol;
+    double beta1, beta2, invqmin, invqmax, gamma, qsmin, qsmax, qoldinit;
+    int clamp, posdom;
+} RsPercellAdjArgs;
+
 /* One ROS23 step VJP over all cells (stages recomputed from u). */
 typedef struct RsChemVjpArgs {
     const double* u;        /* step input state            */

@@ -176,6 +176,9 @@ ABI = {
     "rs_get_step_log": [_P, C.c_int, _D, C.c_longlong, C.POINTER(C.c_longlong)],
     "rs_chem_percell": [_P, C.c_double, C.c_double, C.c_double, _D, C.POINTER(PICtrl),
                         C.c_double, C.c_double, C.c_int, C.c_longlong, C.POINTER(PercellStats)],
+    "rs_chem_percell_vjp": [_P, C.c_double, C.c_double, C.c_double, C.POINTER(PICtrl),
+                            C.c_double, C.c_double, C.c_int, C.c_longlong,
+                            C.POINTER(PercellStats)],
     "rs_macro_step": [_P, C.c_double, C.c_double, _D, _D, C.POINTER(SplitOpts),
                       C.POINTER(Counts), C.POINTER(Counts)],
     "rs_set_lambda": [_P, _D],
@@ -460,6 +463,16 @@ class Context:
         dc = None if dt_cells is None else np.ascontiguousarray(dt_cells, np.float64)
         self._chk(self.lib.rs_chem_percell(self.h, t0, t1, dt0, _dp(dc), C.byref(ctrl), atol,
                                            rtol, _flags(clamp), maxiters, C.byref(st)))
+        return st
+
+    def chem_percell_vjp(self, t0, t1, dt0, ctrl=None, atol=1e-6, rtol=1e-3, clamp=True,
+                         maxiters=20000):
+        """lambda <- VJP of the per-cell chemistry over [t0, t1] from the
+        current state (unchanged); the emission gradient accumulates."""
+        st = PercellStats()
+        ctrl = pictrl_ros23() if ctrl is None else ctrl
+        self._chk(self.lib.rs_chem_percell_vjp(self.h, t0, t1, dt0, C.byref(ctrl), atol, rtol,
+                                               _flags(clamp), maxiters, C.byref(st)))
         return st
 
     def macro_step(self, t0, t1, dtT, dtC, opts: SplitOpts):
