@@ -171,7 +171,8 @@ RS_API int rs_set_ppm_geometry(rs_ctx* ctx, int lon_periodic, double dlon,
  *      periodic longitude Mx[..NLON] must equal Mx[..0]);
  *   My[(k*(NLAT+1) + j)*NLON + i], j = 0..NLAT (south face of row j);
  *   Mz[(k*NLAT + j)*NLON + i],     k = 0..NLEV (lower face of level k, +ve
- *      upward; zero at k = 0 and k = NLEV: no flux through surface and lid);
+ *      upward; zero at k = 0 and k = NLEV: no flux through surface and lid;
+ *      NULL: zero everywhere until rs_diagnose_mz diagnoses it);
  *   dp[(k*NLAT + j)*NLON + i] > 0, the widths of the vertical parabolae
  *      (reseact.esm:178-197 Mx, My; :225-234 dp; :282-291 Mz). */
 RS_API int rs_set_mass_fluxes(rs_ctx* ctx, const double* Mx, const double* My,
@@ -196,6 +197,20 @@ RS_API int rs_set_inflow(rs_ctx* ctx, const double* qbc_w, const double* qbc_e,
  * NULL switches it off.  Recomputed whenever the fluxes or the geometry are
  * set again; call after both. */
 RS_API int rs_set_pressure_fixer(rs_ctx* ctx, const double* dPSdt);
+
+/* Vertical air-mass flux diagnosed from continuity, replacing Mz = -OMEGA
+ * (reseact.esm:262-291: divh, divh_col, dp_col, divh_fix, Mz), from the
+ * horizontal fluxes and dp of rs_set_mass_fluxes and the geometry:
+ *   divh_fix(k) = divh(k) + (-dPSdt - divh_col) dp(k) / dp_col,
+ *   Mz[k] = -sum_{k' < k} (dB[k'] dPSdt + divh_fix(k'))   (exclusive scan),
+ * sums in level order.  The scan reaches dPSdt (1 - sum dB) at the lid up to
+ * round-off; the lid face is set to 0 and the largest |scan at the lid| over
+ * the columns is returned in *lid_resid (may be NULL).  dB[NLEV] (the
+ * hybrid B differences) and dPSdt[NLAT*NLON] (Pa/s, the pressure fixer's)
+ * may be NULL (zeros).  The result becomes the context's Mz; Mz_out (may be
+ * NULL) receives a copy in the layout of rs_set_mass_fluxes. */
+RS_API int rs_diagnose_mz(rs_ctx* ctx, const double* dB, const double* dPSdt, double* Mz_out,
+                          double* lid_resid);
 
 /* PBL mixing of the mixing ratios (reseact.esm:1563-1690, the tracer
  * tendency's GEOS-Chem-style full PBL mixing; weights :7536 pbl_f, :7698

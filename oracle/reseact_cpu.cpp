@@ -1741,11 +1741,13 @@ int rs_set_ppm_geometry(rs_ctx* c, int lon_periodic, double dlon, const double* 
 int rs_set_mass_fluxes(rs_ctx* c, const double* Mx, const double* My, const double* Mz,
                        const double* dp) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
-    if (!Mx || !My || !Mz || !dp) return fail(c, RS_E_ARG, "null flux array");
+    if (!Mx || !My || !dp) return fail(c, RS_E_ARG, "null flux array");
     const size_t nlon = c->nlon, nlat = c->nlat, nlev = c->nlev;
     c->Mx.assign(Mx, Mx + nlev * nlat * (nlon + 1));
     c->My.assign(My, My + nlev * (nlat + 1) * nlon);
-    c->Mz.assign(Mz, Mz + (nlev + 1) * nlat * nlon);
+    // Mz NULL: no vertical flux until rs_diagnose_mz supplies it
+    if (Mz) c->Mz.assign(Mz, Mz + (nlev + 1) * nlat * nlon);
+    else c->Mz.assign((nlev + 1) * nlat * nlon, 0.0);
     c->dp.assign(dp, dp + c->nc);
     if (!all_finite(c->Mx) || !all_finite(c->My) || !all_finite(c->Mz))
         return fail(c, RS_E_NONFINITE, "mass flux");
@@ -1801,6 +1803,75 @@ int rs_set_pressure_fixer(rs_ctx* c, const double* dPSdt) {
     c->dPSdt.assign(dPSdt, dPSdt + c->ncol);
     if (!all_finite(c->dPSdt)) return fail(c, RS_E_NONFINITE, "dPSdt");
     ppm_update_source(c);
+    return RS_OK;
+}
+
+// Vertical air-mass flux diagnosed from continuity with the column pressure
+// fixer (reseact.esm:262-291, variables divh, divh_col, dp_col, divh_fix, Mz).
+// Per column, with divh(k) the horizontal divergence of the fluxes set by
+// rs_set_mass_fluxes (the expression of ppm_update_source) and divh_col,
+// dp_col their sums in level order:
+//   a = (dPSdt + divh_col) / dp_col,   divh_fix(k) = divh(k) - a dp(k)
+//     (= divh(k) + (-dPSdt - divh_col) dp(k)/dp_col),
+//   Mz[0] = 0,  Mz[k+1] = Mz[k] - (dB[k] dPSdt + divh_fix(k))
+//     (the exclusive scan Mz[ke] = -sum_{k<ke} (dB[k] dPSdt + divh_fix(k))).
+// Since sum_k divh_fix(k) = -dPSdt, the scan reaches dPSdt (1 - sum_k dB)
+// at the lid up to round-off: zero when sum_k dB = 1.  The lid face is then
+// set to 0 (the no-flux wall the top face stencil assumes) and the largest
+// |scan value at the lid| over the columns goes to lid_resid.  dB (NLEV) and
+// dPSdt (NCOL) may be NULL (zeros).
+int rs_diagnose_mz(rs_ctx* c, const double* dB, const double* dPSdt, double* Mz_out,
+                   double* lid_resid) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!c->have_ppm_geom || !c->have_ppm_flux)
+        return fail(c, RS_E_STATE, "set the PPM geometry and mass fluxes first");
+    const int nlon = c->nlon, nlat = c->nlat, nlev = c->nlev;
+    const size_t plane = (size_t)nlon * nlat;
+    if (dB)
+        for (int k = 0; k < nlev; ++k)
+            if (!std::isfinite(dB[k])) return fail(c, RS_E_NONFINITE, "dB");
+    if (dPSdt)
+        for (size_t x = 0; x < plane; ++x)
+            if (!std::isfinite(dPSdt[x])) return fail(c, RS_E_NONFINITE, "dPSdt");
+    double worst = 0.0;
+    for (int j = 0; j < nlat; ++j) {
+        const double gx = c->dphi_lat[j] / (c->dlon * c->dS_lat[j]);
+        const double rdS = 1.0 / c->dS_lat[j];
+        const double ces = c->coslat_e[j], cen = c->coslat_e[j + 1];
+        for (int i = 0; i < nlon; ++i) {
+            const size_t col = (size_t)j * nlon + i;
+            auto divh = [&](int k) {
+                const double* mxr = c->Mx.data() + ((size_t)k * nlat + j) * (nlon + 1);
+                const double ms = c->My[((size_t)k * (nlat + 1) + j) * nlon + i];
+                const double mn = c->My[((size_t)k * (nlat + 1) + j + 1) * nlon + i];
+                const double dx = (mxr[i + 1] - mxr[i]) * gx;
+                const double dy = (cen * mn - ces * ms) * rdS;
+                return dx + dy;
+            };
+            double dcol = 0.0, pcol = 0.0;
+            for (int k = 0; k < nlev; ++k) {
+                dcol += divh(k);
+                pcol += c->dp[(size_t)k * plane + col];
+            }
+            const double ps = dPSdt ? dPSdt[col] : 0.0;
+            const double a = (ps + dcol) / pcol;
+            double mz = 0.0;
+            c->Mz[col] = 0.0;
+            for (int k = 0; k < nlev; ++k) {
+                const double fix = divh(k) - a * c->dp[(size_t)k * plane + col];
+                const double db = dB ? dB[k] : 0.0;
+                const double t = db * ps + fix;
+                mz = mz - t;
+                if (k + 1 < nlev) c->Mz[(size_t)(k + 1) * plane + col] = mz;
+            }
+            c->Mz[(size_t)nlev * plane + col] = 0.0;
+            // NaN-propagating max of |lid value|
+            const double r = std::fabs(mz);
+            if (!(r <= worst)) worst = r;
+        }
+    }
+    if (Mz_out) std::copy(c->Mz.begin(), c->Mz.end(), Mz_out);
+    if (lid_resid) *lid_resid = worst;
     return RS_OK;
 }
 

@@ -448,6 +448,56 @@ __global__ void __launch_bounds__(PB) rs_ppm_source(const double* __restrict__ M
     for (int k = 0; k < nlev; ++k) msrc[(long long)k * plane + col] = a * dp[(long long)k * plane + col];
 }
 
+// Vertical air-mass flux from continuity with the column pressure fixer
+// (reseact.esm:262-291), one lane per column: divh(k) as in rs_ppm_source,
+// a = (dPSdt + divh_col) / dp_col, divh_fix(k) = divh(k) - a dp(k), and the
+// exclusive scan Mz[k+1] = Mz[k] - (dB[k] dPSdt + divh_fix(k)); the surface
+// and lid faces are written 0 and the scan's value at the lid goes to
+// lid[col].  dB and dPSdt may be null (zeros).
+__global__ void __launch_bounds__(PB) rs_ppm_diag_mz(const double* __restrict__ Mx,
+                                                     const double* __restrict__ My,
+                                                     const double* __restrict__ dp,
+                                                     const double* __restrict__ gxa,
+                                                     const double* __restrict__ ce,
+                                                     const double* __restrict__ rdSa,
+                                                     const double* __restrict__ dB,
+                                                     const double* __restrict__ dPSdt,
+                                                     double* __restrict__ Mz,
+                                                     double* __restrict__ lid, int nlon,
+                                                     int nlat, int nlev) {
+    const long long plane = (long long)nlon * nlat;
+    const long long col = (long long)blockIdx.x * PB + threadIdx.x;
+    if (col >= plane) return;
+    const int i = (int)(col % nlon), j = (int)(col / nlon);
+    const double gx = gxa[j], rdS = rdSa[j], ces = ce[j], cen = ce[j + 1];
+    auto divh = [&](int k) {
+        const double* mxr = Mx + ((long long)k * nlat + j) * (nlon + 1);
+        const double ms = My[((long long)k * (nlat + 1) + j) * nlon + i];
+        const double mn = My[((long long)k * (nlat + 1) + j + 1) * nlon + i];
+        const double dx = (mxr[i + 1] - mxr[i]) * gx;
+        const double dy = (cen * mn - ces * ms) * rdS;
+        return dx + dy;
+    };
+    double dcol = 0.0, pcol = 0.0;
+    for (int k = 0; k < nlev; ++k) {
+        dcol += divh(k);
+        pcol += dp[(long long)k * plane + col];
+    }
+    const double ps = dPSdt ? dPSdt[col] : 0.0;
+    const double a = (ps + dcol) / pcol;
+    double mz = 0.0;
+    Mz[col] = 0.0;
+    for (int k = 0; k < nlev; ++k) {
+        const double fix = divh(k) - a * dp[(long long)k * plane + col];
+        const double db = dB ? dB[k] : 0.0;
+        const double t = db * ps + fix;
+        mz = mz - t;
+        if (k + 1 < nlev) Mz[(long long)(k + 1) * plane + col] = mz;
+    }
+    Mz[(long long)nlev * plane + col] = 0.0;
+    lid[col] = mz;
+}
+
 // ---- adjoint of the tendency ------------------------------------------------
 // Derivatives on the branch the forward takes at the linearisation state (the
 // limiters switch on products of neighbour differences, so the tendency is
@@ -1132,6 +1182,16 @@ hipError_t rs_launch_ppm_source(const double* Mx, const double* My, const double
     const long long n = (long long)nlon * nlat;
     hipLaunchKernelGGL(rs_ppm_source, dim3((unsigned)((n + PB - 1) / PB)), dim3(PB), 0, st, Mx, My,
                        dp, gx, ce, rdS, dPSdt, msrc, nlon, nlat, nlev);
+    return hipGetLastError();
+}
+
+hipError_t rs_launch_ppm_diag_mz(const double* Mx, const double* My, const double* dp,
+                                 const double* gx, const double* ce, const double* rdS,
+                                 const double* dB, const double* dPSdt, double* Mz, double* lid,
+                                 int nlon, int nlat, int nlev, hipStream_t st) {
+    const long long n = (long long)nlon * nlat;
+    hipLaunchKernelGGL(rs_ppm_diag_mz, dim3((unsigned)((n + PB - 1) / PB)), dim3(PB), 0, st, Mx,
+                       My, dp, gx, ce, rdS, dB, dPSdt, Mz, lid, nlon, nlat, nlev);
     return hipGetLastError();
 }
 

@@ -77,6 +77,10 @@ hipError_t rs_launch_ppm_source(const double* Mx, const double* My, const double
                                 const double* gx, const double* ce, const double* rdS,
                                 const double* dPSdt, double* msrc, int nlon, int nlat, int nlev,
                                 hipStream_t st);
+hipError_t rs_launch_ppm_diag_mz(const double* Mx, const double* My, const double* dp,
+                                 const double* gx, const double* ce, const double* rdS,
+                                 const double* dB, const double* dPSdt, double* Mz, double* lid,
+                                 int nlon, int nlat, int nlev, hipStream_t st);
 
 // embedded sources of the runtime-compiled chemistry kernels
 #include "jit_sources.inc"
@@ -1605,6 +1609,44 @@ int rs_set_pressure_fixer(rs_ctx* c, const double* dPSdt) {
     });
 }
 
+int rs_diagnose_mz(rs_ctx* c, const double* dB, const double* dPSdt, double* Mz_out,
+                   double* lid_resid) {
+    if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
+    if (!c->have_ppm_geom || !c->have_ppm_flux)
+        return fail(c, RS_E_STATE, "set the PPM geometry and mass fluxes first");
+    const int nlev = c->nlev;
+    const long long ncol = c->ncol;
+    if (dB)
+        for (int k = 0; k < nlev; ++k)
+            if (!std::isfinite(dB[k])) return fail(c, RS_E_NONFINITE, "dB");
+    if (dPSdt)
+        for (long long x = 0; x < ncol; ++x)
+            if (!std::isfinite(dPSdt[x])) return fail(c, RS_E_NONFINITE, "dPSdt");
+    return guarded(c, [&]() {
+        DevArray<double> ddB, dps, dlid;
+        if (dB) { ddB.resize(nlev); h2d(ddB.p, dB, nlev, c->stream); }
+        if (dPSdt) { dps.resize(ncol); h2d(dps.p, dPSdt, ncol, c->stream); }
+        dlid.resize(ncol);
+        HIPCHK(rs_launch_ppm_diag_mz(c->pMx.p, c->pMy.p, c->pdp.p, c->pgx.p, c->pce.p, c->prdS.p,
+                                     dB ? ddB.p : nullptr, dPSdt ? dps.p : nullptr, c->pMz.p,
+                                     dlid.p, c->nlon, c->nlat, nlev, c->stream));
+        std::vector<double> lid((size_t)ncol);
+        HIPCHK(hipMemcpyAsync(lid.data(), dlid.p, sizeof(double) * ncol, hipMemcpyDeviceToHost,
+                              c->stream));
+        if (Mz_out)
+            HIPCHK(hipMemcpyAsync(Mz_out, c->pMz.p, sizeof(double) * (size_t)(nlev + 1) * ncol,
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double worst = 0.0;
+        for (double v : lid) {
+            const double r = std::fabs(v);
+            if (!(r <= worst)) worst = r;      // NaN-propagating
+        }
+        if (lid_resid) *lid_resid = worst;
+        return RS_OK;
+    });
+}
+
 int rs_set_ppm_geometry(rs_ctx* c, int lon_periodic, double dlon, const double* coslat_e,
                         const double* dS_lat, const double* dphi_lat) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
@@ -1638,23 +1680,27 @@ int rs_set_ppm_geometry(rs_ctx* c, int lon_periodic, double dlon, const double* 
 int rs_set_mass_fluxes(rs_ctx* c, const double* Mx, const double* My, const double* Mz,
                        const double* dp) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
-    if (!Mx || !My || !Mz || !dp) return fail(c, RS_E_ARG, "null flux array");
+    if (!Mx || !My || !dp) return fail(c, RS_E_ARG, "null flux array");
     const size_t nlon = c->nlon, nlat = c->nlat, nlev = c->nlev, ncol = c->ncol;
     const size_t nx = nlev * nlat * (nlon + 1), ny = nlev * (nlat + 1) * nlon, nz = (nlev + 1) * ncol;
     for (size_t x = 0; x < nx; ++x) if (!std::isfinite(Mx[x])) return fail(c, RS_E_NONFINITE, "mass flux");
     for (size_t x = 0; x < ny; ++x) if (!std::isfinite(My[x])) return fail(c, RS_E_NONFINITE, "mass flux");
-    for (size_t x = 0; x < nz; ++x) if (!std::isfinite(Mz[x])) return fail(c, RS_E_NONFINITE, "mass flux");
+    if (Mz)
+        for (size_t x = 0; x < nz; ++x) if (!std::isfinite(Mz[x])) return fail(c, RS_E_NONFINITE, "mass flux");
     for (size_t x = 0; x < (size_t)c->nc; ++x)
         if (!(dp[x] > 0.0) || !std::isfinite(dp[x])) return fail(c, RS_E_ARG, "dp must be positive");
-    for (size_t x = 0; x < ncol; ++x)
-        if (Mz[x] != 0.0 || Mz[nlev * ncol + x] != 0.0)
-            return fail(c, RS_E_ARG, "Mz must vanish at the surface and the lid");
+    if (Mz)
+        for (size_t x = 0; x < ncol; ++x)
+            if (Mz[x] != 0.0 || Mz[nlev * ncol + x] != 0.0)
+                return fail(c, RS_E_ARG, "Mz must vanish at the surface and the lid");
     return guarded(c, [&]() {
         c->pMx.resize(nx); c->pMy.resize(ny); c->pMz.resize(nz); c->pdp.resize(c->nc);
         c->pvs.resize(2 * (nlev + 4) * ncol); c->pve.resize(3 * (nlev + 3) * ncol);
         h2d(c->pMx.p, Mx, nx, c->stream);
         h2d(c->pMy.p, My, ny, c->stream);
-        h2d(c->pMz.p, Mz, nz, c->stream);
+        // Mz NULL: no vertical flux until rs_diagnose_mz supplies it
+        if (Mz) h2d(c->pMz.p, Mz, nz, c->stream);
+        else HIPCHK(rs_launch_fill(c->pMz.p, (long long)nz, 0.0, c->stream));
         h2d(c->pdp.p, dp, c->nc, c->stream);
         HIPCHK(rs_launch_ppm_coefs(c->pdp.p, c->pvs.p, c->pve.p, c->nlon, c->nlat, c->nlev,
                                    c->stream));

@@ -155,6 +155,7 @@ ABI = {
     "rs_set_mass_fluxes": [_P, _D, _D, _D, _D],
     "rs_set_inflow": [_P, _D, _D, _D, _D],
     "rs_set_pressure_fixer": [_P, _D],
+    "rs_diagnose_mz": [_P, _D, _D, _D, _D],
     "rs_set_pbl_mixing": [_P, _D, C.c_double],
     "rs_set_mechanism": [_P, C.c_int, _I, _I, _D, _I],
     "rs_set_mechanism_stoich": [_P, C.c_int, _I, _I, _I, _D, _D, _I],
@@ -375,6 +376,25 @@ class Context:
         self._ppm_keep.append(fix)
         self._chk(self.lib.rs_set_pressure_fixer(self.h, _dp(fix)))
         self.set_pbl_mixing(ppm.pbl_f, ppm.tau_pbl)
+
+    def set_pressure_fixer(self, dPSdt):
+        """Column pressure fixer of the air mass (dPSdt per column; None: off)."""
+        a = None if dPSdt is None else np.ascontiguousarray(dPSdt, np.float64).reshape(-1)
+        self._fix_keep = a
+        self._chk(self.lib.rs_set_pressure_fixer(self.h, _dp(a)))
+
+    def diagnose_mz(self, dB=None, dPSdt=None):
+        """Diagnose the vertical air-mass flux from continuity with the column
+        pressure fixer (rs_diagnose_mz); it becomes the context's Mz.  Returns
+        (Mz[(nlev+1), nlat, nlon], largest |scan value at the lid|)."""
+        f = lambda a: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)
+        dB, dPSdt = f(dB), f(dPSdt)
+        g = self.grid
+        out = np.empty((g.nlev + 1) * g.nlat * g.nlon)
+        resid = C.c_double(0.0)
+        self._chk(self.lib.rs_diagnose_mz(self.h, _dp(dB), _dp(dPSdt), _dp(out),
+                                          C.byref(resid)))
+        return out.reshape(g.nlev + 1, g.nlat, g.nlon), resid.value
 
     def set_pbl_mixing(self, pbl_f, tau=900.0):
         """PBL mixing of the mixing ratios (pbl_f per cell, None: off)."""

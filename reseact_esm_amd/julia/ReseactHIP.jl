@@ -129,6 +129,18 @@ set_inflow!(c::Context, qw, qe, qs, qn) =
 set_pressure_fixer!(c::Context, dPSdt::Union{Nothing,Vector{Float64}}) =
     check(c, ccall((:rs_set_pressure_fixer, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr,
                    _ornull(dPSdt)))
+# Mz from continuity with the column pressure fixer (reseact.esm:262-291);
+# returns (Mz, largest |scan value at the lid|) and installs Mz in the context
+function diagnose_mz!(c::Context, nlon::Int, nlat::Int, nlev::Int;
+                      dB::Union{Nothing,Vector{Float64}}=nothing,
+                      dPSdt::Union{Nothing,Vector{Float64}}=nothing)
+    Mz = Vector{Float64}(undef, (nlev + 1) * nlat * nlon)
+    resid = Ref{Float64}(0.0)
+    check(c, ccall((:rs_diagnose_mz, LIB), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{Float64}),
+                   c.ptr, _ornull(dB), _ornull(dPSdt), Mz, resid))
+    return Mz, resid[]
+end
 # pbl_f: layer fraction inside the PBL per cell (nothing: off); tau_pblmix in s
 set_pbl_mixing!(c::Context, pbl_f::Union{Nothing,Vector{Float64}}, tau_pblmix::Float64=900.0) =
     check(c, ccall((:rs_set_pbl_mixing, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble), c.ptr,
