@@ -95,24 +95,23 @@ __device__ __forceinline__ void limit(double a0, double& aL, double& aR) {
     else if (-dd > da6) aR = 3.0 * a0 - 2.0 * aL;
 }
 
-// Uniform face value from v[0..5] = cells f-3 .. f+2 around face f.
+// Uniform face value from v[0..5] = cells f-3 .. f+2 around face f: the
+// limited right edge of cell f-1 for M >= 0, the limited left edge of cell f
+// otherwise.  Both cases are the same statements on the five cells around
+// the upwind cell (w = v[0..4] or v[1..5]), so the window is selected first
+// and one reconstruction serves lanes of either flux sign (no divergent
+// branches); every value rounds as in the two-branch form of the oracle.
 __device__ __forceinline__ double face_u(const double* v, double M) {
-    if (M >= 0.0) {
-        const double dm2 = slope_u(v[0], v[1], v[2]);
-        const double dm1 = slope_u(v[1], v[2], v[3]);
-        const double d0 = slope_u(v[2], v[3], v[4]);
-        double aL = edge_u(v[1], v[2], dm2, dm1);
-        double aR = edge_u(v[2], v[3], dm1, d0);
-        limit(v[2], aL, aR);
-        return aR;
-    }
-    const double dm1 = slope_u(v[1], v[2], v[3]);
-    const double d0 = slope_u(v[2], v[3], v[4]);
-    const double dp1 = slope_u(v[3], v[4], v[5]);
-    double aL = edge_u(v[2], v[3], dm1, d0);
-    double aR = edge_u(v[3], v[4], d0, dp1);
-    limit(v[3], aL, aR);
-    return aL;
+    const bool pos = M >= 0.0;
+    const double w0 = pos ? v[0] : v[1], w1 = pos ? v[1] : v[2], w2 = pos ? v[2] : v[3];
+    const double w3 = pos ? v[3] : v[4], w4 = pos ? v[4] : v[5];
+    const double s1 = slope_u(w0, w1, w2);
+    const double s2 = slope_u(w1, w2, w3);
+    const double s3 = slope_u(w2, w3, w4);
+    double aL = edge_u(w1, w2, s1, s2);
+    double aR = edge_u(w2, w3, s2, s3);
+    limit(w2, aL, aR);
+    return pos ? aR : aL;
 }
 
 __device__ __forceinline__ double div6(double fw, double fe, double fs, double fn, double fb,
