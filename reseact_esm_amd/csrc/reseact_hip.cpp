@@ -2392,6 +2392,11 @@ double time_ms(rs_ctx* c, const std::function<void()>& body, int reps) {
     return ms / reps;
 }
 
+// A candidate replaces the incumbent only when at least 3% faster, so that
+// timing noise between near-equal launch shapes does not flip the choice
+// from run to run.
+bool clearly_faster(double ms, double incumbent_ms) { return ms < 0.97 * incumbent_ms; }
+
 int autotune(rs_ctx* c, rs_tune_result* out) {
     rs_tune_result r{};
     r.transport_tile_levels = c->tk;
@@ -2413,9 +2418,15 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
         }
         c->tk = r.transport_tile_levels;
         // the fused kernel with 1024- and 512-lane workgroups (two of the
-        // smaller fit a CU, so one's barrier waits overlap the other's work)
+        // smaller fit a CU, so one's barrier waits overlap the other's work).
+        // The plan's own size is kept unless the other is clearly faster:
+        // the two sizes sum the error norm over different blocks, so a
+        // choice that flipped between runs on timing noise would change the
+        // rounding of EEst, and with it a long run's step sequence.
+        select_fused(c, 0);
+        const int defnth = c->fnth;
         int bestf = 0;
-        for (int nth : {1024, 512}) {
+        for (int nth : {defnth, defnth == 1024 ? 512 : 1024}) {
             select_fused(c, nth);
             if (!c->fplan_ok) continue;
             c->tmode = 1;
@@ -2424,7 +2435,10 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
             const double ms = time_ms(c, [&]() {
                 enqueue_ssprk43(c, nullptr, in, cand, 60.0, 1e-6, 1e-3, 0);
             }, 10);
-            if (bestf == 0 || ms < r.transport_fused_ms) { r.transport_fused_ms = ms; bestf = nth; }
+            if (bestf == 0 || clearly_faster(ms, r.transport_fused_ms)) {
+                r.transport_fused_ms = ms;
+                bestf = nth;
+            }
         }
         const bool fused = bestf != 0 && r.transport_fused_ms < best;
         c->tmode = fused ? 1 : 0;
@@ -2434,6 +2448,8 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
         r.transport_fused_threads = fused ? bestf : 0;
     }
     if (c->have_mech) {
+        // smallest block first; a larger one (another layout of the error
+        // norm's partial sums) only when clearly faster, as above
         const unsigned bl[3] = {64, 128, 256};
         double best = 1e300;
         for (int q = 0; q < 3; ++q) {
@@ -2442,7 +2458,10 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
             r.chem_ms[q] = time_ms(c, [&]() {
                 enqueue_ros23(c, nullptr, c->S[0].p, c->S[1].p, c->cur, 1.0, 1e-6, 1e-3, 0);
             }, 3);
-            if (r.chem_ms[q] < best) { best = r.chem_ms[q]; r.chem_block = (int)bl[q]; }
+            if (best == 1e300 || clearly_faster(r.chem_ms[q], best)) {
+                best = r.chem_ms[q];
+                r.chem_block = (int)bl[q];
+            }
         }
         c->chem_block = (unsigned)r.chem_block;
         // the same attempt with the rate constants through a pointer
@@ -2451,7 +2470,7 @@ int autotune(rs_ctx* c, rs_tune_result* out) {
         const double alt = time_ms(c, [&]() {
             enqueue_ros23(c, nullptr, c->S[0].p, c->S[1].p, c->cur, 1.0, 1e-6, 1e-3, 0);
         }, 3);
-        if (alt >= best) c->chem_kp = kp0;
+        if (!clearly_faster(alt, best)) c->chem_kp = kp0;
         r.chem_kpointer = c->chem_kp ? 1 : 0;
         r.chem_kp_ms = kp0 ? best : alt;
     }
