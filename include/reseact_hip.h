@@ -354,6 +354,17 @@ RS_API int rs_get_grad_emis(rs_ctx* ctx, double* g);           /* NS*NLAT*NLON  
 RS_API int rs_ssprk43_vjp(rs_ctx* ctx, double t, double dt);
 RS_API int rs_ros23_vjp(rs_ctx* ctx, double t, double dt);
 
+/* PPM transport: keep != 0 makes rs_ssprk43_step also keep its stage states
+ * and the tendencies at them (7 transported blocks of device memory, already
+ * held as scratch by the step VJP), so that a following rs_ssprk43_vjp of
+ * the same step -- same state, same dt, with nothing but lambda/gradient
+ * accessors or reads in between -- skips recomputing them: the primal values
+ * a pullback holds on to (the reference's Enzyme ssprk43_step_vjp takes them
+ * from its forward trace, rx_traced_integrator.jl:618-623).  The step costs
+ * the extra writes; the VJP then does its four adjoint applications only.
+ * Results are bit for bit those of the recomputing VJP.  Default 0. */
+RS_API int rs_keep_transport_stages(rs_ctx* ctx, int keep);
+
 /* ---- time-loop adjoint ----------------------------------------------------
  * Forward pass over macro stops tstops[0..nmacro] with a checkpoint of the
  * state and the accepted (t, dt) sequences at every macro step, then the

@@ -2153,6 +2153,13 @@ int rs_get_grad_emis(rs_ctx* c, double* g) {
     return RS_OK;
 }
 
+// A performance hint of the device library (keep the PPM stage states of a
+// step for its VJP); the oracle always recomputes them, with the same result.
+int rs_keep_transport_stages(rs_ctx* c, int keep) {
+    (void)keep;
+    return c ? RS_OK : RS_E_ARG;
+}
+
 int rs_ssprk43_vjp(rs_ctx* c, double t, double dt) {
     (void)t;
     int rc = check_ready(c, false, true);

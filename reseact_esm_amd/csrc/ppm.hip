@@ -393,6 +393,7 @@ __global__ void __launch_bounds__(PB) rs_ppm_stage(RsPPMArgs A) {
                 sq += at * at;
                 if (A.posdom && un < 0.0) ng += 1.0;
                 po[g] = A.clamp ? dmax(un, 0.0) : un;
+                if (A.outf) A.outf[g] = fa;    // the tendency at a4, kept for a step VJP
             } else {
                 po[g] = un;
                 if (A.outf) A.outf[g] = fa;

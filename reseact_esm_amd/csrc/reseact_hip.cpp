@@ -189,6 +189,8 @@ struct rs_ctx {
     hipStream_t stream = nullptr;
     int ncu = 0;
     size_t lds_max = 65536;      // dynamic LDS a workgroup may take on this device
+    size_t devmem = 0;           // device memory, bytes
+    bool tape_stages = false;    // the replay tape keeps each PPM step's stage states
     std::vector<double> hK;      // rate constants (host copy: inline kernel argument)
     double* tape_view = nullptr; // replay tape lent by an ensemble (else `tape`)
     size_t tape_view_n = 0;
@@ -227,6 +229,12 @@ struct rs_ctx {
     DevArray<double> dtcell;
     RsCtrl* dctrl = nullptr;
     RsCtrl* hctrl = nullptr;       // pinned
+    // Version of the state and the transport operator: every entry point
+    // that may change S[cur], the operator or the transport scratch counts
+    // it up.  A PPM step that keeps its stages records the version it left.
+    unsigned long long sver = 0, kept_ver = ~0ull;
+    bool keep_stages = false;
+    double kept_dt = 0.0;
     long long* hstats = nullptr;   // pinned, 8
     double* hred = nullptr;        // pinned, 1024
     bool has_jfac = false, has_E = false;
@@ -515,8 +523,26 @@ RsPPMArgs ppm_args(rs_ctx* c, double dt) {
     return a;
 }
 
+// Stage states of one PPM step as its VJP reads them: u1, u2, u4 = a4 and
+// the tendencies at u, u1, u2, u4 (F: four transported blocks).  The
+// context's own set is X, Y, Z and the F slots; the time-loop adjoint keeps
+// one set per replayed step in the tape.
+struct PPMStages {
+    double *u1, *u2, *u4, *F;
+};
+
+PPMStages ppm_ctx_stages(rs_ctx* c) { return PPMStages{c->X.p, c->Y.p, c->pZ.p, c->pF.p}; }
+
+// keep (not null): also write the stage states and tendencies into *keep
+// (u3 goes to LA), so that a VJP of this step at the same state reuses them.
 void enqueue_ssprk43_ppm(rs_ctx* c, const RsCtrl* ctrl, const double* uin, double* uout,
-                         double dt, double atol, double rtol, int flags) {
+                         double dt, double atol, double rtol, int flags,
+                         const PPMStages* keep = nullptr) {
+    const size_t NTC = (size_t)c->nt * c->nc;
+    auto F = [&](int q) { return keep ? keep->F + (size_t)q * NTC : nullptr; };
+    double* s1 = keep ? keep->u1 : c->X.p;
+    double* s2 = keep ? keep->u2 : c->Y.p;
+    double* s4 = keep ? keep->u4 : c->pZ.p;
     RsPPMArgs a = ppm_args(c, dt);
     a.ctrl = ctrl;
     a.posdom = posdom_of(flags);
@@ -527,30 +553,35 @@ void enqueue_ssprk43_ppm(rs_ctx* c, const RsCtrl* ctrl, const double* uin, doubl
     a.a_is_state = st ? 1 : 0;
     a.a = uin;
     a.out_is_state = 0;
-    a.out = c->X.p;
+    a.out = s1;
     a.clamp = 0;
+    a.outf = F(0);
     { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
     a.a_is_state = 0;
-    a.a = c->X.p;
-    a.out = c->Y.p;
+    a.a = s1;
+    a.out = s2;
+    a.outf = F(1);
     { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(0, &a, c->stream)); }
-    // third stage: u3 into X and a4 = 2/3 u + 1/3 u3 into Z
-    a.a = c->Y.p;
-    a.out = c->X.p;
+    // third stage: u3 into X (LA when keeping u1) and a4 = 2/3 u + 1/3 u3
+    double* u3 = keep ? c->LA.p : c->X.p;
+    a.a = s2;
+    a.out = u3;
     a.b_is_state = st ? 1 : 0;
     a.b = uin;
-    a.out2 = c->pZ.p;
+    a.out2 = s4;
+    a.outf = F(2);
     { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(2, &a, c->stream)); }
     // last stage: stencils on a4, error from u and u3
     a.a_is_state = st ? 1 : 0;
     a.a = uin;
     a.b_is_state = 0;
-    a.b = c->X.p;
-    a.c4 = c->pZ.p;
+    a.b = u3;
+    a.c4 = s4;
     a.out2 = nullptr;
     a.out_is_state = st ? 1 : 0;
     a.out = uout;
     a.clamp = clamp_nonneg_of(flags);
+    a.outf = F(3);
     { ProfScope ps(c, P_TLAST); HIPCHK(rs_launch_ppm_stage(1, &a, c->stream)); }
 }
 
@@ -1044,12 +1075,19 @@ void ppm_vjp_scratch(rs_ctx* c) {
     c->pdivm.ensure((size_t)c->nc);
 }
 
-void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
+// kept (not null): the stages of this step at u, written by a step taken
+// with keep (sweeps only); otherwise they are recomputed into the context's
+// set.
+void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt,
+                             const PPMStages* kept = nullptr) {
     const bool sweeps = ppm_vjp_sweeps(c);
     const size_t NT = (size_t)c->nt * c->nc;
     ppm_vjp_scratch(c);
-    auto Fslot = [&](int q) { return c->pF.p + (size_t)q * NT; };
+    const PPMStages sg = kept ? *kept : ppm_ctx_stages(c);
+    const bool stages = kept == nullptr;
+    auto Fslot = [&](int q) { return sg.F + (size_t)q * NT; };
     RsPPMArgs a = ppm_args(c, dt);
+    if (stages) {
     a.a = u;
     a.out = c->X.p;
     a.outf = sweeps ? Fslot(0) : nullptr;
@@ -1064,6 +1102,7 @@ void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
     a.out2 = c->pZ.p;
     a.outf = sweeps ? Fslot(2) : nullptr;
     { ProfScope ps(c, P_TSTAGE); HIPCHK(rs_launch_ppm_stage(2, &a, c->stream)); }
+    }
     RsPPMVjpArgs v{};
     v.Mx = c->pMx.p; v.My = c->pMy.p; v.Mz = c->pMz.p;
     v.ce = c->pce.p; v.gx = c->pgx.p; v.rdS = c->prdS.p;
@@ -1076,7 +1115,7 @@ void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
     if (c->have_pbl) {
         v.pblf = c->ppblf.p; v.pblw = c->ppblw.p; v.pblWp = c->ppblWp.p; v.pbl_tau = c->pbl_tau;
     }
-    if (sweeps) {
+    if (sweeps && stages) {
         // the tendency at u4 (tendency-only stage)
         RsPPMArgs tf = ppm_args(c, dt);
         tf.a = c->pZ.p;
@@ -1091,9 +1130,9 @@ void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt) {
         if (sweeps) HIPCHK(rs_launch_ppm_vjp_sweeps(&v, Fslot(fq), c->pG.p, c->pdivm.p, c->stream));
         else HIPCHK(rs_launch_ppm_vjp(&v, c->stream));
     };
-    apply(c->pZ.p, 3, c->lam.p, 1.0, nullptr, 0.0, c->LB.p);
-    apply(c->Y.p, 2, c->LB.p, 1.0 / 3.0, nullptr, 0.0, c->LC.p);
-    apply(c->X.p, 1, c->LC.p, 1.0, nullptr, 0.0, c->LA.p);
+    apply(sg.u4, 3, c->lam.p, 1.0, nullptr, 0.0, c->LB.p);
+    apply(sg.u2, 2, c->LB.p, 1.0 / 3.0, nullptr, 0.0, c->LC.p);
+    apply(sg.u1, 1, c->LC.p, 1.0, nullptr, 0.0, c->LA.p);
     apply(u, 0, c->LA.p, 1.0, c->LB.p, 2.0 / 3.0, c->lam.p);
 }
 
@@ -1162,19 +1201,53 @@ double* tape_ptr(rs_ctx* c, size_t n) {
 // with pc_forward the per-cell solve runs from slot nT into the last slot;
 // without, the last slot is left to the per-cell adjoint (which writes the
 // cells' end states there).
+// Doubles of the tape for a macro step of nT transport and nC chemistry
+// steps: nT + nC + 1 state slots, then (stages) the stage set of every PPM
+// transport step: u1, u2, u4 and the four tendencies, 7 transported blocks.
+size_t tape_doubles(rs_ctx* c, size_t nT, size_t nC, bool stages) {
+    size_t n = (nT + nC + 1) * c->N();
+    if (stages) n += nT * 7 * (size_t)c->nt * (size_t)c->nc;
+    return n;
+}
+
+// Stage set of PPM transport step j in a tape of nslots state slots.
+PPMStages tape_stage_set(rs_ctx* c, double* tp, size_t nslots, size_t j) {
+    const size_t NTC = (size_t)c->nt * (size_t)c->nc;
+    double* b = tp + nslots * c->N() + j * 7 * NTC;
+    return PPMStages{b, b + NTC, b + 2 * NTC, b + 3 * NTC};
+}
+
+// Whether the backward sweeps keep the replayed PPM steps' stage states (their
+// VJPs then skip the recompute): PPM transport with the sweep adjoint, and the
+// stage sets of the longest recorded macro step within a quarter of the
+// device memory.  Decided where the tape is sized.
+bool want_tape_stages(rs_ctx* c, int nmacro) {
+    if (!(c->nt > 0 && trans_ppm(c) && ppm_vjp_sweeps(c))) return false;
+    size_t mx = 0;
+    for (int m = 0; m < nmacro; ++m) mx = std::max(mx, c->seqT[m].size());
+    const double bytes = (double)mx * 7.0 * (double)c->nt * (double)c->nc * sizeof(double);
+    return bytes <= 0.25 * (double)c->devmem;
+}
+
 size_t replay_into_tape(rs_ctx* c, const double* u0, const std::vector<std::pair<double, double>>& sT,
                         const std::vector<std::pair<double, double>>& sC, const rs_split_opts& o,
-                        bool pc_forward = true) {
+                        bool pc_forward = true, bool stages = false) {
     const size_t N = c->N();
     const size_t nslots = sT.size() + sC.size() + 1;
-    double* tp = tape_ptr(c, nslots * N);
+    double* tp = tape_ptr(c, tape_doubles(c, sT.size(), sC.size(), stages));
     d2d(tp, u0, N, c->stream);
     size_t j = 0;
     for (auto& p : sT) {
         double* in = tp + j * N;
         double* outp = tp + (j + 1) * N;
-        enqueue_ssprk43(c, nullptr, in, outp, p.second, o.atol_T, o.rtol_T,
-                        clamp_nonneg_of(o.clamp_nonneg));
+        if (stages) {
+            const PPMStages sg = tape_stage_set(c, tp, nslots, j);
+            enqueue_ssprk43_ppm(c, nullptr, in, outp, p.second, o.atol_T, o.rtol_T,
+                                clamp_nonneg_of(o.clamp_nonneg), &sg);
+        } else {
+            enqueue_ssprk43(c, nullptr, in, outp, p.second, o.atol_T, o.rtol_T,
+                            clamp_nonneg_of(o.clamp_nonneg));
+        }
         if (c->nt < c->ns) {
             const size_t off = (size_t)c->nt * c->nc;
             d2d(outp + off, in + off, (size_t)(c->ns - c->nt) * c->nc, c->stream);
@@ -1200,18 +1273,17 @@ size_t replay_into_tape(rs_ctx* c, const double* u0, const std::vector<std::pair
 
 // Size the replay tape once for the longest recorded macro step (before any
 // graph capture: no allocation may happen while a stream is captured).
-void size_tape(rs_ctx* c, int nmacro) {
-    size_t maxslots = 1;
+size_t tape_bytes(rs_ctx* c, int nmacro) {
+    c->tape_stages = want_tape_stages(c, nmacro);
+    size_t mx = c->N();
     for (int m = 0; m < nmacro; ++m)
-        maxslots = std::max(maxslots, c->seqT[m].size() + c->seqC[m].size() + 1);
-    if (!c->tape_view) c->tape.ensure(maxslots * c->N());
+        mx = std::max(mx, tape_doubles(c, c->seqT[m].size(), c->seqC[m].size(), c->tape_stages));
+    return mx * sizeof(double);
 }
 
-size_t tape_bytes(rs_ctx* c, int nmacro) {
-    size_t maxslots = 1;
-    for (int m = 0; m < nmacro; ++m)
-        maxslots = std::max(maxslots, c->seqT[m].size() + c->seqC[m].size() + 1);
-    return maxslots * c->N() * sizeof(double);
+void size_tape(rs_ctx* c, int nmacro) {
+    const size_t n = tape_bytes(c, nmacro) / sizeof(double);
+    if (!c->tape_view) c->tape.ensure(n);
 }
 
 // lambda = dJ/du(t_end) from the final checkpoint: 2 u[species] on level-0
@@ -1305,7 +1377,8 @@ void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* re
     const auto& sT = c->seqT[m];
     const auto& sC = c->seqC[m];
     if (ev0) HIPCHK(hipEventRecord(ev0, c->stream));
-    const size_t ns = replay_into_tape(c, c->ckpt.p + (size_t)m * N, sT, sC, o, false);
+    const size_t ns = replay_into_tape(c, c->ckpt.p + (size_t)m * N, sT, sC, o, false,
+                                       c->tape_stages);
     double* tp = tape_ptr(c, ns * N);
     const size_t nT = sT.size();
     if (c->rec_percell) {
@@ -1336,8 +1409,12 @@ void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* re
             ProfScope ps(c, P_UTIL);
             HIPCHK(rs_launch_mask(c->lam.p, tp + (q + 1) * N, (long long)N, c->stream));
         }
-        if (trans_ppm(c)) enqueue_ssprk43_vjp_ppm(c, tp + q * N, sT[q].second);
-        else enqueue_ssprk43_vjp(c, sT[q].second);
+        if (trans_ppm(c)) {
+            const PPMStages sg = tape_stage_set(c, tp, ns, q);
+            enqueue_ssprk43_vjp_ppm(c, tp + q * N, sT[q].second, c->tape_stages ? &sg : nullptr);
+        } else {
+            enqueue_ssprk43_vjp(c, sT[q].second);
+        }
         ProfScope ps(c, P_UTIL);
         HIPCHK(rs_launch_nonfinite(c->lam.p, (long long)N, slot, c->stream));
     }
@@ -1372,7 +1449,16 @@ std::string replay_failure(double d, int m) {
 }  // namespace
 
 namespace {
+int guarded_keep(rs_ctx* c, const std::function<int()>& f);
+
+// Every entry point runs through guarded (or guarded_keep, for the ones that
+// leave the state, the transport operator and its scratch alone).
 int guarded(rs_ctx* c, const std::function<int()>& f) {
+    if (c) ++c->sver;
+    return guarded_keep(c, f);
+}
+
+int guarded_keep(rs_ctx* c, const std::function<int()>& f) {
     try {
         return f();
     } catch (const HipError& e) {
@@ -1406,6 +1492,7 @@ int rs_create(rs_ctx** out, int device) {
         HIPCHK(hipGetDeviceProperties(&prop, device));
         c->ncu = prop.multiProcessorCount;
         c->lds_max = prop.sharedMemPerBlock;
+        c->devmem = prop.totalGlobalMem;
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCHK(hipMalloc((void**)&c->dctrl, sizeof(RsCtrl)));
         HIPCHK(hipHostMalloc((void**)&c->hctrl, sizeof(RsCtrl), hipHostMallocDefault));
@@ -1870,7 +1957,7 @@ int rs_set_state(rs_ctx* c, const double* u) {
 int rs_get_state(rs_ctx* c, double* u) {
     if (!c || !c->have_state) return fail(c, RS_E_STATE, "state not set");
     if (!u) return fail(c, RS_E_ARG, "null output");
-    return guarded(c, [&]() { d2h(u, c->S[c->cur].p, c->N(), c->stream); return RS_OK; });
+    return guarded_keep(c, [&]() { d2h(u, c->S[c->cur].p, c->N(), c->stream); return RS_OK; });
 }
 
 int rs_clamp_state(rs_ctx* c) {
@@ -1889,7 +1976,12 @@ int rs_ssprk43_step(rs_ctx* c, double t, double dt, double atol, double rtol, do
     return guarded(c, [&]() {
         double* in = c->S[c->cur].p;
         double* outp = c->S[c->cur ^ 1].p;
-        if (c->nt > 0) {
+        const bool keep = c->keep_stages && c->nt > 0 && trans_ppm(c) && ppm_vjp_sweeps(c);
+        if (keep) {
+            ppm_vjp_scratch(c);
+            const PPMStages sg = ppm_ctx_stages(c);
+            enqueue_ssprk43_ppm(c, nullptr, in, outp, dt, atol, rtol, 0, &sg);
+        } else if (c->nt > 0) {
             enqueue_ssprk43(c, nullptr, in, outp, dt, atol, rtol, 0);
         }
         if (c->nt < c->ns) {
@@ -1899,8 +1991,18 @@ int rs_ssprk43_step(rs_ctx* c, double t, double dt, double atol, double rtol, do
         double sse = c->nt > 0 ? host_sum(c, c->partial.p, (int)trans_nblocks(c)) : 0.0;
         if (eest) *eest = std::sqrt(sse / ((double)c->ns * (double)c->nc));
         c->have_cand = true;
+        if (keep) {
+            c->kept_ver = c->sver;
+            c->kept_dt = dt;
+        }
         return RS_OK;
     });
+}
+
+int rs_keep_transport_stages(rs_ctx* c, int keep) {
+    if (!c) return RS_E_ARG;
+    c->keep_stages = keep != 0;
+    return RS_OK;
 }
 
 int rs_ros23_step(rs_ctx* c, double t, double dt, double atol, double rtol, double* eest) {
@@ -1982,7 +2084,7 @@ int rs_ros23_step_noalloc(rs_ctx* c, const double* u_dev, double* unew_dev, long
 
 int rs_get_candidate(rs_ctx* c, double* u) {
     if (!c || !c->have_cand) return fail(c, RS_E_STATE, "no candidate");
-    return guarded(c, [&]() { d2h(u, c->S[c->cur ^ 1].p, c->N(), c->stream); return RS_OK; });
+    return guarded_keep(c, [&]() { d2h(u, c->S[c->cur ^ 1].p, c->N(), c->stream); return RS_OK; });
 }
 
 int rs_accept_candidate(rs_ctx* c, int clamp) {
@@ -2100,7 +2202,7 @@ int rs_macro_step(rs_ctx* c, double t0, double t1, double* dtT, double* dtC,
 
 int rs_set_lambda(rs_ctx* c, const double* lam) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
-    return guarded(c, [&]() {
+    return guarded_keep(c, [&]() {
         h2d(c->lam.p, lam, c->N(), c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
         return RS_OK;
@@ -2109,12 +2211,12 @@ int rs_set_lambda(rs_ctx* c, const double* lam) {
 
 int rs_get_lambda(rs_ctx* c, double* lam) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
-    return guarded(c, [&]() { d2h(lam, c->lam.p, c->N(), c->stream); return RS_OK; });
+    return guarded_keep(c, [&]() { d2h(lam, c->lam.p, c->N(), c->stream); return RS_OK; });
 }
 
 int rs_zero_grad(rs_ctx* c) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
-    return guarded(c, [&]() {
+    return guarded_keep(c, [&]() {
         HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         return RS_OK;
@@ -2123,17 +2225,32 @@ int rs_zero_grad(rs_ctx* c) {
 
 int rs_get_grad_emis(rs_ctx* c, double* g) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
-    return guarded(c, [&]() { d2h(g, c->gE.p, c->gE.n, c->stream); return RS_OK; });
+    return guarded_keep(c, [&]() { d2h(g, c->gE.p, c->gE.n, c->stream); return RS_OK; });
 }
 
 int rs_ssprk43_vjp(rs_ctx* c, double t, double dt) {
     (void)t;
     int rc = check_ready(c, false, true);
     if (rc) return rc;
-    return guarded(c, [&]() {
-        // PPM: the tendency is nonlinear; the step input is the context's state
-        if (c->nt > 0 && trans_ppm(c)) enqueue_ssprk43_vjp_ppm(c, c->S[c->cur].p, dt);
-        else if (c->nt > 0) enqueue_ssprk43_vjp(c, dt);
+    // PPM: the tendency is nonlinear, the VJP needs the stage states of the
+    // step at the context's state; a step kept with rs_keep_transport_stages
+    // at this state and dt left them in place (nothing has run since), and
+    // a recompute leaves them for the next VJP of the same step
+    const bool ppm = c->nt > 0 && trans_ppm(c);
+    const bool reuse = ppm && c->kept_ver == c->sver && c->kept_dt == dt && ppm_vjp_sweeps(c);
+    return guarded_keep(c, [&]() {
+        if (ppm) {
+            if (!reuse) ++c->sver;
+            const PPMStages sg = ppm_ctx_stages(c);
+            enqueue_ssprk43_vjp_ppm(c, c->S[c->cur].p, dt, reuse ? &sg : nullptr);
+            if (ppm_vjp_sweeps(c)) {
+                c->kept_ver = c->sver;
+                c->kept_dt = dt;
+            }
+        } else if (c->nt > 0) {
+            ++c->sver;
+            enqueue_ssprk43_vjp(c, dt);
+        }
         HIPCHK(hipStreamSynchronize(c->stream));
         return RS_OK;
     });
@@ -2143,7 +2260,7 @@ int rs_ros23_vjp(rs_ctx* c, double t, double dt) {
     (void)t;
     int rc = check_ready(c, true, false);
     if (rc) return rc;
-    return guarded(c, [&]() {
+    return guarded_keep(c, [&]() {
         enqueue_ros23_vjp(c, c->S[c->cur].p, nullptr, dt);
         HIPCHK(hipStreamSynchronize(c->stream));
         return RS_OK;
@@ -2312,7 +2429,7 @@ int rs_replay_loss(rs_ctx* c, int loss_kind, int loss_species, double* J_out) {
 
 int rs_mechanism_info(rs_ctx* c, rs_mech_info* out) {
     if (!c || !c->have_mech || !out) return fail(c, RS_E_STATE, "mechanism not set");
-    return guarded(c, [&]() {
+    return guarded_keep(c, [&]() {
         const rs::MechPlan& p = c->chem->plan;
         rs_mech_info m{};
         m.ns = p.ns;
@@ -2341,7 +2458,7 @@ int rs_mechanism_info(rs_ctx* c, rs_mech_info* out) {
 
 int rs_profile(rs_ctx* c, int enable) {
     if (!c) return RS_E_ARG;
-    return guarded(c, [&]() {
+    return guarded_keep(c, [&]() {
         if (enable && !c->prof) {
             prof_flush(c);
             for (int k = 0; k < P_COUNT; ++k) { c->prof_n[k] = 0; c->prof_ms[k] = 0; }
@@ -2354,7 +2471,7 @@ int rs_profile(rs_ctx* c, int enable) {
 
 int rs_profile_read(rs_ctx* c, rs_kernel_time* out, int max, int* n_out) {
     if (!c) return RS_E_ARG;
-    return guarded(c, [&]() {
+    return guarded_keep(c, [&]() {
         prof_flush(c);
         int n = 0;
         for (int k = 0; k < P_COUNT && n < max; ++k) {
@@ -2592,6 +2709,7 @@ int rs_ensemble_adjoint_run(rs_ensemble* e, int nmacro, const double* tstops, do
     if (!e || !o || !tstops || nmacro < 1) return ens_fail(e, RS_E_ARG, "bad arguments");
     if (o->chem_percell) return ens_fail(e, RS_E_ARG, "adjoint needs the global-norm chemistry");
     const int n = (int)e->m.size();
+    for (int d = 0; d < n; ++d) ++e->m[d]->sver;   // member states and scratch change
     for (int d = 0; d < n; ++d) {
         int rc = check_ready(e->m[d], true, true);
         if (!rc) rc = check_loss(e->m[d], loss_kind, loss_species);

@@ -187,6 +187,7 @@ ABI = {
     "rs_zero_grad": [_P],
     "rs_get_grad_emis": [_P, _D],
     "rs_ssprk43_vjp": [_P, C.c_double, C.c_double],
+    "rs_keep_transport_stages": [_P, C.c_int],
     "rs_ros23_vjp": [_P, C.c_double, C.c_double],
     "rs_adjoint_run": [_P, C.c_int, _D, C.c_double, C.c_double, C.POINTER(SplitOpts), C.c_int,
                        C.c_int, _D, _D, _D, C.POINTER(AdjointStats)],
@@ -522,6 +523,10 @@ class Context:
 
     def ssprk43_vjp(self, t, dt):
         self._chk(self.lib.rs_ssprk43_vjp(self.h, t, dt))
+
+    def keep_transport_stages(self, keep: bool):
+        """PPM: keep the stages of rs_ssprk43_step for a following step VJP."""
+        self._chk(self.lib.rs_keep_transport_stages(self.h, 1 if keep else 0))
 
     def ros23_vjp(self, t, dt):
         self._chk(self.lib.rs_ros23_vjp(self.h, t, dt))

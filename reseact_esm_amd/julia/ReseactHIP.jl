@@ -265,6 +265,31 @@ function lie_trotter_solve_bisect(c::Context, u0, tspan; macro_dt, min_macro_dt:
 end
 
 # ---- one-step adjoints (:610-623) ---------------------------------------------
+# PPM: keep the stage states of rs_ssprk43_step for the VJP of the same step
+keep_transport_stages!(c::Context, keep::Bool) =
+    check(c, ccall((:rs_keep_transport_stages, LIB), Cint, (Ptr{Cvoid}, Cint), c.ptr, keep ? 1 : 0))
+
+# A pullback of one SSPRK43 step: the forward keeps its stage states, the
+# pullback (called before anything else touches the context) reuses them.
+function ssprk43_step_with_pullback(c::Context, u, t, dt, abstol, reltol)
+    keep_transport_stages!(c, true)
+    set_state!(c, u)
+    eest = Ref{Float64}(0.0)
+    check(c, ccall((:rs_ssprk43_step, LIB), Cint,
+                   (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Ref{Float64}),
+                   c.ptr, t, dt, abstol, reltol, eest))
+    unew = similar(u)
+    check(c, ccall((:rs_get_candidate, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, unew))
+    function pullback(lambda)
+        check(c, ccall((:rs_set_lambda, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, lambda))
+        check(c, ccall((:rs_ssprk43_vjp, LIB), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), c.ptr, t, dt))
+        lam = similar(lambda)
+        check(c, ccall((:rs_get_lambda, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, lam))
+        return lam
+    end
+    return (unew, eest[]), pullback
+end
+
 function ssprk43_step_vjp(c::Context, u, t, dt, lambda)
     set_state!(c, u)
     check(c, ccall((:rs_set_lambda, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), c.ptr, lambda))

@@ -165,12 +165,66 @@ def test_ppm_adjoint_run(hip, oracle, pbl):
     if pbl:
         P.with_pbl_mixing(p, pblh_mean=800.0, pblh_amp=400.0, tau=0.05)
     res = []
+    prof = None
     for lib in (hip, oracle):
         with lib.context() as c:
             c.load(p)
+            if lib is hip:
+                c.profile(True)
             res.append(c.adjoint_run(p.tstops(), p.dt0_T, p.dt0_C, split_opts(p), loss_species=1))
+            if lib is hip:
+                prof = c.profile_read()
     (Jg, gug, _, sg), (Jo, guo, _, so) = res
     assert (sg.naccept_T, sg.nreject_T, sg.naccept_C) == (so.naccept_T, so.nreject_T, so.naccept_C)
     assert abs(Jg - Jo) <= 1e-10 * abs(Jo)
     assert _relnorm(gug, guo) <= 1e-10
     assert sg.replay_maxrel == 0.0
+    # the replay keeps every transport step's stage states in the tape, so the
+    # VJPs recompute none: every stage launch belongs to an attempt (three
+    # stages and the last; a recomputing VJP would add four stage launches)
+    n_stage = prof.get("transport_stage", (0, 0.0))[0]
+    n_last = prof.get("transport_last", (0, 0.0))[0]
+    assert sg.nvjp_T > 0 and n_last > 0 and n_stage == 3 * n_last, (n_stage, n_last, sg.nvjp_T)
+
+
+KEEP_CASES = ["regional-20x12x9", "global-48x24x10", "global-40x12x53", "global-pbl-40x12x40"]
+
+
+@pytest.mark.parametrize("name", KEEP_CASES)
+def test_kept_stages_vjp_bitwise(hip, name):
+    """rs_keep_transport_stages: the step's candidate is unchanged, the VJP that
+    reuses the kept stages equals the recomputing VJP bit for bit, a second
+    VJP of the same step reuses them again, and anything that may change the
+    state (here rs_set_state) or another dt makes the VJP recompute."""
+    print(f"kept stages {name}", flush=True)
+    p = CASES[name]()
+    dt = 30.0 if name.startswith("global") else 0.01
+    rng = np.random.default_rng(7)
+    lam0 = rng.normal(size=p.ns * p.nc)
+    out = {}
+    for keep in (False, True):
+        c = hip.context().load(p)
+        c.keep_transport_stages(keep)
+        c.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+        cand = c.get_candidate()
+        c.set_lambda(lam0)
+        c.ssprk43_vjp(0.0, dt)
+        l1 = c.get_lambda()
+        c.set_lambda(lam0)
+        c.ssprk43_vjp(0.0, dt)                    # the same step again
+        l2 = c.get_lambda()
+        c.set_lambda(lam0)
+        c.ssprk43_vjp(0.0, 0.5 * dt)              # another dt: recomputed
+        l3 = c.get_lambda()
+        c.ssprk43_step(0.0, dt, 1e-6, 1e-3)
+        c.set_state(p.u0 * 1.001)                 # state changed: recomputed
+        c.set_lambda(lam0)
+        c.ssprk43_vjp(0.0, dt)
+        l4 = c.get_lambda()
+        out[keep] = (cand, l1, l2, l3, l4)
+        c.close()
+    for a, b in zip(out[False], out[True]):
+        assert np.array_equal(a, b), float(np.max(np.abs(a - b)))
+    assert np.array_equal(out[True][1], out[True][2])
+    assert not np.array_equal(out[True][1], out[True][3])
+    assert not np.array_equal(out[True][1], out[True][4])
