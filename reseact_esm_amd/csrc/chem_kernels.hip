@@ -10,7 +10,8 @@
 
 /* One ROS23 attempt on one cell (ros23_step, rx_traced_integrator.jl:279-349,
  * mass matrix I, dT = 0).  Returns the cell's sum of squared scaled errors.
- * E (may be 0) points at this cell's emission of species 0, stride es.
+ * E (may be 0) points at this cell's NS emissions (contiguous: the device
+ * keeps the surface emissions cell-major, [ncol][NS]).
  *
  * Storage: three long-lived vectors sit in a slab sv with stride ST between
  * species (the lane's LDS slab on the device, ST = 64; a local array on the
@@ -53,12 +54,12 @@
 #define RS_SC(i) sv[(2 * RS_NS + (i)) * ST]
 template <int ST, int WS, bool KEEP_U, class Out>
 RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
-                              const double* __restrict__ E, long long es,
+                              const double* __restrict__ E,
                               double* __restrict__ sv, double* __restrict__ W, double dt,
                               double atol, double rtol, Out out) {
     /* two short-lived register vectors; the solves work in place */
     double f[RS_NS], y[RS_NS];
-    rs_rhs<ST>(K, jf, E, es, &RS_SC(0), f);                        /* f0, :282 */
+    rs_rhs<ST>(K, jf, E, &RS_SC(0), f);                        /* f0, :282 */
     RS_AFENCE(1);
     const double hg = dt * RS_ROS23_D;                             /* :291 */
     rs_jac_W<ST, WS>(K, jf, &RS_SC(0), hg, W);                     /* :287-296 */
@@ -78,7 +79,7 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) { const double du1 = dto2 * f[i]; y[i] = RS_SC(i) + du1; }
     RS_AFENCE(8);
-    rs_rhs(K, jf, E, es, y, f);                                    /* f1, :303 */
+    rs_rhs(K, jf, E, y, f);                                    /* f1, :303 */
     RS_AFENCE(8);
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) y[i] = f[i] - RS_SB(i);        /* :306-309 */
@@ -119,7 +120,7 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
         y[i] = un;
     }
     RS_AFENCE(32);
-    rs_rhs(K, jf, E, es, y, f);                                    /* f2, :318 */
+    rs_rhs(K, jf, E, y, f);                                    /* f2, :318 */
     RS_AFENCE(32);
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) {                              /* :325-326 */
@@ -152,8 +153,9 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
  * (f2, k3 and EEst are dead, rx_traced_integrator.jl:540-547):
  *   f0 = f(u); k1 = W\f0; u1 = u + h/2 k1; f1 = f(u1); k2 = W\(f1-k1) + k1;
  *   unew = u + h k2,   W = I - h*gamma*J(u).
- * Given lam = dL/dunew, returns dL/du in lam and adds dL/dE to G[i*es] (G
- * may be 0; E enters f additively on the cells that have it); returns the
+ * Given lam = dL/dunew, returns dL/du in lam and adds dL/dE to G[i*gs] (G
+ * may be 0; E, this cell's emissions as for the attempt, enters f additively
+ * on the cells that have it); returns the
  * number of non-finite values among those dL/dE contributions.  Reverse:
  *   z2 = W^-T (h lam)             = f1bar;  k1bar = h lam - z2
  *   u1bar = J(u1)^T z2;  ubar = lam + u1bar;  k1bar += h/2 u1bar
@@ -169,10 +171,10 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
 #define RS_SC(i) sv[(2 * RS_NS + (i)) * ST]
 template <int ST, int WS, class LIn, class LOut>
 RS_FN int rs_ros23_vjp(const double* __restrict__ K, double jf, const double* __restrict__ E,
-                        long long es, double* __restrict__ sv, double* __restrict__ W, double dt,
+                        long long gs, double* __restrict__ sv, double* __restrict__ W, double dt,
                         LIn lam_in, LOut lam_out, double* __restrict__ G) {
     double f[RS_NS], x[RS_NS], y[RS_NS];
-    rs_rhs<ST>(K, jf, E, es, &RS_SA(0), f);
+    rs_rhs<ST>(K, jf, E, &RS_SA(0), f);
     const double hg = dt * RS_ROS23_D;
     RS_PHASE();
     rs_jac_W<ST, WS>(K, jf, &RS_SA(0), hg, W);
@@ -187,7 +189,7 @@ RS_FN int rs_ros23_vjp(const double* __restrict__ K, double jf, const double* __
         RS_SB(i) = RS_SA(i) + du1;                                 /* u1 */
     }
     RS_PHASE();
-    rs_rhs<ST>(K, jf, E, es, &RS_SB(0), f);                        /* f1 */
+    rs_rhs<ST>(K, jf, E, &RS_SB(0), f);                        /* f1 */
     RS_PHASE();
 #pragma unroll
     for (int i = 0; i < RS_NS; ++i) f[i] = f[i] - x[i];
@@ -222,7 +224,7 @@ RS_FN int rs_ros23_vjp(const double* __restrict__ K, double jf, const double* __
         for (int i = 0; i < RS_NS; ++i) {
             const double g = x[i] + y[i];
             nbad += !RS_ISFINITE(g);
-            G[i * es] += g;
+            G[i * gs] += g;
         }
     }
     return nbad;
@@ -328,10 +330,10 @@ typedef struct { double k[RS_NR]; } RsKTab;
             _Pragma("unroll") for (int s = 0; s < RS_NS; ++s)                               \
                 sv[(2 * RS_NS + s) * RS_SLAB_ST] = src[(long long)s * a.nc + cell];         \
             const double jf = a.jfac ? a.jfac[cell] : 1.0;                                  \
-            const double* Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;               \
+            const double* Ec = (a.E && cell < a.ncol) ? a.E + cell * RS_NS : nullptr;       \
             const int clamp = a.clamp;                                                      \
             sse = rs_ros23_attempt<RS_SLAB_ST, RS_W_ST, false>(                             \
-                KEXPR, jf, Ec, a.ncol, sv, W, dt, a.atol, a.rtol, [&](int s, double un) {   \
+                KEXPR, jf, Ec, sv, W, dt, a.atol, a.rtol, [&](int s, double un) {           \
                     if (POSDOM && un < 0.0) neg += 1.0;                                     \
                     dst[(long long)s * a.nc + cell] = clamp ? rs_max(un, 0.0) : un;         \
                 });                                                                         \
@@ -387,7 +389,7 @@ rs_chem_percell(RsPercellArgs a RS_KPARAM) {
 #pragma unroll
                     for (int s = 0; s < RS_NS; ++s) RS_U(s) = a.u[(long long)s * a.nc + cell];
                     jf = a.jfac ? a.jfac[cell] : 1.0;
-                    Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;
+                    Ec = (a.E && cell < a.ncol) ? a.E + cell * RS_NS : nullptr;
                     t = a.t0;
                     dt = a.dtcell ? a.dtcell[cell] : a.dt0;
                     qold = a.qoldinit;
@@ -404,7 +406,7 @@ rs_chem_percell(RsPercellArgs a RS_KPARAM) {
                 double un[RS_NS];
 #define RS_UN(s) un[s]
                 const double sse = rs_ros23_attempt<RS_SLAB_ST, RS_W_ST, true>(
-                    RS_KPTR(a), jf, Ec, a.ncol, sv, W, dtc, a.atol, a.rtol,
+                    RS_KPTR(a), jf, Ec, sv, W, dtc, a.atol, a.rtol,
                     [&](int s, double v) { RS_UN(s) = v; });
                 const double eest = RS_SQRT(sse / (double)RS_NS);
                 bool isout = false;
@@ -510,7 +512,7 @@ rs_chem_percell_adj(RsPercellAdjArgs a RS_KPARAM) {
                         RS_REC(0, s) = v;
                     }
                     jf = a.jfac ? a.jfac[cell] : 1.0;
-                    Ec = (a.E && cell < a.ncol) ? a.E + cell : nullptr;
+                    Ec = (a.E && cell < a.ncol) ? a.E + cell * RS_NS : nullptr;
                     Gc = (a.gE && cell < a.ncol) ? a.gE + cell : nullptr;
                     t = a.t0;
                     dt = a.dt0;
@@ -529,7 +531,7 @@ rs_chem_percell_adj(RsPercellAdjArgs a RS_KPARAM) {
                 const double dtc = (dt < a.t1 - t) ? dt : a.t1 - t;
                 double un[RS_NS];
                 const double sse = rs_ros23_attempt<RS_SLAB_ST, RS_W_ST, true>(
-                    RS_KPTR(a), jf, Ec, a.ncol, sv, W, dtc, a.atol, a.rtol,
+                    RS_KPTR(a), jf, Ec, sv, W, dtc, a.atol, a.rtol,
                     [&](int s, double v) { un[s] = v; });
                 const double eest = RS_SQRT(sse / (double)RS_NS);
                 bool isout = false;
@@ -643,7 +645,7 @@ rs_chem_vjp(RsChemVjpArgs a RS_KPARAM) {
     for (int s = 0; s < RS_NS; ++s) sv[s * RS_SLAB_ST] = a.u[(long long)s * a.nc + cell];
     const double jf = a.jfac ? a.jfac[cell] : 1.0;
     const bool surf = cell < a.ncol;
-    const double* Ec = (surf && a.E) ? a.E + cell : nullptr;
+    const double* Ec = (surf && a.E) ? a.E + cell * RS_NS : nullptr;
     double* Gc = (surf && a.gE) ? a.gE + cell : nullptr;
     unsigned nlam = 0;
     const int ngrad = rs_ros23_vjp<RS_SLAB_ST, RS_W_ST>(

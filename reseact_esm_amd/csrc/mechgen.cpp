@@ -206,8 +206,13 @@ std::string generate_mech_source(const MechSpec& m, const MechPlan& p) {
     // ---- RHS: rates folded into the accumulation, reaction order per species
     e << "template <int US = 1>\n";
     e << "RS_FN void rs_rhs(const double* __restrict__ K, double jf, const double* __restrict__ E,"
-         " long long es, const double* __restrict__ u, double* __restrict__ f) {\n";
-    for (int i = 0; i < n; ++i) e << "  f[" << i << "] = E ? E[" << i << " * es] : 0.0;\n";
+         " const double* __restrict__ u, double* __restrict__ f) {\n";
+    // emissions of this cell (E: its NS values, contiguous), one branch for all
+    e << "  if (E) {\n";
+    for (int i = 0; i < n; ++i) e << "    f[" << i << "] = E[" << i << "];\n";
+    e << "  } else {\n";
+    for (int i = 0; i < n; ++i) e << "    f[" << i << "] = 0.0;\n";
+    e << "  }\n";
     for (int r = 0; r < m.nr; ++r) {
         e << "  { const double rt = ";
         if (m.rb[r] < 0) e << kexpr(m, r) << " * u[" << m.ra[r] << " * US];";

@@ -1934,8 +1934,14 @@ int rs_set_emissions(rs_ctx* c, const double* e) {
     if (!c || c->nc == 0) return fail(c, RS_E_STATE, "grid not set");
     return guarded(c, [&]() {
         if (!e) { c->has_E = false; return RS_OK; }
-        c->E.resize((size_t)c->ns * c->ncol);
-        h2d(c->E.p, e, c->E.n, c->stream);
+        // kept cell-major on the device, [ncol][ns]: a chemistry lane reads
+        // its cell's ns values at fixed offsets from one address
+        const size_t ns = (size_t)c->ns, ncol = (size_t)c->ncol;
+        std::vector<double> cm(ns * ncol);
+        for (size_t s = 0; s < ns; ++s)
+            for (size_t q = 0; q < ncol; ++q) cm[q * ns + s] = e[s * ncol + q];
+        c->E.resize(ns * ncol);
+        h2d(c->E.p, cm.data(), c->E.n, c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
         c->has_E = true;
         return RS_OK;

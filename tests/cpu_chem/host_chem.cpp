@@ -9,6 +9,14 @@
 
 extern "C" int hc_ns(void) { return RS_NS; }
 
+// Cell c's emissions gathered from the species-major E (stride ncol) into ec,
+// the contiguous per-cell form the generated code reads; 0 without any.
+static const double* cell_emis(const double* E, long long c, long long ncol, double* ec) {
+    if (!E || c >= ncol) return nullptr;
+    for (int s = 0; s < RS_NS; ++s) ec[s] = E[s * ncol + c];
+    return ec;
+}
+
 // One attempt for every cell; unew raw; returns the sum of per-cell sse in
 // cell order.
 extern "C" double hc_attempt(long long nc, long long ncol, const double* K, const double* jfac,
@@ -19,9 +27,10 @@ extern "C" double hc_attempt(long long nc, long long ncol, const double* K, cons
         double sv[3 * RS_NS];
         for (int s = 0; s < RS_NS; ++s) sv[2 * RS_NS + s] = u[s * nc + c];
         const double jf = jfac ? jfac[c] : 1.0;
-        const double* Ec = (E && c < ncol) ? E + c : nullptr;
+        double ec[RS_NS];
+        const double* Ec = cell_emis(E, c, ncol, ec);
         double W[RS_NNZ];
-        tot += rs_ros23_attempt<1, 1, false>(K, jf, Ec, ncol, sv, W, dt, atol, rtol,
+        tot += rs_ros23_attempt<1, 1, false>(K, jf, Ec, sv, W, dt, atol, rtol,
                                              [&](int s, double un) { unew[s * nc + c] = un; });
     }
     return tot;
@@ -36,7 +45,8 @@ extern "C" void hc_vjp(long long nc, long long ncol, const double* K, const doub
             lc[s] = lam[s * nc + c];
         }
         const double jf = jfac ? jfac[c] : 1.0;
-        const double* Ec = (E && c < ncol) ? E + c : nullptr;
+        double ec[RS_NS];
+        const double* Ec = cell_emis(E, c, ncol, ec);
         double* Gc = (c < ncol) ? gE + c : nullptr;
         double W[RS_NNZ];
         rs_ros23_vjp<1, 1>(K, jf, Ec, ncol, sv, W, dt, [&](int s) { return lc[s]; },
@@ -68,9 +78,10 @@ extern "C" long long hc_attempt_wave(long long nc, long long ncol, const double*
             double* sv = slab + lane;
             for (int s = 0; s < RS_NS; ++s) sv[(2 * RS_NS + s) * 64] = u[s * nc + c];
             const double jf = jfac ? jfac[c] : 1.0;
-            const double* Ec = (E && c < ncol) ? E + c : nullptr;
+            double ec[RS_NS];
+            const double* Ec = cell_emis(E, c, ncol, ec);
             double W[RS_NNZ];
-            tot += rs_ros23_attempt<64, 1, false>(K, jf, Ec, ncol, sv, W, dt, atol, rtol,
+            tot += rs_ros23_attempt<64, 1, false>(K, jf, Ec, sv, W, dt, atol, rtol,
                                                   [&](int s, double un) { unew[s * nc + c] = un; });
             for (int x = 0; x < 3 * RS_NS * 64; ++x)
                 if ((x & 63) != lane && slab[x] != sentinel) ++bad;
