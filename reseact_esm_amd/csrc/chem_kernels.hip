@@ -635,34 +635,54 @@ rs_chem_percell_adj(RsPercellAdjArgs a RS_KPARAM) {
  * non-finite entry stays non-finite) applied first.  With a.nfc the step's
  * non-finite outputs are counted (adjoint_gradient.jl:763-779): a lane with
  * none issues no atomic. */
-extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK)
-rs_chem_vjp(RsChemVjpArgs a RS_KPARAM) {
-    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= a.nc) return;
-    RS_SLAB(sv);
-    RS_WDECL(W);
-#pragma unroll
-    for (int s = 0; s < RS_NS; ++s) sv[s * RS_SLAB_ST] = a.u[(long long)s * a.nc + cell];
-    const double jf = a.jfac ? a.jfac[cell] : 1.0;
-    const bool surf = cell < a.ncol;
-    const double* Ec = (surf && a.E) ? a.E + cell * RS_NS : nullptr;
-    double* Gc = (surf && a.gE) ? a.gE + cell : nullptr;
-    unsigned nlam = 0;
-    const int ngrad = rs_ros23_vjp<RS_SLAB_ST, RS_W_ST>(
-        RS_KPTR(a), jf, Ec, a.ncol, sv, W, a.dt,
-        [&](int s) {
-            double l = a.lam[(long long)s * a.nc + cell];
-            if (a.uout) l = l * ((a.uout[(long long)s * a.nc + cell] > 0.0) ? 1.0 : 0.0);
-            return l;
-        },
-        [&](int s, double v) {
-            nlam += !RS_ISFINITE(v);
-            a.lam[(long long)s * a.nc + cell] = v;
-        }, Gc);
-    if (a.nfc) {
-        if (nlam) atomicAdd(&a.nfc[0], (unsigned long long)nlam);
-        if (ngrad) atomicAdd(&a.nfc[1], (unsigned long long)ngrad);
+/* MB = 1 (rs_chem_vjp_mb, species counts up to 32): the clamp mask comes
+ * from a.mask_in, the sign bits of the step output written by the VJP of the
+ * step after (one 32-bit word per cell instead of NS doubles); MB = 0
+ * (rs_chem_vjp): from a.uout, or none.  Either writes the sign bits of its
+ * own input u to a.mask_out when that is given.  The factor is 1.0 or 0.0
+ * from the same predicate u > 0, so both give the same lambda. */
+#define RS_CHEM_VJP_KERNEL(NAME, MB)                                                       \
+    extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK) NAME(RsChemVjpArgs a RS_KPARAM) { \
+        const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;          \
+        if (cell >= a.nc) return;                                                         \
+        RS_SLAB(sv);                                                                      \
+        RS_WDECL(W);                                                                      \
+        _Pragma("unroll") for (int s = 0; s < RS_NS; ++s)                                 \
+            sv[s * RS_SLAB_ST] = a.u[(long long)s * a.nc + cell];                         \
+        if (RS_NS <= 32 && a.mask_out) {                                                  \
+            unsigned mo = 0;                                                              \
+            _Pragma("unroll") for (int s = 0; s < RS_NS; ++s)                             \
+                mo |= (sv[s * RS_SLAB_ST] > 0.0 ? 1u : 0u) << (s & 31);                   \
+            a.mask_out[cell] = mo;                                                        \
+        }                                                                                 \
+        const double jf = a.jfac ? a.jfac[cell] : 1.0;                                    \
+        const bool surf = cell < a.ncol;                                                  \
+        const double* Ec = (surf && a.E) ? a.E + cell * RS_NS : nullptr;                  \
+        double* Gc = (surf && a.gE) ? a.gE + cell : nullptr;                              \
+        unsigned nlam = 0;                                                                \
+        const int ngrad = rs_ros23_vjp<RS_SLAB_ST, RS_W_ST>(                              \
+            RS_KPTR(a), jf, Ec, a.ncol, sv, W, a.dt,                                      \
+            [&](int s) {                                                                  \
+                double l = a.lam[(long long)s * a.nc + cell];                             \
+                if (MB) {                                                                 \
+                    const unsigned bit = (a.mask_in[cell] >> (s & 31)) & 1u;              \
+                    l = l * (bit ? 1.0 : 0.0);                                            \
+                } else if (a.uout) {                                                      \
+                    l = l * ((a.uout[(long long)s * a.nc + cell] > 0.0) ? 1.0 : 0.0);     \
+                }                                                                         \
+                return l;                                                                 \
+            },                                                                            \
+            [&](int s, double v) {                                                        \
+                nlam += !RS_ISFINITE(v);                                                  \
+                a.lam[(long long)s * a.nc + cell] = v;                                    \
+            }, Gc);                                                                       \
+        if (a.nfc) {                                                                      \
+            if (nlam) atomicAdd(&a.nfc[0], (unsigned long long)nlam);                     \
+            if (ngrad) atomicAdd(&a.nfc[1], (unsigned long long)ngrad);                   \
+        }                                                                                 \
     }
-}
+
+RS_CHEM_VJP_KERNEL(rs_chem_vjp, 0)
+RS_CHEM_VJP_KERNEL(rs_chem_vjp_mb, 1)
 
 #endif /* RS_HOST_BUILD */

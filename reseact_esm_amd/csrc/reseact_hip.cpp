@@ -139,7 +139,7 @@ struct DevArray {
 struct ChemModule {
     hipModule_t mod = nullptr;
     hipFunction_t attempt = nullptr, attempt_kp = nullptr, attempt_pd = nullptr, percell = nullptr,
-                  vjp = nullptr, percell_adj = nullptr;
+                  vjp = nullptr, vjp_mb = nullptr, percell_adj = nullptr;
     rs::MechPlan plan;
     ~ChemModule() { if (mod) (void)hipModuleUnload(mod); }
 };
@@ -225,6 +225,7 @@ struct rs_ctx {
     bool have_pbl = false;
     DevArray<double> red;
     DevArray<unsigned long long> head;
+    DevArray<unsigned> cmask;   // backward sweep: clamp masks as sign bits, two buffers
     DevArray<long long> stats;
     DevArray<double> dtcell;
     RsCtrl* dctrl = nullptr;
@@ -1137,9 +1138,12 @@ void enqueue_ssprk43_vjp_ppm(rs_ctx* c, const double* u, double dt,
 }
 
 void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt,
-                       unsigned long long* nfc = nullptr) {
+                       unsigned long long* nfc = nullptr,
+                       const unsigned* mask_in = nullptr, unsigned* mask_out = nullptr) {
     RsChemVjpArgs a{};
     a.nfc = nfc;
+    a.mask_in = mask_in;
+    a.mask_out = mask_out;
     a.u = u;
     a.uout = uout;
     a.lam = c->lam.p;
@@ -1150,7 +1154,8 @@ void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt
     a.nc = c->nc;
     a.ncol = c->ncol;
     a.dt = dt;
-    chem_launch(c, c->chem->vjp, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEMVJP);
+    chem_launch(c, mask_in ? c->chem->vjp_mb : c->chem->vjp, &a, sizeof(a), chem_grid(c),
+                chem_threads(c), P_CHEMVJP);
 }
 
 // The objective at state u: loss kind 0, J = sum over level-0 cells of
@@ -1298,6 +1303,7 @@ void backward_seed(rs_ctx* c, int kind, int species, int nvjp) {
                                     (long long)species * c->nc, c->ncol, c->stream));
     HIPCHK(rs_launch_fill(c->gE.p, (long long)c->gE.n, 0.0, c->stream));
     HIPCHK(rs_launch_fill_u64(c->nfc.p, 2 * (nvjp + 1), 0ULL, c->stream));
+    if (c->ns <= 32 && c->have_mech) c->cmask.ensure(2 * (size_t)c->nc);   // sign bits, before any capture
     HIPCHK(rs_launch_nonfinite(c->lam.p, (long long)N, c->nfc.p, c->stream));
 }
 
@@ -1399,10 +1405,19 @@ void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* re
                                 relpart, c->stream));
         if (ev1) HIPCHK(hipEventRecord(ev1, c->stream));
     }
+    // The clamp mask of chemistry step q is 1[u_{q+1} > 0], and u_{q+1} is the
+    // input of step q + 1, whose VJP runs just before: that VJP writes its
+    // input's sign bits (one word per cell, two buffers used in turn) and step
+    // q reads them instead of u_{q+1} itself.  The last step reads its output.
+    const bool clampC = clamp_nonneg_of(o.clamp_nonneg) != 0;
+    const bool bits = clampC && c->ns <= 32 && c->cmask.n >= 2 * (size_t)c->nc;
     for (size_t q = c->rec_percell ? 0 : sC.size(); q-- > 0; slot += 2) {
         const size_t ts = nT + q;
-        enqueue_ros23_vjp(c, tp + ts * N, clamp_nonneg_of(o.clamp_nonneg) ? tp + (ts + 1) * N : nullptr,
-                          sC[q].second, slot);
+        const bool last = q + 1 == sC.size();
+        const unsigned* mi = (bits && !last) ? c->cmask.p + ((q + 1) & 1) * c->nc : nullptr;
+        unsigned* mo = (bits && q > 0) ? c->cmask.p + (q & 1) * c->nc : nullptr;
+        enqueue_ros23_vjp(c, tp + ts * N, (clampC && !mi) ? tp + (ts + 1) * N : nullptr,
+                          sC[q].second, slot, mi, mo);
     }
     for (size_t q = nT; q-- > 0; slot += 2) {
         if (clamp_nonneg_of(o.clamp_nonneg)) {
@@ -1863,6 +1878,7 @@ int install_mechanism(rs_ctx* c, rs::MechSpec& m, const double* k) {
         HIPCHK(hipModuleGetFunction(&cm->attempt_pd, cm->mod, "rs_chem_attempt_pd"));
         HIPCHK(hipModuleGetFunction(&cm->percell, cm->mod, "rs_chem_percell"));
         HIPCHK(hipModuleGetFunction(&cm->vjp, cm->mod, "rs_chem_vjp"));
+        HIPCHK(hipModuleGetFunction(&cm->vjp_mb, cm->mod, "rs_chem_vjp_mb"));
         HIPCHK(hipModuleGetFunction(&cm->percell_adj, cm->mod, "rs_chem_percell_adj"));
         c->K.resize(std::max(nr, 1));
         if (nr) h2d(c->K.p, k, nr, c->stream);

@@ -94,6 +94,12 @@ typedef struct RsChemVjpArgs {
     double dt;
     unsigned long long* nfc;    /* or 0: [0] += non-finite lam outputs, [1] += non-finite
                                    emission-gradient contributions of this step */
+    /* The clamp mask as sign bits, one word per cell (bit s: species s > 0;
+     * species counts up to 32): rs_chem_vjp_mb reads mask_in instead of uout;
+     * mask_out, if given, gets the bits of u, which are the mask of the step
+     * before in the backward sweep. */
+    const unsigned* mask_in;
+    unsigned* mask_out;
 } RsChemVjpArgs;
 
 /* One transport stage over all transported species (see transport.hip). */
