@@ -1409,8 +1409,12 @@ void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* re
     // input of step q + 1, whose VJP runs just before: that VJP writes its
     // input's sign bits (one word per cell, two buffers used in turn) and step
     // q reads them instead of u_{q+1} itself.  The last step reads its output.
+    // RS_VJP_SIGN_BITS=0 turns this off (every VJP reads its output), so the
+    // two paths can be compared.
+    const char* sb = std::getenv("RS_VJP_SIGN_BITS");
     const bool clampC = clamp_nonneg_of(o.clamp_nonneg) != 0;
-    const bool bits = clampC && c->ns <= 32 && c->cmask.n >= 2 * (size_t)c->nc;
+    const bool bits = clampC && c->ns <= 32 && c->cmask.n >= 2 * (size_t)c->nc &&
+                      !(sb && sb[0] == '0');
     for (size_t q = c->rec_percell ? 0 : sC.size(); q-- > 0; slot += 2) {
         const size_t ts = nT + q;
         const bool last = q + 1 == sC.size();
