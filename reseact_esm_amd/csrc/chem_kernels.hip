@@ -56,7 +56,7 @@ template <int ST, int WS, bool KEEP_U, class Out>
 RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
                               const double* __restrict__ E,
                               double* __restrict__ sv, double* __restrict__ W, double dt,
-                              double atol, double rtol, Out out) {
+                              double atol, double rtol, Out out, bool unew_only = false) {
     /* two short-lived register vectors; the solves work in place */
     double f[RS_NS], y[RS_NS];
     rs_rhs<ST>(K, jf, E, &RS_SC(0), f);                        /* f0, :282 */
@@ -119,6 +119,12 @@ RS_FN double rs_ros23_attempt(const double* __restrict__ K, double jf,
         RS_SB(i) = RS_SB(i) - w2;
         y[i] = un;
     }
+    /* A replay on a known step sequence needs unew alone: it stops here,
+       before f2, k3 and the error estimate.  A uniform run-time flag rather
+       than a second kernel, so that the replay executes the forward's own
+       instructions up to this point and reproduces its unew bit for bit
+       (separately compiled code may contract multiply-adds differently). */
+    if (unew_only) return 0.0;
     RS_AFENCE(32);
     rs_rhs(K, jf, E, y, f);                                    /* f2, :318 */
     RS_AFENCE(32);
@@ -336,7 +342,7 @@ typedef struct { double k[RS_NR]; } RsKTab;
                 KEXPR, jf, Ec, sv, W, dt, a.atol, a.rtol, [&](int s, double un) {           \
                     if (POSDOM && un < 0.0) neg += 1.0;                                     \
                     dst[(long long)s * a.nc + cell] = clamp ? rs_max(un, 0.0) : un;         \
-                });                                                                         \
+                }, a.unew_only != 0);                                                       \
         }                                                                                   \
         const double tot = rs_block_sum(sse, sh);                                           \
         if (threadIdx.x == 0) a.partial[blockIdx.x] = tot;                                  \

@@ -95,11 +95,11 @@ hipError_t rs_launch_ppm_diag_mz(const double* Mx, const double* My, const doubl
 namespace {
 
 enum ProfKind { P_TSTAGE, P_TLAST, P_CTRL, P_CHEM, P_PERCELL, P_CHEMVJP, P_TADJ, P_UTIL,
-                P_TFUSED, P_TFUSEDADJ, P_TPPMADJ, P_COUNT };
+                P_TFUSED, P_TFUSEDADJ, P_TPPMADJ, P_CHEMUNEW, P_COUNT };
 const char* kProfNames[P_COUNT] = {"transport_stage", "transport_last", "ctrl_update",
                                    "chem_attempt",    "chem_percell",   "chem_vjp",
                                    "transport_adj",   "util",           "transport_fused",
-                                   "transport_fused_adj", "ppm_vjp"};
+                                   "transport_fused_adj", "ppm_vjp",  "chem_unew"};
 
 struct HipError {
     hipError_t e;
@@ -226,6 +226,9 @@ struct rs_ctx {
     DevArray<double> red;
     DevArray<unsigned long long> head;
     DevArray<unsigned> cmask;   // backward sweep: clamp masks as sign bits, two buffers
+    // replayed chemistry steps run the unew-only kernels (RS_REPLAY_UNEW=0: the
+    // full attempt, for comparison)
+    bool replay_unew = true;
     DevArray<long long> stats;
     DevArray<double> dtcell;
     RsCtrl* dctrl = nullptr;
@@ -674,8 +677,11 @@ void chem_launch(rs_ctx* c, hipFunction_t f, const void* argp, size_t argsz, uns
 unsigned chem_grid(rs_ctx* c) { return (unsigned)((c->nc + chem_threads(c) - 1) / chem_threads(c)); }
 
 
+// unew_only (static dt only): the step's new state alone, for the
+// fixed-sequence replay; the attempt kernel returns before f2, k3 and the
+// error estimate (its partials are then zero).
 void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cur_static,
-                   double dt, double atol, double rtol, int flags) {
+                   double dt, double atol, double rtol, int flags, bool unew_only = false) {
     const int clamp = clamp_nonneg_of(flags);
     RsChemArgs a{};
     a.posdom = posdom_of(flags);
@@ -694,9 +700,11 @@ void enqueue_ros23(rs_ctx* c, const RsCtrl* ctrl, double* S0, double* S1, int cu
     a.rtol = rtol;
     a.cur_static = cur_static;
     a.clamp = clamp;
-    if (a.posdom) chem_launch(c, c->chem->attempt_pd, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEM, false);
-    else if (c->chem_kp) chem_launch(c, c->chem->attempt_kp, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEM, false);
-    else chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), chem_threads(c), P_CHEM);
+    a.unew_only = (unew_only && !ctrl) ? 1 : 0;
+    const int kind = a.unew_only ? P_CHEMUNEW : P_CHEM;
+    if (a.posdom) chem_launch(c, c->chem->attempt_pd, &a, sizeof(a), chem_grid(c), chem_threads(c), kind, false);
+    else if (c->chem_kp) chem_launch(c, c->chem->attempt_kp, &a, sizeof(a), chem_grid(c), chem_threads(c), kind, false);
+    else chem_launch(c, c->chem->attempt, &a, sizeof(a), chem_grid(c), chem_threads(c), kind);
 }
 
 // Sum of n device partials: folded on the device to at most 1024 values first.
@@ -1270,7 +1278,7 @@ size_t replay_into_tape(rs_ctx* c, const double* u0, const std::vector<std::pair
     }
     for (auto& p : sC) {
         enqueue_ros23(c, nullptr, tp + j * N, tp + (j + 1) * N, 0, p.second, o.atol_C, o.rtol_C,
-                      clamp_nonneg_of(o.clamp_nonneg));
+                      clamp_nonneg_of(o.clamp_nonneg), c->replay_unew);
         ++j;
     }
     return nslots;
@@ -1505,6 +1513,7 @@ int rs_create(rs_ctx** out, int device) {
     if (device < 0 || device >= ndev) return RS_E_ARG;
     auto* c = new rs_ctx();
     c->device = device;
+    if (const char* ev = std::getenv("RS_REPLAY_UNEW")) c->replay_unew = ev[0] != '0';
     int rc = guarded(c, [&]() {
         HIPCHK(hipSetDevice(device));
         hipDeviceProp_t prop;

@@ -38,7 +38,8 @@ typedef struct RsChemArgs {
     double dt_static, atol, rtol;
     int cur_static, clamp;
     double* negpart;        /* positive domain: negative entries per block, or 0 */
-    int posdom, pad;
+    int posdom;
+    int unew_only;          /* 1: stop after unew (the fixed-sequence replay; static dt) */
 } RsChemArgs;
 
 /* Per-cell adaptive chemistry, persistent work-stealing launch. */
