@@ -833,7 +833,16 @@ int run_adaptive(rs_ctx* c, Op op, double t0, double t1, double* dt_io, const rs
         adaptive_chunk(c, ar, chunk);
         HIPCHK(hipMemcpyAsync(c->hctrl, c->dctrl, sizeof(RsCtrl), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        chunk = std::min(chunk * 2, 64);
+        // doubling, but no more than the steps left at the current step size
+        // plus one (a rejection): launches after the controller is done only
+        // return, and the last doubling used to overshoot by up to 63
+        int next = std::min(chunk * 2, 64);
+        const RsCtrl& h = *c->hctrl;
+        if (!h.done && h.dt > 0.0) {
+            const double left = (h.t1 - h.t) / h.dt;
+            if (left < next - 1) next = std::max(2, (int)std::ceil(left) + 1);
+        }
+        chunk = next;
     }
     return adaptive_finish(c, ar, dt_io, out, seq);
 }
