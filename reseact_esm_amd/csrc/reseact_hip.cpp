@@ -1035,13 +1035,18 @@ int macro(rs_ctx* c, double t0, double t1, double* dtT, double* dtC, const rs_sp
 }
 
 // lam <- SSPRK43 VJP (lam in c->lam; species >= nt untouched: identity)
-void enqueue_ssprk43_vjp(rs_ctx* c, double dt) {
+// lmask / nfcount (fused kernel only): the clamp mask of the step output
+// applied as lambda is read, and the count of non-finite values written.
+void enqueue_ssprk43_vjp(rs_ctx* c, double dt, const double* lmask = nullptr,
+                         unsigned long long* nfcount = nullptr) {
     RsTransArgs a = trans_args(c);
     a.dt_static = dt;
     a.a_is_state = 0;
     a.out_is_state = 0;
     if (trans_fused(c)) {
         set_fused_plan(a, c);
+        a.lmask = lmask;
+        a.nfcount = nfcount;
         a.a = c->lam.p;
         if (c->fplan[2] == 0) {
             // whole periodic rows: each workgroup reads its slab before it
@@ -1440,7 +1445,16 @@ void enqueue_backward_macro(rs_ctx* c, int m, const rs_split_opts& o, double* re
         enqueue_ros23_vjp(c, tp + ts * N, (clampC && !mi) ? tp + (ts + 1) * N : nullptr,
                           sC[q].second, slot, mi, mo);
     }
+    // The fused upwind VJP applies the mask as it reads lambda and counts the
+    // non-finite values it writes, when it writes every entry (all species
+    // transported); otherwise a mask pass before and a count pass after.
+    const bool fold = trans_fused(c) && !trans_ppm(c) && c->nt == c->ns;
     for (size_t q = nT; q-- > 0; slot += 2) {
+        if (fold) {
+            enqueue_ssprk43_vjp(c, sT[q].second,
+                                clamp_nonneg_of(o.clamp_nonneg) ? tp + (q + 1) * N : nullptr, slot);
+            continue;
+        }
         if (clamp_nonneg_of(o.clamp_nonneg)) {
             ProfScope ps(c, P_UTIL);
             HIPCHK(rs_launch_mask(c->lam.p, tp + (q + 1) * N, (long long)N, c->stream));

@@ -127,6 +127,11 @@ typedef struct RsTransArgs {
     double* negpart;        /* positive domain: negative entries per block, or 0 */
     int posdom;
     int fnth;               /* fused kernels: lanes per workgroup (1024 or 512) */
+    /* fused adjoint only: lambda is multiplied by 1[lmask > 0] as it is read
+     * (the clamp mask of the step output), and *nfcount gets the number of
+     * non-finite values written; 0 = neither */
+    const double* lmask;
+    unsigned long long* nfcount;
 } RsTransArgs;
 
 /* Monotone PPM transport stage (CWC mixing-ratio form; species 0 = air

@@ -449,7 +449,13 @@ __global__ void __launch_bounds__(NTH) rs_transport_fused(RsTransArgs P) {
     for (int jj = 0; jj < FL; ++jj) {
         v[jj] = 0.0;
         y[jj] = 0.0;
-        if (jj < nk) v[jj] = pa[g0 + (k0 + jj) * plane];
+        if (jj < nk) {
+            const long long ix = g0 + (k0 + jj) * plane;
+            v[jj] = pa[ix];
+            // the clamp mask of the step output, as the separate mask pass
+            // computes it: lambda * (u_out > 0 ? 1 : 0)
+            if (ADJ && P.lmask) v[jj] = v[jj] * ((P.lmask[ix] > 0.0) ? 1.0 : 0.0);
+        }
     }
     put([&](int jj) { return v[jj]; });
     __syncthreads();
@@ -494,9 +500,18 @@ __global__ void __launch_bounds__(NTH) rs_transport_fused(RsTransArgs P) {
     double sq = 0.0, ng = 0.0;
     if (ADJ) {
         // out = 2/3 lambda4 + (x + dt/2 L^T x)
+        unsigned nbad = 0;
         fstage_adj(C, nk, h2, [&](int jj, double z) {
-            if (keep) po[g0 + (k0 + jj) * plane] = C23 * v[jj] + z;
+            if (!keep) return;
+            const double o = C23 * v[jj] + z;
+            po[g0 + (k0 + jj) * plane] = o;
+            nbad += !__builtin_isfinite(o);
         });
+        if (P.nfcount) {
+            // one atomic per wave that wrote a non-finite value
+            for (int off = 32; off > 0; off >>= 1) nbad += __shfl_down(nbad, off, 64);
+            if ((tid & 63) == 0 && nbad) atomicAdd(P.nfcount, (unsigned long long)nbad);
+        }
     } else {
         fstage_fwd(C, nk, h2, [&](int jj, double un) {
             if (!keep) return;
