@@ -226,8 +226,8 @@ struct rs_ctx {
     DevArray<double> red;
     DevArray<unsigned long long> head;
     DevArray<unsigned> cmask;   // backward sweep: clamp masks as sign bits, two buffers
-    // replayed chemistry steps run the unew-only kernels (RS_REPLAY_UNEW=0: the
-    // full attempt, for comparison)
+    // replayed chemistry attempts stop at unew (RsChemArgs.unew_only;
+    // RS_REPLAY_UNEW=0: full attempts, for comparison)
     bool replay_unew = true;
     DevArray<long long> stats;
     DevArray<double> dtcell;
