@@ -311,6 +311,15 @@ typedef struct { double k[RS_NR]; } RsKTab;
 #define RS_SLAB(name) double name##_arr[3 * RS_NS]; double* name = name##_arr
 #endif
 
+/* Kernel groups of a split compile: with RS_JIT_ONLY=n only group n is
+ * emitted (1-3 the three attempts, 4 the per-cell solve, 5 its adjoint, 6 and
+ * 7 the two VJPs), so the groups compile in parallel; without it, all. */
+#ifndef RS_JIT_ONLY
+#define RS_JIT_PART(n) 1
+#else
+#define RS_JIT_PART(n) (RS_JIT_ONLY == (n))
+#endif
+
 /* One attempt over all cells, in three entry points: the rate constants as a
  * kernel argument (scalar loads; RS_KPARAM) or through the pointer in the
  * argument block (vector loads, fewer SGPRs held), the autotuner keeping the
@@ -353,9 +362,15 @@ typedef struct { double k[RS_NR]; } RsKTab;
         }                                                                                   \
     }
 
+#if RS_JIT_PART(1)
 RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt, RS_KPARAM, RS_KPTR(a), 0)
+#endif
+#if RS_JIT_PART(2)
 RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt_kp, , a.K, 0)
+#endif
+#if RS_JIT_PART(3)
 RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt_pd, , a.K, 1)
+#endif
 
 /* Persistent per-cell adaptive solve with wave-aggregated work stealing: a
  * lane that finishes its cell takes the next index from the queue head; all
@@ -367,6 +382,7 @@ RS_CHEM_ATTEMPT_KERNEL(rs_chem_attempt_pd, , a.K, 1)
 #else
 #define RS_PERCELL_ATTR
 #endif
+#if RS_JIT_PART(4)
 extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK) RS_PERCELL_ATTR
 rs_chem_percell(RsPercellArgs a RS_KPARAM) {
     const int lane = threadIdx.x & 63;
@@ -466,6 +482,7 @@ rs_chem_percell(RsPercellArgs a RS_KPARAM) {
 }
 
 #undef RS_U
+#endif
 
 /* Adjoint of the per-cell solve (the reference's backward sweep,
  * adjoint_gradient.jl:731-834, over each cell's own accepted sequence).
@@ -479,6 +496,7 @@ rs_chem_percell(RsPercellArgs a RS_KPARAM) {
  * independently.  A cell that would need more than a.nmax recorded states
  * is counted in stats[6] and left without its sweep (the host fails the
  * call). */
+#if RS_JIT_PART(5)
 extern "C" __global__ void __launch_bounds__(RS_CHEM_BLOCK) RS_PERCELL_ATTR
 rs_chem_percell_adj(RsPercellAdjArgs a RS_KPARAM) {
     const int lane = threadIdx.x & 63;
@@ -635,6 +653,7 @@ rs_chem_percell_adj(RsPercellAdjArgs a RS_KPARAM) {
 }
 
 #undef RS_U
+#endif
 
 /* One ROS23 step VJP for every cell, with the clamp mask of that step
  * (lam .*= 1[uout > 0], adjoint_gradient.jl:736; a multiplication, so a
@@ -688,7 +707,11 @@ rs_chem_percell_adj(RsPercellAdjArgs a RS_KPARAM) {
         }                                                                                 \
     }
 
+#if RS_JIT_PART(6)
 RS_CHEM_VJP_KERNEL(rs_chem_vjp, 0)
+#endif
+#if RS_JIT_PART(7) && RS_NS <= 32   /* one mask bit per species */
 RS_CHEM_VJP_KERNEL(rs_chem_vjp_mb, 1)
+#endif
 
 #endif /* RS_HOST_BUILD */

@@ -31,6 +31,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <sys/stat.h>
@@ -136,12 +137,18 @@ struct DevArray {
     void ensure(size_t m) { if (m > n) resize(m); }
 };
 
+// The generated kernels come in RS_JIT_GROUPS code objects (compiled side by
+// side), one module each.
+#define RS_JIT_GROUPS 7
 struct ChemModule {
-    hipModule_t mod = nullptr;
+    hipModule_t mod[RS_JIT_GROUPS] = {};
     hipFunction_t attempt = nullptr, attempt_kp = nullptr, attempt_pd = nullptr, percell = nullptr,
                   vjp = nullptr, vjp_mb = nullptr, percell_adj = nullptr;
     rs::MechPlan plan;
-    ~ChemModule() { if (mod) (void)hipModuleUnload(mod); }
+    ~ChemModule() {
+        for (hipModule_t m : mod)
+            if (m) (void)hipModuleUnload(m);
+    }
 };
 
 // Device memory of rs_ros23_step_noalloc, for callers that cannot allocate:
@@ -363,8 +370,10 @@ std::vector<std::string> jit_extra_options() {
     return out;
 }
 
-std::vector<char> jit_compile(const std::string& src, bool exact, std::string& log) {
-    const std::vector<std::string> extra = jit_extra_options();
+std::vector<char> jit_compile(const std::string& src, bool exact, std::string& log,
+                              const std::string& define = "") {
+    std::vector<std::string> extra = jit_extra_options();
+    if (!define.empty()) extra.push_back(define);
     std::string xk;
     for (auto& w : extra) xk += w + " ";
     const std::string key = std::to_string(fnv1a(src + "|" + xk)) + "_" + std::to_string(src.size()) +
@@ -1176,6 +1185,8 @@ void enqueue_ros23_vjp(rs_ctx* c, const double* u, const double* uout, double dt
     a.nc = c->nc;
     a.ncol = c->ncol;
     a.dt = dt;
+    if (mask_in && !c->chem->vjp_mb)
+        throw ArgError{"sign-bit clamp masks need at most 32 species", RS_E_STATE};
     chem_launch(c, mask_in ? c->chem->vjp_mb : c->chem->vjp, &a, sizeof(a), chem_grid(c),
                 chem_threads(c), P_CHEMVJP);
 }
@@ -1905,17 +1916,46 @@ int install_mechanism(rs_ctx* c, rs::MechSpec& m, const double* k) {
                 return fail(c, RS_E_ARG, "RS_JIT_EXACT=1: the exact-arithmetic chemistry build "
                                          "is disabled (wrong lanes on repeated launches, cause "
                                          "not found)");
-        std::vector<char> code = jit_compile(chem_source(m, cm->plan), false, log);
-        if (code.empty()) return fail(c, RS_E_JIT, "hipRTC compile failed: " + log.substr(0, 2000));
+        // One program per kernel group (RS_JIT_ONLY), compiled on threads of
+        // their own: a large mechanism's straight-line kernels take minutes
+        // one after the other (RS_JIT_THREADS=0: in turn, on this thread).
+        const std::string src = chem_source(m, cm->plan);
+        std::vector<char> code[RS_JIT_GROUPS];
+        std::string logs[RS_JIT_GROUPS];
+        // the sign-bit VJP (group 7) exists for at most 32 species only
+        const int ngroups = m.ns <= 32 ? RS_JIT_GROUPS : RS_JIT_GROUPS - 1;
+        double secs[RS_JIT_GROUPS] = {};
+        auto build = [&](int g) {
+            const auto t0 = std::chrono::steady_clock::now();
+            code[g] = jit_compile(src, false, logs[g], "-DRS_JIT_ONLY=" + std::to_string(g + 1));
+            secs[g] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        };
+        const char* jt = std::getenv("RS_JIT_THREADS");
+        if (jt && jt[0] == '0') {
+            for (int g = 0; g < ngroups; ++g) build(g);
+        } else {
+            std::vector<std::thread> th;
+            for (int g = 0; g < ngroups; ++g) th.emplace_back(build, g);
+            for (auto& t : th) t.join();
+        }
+        if (const char* tv = std::getenv("RS_JIT_TIMES"))   // developer knob: compile times
+            if (tv[0] == '1')
+                for (int g = 0; g < ngroups; ++g)
+                    std::fprintf(stderr, "[reseact] chemistry kernel group %d: %.1f s\n", g + 1,
+                                 secs[g]);
+        for (int g = 0; g < ngroups; ++g)
+            if (code[g].empty())
+                return fail(c, RS_E_JIT, "hipRTC compile failed: " + logs[g].substr(0, 2000));
         HIPCHK(hipSetDevice(c->device));
-        HIPCHK(hipModuleLoadData(&cm->mod, code.data()));
-        HIPCHK(hipModuleGetFunction(&cm->attempt, cm->mod, "rs_chem_attempt"));
-        HIPCHK(hipModuleGetFunction(&cm->attempt_kp, cm->mod, "rs_chem_attempt_kp"));
-        HIPCHK(hipModuleGetFunction(&cm->attempt_pd, cm->mod, "rs_chem_attempt_pd"));
-        HIPCHK(hipModuleGetFunction(&cm->percell, cm->mod, "rs_chem_percell"));
-        HIPCHK(hipModuleGetFunction(&cm->vjp, cm->mod, "rs_chem_vjp"));
-        HIPCHK(hipModuleGetFunction(&cm->vjp_mb, cm->mod, "rs_chem_vjp_mb"));
-        HIPCHK(hipModuleGetFunction(&cm->percell_adj, cm->mod, "rs_chem_percell_adj"));
+        for (int g = 0; g < ngroups; ++g) HIPCHK(hipModuleLoadData(&cm->mod[g], code[g].data()));
+        HIPCHK(hipModuleGetFunction(&cm->attempt, cm->mod[0], "rs_chem_attempt"));
+        HIPCHK(hipModuleGetFunction(&cm->attempt_kp, cm->mod[1], "rs_chem_attempt_kp"));
+        HIPCHK(hipModuleGetFunction(&cm->attempt_pd, cm->mod[2], "rs_chem_attempt_pd"));
+        HIPCHK(hipModuleGetFunction(&cm->percell, cm->mod[3], "rs_chem_percell"));
+        HIPCHK(hipModuleGetFunction(&cm->percell_adj, cm->mod[4], "rs_chem_percell_adj"));
+        HIPCHK(hipModuleGetFunction(&cm->vjp, cm->mod[5], "rs_chem_vjp"));
+        if (ngroups == RS_JIT_GROUPS)
+            HIPCHK(hipModuleGetFunction(&cm->vjp_mb, cm->mod[6], "rs_chem_vjp_mb"));
         c->K.resize(std::max(nr, 1));
         if (nr) h2d(c->K.p, k, nr, c->stream);
         c->hK.assign(k, k + nr);
